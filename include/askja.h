@@ -1,0 +1,182 @@
+/*
+ * askja.h -- C ABI of the MI355X SU(N) HMC engine (libaskja.so).
+ *
+ * The reference (paro8929/askja) has no library API: its hot path is a set of
+ * void functions over global arrays in sun.cpp.  Each entry point below
+ * replaces one of them (cited), operating on device-resident fields owned by a
+ * context.  Plain pointers and sizes only; no C++ or torch types.
+ *
+ * Conventions
+ *  - Every call returns 0 on success or a negative ASKJA_E* code; the message
+ *    is kept per context (askja_last_error) and errors are sticky: after a
+ *    failure every further call on that context returns ASKJA_ESTICKY until
+ *    askja_clear_error.
+ *  - One host thread per context.  Kernels run on the context's HIP stream;
+ *    only calls that return scalars or fill host buffers synchronise.
+ *  - Host buffers are caller-owned and copied.  Links and momenta cross the
+ *    boundary in the reference's U-*.conf layout: for each site in
+ *    lexicographic order (x[D-1] fastest), for each direction mu = 0..D-1, an
+ *    N x N complex matrix, row-major, (re, im) interleaved doubles
+ *    (sun.cpp:289-302, structs.h:130-165).  Length V*D*N*N*2 doubles.
+ *  - Device layout is private: links grouped by direction, sites in
+ *    checkerboard order (even sites, then odd, each in lexicographic order),
+ *    structure-of-arrays across 64-site tiles for N <= 3, matrix-contiguous
+ *    for N >= 4; momenta stored as Hermitian halves (N^2 reals per link).
+ */
+#ifndef ASKJA_H
+#define ASKJA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct askja_ctx askja_ctx;
+
+/* precision modes */
+enum {
+    ASKJA_FP64 = 0,       /* FP64 storage and arithmetic, FMA, factored staples */
+    ASKJA_FP64_EXACT = 1, /* FP64 in the reference's operation order, no FMA:
+                             bit-identical to the reference's arithmetic */
+    ASKJA_FP32 = 2,       /* FP32 links/momenta/MD; FP64 sums, reunitarisation */
+    ASKJA_BF16X3 = 3      /* FP32 storage, link products as split BF16 (hi*hi +
+                             hi*lo + lo*hi) on MFMA with FP32 accumulation */
+};
+
+/* error codes */
+enum {
+    ASKJA_OK = 0,
+    ASKJA_EINVAL = -1,   /* bad argument */
+    ASKJA_EHIP = -2,     /* HIP runtime error */
+    ASKJA_ENOMEM = -3,   /* device allocation failed */
+    ASKJA_EUNSUP = -4,   /* unsupported N / mode combination */
+    ASKJA_ESTICKY = -5,  /* an earlier call on this context failed */
+    ASKJA_EIO = -6       /* file read/write failed */
+};
+
+/* Context: allocate_memory + generators + xygrid.init
+ * (sun.cpp:96-125, 142-189, 1086-1097; structs.h:109-147).
+ * extents[0..dim-1] = N0..N_{D-1}; extents of 1 are allowed.
+ * Returns ASKJA_EUNSUP for nc > 64 or dim > 32. */
+int askja_create(int nc, int dim, const int *extents, int precision, askja_ctx **out);
+/* free_memory (sun.cpp:128-139) */
+int askja_destroy(askja_ctx *ctx);
+const char *askja_last_error(const askja_ctx *ctx);
+int askja_clear_error(askja_ctx *ctx);
+/* device ordinal, stream, sizes */
+int askja_info(const askja_ctx *ctx, long *volume, int *nc, int *dim, int *precision);
+/* Number of doubles in one host field buffer (V*D*N*N*2). */
+size_t askja_field_doubles(const askja_ctx *ctx);
+
+/* ---- links ---------------------------------------------------------------- */
+/* loadconf (sun.cpp:744-765), its in-memory half: links <- host buffer */
+int askja_links_upload(askja_ctx *ctx, const double *aos);
+/* storeconf (sun.cpp:289-302), its in-memory half: host buffer <- links */
+int askja_links_download(askja_ctx *ctx, double *aos);
+/* initializeU (sun.cpp:194-207): cold start, all links = identity */
+int askja_links_cold(askja_ctx *ctx);
+/* Haar-random SU(N) start (no reference counterpart): Ginibre matrix from
+ * Philox-4x32-10 (key = seed, stream selects the counter space), then the
+ * SU(N) projection of askja_reunitarize. */
+int askja_links_haar(askja_ctx *ctx, uint64_t seed, uint64_t stream);
+/* Project every link onto SU(N) in FP64 (no reference counterpart; the
+ * reference only monitors unitarity, sun.cpp:441-471): modified Gram-Schmidt
+ * on the rows, then multiply by exp(-i arg(det)/N). */
+int askja_reunitarize(askja_ctx *ctx);
+/* takeconfiguration / loadconfiguration (sun.cpp:725-742):
+ * op 0 = save links to the snapshot, 1 = restore links from it. */
+int askja_snapshot(askja_ctx *ctx, int op);
+
+/* ---- momenta --------------------------------------------------------------- */
+/* initializeE (sun.cpp:209-252), host-stream form: momenta <- host buffer
+ * (Hermitian matrices, same layout as links).  With the reference's own
+ * MT19937/polar stream generated on the host this reproduces its momenta
+ * exactly.  Resets the pre-kick momentum to the same field. */
+int askja_mom_upload(askja_ctx *ctx, const double *aos);
+/* host buffer <- current momentum field (the reference's Pi) */
+int askja_mom_download(askja_ctx *ctx, double *aos);
+/* host buffer <- momentum before the last kick (the reference's pi) */
+int askja_mom_download_prev(askja_ctx *ctx, double *aos);
+/* initializeE, counter-based form: pi = sqrt(2N/beta) sum_a xi_a T^a with
+ * xi from Philox-4x32-10 (key = seed, counter = traj_counter) and Box-Muller;
+ * generator order as sun.cpp:142-189. */
+int askja_mom_refresh(askja_ctx *ctx, uint64_t seed, uint64_t traj_counter, double beta);
+/* Flip the sign of the momenta (for reversibility checks). */
+int askja_mom_negate(askja_ctx *ctx);
+
+/* ---- molecular dynamics ------------------------------------------------------ */
+/* updateE (sun.cpp:672-723): Pi = pi - eps * TAH(U_mu(x) sum_nu staples).
+ * Out of place like the reference: the momentum before the kick stays
+ * available to the E_L and Gauss-law diagnostics. */
+int askja_kick(askja_ctx *ctx, double eps);
+/* updateUandcopyDown (sun.cpp:774-790): U = T_L(i eps Pi) U with T_L the
+ * order-`level` Taylor exp by Horner (sunmat.h:200-248; level outside 0..8
+ * means 8), then pi = Pi. */
+int askja_drift(askja_ctx *ctx, double eps, int level);
+/* The MD part of one trajectory (sun.cpp:1178-1218, 1249-1251):
+ * kick(dt/2), ndrift x [drift(dt); kick(dt) except after the last drift],
+ * kick(dt/2).  The reference's first trajectory has ndrift = UPDATE2-1. */
+int askja_md_trajectory(askja_ctx *ctx, double dt, int ndrift, int level);
+
+/* ---- reductions (FP64, compensated) ------------------------------------------ */
+/* EnergyMag (sun.cpp:507-537) split as PlaqExp (measurements4d.cpp:885-927):
+ * s_t = sum_x sum_{k>=1} (1 - ReTr P_0k/N), s_s = sum over 1 <= j < k.
+ * EnergyMag = s_t + s_s.  Either pointer may be NULL. */
+int askja_action(askja_ctx *ctx, double *s_t, double *s_s);
+/* simpleenergyEl (sun.cpp:540-548): sum Tr(Pi^2) of the current momenta */
+int askja_kinetic(askja_ctx *ctx, double *out);
+/* EnergyEl (sun.cpp:560-567): 1/4 sum tr2(pi + Pi) */
+int askja_energy_el(askja_ctx *ctx, double *out);
+/* GaussLaw (sun.cpp:415-438): tr2 of sum_{x,mu} [Pi_mu(x) - U^+ Pi U at x-mu] */
+int askja_gauss_law(askja_ctx *ctx, double *out);
+/* UViol (sun.cpp:441-471): sum ||U U^+ - 1||_F^2 */
+int askja_uviol(askja_ctx *ctx, double *out);
+
+/* ---- Wilson lines ------------------------------------------------------------- */
+/* Traces Tr(prod_{t} U_dir(x + t dir))/N of every straight line in direction
+ * dir that starts at x_dir = 0, ordered lexicographically by start site.
+ * out holds 2*V/extent[dir] doubles (re, im).  dir = 0 gives the Polyakov
+ * loops; WilsonLoop (sun.cpp:307-345) is entry 0, PolyakovLoop
+ * (measurements4d.cpp:410-466) the mean of |.| over x1 at x2.. = 0, and
+ * WilsonLoop(evs, dir) (measurements4d.cpp:311-355) the mean over x0. */
+int askja_wilson_lines(askja_ctx *ctx, int dir, double *out);
+
+/* ---- HMC -------------------------------------------------------------------- */
+typedef struct {
+    double beta;
+    double dt;
+    int ndrift;      /* drifts per trajectory (UPDATE2) */
+    int level;       /* Taylor order */
+    int reunit;      /* project the chain state onto SU(N) before each trajectory */
+    uint64_t seed;   /* Philox key for momenta and the Metropolis draw */
+} askja_hmc_params;
+
+typedef struct {
+    double h_old, h_new, dh;
+    double uniform;    /* Metropolis draw (only used when dh > 0) */
+    int accepted;
+    int accept_case;   /* 1 = H decreased, 2 = uniform < exp(-dH), 0 = rejected */
+} askja_hmc_result;
+
+/* One HMC trajectory of the reference's loop (sun.cpp:1249-1318) with a
+ * counter-based stream in place of the time(NULL) seeds: snapshot, momentum
+ * refresh (counter = traj_counter), H_old, MD, H_new, accept if H_new <
+ * H_old or u < exp(H_old - H_new) with u from Philox; on reject restore the
+ * snapshot.  NaN energies reject, as in the reference. */
+int askja_hmc_trajectory(askja_ctx *ctx, const askja_hmc_params *p, uint64_t traj_counter,
+                         askja_hmc_result *res);
+
+/* ---- raw-pointer entry points (device buffers owned by the caller) ----------- */
+/* Synchronise the context's stream. */
+int askja_sync(askja_ctx *ctx);
+/* Device pointers of the context's fields in the private layout (for callers
+ * that share device memory, e.g. a torch extension). */
+int askja_device_fields(askja_ctx *ctx, void **links, void **mom, void **mom_prev);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
