@@ -1,0 +1,770 @@
+// capi.hip -- the C ABI of include/askja.h: a context owning the
+// device-resident lattice (geometry tables, three link buffers, momenta,
+// reduction buffers) and the host side of every entry point.
+#include "../../include/askja.h"
+#include "kernels.hpp"
+#include "common.hpp"
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+using namespace askja;
+
+struct askja_ctx {
+    int nc = 0, dim = 0, prec = 0, mode = 0;
+    int ext[32] = {0};
+    long V = 0, NL = 0;
+    long helparr[32] = {0};
+    int tile = 1;
+    size_t link_reals = 0, mom_reals = 0;  // per field, padded to the tile
+    std::vector<int> lex2dev, dev2lex;
+
+    int *d_nbr = nullptr, *d_dev2lex = nullptr, *d_lex2dev = nullptr;
+    void *links[3] = {nullptr, nullptr, nullptr};
+    int cur = 0;
+    void *snap = nullptr;
+    void *mom = nullptr, *mom_prev = nullptr;
+    double *part = nullptr;
+    size_t part_doubles = 0;
+    double *res = nullptr;      // device, 64 doubles
+    double *scratch = nullptr;  // FP64 working memory for generic kernels
+    size_t scratch_doubles = 0;
+    double *staging = nullptr;  // host<->device AoS staging
+    size_t staging_doubles = 0;
+    hipStream_t st = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int sticky = 0;
+    std::string err;
+    Geom geom{};
+};
+
+namespace {
+
+int fail(askja_ctx *c, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) {
+        c->err = buf;
+        c->sticky = code;
+    }
+    return code;
+}
+
+#define HIPCHK(c, expr)                                                                        \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) return fail(c, ASKJA_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+#define ENTER(c)                                                               \
+    do {                                                                       \
+        if (!(c)) return ASKJA_EINVAL;                                         \
+        if ((c)->sticky) return ASKJA_ESTICKY;                                 \
+    } while (0)
+
+size_t elem_bytes(const askja_ctx *c) { return c->mode == kModeF32 ? 4 : 8; }
+
+int ensure_part(askja_ctx *c, size_t nd)
+{
+    if (nd <= c->part_doubles) return ASKJA_OK;
+    if (c->part) (void)hipFree(c->part);
+    c->part = nullptr;
+    c->part_doubles = 0;
+    HIPCHK(c, hipMalloc(&c->part, nd * sizeof(double)));
+    c->part_doubles = nd;
+    return ASKJA_OK;
+}
+
+int ensure_scratch(askja_ctx *c)
+{
+    if (c->scratch) return ASKJA_OK;
+    // enough for one FP64 working set per link, capped at 1 GiB
+    size_t want = (size_t)c->NL * (size_t)generic_scratch_per_thread(c->nc);
+    const size_t cap = (size_t)1 << 27, lo = (size_t)1 << 20;
+    if (want > cap) want = cap;
+    if (want < lo) want = lo;
+    HIPCHK(c, hipMalloc(&c->scratch, want * sizeof(double)));
+    c->scratch_doubles = want;
+    return ASKJA_OK;
+}
+
+int ensure_staging(askja_ctx *c)
+{
+    if (c->staging) return ASKJA_OK;
+    const size_t nd = (size_t)c->V * c->dim * c->nc * c->nc * 2;
+    HIPCHK(c, hipMalloc(&c->staging, nd * sizeof(double)));
+    c->staging_doubles = nd;
+    return ASKJA_OK;
+}
+
+// sum of the (s, c) pair at res[2k], res[2k+1] after a D2H copy
+double pair_val(const double *h, int k) { return h[2 * k] + h[2 * k + 1]; }
+
+int read_res(askja_ctx *c, double *host, int n)
+{
+    HIPCHK(c, hipMemcpyAsync(host, c->res, n * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return ASKJA_OK;
+}
+
+void fill_cinv(double *ci)
+{
+    ci[0] = 1.0;
+    for (int k = 1; k < 9; k++) ci[k] = 1. / (double)k;
+}
+
+KickArgs kick_args(askja_ctx *c, const void *U, void *Uout, void *Pprev, double ek, double ed, int level)
+{
+    KickArgs a{};
+    a.g = c->geom;
+    a.U = U;
+    a.Uout = Uout;
+    a.P = c->mom;
+    a.Pprev = Pprev;
+    a.eps_kick = ek;
+    a.eps_drift = ed;
+    a.level = level;
+    fill_cinv(a.cinv);
+    a.inv_n = 1. / (double)c->nc;
+    a.part = c->part;
+    return a;
+}
+
+// kick (optionally fused with a drift into `out`) + optional finalize of its
+// three sums into res[off .. off+5]
+int do_kick(askja_ctx *c, const void *U, void *Uout, void *Pprev, double ek, double ed, int level, bool drift,
+            int res_off)
+{
+    const long np = kick_partials(c->geom);
+    int rc = ensure_part(c, (size_t)np * 6);
+    if (rc) return rc;
+    KickArgs a = kick_args(c, U, Uout, Pprev, ek, ed, level);
+    HIPCHK(c, launch_kick(a, c->mode, drift, c->st));
+    if (res_off >= 0) HIPCHK(c, launch_finalize(c->part, np, 3, c->res + res_off, c->st));
+    return ASKJA_OK;
+}
+
+long n_plaquettes(const askja_ctx *c) { return c->V * (long)c->dim * (c->dim - 1) / 2; }
+
+// MD of one trajectory with the chain-state buffer `in0` left untouched;
+// returns the buffer index that holds the final links.  Energies of the
+// first and last kick go to res[0..5] and res[6..11].
+int md_pingpong(askja_ctx *c, double dt, int ndrift, int level, int *final_buf)
+{
+    const int a = c->cur, b = (a + 1) % 3, d = (a + 2) % 3;
+    int rc;
+    if (ndrift <= 0) {
+        // two half kicks on the same links
+        if ((rc = do_kick(c, c->links[a], nullptr, nullptr, dt / 2, 0.0, level, false, 0))) return rc;
+        if ((rc = do_kick(c, c->links[a], nullptr, nullptr, dt / 2, 0.0, level, false, 6))) return rc;
+        *final_buf = a;
+        return ASKJA_OK;
+    }
+    if ((rc = do_kick(c, c->links[a], c->links[b], nullptr, dt / 2, dt, level, true, 0))) return rc;
+    int in = b, out = d;
+    for (int t = 2; t <= ndrift; t++) {
+        if ((rc = do_kick(c, c->links[in], c->links[out], nullptr, dt, dt, level, true, -1))) return rc;
+        const int tmp = in;
+        in = out;
+        out = tmp;
+    }
+    if ((rc = do_kick(c, c->links[in], nullptr, nullptr, dt / 2, 0.0, level, false, 6))) return rc;
+    *final_buf = in;
+    return ASKJA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int askja_create(int nc, int dim, const int *extents, int precision, askja_ctx **out)
+{
+    if (!out || !extents) return ASKJA_EINVAL;
+    *out = nullptr;
+    if (nc < 1 || nc > 64 || dim < 1 || dim > 32) return ASKJA_EUNSUP;
+    if (precision < ASKJA_FP64 || precision > ASKJA_FP32) return ASKJA_EUNSUP;
+    long V = 1;
+    for (int i = 0; i < dim; i++) {
+        if (extents[i] < 1) return ASKJA_EINVAL;
+        V *= extents[i];
+        if (V * dim > (1L << 30)) return ASKJA_EUNSUP;
+    }
+    askja_ctx *c = new askja_ctx();
+    c->nc = nc;
+    c->dim = dim;
+    c->prec = precision;
+    c->mode = precision == ASKJA_FP64 ? kModeF64 : precision == ASKJA_FP64_EXACT ? kModeF64Exact : kModeF32;
+    c->V = V;
+    c->NL = V * dim;
+    for (int i = 0; i < dim; i++) c->ext[i] = extents[i];
+    for (int i = 0; i < dim; i++) {
+        c->helparr[i] = 1;
+        for (int j = i + 1; j < dim; j++) c->helparr[i] *= extents[j];
+    }
+    c->tile = nc <= 3 ? 64 : 1;
+    const size_t ntiles = (size_t)((c->NL + c->tile - 1) / c->tile);
+    c->link_reals = ntiles * c->tile * 2 * nc * nc;
+    c->mom_reals = ntiles * c->tile * nc * nc;
+
+    // checkerboard order: parity-0 sites then parity-1, each lexicographic
+    c->lex2dev.resize(V);
+    c->dev2lex.resize(V);
+    std::vector<int> x(dim);
+    long n0 = 0;
+    for (long i = 0; i < V; i++) {
+        long r = i;
+        int par = 0;
+        for (int d = 0; d < dim; d++) {
+            x[d] = (int)(r / c->helparr[d]);
+            r %= c->helparr[d];
+            par += x[d];
+        }
+        if ((par & 1) == 0) n0++;
+    }
+    long k0 = 0, k1 = n0;
+    for (long i = 0; i < V; i++) {
+        long r = i;
+        int par = 0;
+        for (int d = 0; d < dim; d++) {
+            par += (int)(r / c->helparr[d]);
+            r %= c->helparr[d];
+        }
+        const long s = (par & 1) == 0 ? k0++ : k1++;
+        c->lex2dev[i] = (int)s;
+        c->dev2lex[s] = (int)i;
+    }
+    // neighbour table in device order (periodic +-1 steps, structs.h:169-218)
+    std::vector<int> nbr((size_t)2 * dim * V);
+    for (long s = 0; s < V; s++) {
+        const long i = c->dev2lex[s];
+        for (int d = 0; d < dim; d++) {
+            const long xd = (i / c->helparr[d]) % extents[d];
+            const long up = (xd + 1) % extents[d], dn = (xd + extents[d] - 1) % extents[d];
+            nbr[(size_t)(2 * d) * V + s] = c->lex2dev[i + (up - xd) * c->helparr[d]];
+            nbr[(size_t)(2 * d + 1) * V + s] = c->lex2dev[i + (dn - xd) * c->helparr[d]];
+        }
+    }
+#define CREATE_CHK(expr)                                                        \
+    do {                                                                        \
+        hipError_t e_ = (expr);                                                 \
+        if (e_ != hipSuccess) {                                                 \
+            fail(c, ASKJA_EHIP, "%s: %s", #expr, hipGetErrorString(e_));        \
+            askja_destroy(c);                                                   \
+            return e_ == hipErrorOutOfMemory ? ASKJA_ENOMEM : ASKJA_EHIP;       \
+        }                                                                       \
+    } while (0)
+    CREATE_CHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+    CREATE_CHK(hipEventCreate(&c->ev0));
+    CREATE_CHK(hipEventCreate(&c->ev1));
+    CREATE_CHK(hipMalloc(&c->d_nbr, nbr.size() * sizeof(int)));
+    CREATE_CHK(hipMalloc(&c->d_dev2lex, V * sizeof(int)));
+    CREATE_CHK(hipMalloc(&c->d_lex2dev, V * sizeof(int)));
+    CREATE_CHK(hipMemcpy(c->d_nbr, nbr.data(), nbr.size() * sizeof(int), hipMemcpyHostToDevice));
+    CREATE_CHK(hipMemcpy(c->d_dev2lex, c->dev2lex.data(), V * sizeof(int), hipMemcpyHostToDevice));
+    CREATE_CHK(hipMemcpy(c->d_lex2dev, c->lex2dev.data(), V * sizeof(int), hipMemcpyHostToDevice));
+    const size_t eb = elem_bytes(c);
+    for (int b = 0; b < 3; b++) {
+        CREATE_CHK(hipMalloc(&c->links[b], c->link_reals * eb));
+        CREATE_CHK(hipMemset(c->links[b], 0, c->link_reals * eb));
+    }
+    CREATE_CHK(hipMalloc(&c->mom, c->mom_reals * eb));
+    CREATE_CHK(hipMalloc(&c->mom_prev, c->mom_reals * eb));
+    CREATE_CHK(hipMemset(c->mom, 0, c->mom_reals * eb));
+    CREATE_CHK(hipMemset(c->mom_prev, 0, c->mom_reals * eb));
+    CREATE_CHK(hipMalloc(&c->res, 64 * sizeof(double)));
+#undef CREATE_CHK
+    Geom &g = c->geom;
+    g.V = V;
+    g.D = dim;
+    g.N = nc;
+    g.tile = c->tile;
+    g.nbr = c->d_nbr;
+    g.dev2lex = c->d_dev2lex;
+    g.lex2dev = c->d_lex2dev;
+    for (int i = 0; i < dim; i++) {
+        g.ext[i] = extents[i];
+        g.helparr[i] = c->helparr[i];
+    }
+    if (askja_links_cold(c) != ASKJA_OK) {
+        askja_destroy(c);
+        return ASKJA_EHIP;
+    }
+    *out = c;
+    return ASKJA_OK;
+}
+
+int askja_destroy(askja_ctx *c)
+{
+    if (!c) return ASKJA_EINVAL;
+    if (c->st) (void)hipStreamSynchronize(c->st);
+    for (int b = 0; b < 3; b++)
+        if (c->links[b]) (void)hipFree(c->links[b]);
+    void *bufs[] = {c->snap, c->mom, c->mom_prev, c->part, c->res, c->scratch, c->staging,
+                    c->d_nbr, c->d_dev2lex, c->d_lex2dev};
+    for (void *p : bufs)
+        if (p) (void)hipFree(p);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->st) (void)hipStreamDestroy(c->st);
+    delete c;
+    return ASKJA_OK;
+}
+
+const char *askja_last_error(const askja_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int askja_clear_error(askja_ctx *c)
+{
+    if (!c) return ASKJA_EINVAL;
+    c->sticky = 0;
+    c->err.clear();
+    return ASKJA_OK;
+}
+
+int askja_info(const askja_ctx *c, long *volume, int *nc, int *dim, int *precision)
+{
+    if (!c) return ASKJA_EINVAL;
+    if (volume) *volume = c->V;
+    if (nc) *nc = c->nc;
+    if (dim) *dim = c->dim;
+    if (precision) *precision = c->prec;
+    return ASKJA_OK;
+}
+
+size_t askja_field_doubles(const askja_ctx *c) { return c ? (size_t)c->V * c->dim * c->nc * c->nc * 2 : 0; }
+
+int askja_links_upload(askja_ctx *c, const double *aos)
+{
+    ENTER(c);
+    if (!aos) return fail(c, ASKJA_EINVAL, "links_upload: null buffer");
+    int rc = ensure_staging(c);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->staging, aos, c->staging_doubles * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, launch_links_from_aos(c->geom, c->staging, c->links[c->cur], c->mode, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return ASKJA_OK;
+}
+
+int askja_links_download(askja_ctx *c, double *aos)
+{
+    ENTER(c);
+    if (!aos) return fail(c, ASKJA_EINVAL, "links_download: null buffer");
+    int rc = ensure_staging(c);
+    if (rc) return rc;
+    HIPCHK(c, launch_links_to_aos(c->geom, c->links[c->cur], c->staging, c->mode, c->st));
+    HIPCHK(c, hipMemcpyAsync(aos, c->staging, c->staging_doubles * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return ASKJA_OK;
+}
+
+int askja_links_cold(askja_ctx *c)
+{
+    ENTER(c);
+    HIPCHK(c, launch_links_cold(c->geom, c->links[c->cur], c->mode, c->st));
+    return ASKJA_OK;
+}
+
+int askja_links_haar(askja_ctx *c, uint64_t seed, uint64_t stream)
+{
+    ENTER(c);
+    int rc = ensure_scratch(c);
+    if (rc) return rc;
+    HIPCHK(c, launch_project(c->geom, c->links[c->cur], c->mode, 1, seed, stream, c->scratch, (long)c->scratch_doubles,
+                             c->st));
+    return ASKJA_OK;
+}
+
+int askja_reunitarize(askja_ctx *c)
+{
+    ENTER(c);
+    int rc = ensure_scratch(c);
+    if (rc) return rc;
+    HIPCHK(c, launch_project(c->geom, c->links[c->cur], c->mode, 0, 0, 0, c->scratch, (long)c->scratch_doubles, c->st));
+    return ASKJA_OK;
+}
+
+int askja_snapshot(askja_ctx *c, int op)
+{
+    ENTER(c);
+    const size_t bytes = c->link_reals * elem_bytes(c);
+    if (op == 0) {
+        if (!c->snap) HIPCHK(c, hipMalloc(&c->snap, bytes));
+        HIPCHK(c, hipMemcpyAsync(c->snap, c->links[c->cur], bytes, hipMemcpyDeviceToDevice, c->st));
+        return ASKJA_OK;
+    }
+    if (op == 1) {
+        if (!c->snap) return fail(c, ASKJA_EINVAL, "snapshot restore without a saved snapshot");
+        HIPCHK(c, hipMemcpyAsync(c->links[c->cur], c->snap, bytes, hipMemcpyDeviceToDevice, c->st));
+        return ASKJA_OK;
+    }
+    return fail(c, ASKJA_EINVAL, "snapshot: op must be 0 (save) or 1 (restore)");
+}
+
+int askja_mom_upload(askja_ctx *c, const double *aos)
+{
+    ENTER(c);
+    if (!aos) return fail(c, ASKJA_EINVAL, "mom_upload: null buffer");
+    int rc = ensure_staging(c);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->staging, aos, c->staging_doubles * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, launch_mom_from_aos(c->geom, c->staging, c->mom, c->mode, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->mom_prev, c->mom, c->mom_reals * elem_bytes(c), hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return ASKJA_OK;
+}
+
+static int mom_down(askja_ctx *c, const void *src, double *aos)
+{
+    int rc = ensure_staging(c);
+    if (rc) return rc;
+    HIPCHK(c, launch_mom_to_aos(c->geom, src, c->staging, c->mode, c->st));
+    HIPCHK(c, hipMemcpyAsync(aos, c->staging, c->staging_doubles * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return ASKJA_OK;
+}
+
+int askja_mom_download(askja_ctx *c, double *aos)
+{
+    ENTER(c);
+    if (!aos) return fail(c, ASKJA_EINVAL, "mom_download: null buffer");
+    return mom_down(c, c->mom, aos);
+}
+
+int askja_mom_download_prev(askja_ctx *c, double *aos)
+{
+    ENTER(c);
+    if (!aos) return fail(c, ASKJA_EINVAL, "mom_download_prev: null buffer");
+    return mom_down(c, c->mom_prev, aos);
+}
+
+int askja_mom_gsl(askja_ctx *c, unsigned long seed, double beta)
+{
+#pragma clang fp contract(off)
+    ENTER(c);
+    if (!(beta > 0)) return fail(c, ASKJA_EINVAL, "mom_gsl: beta must be positive");
+    // GSL's mt19937: gsl_rng_set(r, 0) seeds with 4357; the state recursion
+    // is the standard one, so std::mt19937 reproduces the stream.
+    std::mt19937 mt((std::mt19937::result_type)(seed == 0 ? 4357UL : (seed & 0xffffffffUL)));
+    auto uniform_pos = [&mt]() {
+        double x;
+        do x = mt() / 4294967296.0;
+        while (x == 0);
+        return x;
+    };
+    // gsl_ran_gaussian: polar (Marsaglia) method, returning y * sqrt(-2 ln r2 / r2)
+    auto gaussian = [&]() {
+        double x, y, r2;
+        do {
+            x = -1 + 2 * uniform_pos();
+            y = -1 + 2 * uniform_pos();
+            r2 = x * x + y * y;
+        } while (r2 > 1.0 || r2 == 0);
+        return 1.0 * y * std::sqrt(-2.0 * std::log(r2) / r2);
+    };
+    const int n = c->nc, ng = n * n - 1;
+    // normalised Cartan generator entries (sun.cpp:170-189): 1*norm and
+    // -(m+1)*norm with norm = sqrt(0.5 / TrR(g g)), TrR(g g) = (m+1) + (m+1)^2
+    std::vector<double> cn(n > 1 ? n - 1 : 1), cd(n > 1 ? n - 1 : 1);
+    for (int m = 0; m + 1 < n; m++) {
+        const double tr = (double)(m + 1) + (double)(m + 1) * (double)(m + 1);
+        const double norm = std::sqrt(0.5 / tr);
+        cn[m] = 1.0 * norm;
+        cd[m] = (double)(-m - 1) * norm;
+    }
+    const double fac = std::sqrt(2 * n / beta);
+    const size_t nd = askja_field_doubles(c);
+    std::vector<double> aos(nd, 0.0);
+    std::vector<double> xi(ng > 0 ? ng : 1);
+    const size_t ms = (size_t)2 * n * n;
+    for (long i = 0; i < c->V; i++)
+        for (int d = 0; d < c->dim; d++) {
+            for (int a = 0; a < ng; a++) xi[a] = gaussian();
+            double *p = aos.data() + ((size_t)i * c->dim + d) * ms;
+            int a = 0;
+            for (int r = 0; r < n; r++)
+                for (int s = r + 1; s < n; s++) {
+                    // sym generator: 0.5 at (r,s),(s,r); antisym: -0.5i at (r,s), +0.5i at (s,r)
+                    const double vs = 0.5 * xi[a], va = -0.5 * xi[a + 1], vb = 0.5 * xi[a + 1];
+                    p[2 * (r * n + s)] = vs * fac;
+                    p[2 * (r * n + s) + 1] = va * fac;
+                    p[2 * (s * n + r)] = vs * fac;
+                    p[2 * (s * n + r) + 1] = vb * fac;
+                    a += 2;
+                }
+            // diagonal: accumulate in generator order, then scale
+            for (int r = 0; r < n; r++) {
+                double acc = 0.0;
+                bool any = false;
+                for (int m = 0; m + 1 < n; m++) {
+                    double v;
+                    if (r <= m) v = cn[m] * xi[a + m];
+                    else if (r == m + 1) v = cd[m] * xi[a + m];
+                    else continue;
+                    acc = any ? acc + v : v;
+                    any = true;
+                }
+                p[2 * (r * n + r)] = acc * fac;
+            }
+        }
+    int rc = askja_mom_upload(c, aos.data());
+    return rc;
+}
+
+int askja_mom_refresh(askja_ctx *c, uint64_t seed, uint64_t counter, double beta)
+{
+    ENTER(c);
+    if (!(beta > 0)) return fail(c, ASKJA_EINVAL, "mom_refresh: beta must be positive");
+    HIPCHK(c, launch_mom_refresh(c->geom, c->mom, seed, counter, std::sqrt(2.0 * c->nc / beta), c->mode, c->st));
+    return ASKJA_OK;
+}
+
+int askja_mom_negate(askja_ctx *c)
+{
+    ENTER(c);
+    HIPCHK(c, launch_mom_scale(c->geom, c->mom, -1.0, c->mode, c->st));
+    return ASKJA_OK;
+}
+
+int askja_kick(askja_ctx *c, double eps)
+{
+    ENTER(c);
+    return do_kick(c, c->links[c->cur], nullptr, c->mom_prev, eps, 0.0, 0, false, -1);
+}
+
+int askja_drift(askja_ctx *c, double eps, int level)
+{
+    ENTER(c);
+    double ci[9];
+    fill_cinv(ci);
+    HIPCHK(c, launch_drift(c->geom, c->links[c->cur], c->mom, eps, level, ci, c->mode, c->st));
+    return ASKJA_OK;
+}
+
+int askja_kick_drift(askja_ctx *c, double eps_kick, double eps_drift, int level)
+{
+    ENTER(c);
+    const int out = (c->cur + 1) % 3;
+    int rc = do_kick(c, c->links[c->cur], c->links[out], nullptr, eps_kick, eps_drift, level, true, -1);
+    if (rc) return rc;
+    c->cur = out;
+    return ASKJA_OK;
+}
+
+int askja_md_trajectory(askja_ctx *c, double dt, int ndrift, int level, double *energies)
+{
+    ENTER(c);
+    int fin = c->cur;
+    int rc = md_pingpong(c, dt, ndrift, level, &fin);
+    if (rc) return rc;
+    c->cur = fin;
+    if (energies) {
+        double h[12];
+        if ((rc = read_res(c, h, 12))) return rc;
+        const double np = (double)n_plaquettes(c), n4 = 4.0 * c->nc;
+        energies[0] = np - pair_val(h, 0) / n4;
+        energies[1] = pair_val(h, 1);
+        energies[2] = np - pair_val(h + 6, 0) / n4;
+        energies[3] = pair_val(h + 6, 2);
+    }
+    return ASKJA_OK;
+}
+
+int askja_action(askja_ctx *c, double *s_t, double *s_s)
+{
+    ENTER(c);
+    int rc = ensure_scratch(c);
+    if (rc) return rc;
+    long np = 0;
+    const long cap = (c->V + 255) / 256 + 1024;
+    if ((rc = ensure_part(c, (size_t)cap * 4))) return rc;
+    HIPCHK(c, launch_plaq(c->geom, c->links[c->cur], c->mode, 1. / (double)c->nc, c->part, &np, c->scratch,
+                          (long)c->scratch_doubles, c->st));
+    HIPCHK(c, launch_finalize(c->part, np, 2, c->res, c->st));
+    double h[4];
+    if ((rc = read_res(c, h, 4))) return rc;
+    if (s_t) *s_t = pair_val(h, 0);
+    if (s_s) *s_s = pair_val(h, 1);
+    return ASKJA_OK;
+}
+
+static int kinetic2(askja_ctx *c, const void *Q, double *out2)
+{
+    long np = 0;
+    int rc = ensure_part(c, (size_t)((c->NL + 255) / 256) * 4);
+    if (rc) return rc;
+    HIPCHK(c, launch_kinetic(c->geom, c->mom, Q, c->mode, c->part, &np, c->st));
+    HIPCHK(c, launch_finalize(c->part, np, 2, c->res, c->st));
+    double h[4];
+    if ((rc = read_res(c, h, 4))) return rc;
+    out2[0] = pair_val(h, 0);
+    out2[1] = pair_val(h, 1);
+    return ASKJA_OK;
+}
+
+int askja_kinetic(askja_ctx *c, double *out)
+{
+    ENTER(c);
+    if (!out) return fail(c, ASKJA_EINVAL, "kinetic: null output");
+    double v[2];
+    int rc = kinetic2(c, nullptr, v);
+    if (rc) return rc;
+    *out = v[0];
+    return ASKJA_OK;
+}
+
+int askja_energy_el(askja_ctx *c, double *out)
+{
+    ENTER(c);
+    if (!out) return fail(c, ASKJA_EINVAL, "energy_el: null output");
+    double v[2];
+    int rc = kinetic2(c, c->mom_prev, v);
+    if (rc) return rc;
+    *out = v[1];
+    return ASKJA_OK;
+}
+
+int askja_gauss_law(askja_ctx *c, double *out)
+{
+    ENTER(c);
+    if (!out) return fail(c, ASKJA_EINVAL, "gauss_law: null output");
+    int rc = ensure_scratch(c);
+    if (rc) return rc;
+    const int n = c->nc, nr = 2 * n * n;
+    const size_t small_need = (size_t)((c->NL + 255) / 256) * 2 * nr;
+    const size_t gen_need = c->scratch_doubles / 3 + 2 * (size_t)nr;
+    if ((rc = ensure_part(c, n <= 3 ? small_need : gen_need))) return rc;
+    double *dout = nullptr;
+    HIPCHK(c, hipMallocAsync((void **)&dout, (size_t)2 * nr * sizeof(double), c->st));
+    long np = 0;
+    hipError_t e = launch_gauss(c->geom, c->links[c->cur], c->mom, c->mode, c->part, &np, c->scratch,
+                                (long)c->scratch_doubles, dout, c->st);
+    std::vector<double> h((size_t)2 * nr);
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), dout, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipFreeAsync(dout, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    if (e != hipSuccess) return fail(c, ASKJA_EHIP, "gauss_law: %s", hipGetErrorString(e));
+    // tr2(M) = ReTr(M M) of the summed matrix (sunmat.h:331-338)
+    std::vector<double> m((size_t)nr);
+    for (int q = 0; q < nr; q++) m[q] = h[2 * q] + h[2 * q + 1];
+    double t = 0.0;
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < n; k++) {
+            const double ar = m[2 * (i * n + k)], ai = m[2 * (i * n + k) + 1];
+            const double br = m[2 * (k * n + i)], bi = m[2 * (k * n + i) + 1];
+            t += ar * br - ai * bi;
+        }
+    *out = t;
+    return ASKJA_OK;
+}
+
+int askja_uviol(askja_ctx *c, double *out)
+{
+    ENTER(c);
+    if (!out) return fail(c, ASKJA_EINVAL, "uviol: null output");
+    int rc = ensure_scratch(c);
+    if (rc) return rc;
+    if ((rc = ensure_part(c, (size_t)((c->NL + 255) / 256 + 1024) * 2))) return rc;
+    long np = 0;
+    HIPCHK(c, launch_uviol(c->geom, c->links[c->cur], c->mode, c->part, &np, c->scratch, (long)c->scratch_doubles, c->st));
+    HIPCHK(c, launch_finalize(c->part, np, 1, c->res, c->st));
+    double h[2];
+    if ((rc = read_res(c, h, 2))) return rc;
+    *out = pair_val(h, 0);
+    return ASKJA_OK;
+}
+
+int askja_wilson_lines(askja_ctx *c, int dir, double *out)
+{
+    ENTER(c);
+    if (!out) return fail(c, ASKJA_EINVAL, "wilson_lines: null output");
+    if (dir < 0 || dir >= c->dim) return fail(c, ASKJA_EINVAL, "wilson_lines: dir %d out of range", dir);
+    int rc = ensure_scratch(c);
+    if (rc) return rc;
+    const long nl = c->V / c->ext[dir];
+    double *d = nullptr;
+    HIPCHK(c, hipMallocAsync((void **)&d, (size_t)2 * nl * sizeof(double), c->st));
+    hipError_t e = launch_wlines(c->geom, c->links[c->cur], dir, c->mode, d, c->scratch, (long)c->scratch_doubles, c->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d, (size_t)2 * nl * sizeof(double), hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipFreeAsync(d, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    if (e != hipSuccess) return fail(c, ASKJA_EHIP, "wilson_lines: %s", hipGetErrorString(e));
+    return ASKJA_OK;
+}
+
+int askja_hmc_trajectory(askja_ctx *c, const askja_hmc_params *p, uint64_t traj_counter, askja_hmc_result *r)
+{
+    ENTER(c);
+    if (!p || !r) return fail(c, ASKJA_EINVAL, "hmc_trajectory: null argument");
+    if (!(p->beta > 0) || !(p->dt > 0)) return fail(c, ASKJA_EINVAL, "hmc_trajectory: beta and dt must be positive");
+    int rc;
+    if (p->reunit && (rc = askja_reunitarize(c))) return rc;
+    if ((rc = askja_mom_refresh(c, p->seed, traj_counter, p->beta))) return rc;
+    int fin = c->cur;
+    if ((rc = md_pingpong(c, p->dt, p->ndrift, p->level, &fin))) return rc;
+    double h[12];
+    if ((rc = read_res(c, h, 12))) return rc;
+    const double np = (double)n_plaquettes(c), n4 = 4.0 * c->nc, n2 = 2.0 * c->nc;
+    const double s0 = np - pair_val(h, 0) / n4, k0 = pair_val(h, 1);
+    const double s1 = np - pair_val(h + 6, 0) / n4, k1 = pair_val(h + 6, 2);
+    r->h_old = (s0 + k0 / n2) * p->beta;
+    r->h_new = (s1 + k1 / n2) * p->beta;
+    r->dh = r->h_new - r->h_old;
+    U4 ctr;
+    ctr.x = 0u;
+    ctr.y = 0u;
+    ctr.z = (uint32_t)traj_counter;
+    ctr.w = (uint32_t)(traj_counter >> 32) ^ (kPurposeAccept << 24);
+    const U4 rr = philox4x32_10(ctr, (uint32_t)p->seed, (uint32_t)(p->seed >> 32));
+    r->uniform = u01(rr.x);
+    if (r->h_new < r->h_old) r->accept_case = 1;
+    else if (r->uniform < std::exp(r->h_old - r->h_new)) r->accept_case = 2;
+    else r->accept_case = 0;
+    r->accepted = r->accept_case != 0;
+    if (r->accepted) c->cur = fin;
+    r->action = r->accepted ? s1 : s0;
+    return ASKJA_OK;
+}
+
+int askja_sync(askja_ctx *c)
+{
+    ENTER(c);
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return ASKJA_OK;
+}
+
+int askja_timer_start(askja_ctx *c)
+{
+    ENTER(c);
+    HIPCHK(c, hipEventRecord(c->ev0, c->st));
+    return ASKJA_OK;
+}
+
+int askja_timer_stop(askja_ctx *c, double *ms)
+{
+    ENTER(c);
+    HIPCHK(c, hipEventRecord(c->ev1, c->st));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float f = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&f, c->ev0, c->ev1));
+    if (ms) *ms = f;
+    return ASKJA_OK;
+}
+
+int askja_device_fields(askja_ctx *c, void **links, void **mom, void **mom_prev)
+{
+    ENTER(c);
+    if (links) *links = c->links[c->cur];
+    if (mom) *mom = c->mom;
+    if (mom_prev) *mom_prev = c->mom_prev;
+    return ASKJA_OK;
+}
+
+}  // extern "C"

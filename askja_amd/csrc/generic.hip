@@ -1,0 +1,732 @@
+// generic.hip -- kernels that run for every N: layout conversion, cold and
+// Haar starts, momentum refresh, SU(N) projection, kinetic sums, the final
+// stage of every reduction, and the N >= 4 diagnostics (plaquette split,
+// unitarity violation, Gauss law, Wilson lines) with FP64 working matrices in
+// a per-thread slice of global scratch memory.  Also the top-level launchers
+// that pick the register (N <= 3) or workgroup-per-link (N >= 4) kernels.
+#include "kernels.hpp"
+#include "common.hpp"
+
+namespace askja {
+
+// exported by small.hip and team<NMAX>.hip
+hipError_t small_kick(const KickArgs &a, int mode, bool drift, hipStream_t st);
+hipError_t small_drift(const Geom &g, void *U, const void *P, double eps, int level, const double *c, int mode,
+                       hipStream_t st);
+hipError_t small_plaq(const Geom &g, const void *U, int mode, double inv_n, double *part, long *np, hipStream_t st);
+hipError_t small_uviol(const Geom &g, const void *U, int mode, double *part, long *np, hipStream_t st);
+hipError_t small_gauss(const Geom &g, const void *U, const void *P, int mode, double *part, long *np, double *out,
+                       hipStream_t st);
+hipError_t small_wlines(const Geom &g, const void *U, int dir, int mode, double *out, hipStream_t st);
+#define ASKJA_TEAM_DECL(NM)                                                                                   \
+    hipError_t team_kick_##NM(const KickArgs &a, int mode, bool drift, hipStream_t st);                       \
+    hipError_t team_drift_##NM(const Geom &g, void *U, const void *P, double eps, int level, const double *c, \
+                               int mode, hipStream_t st);
+ASKJA_TEAM_DECL(8)
+ASKJA_TEAM_DECL(16)
+ASKJA_TEAM_DECL(32)
+ASKJA_TEAM_DECL(64)
+#undef ASKJA_TEAM_DECL
+
+namespace {
+
+constexpr int kB = 256;
+
+inline unsigned nblk(long n, int b = kB) { return (unsigned)((n + b - 1) / b); }
+
+template <typename R>
+__device__ inline R &lk(R *f, const Geom &g, long l, int e)
+{
+    return f[foff(l, e, 2 * g.N * g.N, g.tile)];
+}
+template <typename R>
+__device__ inline R lkc(const R *f, const Geom &g, long l, int e)
+{
+    return f[foff(l, e, 2 * g.N * g.N, g.tile)];
+}
+template <typename R>
+__device__ inline R &mk(R *f, const Geom &g, long l, int c)
+{
+    return f[foff(l, c, g.N * g.N, g.tile)];
+}
+
+// (i, j), i < j, of compact pair index p (0-based over the strict upper
+// triangle, row-major).
+__device__ inline void pair_ij(int n, int p, int &i, int &j)
+{
+    i = 0;
+    while (p >= n - 1 - i) {
+        p -= n - 1 - i;
+        i++;
+    }
+    j = i + 1 + p;
+}
+
+// ------------------------------------------------------------- reductions
+template <int K>
+__global__ __launch_bounds__(1024) void k_finalize(const double *part, long np, double *out)
+{
+    double s[K], c[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) s[k] = c[k] = 0.0;
+    for (long p = threadIdx.x; p < np; p += 1024) {
+#pragma unroll
+        for (int k = 0; k < K; k++) pair_add(s[k], c[k], part[(size_t)p * 2 * K + 2 * k], part[(size_t)p * 2 * K + 2 * k + 1]);
+    }
+    block_reduce_store<K>(s, c, out);
+}
+
+// Column sums of np rows of K (s, c) pairs, for large K: thread per column.
+__global__ void k_finalize_cols(const double *part, long np, int K, double *out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    double s = 0.0, c = 0.0;
+    for (long p = 0; p < np; p++) pair_add(s, c, part[(size_t)p * 2 * K + 2 * k], part[(size_t)p * 2 * K + 2 * k + 1]);
+    out[2 * k] = s;
+    out[2 * k + 1] = c;
+}
+
+// ------------------------------------------------------ layout conversion
+template <typename R>
+__global__ void k_links_from_aos(Geom g, const double *aos, R *U)
+{
+    const long t = (long)blockIdx.x * kB + threadIdx.x;
+    const int nr = 2 * g.N * g.N;
+    const long tot = g.V * g.D * nr;
+    if (t >= tot) return;
+    const int e = (int)(t % nr);
+    const long sd = t / nr;
+    const int d = (int)(sd % g.D);
+    const long i = sd / g.D;
+    const long l = d * g.V + g.lex2dev[i];
+    lk(U, g, l, e) = (R)aos[t];
+}
+
+template <typename R>
+__global__ void k_links_to_aos(Geom g, const R *U, double *aos)
+{
+    const long t = (long)blockIdx.x * kB + threadIdx.x;
+    const int nr = 2 * g.N * g.N;
+    const long tot = g.V * g.D * nr;
+    if (t >= tot) return;
+    const int e = (int)(t % nr);
+    const long sd = t / nr;
+    const int d = (int)(sd % g.D);
+    const long i = sd / g.D;
+    const long l = d * g.V + g.lex2dev[i];
+    aos[t] = (double)lkc(U, g, l, e);
+}
+
+// compact element c of link l <- full Hermitian AoS matrix
+template <typename R>
+__global__ void k_mom_from_aos(Geom g, const double *aos, R *P)
+{
+    const long t = (long)blockIdx.x * kB + threadIdx.x;
+    const int n = g.N, nc = n * n;
+    const long tot = g.V * g.D * nc;
+    if (t >= tot) return;
+    const int c = (int)(t % nc);
+    const long sd = t / nc;
+    const int d = (int)(sd % g.D);
+    const long i = sd / g.D;
+    const long l = d * g.V + g.lex2dev[i];
+    const double *m = aos + sd * 2 * nc;
+    double v;
+    if (c < n) {
+        v = m[2 * (c * n + c)];
+    } else {
+        int a, b;
+        pair_ij(n, (c - n) >> 1, a, b);
+        v = m[2 * (a * n + b) + ((c - n) & 1)];
+    }
+    mk(P, g, l, c) = (R)v;
+}
+
+template <typename R>
+__global__ void k_mom_to_aos(Geom g, const R *P, double *aos)
+{
+    const long t = (long)blockIdx.x * kB + threadIdx.x;
+    const int n = g.N, nn = n * n;
+    const long tot = g.V * g.D * nn;
+    if (t >= tot) return;
+    const int e = (int)(t % nn);
+    const long sd = t / nn;
+    const int d = (int)(sd % g.D);
+    const long i = sd / g.D;
+    const long l = d * g.V + g.lex2dev[i];
+    const int a = e / n, b = e % n;
+    double re, im;
+    if (a == b) {
+        re = (double)mk(const_cast<R *>(P), g, l, a);
+        im = 0.0;
+    } else if (a < b) {
+        const int h = hup(n, a, b);
+        re = (double)mk(const_cast<R *>(P), g, l, h);
+        im = (double)mk(const_cast<R *>(P), g, l, h + 1);
+    } else {
+        const int h = hup(n, b, a);
+        re = (double)mk(const_cast<R *>(P), g, l, h);
+        im = -(double)mk(const_cast<R *>(P), g, l, h + 1);
+    }
+    aos[2 * t] = re;
+    aos[2 * t + 1] = im;
+}
+
+template <typename R>
+__global__ void k_cold(Geom g, R *U)
+{
+    const long t = (long)blockIdx.x * kB + threadIdx.x;
+    const int nr = 2 * g.N * g.N;
+    if (t >= g.V * g.D * nr) return;
+    const int e = (int)(t % nr);
+    const long l = t / nr;
+    const int ij = e >> 1;
+    lk(U, g, l, e) = ((e & 1) == 0 && ij / g.N == ij % g.N) ? R(1) : R(0);
+}
+
+// ---------------------------------------------------------- random fields
+// Normal deviate number a of (purpose, link) at `counter`: Philox block a/4,
+// Box-Muller pair (a%4)/2, cos for even a, sin for odd a.
+__device__ inline double normal_at(uint64_t seed, uint64_t counter, uint32_t purpose, uint32_t link, int a)
+{
+    const U4 r = philox_block(seed, counter, purpose, link, (uint32_t)(a >> 2));
+    const int h = (a >> 1) & 1;
+    const uint32_t x0 = h ? r.z : r.x, x1 = h ? r.w : r.y;
+    const double rad = sqrt(-2.0 * log(u01(x0)));
+    const double ang = 6.283185307179586 * u01(x1);
+    return (a & 1) ? rad * sin(ang) : rad * cos(ang);
+}
+
+// pi = c sum_a xi_a T^a with the reference's generator order (sun.cpp:142-189,
+// 209-252): symmetric/antisymmetric pairs for i < j, then the Cartan
+// generators diag(1,..,1,-(m+1),0,..)/sqrt(2(m+1)(m+2)).  RNG link index =
+// dir * V + lexicographic site.
+template <typename R>
+__global__ void k_mom_refresh(Geom g, R *P, uint64_t seed, uint64_t counter, double c)
+{
+    const long l = (long)blockIdx.x * kB + threadIdx.x;
+    if (l >= g.V * g.D) return;
+    const int n = g.N;
+    const int d = (int)(l / g.V);
+    const uint32_t link = (uint32_t)(d * g.V + g.dev2lex[l - d * g.V]);
+    // off-diagonal pairs, normals 0 .. n(n-1)-1 in order
+    int a = 0;
+    U4 blk = {0u, 0u, 0u, 0u};
+    double z[4];
+    for (int i = 0; i < n; i++)
+        for (int j = i + 1; j < n; j++) {
+            if ((a & 3) == 0) {
+                blk = philox_block(seed, counter, kPurposeMomentum, link, (uint32_t)(a >> 2));
+                normals4(blk, z);
+            }
+            const double xs = z[a & 3], xa = z[(a & 3) + 1];
+            const int h = hup(n, i, j);
+            mk(P, g, l, h) = (R)(0.5 * c * xs);
+            mk(P, g, l, h + 1) = (R)(-(0.5 * c * xa));
+            a += 2;
+        }
+    // diagonal: P_rr = sum_{m >= r} h_m - r h_{r-1}, h_m = c xi_m / sqrt(2 (m+1)(m+2))
+    const int a0 = n * (n - 1);
+    double suffix = 0.0;
+    for (int r = n - 1; r >= 0; r--) {
+        double v = suffix;
+        if (r >= 1) {
+            const int m = r - 1;
+            const double hm = c * normal_at(seed, counter, kPurposeMomentum, link, a0 + m) *
+                              sqrt(0.5 / ((double)(m + 1) * (m + 2)));
+            v -= (double)r * hm;
+            suffix += hm;
+        }
+        mk(P, g, l, r) = (R)v;
+    }
+}
+
+template <typename R>
+__global__ void k_mom_scale(Geom g, R *P, double f)
+{
+    const long t = (long)blockIdx.x * kB + threadIdx.x;
+    const int nc = g.N * g.N;
+    if (t >= g.V * g.D * nc) return;
+    const long l = t / nc;
+    const int c = (int)(t % nc);
+    mk(P, g, l, c) = (R)((double)mk(P, g, l, c) * f);
+}
+
+// -------------------------------------------------------- SU(N) projection
+// Modified Gram-Schmidt on the rows, then multiplication by exp(-i arg(det)/N)
+// with det from LU with partial pivoting; FP64, no contraction.  haar: the
+// matrix is first filled with a Ginibre matrix (xi + i eta)/sqrt(2) from the
+// Philox stream (purpose 2, counter = stream): Gram-Schmidt with positive
+// diagonal R is the phase-fixed QR of Mezzadri (2007), so the result is Haar
+// distributed on U(N), and the determinant phase maps it to Haar on SU(N).
+template <typename R>
+__global__ void k_project(Geom g, R *U, int haar, uint64_t seed, uint64_t stream, double *scratch, long nthr)
+{
+#pragma clang fp contract(off)
+    const long t = (long)blockIdx.x * kB + threadIdx.x;
+    if (t >= nthr) return;
+    const int n = g.N;
+    const int nn = n * n;
+    double *A = scratch + (size_t)t * 4 * nn;  // 2 complex matrices
+    double *M = A + 2 * nn;
+    for (long l = t; l < g.V * g.D; l += nthr) {
+        if (haar) {
+            const int d = (int)(l / g.V);
+            const uint32_t link = (uint32_t)(d * g.V + g.dev2lex[l - d * g.V]);
+            for (int b = 0; 4 * b < 2 * nn; b++) {
+                double z[4];
+                normals4(philox_block(seed, stream, kPurposeHaar, link, (uint32_t)b), z);
+                for (int q = 0; q < 4 && 4 * b + q < 2 * nn; q++) A[4 * b + q] = z[q] * 0.7071067811865476;
+            }
+        } else {
+            for (int e = 0; e < 2 * nn; e++) A[e] = (double)lkc(U, g, l, e);
+        }
+        for (int r = 0; r < n; r++) {
+            double *a = A + 2 * r * n;
+            for (int q = 0; q < r; q++) {
+                const double *b = A + 2 * q * n;
+                double dr = 0.0, di = 0.0;
+                for (int k = 0; k < n; k++) {
+                    dr += b[2 * k] * a[2 * k] + b[2 * k + 1] * a[2 * k + 1];
+                    di += b[2 * k] * a[2 * k + 1] - b[2 * k + 1] * a[2 * k];
+                }
+                for (int k = 0; k < n; k++) {
+                    a[2 * k] -= dr * b[2 * k] - di * b[2 * k + 1];
+                    a[2 * k + 1] -= dr * b[2 * k + 1] + di * b[2 * k];
+                }
+            }
+            double nrm = 0.0;
+            for (int k = 0; k < 2 * n; k++) nrm += a[k] * a[k];
+            nrm = 1.0 / sqrt(nrm);
+            for (int k = 0; k < 2 * n; k++) a[k] *= nrm;
+        }
+        // determinant by LU with partial pivoting on a copy
+        for (int e = 0; e < 2 * nn; e++) M[e] = A[e];
+        double rr = 1.0, ri = 0.0;
+        for (int cc = 0; cc < n; cc++) {
+            int p = cc;
+            double best = -1.0;
+            for (int r = cc; r < n; r++) {
+                const double v = hypot(M[2 * (r * n + cc)], M[2 * (r * n + cc) + 1]);
+                if (v > best) {
+                    best = v;
+                    p = r;
+                }
+            }
+            if (p != cc) {
+                for (int k = 0; k < 2 * n; k++) {
+                    const double tmp = M[2 * cc * n + k];
+                    M[2 * cc * n + k] = M[2 * p * n + k];
+                    M[2 * p * n + k] = tmp;
+                }
+                rr = -rr;
+                ri = -ri;
+            }
+            const double pr = M[2 * (cc * n + cc)], pim = M[2 * (cc * n + cc) + 1];
+            const double nr = rr * pr - ri * pim, ni = rr * pim + ri * pr;
+            rr = nr;
+            ri = ni;
+            const double den = pr * pr + pim * pim;
+            if (den == 0.0) {
+                rr = ri = 0.0;
+                break;
+            }
+            for (int r = cc + 1; r < n; r++) {
+                const double ar = M[2 * (r * n + cc)], ai = M[2 * (r * n + cc) + 1];
+                const double fr = (ar * pr + ai * pim) / den, fi = (ai * pr - ar * pim) / den;
+                for (int k = cc; k < n; k++) {
+                    const double br = M[2 * (cc * n + k)], bi = M[2 * (cc * n + k) + 1];
+                    M[2 * (r * n + k)] -= fr * br - fi * bi;
+                    M[2 * (r * n + k) + 1] -= fr * bi + fi * br;
+                }
+            }
+        }
+        const double ph = -atan2(ri, rr) / n;
+        const double cr = cos(ph), ci = sin(ph);
+        for (int q = 0; q < nn; q++) {
+            const double ar = A[2 * q], ai = A[2 * q + 1];
+            lk(U, g, l, 2 * q) = (R)(ar * cr - ai * ci);
+            lk(U, g, l, 2 * q + 1) = (R)(ar * ci + ai * cr);
+        }
+    }
+}
+
+// ------------------------------------------------------------- kinetic
+// {Tr P^2, 1/4 Tr (P + Q)^2} per link from the compact Hermitian layout.
+template <typename R>
+__global__ __launch_bounds__(kB) void k_kinetic(Geom g, const R *P, const R *Q, double *part)
+{
+    const long l = (long)blockIdx.x * kB + threadIdx.x;
+    double s[2] = {0.0, 0.0}, c[2] = {0.0, 0.0};
+    if (l < g.V * g.D) {
+        const int n = g.N, nc = n * n;
+        double a = 0.0, b = 0.0;
+        for (int e = 0; e < nc; e++) {
+            const double w = e < n ? 1.0 : 2.0;
+            const double p = (double)mk(const_cast<R *>(P), g, l, e);
+            a += w * p * p;
+            if (Q) {
+                const double pq = p + (double)mk(const_cast<R *>(Q), g, l, e);
+                b += w * pq * pq;
+            }
+        }
+        s[0] = a;
+        s[1] = 0.25 * b;
+    }
+    block_reduce_store<2>(s, c, part + 4 * (size_t)blockIdx.x);
+}
+
+// ------------------------------------------- N >= 4 diagnostics (FP64 work)
+// C = op(A) op(B), n x n complex, interleaved, k ascending, no contraction.
+__device__ inline void gmul(const double *A, bool dA, const double *B, bool dB, double *C, int n)
+{
+#pragma clang fp contract(off)
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) {
+            double cr = 0.0, ci = 0.0;
+            for (int k = 0; k < n; k++) {
+                const int ia = dA ? k * n + i : i * n + k;
+                const int ib = dB ? j * n + k : k * n + j;
+                const double ar = A[2 * ia], ai = dA ? -A[2 * ia + 1] : A[2 * ia + 1];
+                const double br = B[2 * ib], bi = dB ? -B[2 * ib + 1] : B[2 * ib + 1];
+                cr = cr + (ar * br - ai * bi);
+                ci = ci + (ar * bi + ai * br);
+            }
+            C[2 * (i * n + j)] = cr;
+            C[2 * (i * n + j) + 1] = ci;
+        }
+}
+
+template <typename R>
+__device__ inline void gload(const R *U, const Geom &g, long l, double *A)
+{
+    const int nr = 2 * g.N * g.N;
+    for (int e = 0; e < nr; e++) A[e] = (double)lkc(U, g, l, e);
+}
+
+// plaquette sums, thread per site (grid-stride over sites, 4 work matrices)
+template <typename R>
+__global__ __launch_bounds__(kB) void k_plaq_g(Geom g, const R *U, double inv_n, double *scratch, long nthr,
+                                                double *part)
+{
+    const long t = (long)blockIdx.x * kB + threadIdx.x;
+    double s[2] = {0.0, 0.0}, c[2] = {0.0, 0.0};
+    if (t < nthr) {
+        const int n = g.N, nr = 2 * n * n;
+        double *A = scratch + (size_t)t * 4 * nr, *B = A + nr, *T1 = B + nr, *T2 = T1 + nr;
+        for (long x = t; x < g.V; x += nthr)
+            for (int j = 0; j < g.D; j++) {
+                const long xj = g.nbr[(2 * j) * g.V + x];
+                for (int k = j + 1; k < g.D; k++) {
+                    const long xk = g.nbr[(2 * k) * g.V + x];
+                    gload(U, g, j * g.V + x, A);
+                    gload(U, g, k * g.V + xj, B);
+                    gmul(A, false, B, false, T1, n);
+                    gload(U, g, j * g.V + xk, B);
+                    gmul(T1, false, B, true, T2, n);
+                    gload(U, g, k * g.V + x, B);
+                    gmul(T2, false, B, true, T1, n);
+                    double tr = 0.0;
+                    for (int i = 0; i < n; i++) tr = tr + T1[2 * (i * n + i)];
+                    kahan_add(s[j == 0 ? 0 : 1], c[j == 0 ? 0 : 1], 1.0 - tr * inv_n);
+                }
+            }
+    }
+    block_reduce_store<2>(s, c, part + 4 * (size_t)blockIdx.x);
+}
+
+template <typename R>
+__global__ __launch_bounds__(kB) void k_uviol_g(Geom g, const R *U, double *scratch, long nthr, double *part)
+{
+    const long t = (long)blockIdx.x * kB + threadIdx.x;
+    double s[1] = {0.0}, c[1] = {0.0};
+    if (t < nthr) {
+        const int n = g.N, nr = 2 * n * n;
+        double *A = scratch + (size_t)t * 4 * nr, *T = A + nr, *Q = T + nr;
+        for (long l = t; l < g.V * g.D; l += nthr) {
+            gload(U, g, l, A);
+            gmul(A, false, A, true, T, n);
+            for (int i = 0; i < n; i++) T[2 * (i * n + i)] -= 1.0;
+            gmul(T, false, T, false, Q, n);
+            double tr = 0.0;
+            for (int i = 0; i < n; i++) tr = tr + Q[2 * (i * n + i)];
+            kahan_add(s[0], c[0], tr);
+        }
+    }
+    block_reduce_store<1>(s, c, part + 2 * (size_t)blockIdx.x);
+}
+
+// Gauss law: per thread compensated matrix accumulator of P - U^+ P U in its
+// scratch slice (2N^2 (s, c) pairs, then 4 work matrices), then column sums
+// over threads.
+template <typename R>
+__global__ __launch_bounds__(kB) void k_gauss_g(Geom g, const R *U, const R *P, double *scratch, long nthr)
+{
+    const long t = (long)blockIdx.x * kB + threadIdx.x;
+    if (t >= nthr) return;
+    const int n = g.N, nr = 2 * n * n;
+    double *acc = scratch + (size_t)t * 6 * nr;
+    double *A = acc + 2 * nr, *Pm = A + nr, *T = Pm + nr, *W = T + nr;
+    for (int e = 0; e < 2 * nr; e++) acc[e] = 0.0;
+    for (long l = t; l < g.V * g.D; l += nthr) {
+        gload(U, g, l, A);
+        for (int a = 0; a < n; a++)
+            for (int b = 0; b < n; b++) {
+                double re, im;
+                if (a == b) {
+                    re = (double)mk(const_cast<R *>(P), g, l, a);
+                    im = 0.0;
+                } else {
+                    const int h = hup(n, a < b ? a : b, a < b ? b : a);
+                    re = (double)mk(const_cast<R *>(P), g, l, h);
+                    im = (double)mk(const_cast<R *>(P), g, l, h + 1);
+                    if (a > b) im = -im;
+                }
+                Pm[2 * (a * n + b)] = re;
+                Pm[2 * (a * n + b) + 1] = im;
+            }
+        gmul(A, true, Pm, false, T, n);
+        gmul(T, false, A, false, W, n);
+        for (int e = 0; e < nr; e++) kahan_add(acc[2 * e], acc[2 * e + 1], Pm[e] - W[e]);
+    }
+}
+
+template <typename R>
+__global__ void k_wlines_g(Geom g, const R *U, int dir, long q0, long nq, double *scratch, double *out)
+{
+    const long t = (long)blockIdx.x * kB + threadIdx.x;
+    if (t >= nq) return;
+    const long q = q0 + t;
+    const int n = g.N, nr = 2 * n * n;
+    double *W = scratch + (size_t)t * 3 * nr, *B = W + nr, *T = B + nr;
+    long rem = q, lex = 0;
+    for (int d = g.D - 1; d >= 0; d--) {
+        if (d == dir) continue;
+        const long x = rem % g.ext[d];
+        rem /= g.ext[d];
+        lex += x * g.helparr[d];
+    }
+    long s = g.lex2dev[lex];
+    gload(U, g, dir * g.V + s, W);
+    for (int k = 1; k < g.ext[dir]; k++) {
+        s = g.nbr[(2 * dir) * g.V + s];
+        gload(U, g, dir * g.V + s, B);
+        gmul(W, false, B, false, T, n);
+        for (int e = 0; e < nr; e++) W[e] = T[e];
+    }
+    double re = 0.0, im = 0.0;
+    for (int i = 0; i < n; i++) {
+        re = re + W[2 * (i * n + i)];
+        im = im + W[2 * (i * n + i) + 1];
+    }
+    out[2 * q] = re / n;
+    out[2 * q + 1] = im / n;
+}
+
+inline long threads_for(long work, long per_thread, long scratch_doubles)
+{
+    long t = scratch_doubles / per_thread;
+    if (t > work) t = work;
+    if (t < 1) t = 1;
+    return t;
+}
+
+}  // namespace
+
+long generic_scratch_per_thread(int n) { return 12L * n * n; }
+
+// ---------------------------------------------------------------- dispatch
+long kick_partials(const Geom &g)
+{
+    if (g.N <= 3) return (long)nblk(g.V) * g.D;
+    return g.V * g.D;
+}
+
+hipError_t launch_kick(const KickArgs &a, int mode, bool drift, hipStream_t st)
+{
+    const int n = a.g.N;
+    if (n <= 3) return small_kick(a, mode, drift, st);
+    if (n <= 8) return team_kick_8(a, mode, drift, st);
+    if (n <= 16) return team_kick_16(a, mode, drift, st);
+    if (n <= 32) return team_kick_32(a, mode, drift, st);
+    return team_kick_64(a, mode, drift, st);
+}
+
+hipError_t launch_drift(const Geom &g, void *U, const void *P, double eps, int level, const double *cinv, int mode,
+                        hipStream_t st)
+{
+    const int n = g.N;
+    if (n <= 3) return small_drift(g, U, P, eps, level, cinv, mode, st);
+    if (n <= 8) return team_drift_8(g, U, P, eps, level, cinv, mode, st);
+    if (n <= 16) return team_drift_16(g, U, P, eps, level, cinv, mode, st);
+    if (n <= 32) return team_drift_32(g, U, P, eps, level, cinv, mode, st);
+    return team_drift_64(g, U, P, eps, level, cinv, mode, st);
+}
+
+hipError_t launch_finalize(const double *part, long np, int K, double *out, hipStream_t st)
+{
+    switch (K) {
+    case 1: hipLaunchKernelGGL((k_finalize<1>), dim3(1), dim3(1024), 0, st, part, np, out); break;
+    case 2: hipLaunchKernelGGL((k_finalize<2>), dim3(1), dim3(1024), 0, st, part, np, out); break;
+    case 3: hipLaunchKernelGGL((k_finalize<3>), dim3(1), dim3(1024), 0, st, part, np, out); break;
+    default: hipLaunchKernelGGL(k_finalize_cols, dim3(nblk(K)), dim3(kB), 0, st, part, np, K, out);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_plaq(const Geom &g, const void *U, int mode, double inv_n, double *part, long *np,
+                       double *scratch, long scratch_doubles, hipStream_t st)
+{
+    if (g.N <= 3) return small_plaq(g, U, mode, inv_n, part, np, st);
+    const long nthr = threads_for(g.V, 4L * 2 * g.N * g.N, scratch_doubles);
+    *np = nblk(nthr);
+    if (mode == kModeF32)
+        hipLaunchKernelGGL(k_plaq_g<float>, dim3(nblk(nthr)), dim3(kB), 0, st, g, (const float *)U, inv_n, scratch, nthr, part);
+    else
+        hipLaunchKernelGGL(k_plaq_g<double>, dim3(nblk(nthr)), dim3(kB), 0, st, g, (const double *)U, inv_n, scratch, nthr,
+                           part);
+    return hipGetLastError();
+}
+
+hipError_t launch_uviol(const Geom &g, const void *U, int mode, double *part, long *np, double *scratch,
+                        long scratch_doubles, hipStream_t st)
+{
+    if (g.N <= 3) return small_uviol(g, U, mode, part, np, st);
+    const long nthr = threads_for(g.V * g.D, 4L * 2 * g.N * g.N, scratch_doubles);
+    *np = nblk(nthr);
+    if (mode == kModeF32)
+        hipLaunchKernelGGL(k_uviol_g<float>, dim3(nblk(nthr)), dim3(kB), 0, st, g, (const float *)U, scratch, nthr, part);
+    else
+        hipLaunchKernelGGL(k_uviol_g<double>, dim3(nblk(nthr)), dim3(kB), 0, st, g, (const double *)U, scratch, nthr, part);
+    return hipGetLastError();
+}
+
+hipError_t launch_gauss(const Geom &g, const void *U, const void *P, int mode, double *part, long *np,
+                        double *scratch, long scratch_doubles, double *out, hipStream_t st)
+{
+    if (g.N <= 3) return small_gauss(g, U, P, mode, part, np, out, st);
+    const int nr = 2 * g.N * g.N;
+    const long nthr = threads_for(g.V * g.D, 6L * nr, scratch_doubles);
+    *np = nthr;
+    if (mode == kModeF32)
+        hipLaunchKernelGGL(k_gauss_g<float>, dim3(nblk(nthr)), dim3(kB), 0, st, g, (const float *)U, (const float *)P,
+                           scratch, nthr);
+    else
+        hipLaunchKernelGGL(k_gauss_g<double>, dim3(nblk(nthr)), dim3(kB), 0, st, g, (const double *)U,
+                           (const double *)P, scratch, nthr);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // accumulators are strided by 5*nr doubles: gather the (s, c) pairs into
+    // `part` as np rows of nr pairs, then column-reduce.
+    hipError_t e2 = hipMemcpy2DAsync(part, (size_t)2 * nr * sizeof(double), scratch, (size_t)6 * nr * sizeof(double),
+                                     (size_t)2 * nr * sizeof(double), (size_t)nthr, hipMemcpyDeviceToDevice, st);
+    if (e2 != hipSuccess) return e2;
+    return launch_finalize(part, nthr, nr, out, st);
+}
+
+hipError_t launch_kinetic(const Geom &g, const void *P, const void *Q, int mode, double *part, long *np,
+                          hipStream_t st)
+{
+    const long nl = g.V * g.D;
+    *np = nblk(nl);
+    if (mode == kModeF32)
+        hipLaunchKernelGGL(k_kinetic<float>, dim3(nblk(nl)), dim3(kB), 0, st, g, (const float *)P, (const float *)Q, part);
+    else
+        hipLaunchKernelGGL(k_kinetic<double>, dim3(nblk(nl)), dim3(kB), 0, st, g, (const double *)P, (const double *)Q,
+                           part);
+    return hipGetLastError();
+}
+
+hipError_t launch_wlines(const Geom &g, const void *U, int dir, int mode, double *out, double *scratch,
+                         long scratch_doubles, hipStream_t st)
+{
+    if (g.N <= 3) return small_wlines(g, U, dir, mode, out, st);
+    const long nlines = g.V / g.ext[dir];
+    const long chunk = threads_for(nlines, 3L * 2 * g.N * g.N, scratch_doubles);
+    for (long q0 = 0; q0 < nlines; q0 += chunk) {
+        const long nq = nlines - q0 < chunk ? nlines - q0 : chunk;
+        if (mode == kModeF32)
+            hipLaunchKernelGGL(k_wlines_g<float>, dim3(nblk(nq)), dim3(kB), 0, st, g, (const float *)U, dir, q0, nq,
+                               scratch, out);
+        else
+            hipLaunchKernelGGL(k_wlines_g<double>, dim3(nblk(nq)), dim3(kB), 0, st, g, (const double *)U, dir, q0, nq,
+                               scratch, out);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_links_from_aos(const Geom &g, const double *aos, void *U, int mode, hipStream_t st)
+{
+    const long tot = g.V * g.D * 2L * g.N * g.N;
+    if (mode == kModeF32) hipLaunchKernelGGL(k_links_from_aos<float>, dim3(nblk(tot)), dim3(kB), 0, st, g, aos, (float *)U);
+    else hipLaunchKernelGGL(k_links_from_aos<double>, dim3(nblk(tot)), dim3(kB), 0, st, g, aos, (double *)U);
+    return hipGetLastError();
+}
+
+hipError_t launch_links_to_aos(const Geom &g, const void *U, double *aos, int mode, hipStream_t st)
+{
+    const long tot = g.V * g.D * 2L * g.N * g.N;
+    if (mode == kModeF32) hipLaunchKernelGGL(k_links_to_aos<float>, dim3(nblk(tot)), dim3(kB), 0, st, g, (const float *)U, aos);
+    else hipLaunchKernelGGL(k_links_to_aos<double>, dim3(nblk(tot)), dim3(kB), 0, st, g, (const double *)U, aos);
+    return hipGetLastError();
+}
+
+hipError_t launch_mom_from_aos(const Geom &g, const double *aos, void *P, int mode, hipStream_t st)
+{
+    const long tot = g.V * g.D * (long)g.N * g.N;
+    if (mode == kModeF32) hipLaunchKernelGGL(k_mom_from_aos<float>, dim3(nblk(tot)), dim3(kB), 0, st, g, aos, (float *)P);
+    else hipLaunchKernelGGL(k_mom_from_aos<double>, dim3(nblk(tot)), dim3(kB), 0, st, g, aos, (double *)P);
+    return hipGetLastError();
+}
+
+hipError_t launch_mom_to_aos(const Geom &g, const void *P, double *aos, int mode, hipStream_t st)
+{
+    const long tot = g.V * g.D * (long)g.N * g.N;
+    if (mode == kModeF32) hipLaunchKernelGGL(k_mom_to_aos<float>, dim3(nblk(tot)), dim3(kB), 0, st, g, (const float *)P, aos);
+    else hipLaunchKernelGGL(k_mom_to_aos<double>, dim3(nblk(tot)), dim3(kB), 0, st, g, (const double *)P, aos);
+    return hipGetLastError();
+}
+
+hipError_t launch_links_cold(const Geom &g, void *U, int mode, hipStream_t st)
+{
+    const long tot = g.V * g.D * 2L * g.N * g.N;
+    if (mode == kModeF32) hipLaunchKernelGGL(k_cold<float>, dim3(nblk(tot)), dim3(kB), 0, st, g, (float *)U);
+    else hipLaunchKernelGGL(k_cold<double>, dim3(nblk(tot)), dim3(kB), 0, st, g, (double *)U);
+    return hipGetLastError();
+}
+
+hipError_t launch_mom_refresh(const Geom &g, void *P, uint64_t seed, uint64_t counter, double c, int mode,
+                              hipStream_t st)
+{
+    const long nl = g.V * g.D;
+    if (mode == kModeF32)
+        hipLaunchKernelGGL(k_mom_refresh<float>, dim3(nblk(nl)), dim3(kB), 0, st, g, (float *)P, seed, counter, c);
+    else hipLaunchKernelGGL(k_mom_refresh<double>, dim3(nblk(nl)), dim3(kB), 0, st, g, (double *)P, seed, counter, c);
+    return hipGetLastError();
+}
+
+hipError_t launch_mom_scale(const Geom &g, void *P, double f, int mode, hipStream_t st)
+{
+    const long tot = g.V * g.D * (long)g.N * g.N;
+    if (mode == kModeF32) hipLaunchKernelGGL(k_mom_scale<float>, dim3(nblk(tot)), dim3(kB), 0, st, g, (float *)P, f);
+    else hipLaunchKernelGGL(k_mom_scale<double>, dim3(nblk(tot)), dim3(kB), 0, st, g, (double *)P, f);
+    return hipGetLastError();
+}
+
+hipError_t launch_project(const Geom &g, void *U, int mode, int haar, uint64_t seed, uint64_t stream, double *scratch,
+                          long scratch_doubles, hipStream_t st)
+{
+    const long nthr = threads_for(g.V * g.D, 4L * g.N * g.N, scratch_doubles);
+    if (mode == kModeF32)
+        hipLaunchKernelGGL(k_project<float>, dim3(nblk(nthr)), dim3(kB), 0, st, g, (float *)U, haar, seed, stream, scratch,
+                           nthr);
+    else
+        hipLaunchKernelGGL(k_project<double>, dim3(nblk(nthr)), dim3(kB), 0, st, g, (double *)U, haar, seed, stream,
+                           scratch, nthr);
+    return hipGetLastError();
+}
+
+}  // namespace askja

@@ -1,0 +1,118 @@
+// kernels.hpp -- host-side view of the device layout and the launchers that
+// each kernel file exports.  Plain C++ types only; every launcher returns a
+// hipError_t and launches on `stream`.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+
+namespace askja {
+
+// Arithmetic flavours of the MD kernels.
+enum Mode : int {
+    kModeF64 = 0,       // FP64, FMA, factored staples
+    kModeF64Exact = 1,  // FP64 in the reference's operation order, no FMA
+    kModeF32 = 2,       // FP32 storage and arithmetic, FP64 sums
+};
+
+// Geometry shared by every kernel.  Device site index s: all sites of even
+// parity (sum of coordinates mod 2) in lexicographic order, then all odd ones.
+// Link l = mu * V + s.  nbr[(2*d + 0) * V + s] = s + e_d, nbr[(2*d + 1) * V +
+// s] = s - e_d (periodic), dev2lex[s] = lexicographic index (x[D-1] fastest).
+struct Geom {
+    long V;
+    int D;
+    int N;
+    int tile;       // 64 for N <= 3 (links tiled by wavefront), 1 otherwise
+    const int *nbr;
+    const int *dev2lex;
+    const int *lex2dev;
+    int ext[32];
+    long helparr[32];  // lexicographic strides, structs.h:130-136
+};
+
+// Reals per link (2 N^2) and per momentum (N^2, compact Hermitian).
+__host__ __device__ inline int link_reals(int n) { return 2 * n * n; }
+__host__ __device__ inline int mom_reals(int n) { return n * n; }
+
+// Offset of real number e of link l in a field tiled by `tile` links.
+__host__ __device__ inline size_t foff(long l, int e, int per, int tile)
+{
+    return ((size_t)(l / tile) * (size_t)per + (size_t)e) * (size_t)tile + (size_t)(l % tile);
+}
+
+// Index of the (i < j) pair in the compact Hermitian layout: N diagonal
+// reals, then the strict upper triangle row-major as (re, im).
+__host__ __device__ inline int hup(int n, int i, int j)
+{
+    return n + 2 * (i * n - i * (i + 1) / 2 + (j - i - 1));
+}
+
+struct Cinv9 {
+    double c[9];
+};
+
+struct KickArgs {
+    Geom g;
+    const void *U;     // links read (staples)
+    void *Uout;        // links written by the fused drift (may equal nothing)
+    void *P;           // momenta, updated in place
+    void *Pprev;       // receives P before the kick (kick-only variant), may be null
+    double eps_kick;
+    double eps_drift;
+    int level;
+    double cinv[9];    // 1./k, k = 0..8, computed on the host
+    double inv_n;      // 1./N computed on the host
+    double *part;      // per-block partial sums: {ReTr W, Tr Pin^2, Tr Pout^2} x (s, c)
+};
+
+// Number of partial-sum blocks a kick launch writes (each 3 (s, c) pairs).
+long kick_partials(const Geom &g);
+
+hipError_t launch_kick(const KickArgs &a, int mode, bool drift, hipStream_t st);
+hipError_t launch_drift(const Geom &g, void *U, const void *P, double eps, int level,
+                        const double *cinv, int mode, hipStream_t st);
+
+// ---- reductions -----------------------------------------------------------
+// Per-block partials (K (s, c) pairs each) -> out[2K] on the device.
+hipError_t launch_finalize(const double *part, long nparts, int K, double *out, hipStream_t st);
+
+// Plaquette sums: {sum over (0,k) planes, sum over (j,k), 1 <= j < k} of
+// (1 - ReTr P / N).  Returns the number of partial blocks written.
+hipError_t launch_plaq(const Geom &g, const void *U, int mode, double inv_n, double *part,
+                       long *nparts, double *scratch, long scratch_doubles, hipStream_t st);
+// sum over links of ||U U^+ - 1||_F^2 (tr2 of U U^+ - 1)
+hipError_t launch_uviol(const Geom &g, const void *U, int mode, double *part, long *nparts,
+                        double *scratch, long scratch_doubles, hipStream_t st);
+// sum over links of the N x N matrix P - U^+ P U (2 N^2 reals), written to out
+// (device, 2 N^2 (s, c) pairs).
+hipError_t launch_gauss(const Geom &g, const void *U, const void *P, int mode, double *part,
+                        long *nparts, double *scratch, long scratch_doubles, double *out, hipStream_t st);
+// {sum Tr P^2, sum 1/4 Tr (P + Q)^2} over links, Q may be null
+hipError_t launch_kinetic(const Geom &g, const void *P, const void *Q, int mode, double *part,
+                          long *nparts, hipStream_t st);
+// Straight Wilson lines in direction dir starting at x_dir = 0: out[2q..2q+1]
+// = Tr(prod U)/N of line q (lines in lexicographic order of their start).
+hipError_t launch_wlines(const Geom &g, const void *U, int dir, int mode, double *out,
+                         double *scratch, long scratch_doubles, hipStream_t st);
+
+// ---- field set-up -----------------------------------------------------------
+// Host AoS (reference U-*.conf layout, doubles) <-> device layout.
+hipError_t launch_links_from_aos(const Geom &g, const double *aos, void *U, int mode, hipStream_t st);
+hipError_t launch_links_to_aos(const Geom &g, const void *U, double *aos, int mode, hipStream_t st);
+hipError_t launch_mom_from_aos(const Geom &g, const double *aos, void *P, int mode, hipStream_t st);
+hipError_t launch_mom_to_aos(const Geom &g, const void *P, double *aos, int mode, hipStream_t st);
+hipError_t launch_links_cold(const Geom &g, void *U, int mode, hipStream_t st);
+hipError_t launch_mom_refresh(const Geom &g, void *P, uint64_t seed, uint64_t counter, double c,
+                              int mode, hipStream_t st);
+// Doubles of FP64 working memory each generic (N >= 4) diagnostic or
+// projection thread needs.
+long generic_scratch_per_thread(int n);
+hipError_t launch_mom_scale(const Geom &g, void *P, double f, int mode, hipStream_t st);
+// SU(N) projection (Gram-Schmidt on rows + determinant phase), FP64 working
+// copy in `scratch` (>= projection_scratch_doubles(...)).  haar != 0 first
+// fills every link with a Ginibre matrix from Philox (seed, stream).
+hipError_t launch_project(const Geom &g, void *U, int mode, int haar, uint64_t seed, uint64_t stream,
+                          double *scratch, long scratch_doubles, hipStream_t st);
+
+}  // namespace askja

@@ -257,4 +257,55 @@ __device__ inline void taylor_drift(const CM<N, R> &P, R eps, int level, const R
     else mul_fast<false, false, false>(X, U, out);
 }
 
+// ReTr(A B^+) = sum_ij Re(A_ij conj(B_ij)) (FMA flavour).
+template <int N, typename R>
+__device__ inline R retr_abh(const CM<N, R> &A, const CM<N, R> &B)
+{
+    R t = R(0);
+#pragma unroll
+    for (int i = 0; i < N; i++)
+#pragma unroll
+        for (int j = 0; j < N; j++) {
+            t = fma(A.re[i][j], B.re[i][j], t);
+            t = fma(A.im[i][j], B.im[i][j], t);
+        }
+    return t;
+}
+
+// TrR: real part of the trace, i ascending from 0 (sunmat.h:280-292).
+template <int N, typename R>
+__device__ inline R retr(const CM<N, R> &A)
+{
+#pragma clang fp contract(off)
+    R t = R(0);
+#pragma unroll
+    for (int i = 0; i < N; i++) t = t + A.re[i][i];
+    return t;
+}
+
+template <int N, typename R>
+__device__ inline R imtr(const CM<N, R> &A)
+{
+#pragma clang fp contract(off)
+    R t = R(0);
+#pragma unroll
+    for (int i = 0; i < N; i++) t = t + A.im[i][i];
+    return t;
+}
+
+// Squared Frobenius norm of a compact Hermitian momentum = Tr P^2.
+template <int N, typename R>
+__device__ inline double herm_norm2(const CM<N, R> &P)
+{
+    double t = 0.0;
+#pragma unroll
+    for (int i = 0; i < N; i++) t += (double)P.re[i][i] * (double)P.re[i][i];
+#pragma unroll
+    for (int i = 0; i < N; i++)
+#pragma unroll
+        for (int j = i + 1; j < N; j++)
+            t += 2.0 * ((double)P.re[i][j] * (double)P.re[i][j] + (double)P.im[i][j] * (double)P.im[i][j]);
+    return t;
+}
+
 }  // namespace askja

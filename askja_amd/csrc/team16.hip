@@ -1,0 +1,18 @@
+// team16.hip -- instantiates the workgroup-per-link MD kernels for
+// N <= 16 (LDS slots of 16x16, 64 threads per link).
+#include "team_impl.hpp"
+
+namespace askja {
+
+hipError_t team_kick_16(const KickArgs &a, int mode, bool drift, hipStream_t st)
+{
+    return team::kick<16, 64>(a, mode, drift, st);
+}
+
+hipError_t team_drift_16(const Geom &g, void *U, const void *P, double eps, int level, const double *c, int mode,
+                       hipStream_t st)
+{
+    return team::drift<16, 64>(g, U, P, eps, level, c, mode, st);
+}
+
+}  // namespace askja

@@ -1,0 +1,18 @@
+// team32.hip -- instantiates the workgroup-per-link MD kernels for
+// N <= 32 (LDS slots of 32x32, 256 threads per link).
+#include "team_impl.hpp"
+
+namespace askja {
+
+hipError_t team_kick_32(const KickArgs &a, int mode, bool drift, hipStream_t st)
+{
+    return team::kick<32, 256>(a, mode, drift, st);
+}
+
+hipError_t team_drift_32(const Geom &g, void *U, const void *P, double eps, int level, const double *c, int mode,
+                       hipStream_t st)
+{
+    return team::drift<32, 256>(g, U, P, eps, level, c, mode, st);
+}
+
+}  // namespace askja

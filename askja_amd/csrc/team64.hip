@@ -1,0 +1,18 @@
+// team64.hip -- instantiates the workgroup-per-link MD kernels for
+// N <= 64 (LDS slots of 64x64, 256 threads per link).
+#include "team_impl.hpp"
+
+namespace askja {
+
+hipError_t team_kick_64(const KickArgs &a, int mode, bool drift, hipStream_t st)
+{
+    return team::kick<64, 256>(a, mode, drift, st);
+}
+
+hipError_t team_drift_64(const Geom &g, void *U, const void *P, double eps, int level, const double *c, int mode,
+                       hipStream_t st)
+{
+    return team::drift<64, 256>(g, U, P, eps, level, c, mode, st);
+}
+
+}  // namespace askja

@@ -1,0 +1,18 @@
+// team8.hip -- instantiates the workgroup-per-link MD kernels for
+// N <= 8 (LDS slots of 8x8, 64 threads per link).
+#include "team_impl.hpp"
+
+namespace askja {
+
+hipError_t team_kick_8(const KickArgs &a, int mode, bool drift, hipStream_t st)
+{
+    return team::kick<8, 64>(a, mode, drift, st);
+}
+
+hipError_t team_drift_8(const Geom &g, void *U, const void *P, double eps, int level, const double *c, int mode,
+                       hipStream_t st)
+{
+    return team::drift<8, 64>(g, U, P, eps, level, c, mode, st);
+}
+
+}  // namespace askja

@@ -1,0 +1,450 @@
+// team_impl.hpp -- MD kernels for 4 <= N <= 64: one workgroup of NT threads
+// per link, matrices staged in LDS (two slots, planar re/im), each thread
+// owning the output elements e = tid + q*NT (q < EPL) of every product.
+// Included by team<NMAX>.hip, which instantiates one size class.
+#pragma once
+#include "kernels.hpp"
+#include "common.hpp"
+
+namespace askja {
+namespace team {
+
+template <int NMAX, int NT, typename R>
+struct Cfg {
+    static constexpr int EPL = (NMAX * NMAX + NT - 1) / NT;
+    static constexpr int SLOT = NMAX * NMAX;
+};
+
+// global (matrix-contiguous link) -> LDS slot
+template <int NT, typename R>
+__device__ inline void ld_link(R *sre, R *sim, const R *__restrict__ f, long l, int n)
+{
+    const int nr = 2 * n * n;
+    const R *b = f + (size_t)l * nr;
+    for (int r = threadIdx.x; r < nr; r += NT) {
+        const R v = b[r];
+        if (r & 1) sim[r >> 1] = v;
+        else sre[r >> 1] = v;
+    }
+}
+
+// Owned outputs: element index, row, column, validity.
+template <int EPL, int NT>
+struct Own {
+    int i[EPL], j[EPL];
+    bool ok[EPL];
+    __device__ inline void init(int n)
+    {
+#pragma unroll
+        for (int q = 0; q < EPL; q++) {
+            const int e = threadIdx.x + q * NT;
+            ok[q] = e < n * n;
+            i[q] = ok[q] ? e / n : 0;
+            j[q] = ok[q] ? e % n : 0;
+        }
+    }
+};
+
+// c (+)= op(ck*A) op(B) for the owned elements; k ascending from 0.
+// Exact: no FMA and products rounded as (ar*br - ai*bi), as GSL's zgemm.
+template <bool DA, bool DB, bool Acc, bool Exact, int EPL, int NT, typename R>
+__device__ inline void mul(const Own<EPL, NT> &o, const R *ar_, const R *ai_, const R *br_, const R *bi_, int n,
+                           R ck, R (&cr)[EPL], R (&ci)[EPL])
+{
+    if constexpr (!Acc) {
+#pragma unroll
+        for (int q = 0; q < EPL; q++) cr[q] = ci[q] = R(0);
+    }
+    if constexpr (Exact) {
+#pragma clang fp contract(off)
+        for (int k = 0; k < n; k++) {
+#pragma unroll
+            for (int q = 0; q < EPL; q++) {
+                const int ia = DA ? k * n + o.i[q] : o.i[q] * n + k;
+                const int ib = DB ? o.j[q] * n + k : k * n + o.j[q];
+                const R ar = ar_[ia] * ck, ai = (DA ? -ai_[ia] : ai_[ia]) * ck;
+                const R br = br_[ib], bi = DB ? -bi_[ib] : bi_[ib];
+                cr[q] = cr[q] + (ar * br - ai * bi);
+                ci[q] = ci[q] + (ar * bi + ai * br);
+            }
+        }
+    } else {
+        for (int k = 0; k < n; k++) {
+#pragma unroll
+            for (int q = 0; q < EPL; q++) {
+                const int ia = DA ? k * n + o.i[q] : o.i[q] * n + k;
+                const int ib = DB ? o.j[q] * n + k : k * n + o.j[q];
+                const R ar = ar_[ia] * ck, ai = (DA ? -ai_[ia] : ai_[ia]) * ck;
+                const R br = br_[ib], bi = DB ? -bi_[ib] : bi_[ib];
+                cr[q] = fma(ar, br, cr[q]);
+                cr[q] = fma(-ai, bi, cr[q]);
+                ci[q] = fma(ar, bi, ci[q]);
+                ci[q] = fma(ai, br, ci[q]);
+            }
+        }
+    }
+}
+
+template <int EPL, int NT, typename R>
+__device__ inline void st_regs(const Own<EPL, NT> &o, R *sre, R *sim, int n, const R (&cr)[EPL], const R (&ci)[EPL])
+{
+#pragma unroll
+    for (int q = 0; q < EPL; q++)
+        if (o.ok[q]) {
+            sre[o.i[q] * n + o.j[q]] = cr[q];
+            sim[o.i[q] * n + o.j[q]] = ci[q];
+        }
+}
+
+// Compact Hermitian element (i, j) of momentum link l (tile 1).
+template <typename R>
+__device__ inline void mom_get(const R *__restrict__ P, long l, int n, int i, int j, R &re, R &im)
+{
+    const R *b = P + (size_t)l * n * n;
+    if (i == j) {
+        re = b[i];
+        im = R(0);
+    } else if (i < j) {
+        const int h = hup(n, i, j);
+        re = b[h];
+        im = b[h + 1];
+    } else {
+        const int h = hup(n, j, i);
+        re = b[h];
+        im = -b[h + 1];
+    }
+}
+
+// X = 1 + c*rhs on the owned elements (Horner seed, sunmat.h:200-248)
+template <bool Exact, int EPL, int NT, typename R>
+__device__ inline void horner_seed(const Own<EPL, NT> &o, const R *rre, const R *rim, int n, R c, R (&xr)[EPL],
+                                   R (&xi)[EPL])
+{
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int q = 0; q < EPL; q++) {
+        const int e = o.i[q] * n + o.j[q];
+        xr[q] = (o.i[q] == o.j[q] ? R(1) : R(0)) + rre[e] * c;
+        xi[q] = R(0) + rim[e] * c;
+    }
+}
+
+// Drift of one link whose i*eps*P is in slot A: Uout = T_L(A) U_l.
+// Uses both slots; on return slot contents are undefined.
+template <bool Exact, int EPL, int NT, typename R>
+__device__ inline void drift_core(const Own<EPL, NT> &o, R *Are, R *Aim, R *Bre, R *Bim, int n, int level,
+                                  const double *cinv, const R *__restrict__ Uin, long l, R *__restrict__ Uout)
+{
+#pragma clang fp contract(off)
+    if (level < 0 || level > 8) level = 8;
+    R xr[EPL], xi[EPL];
+    if (level == 0) {
+#pragma unroll
+        for (int q = 0; q < EPL; q++) {
+            xr[q] = o.i[q] == o.j[q] ? R(1) : R(0);
+            xi[q] = R(0);
+        }
+    } else {
+        horner_seed<Exact>(o, Are, Aim, n, level == 1 ? R(1) : (R)cinv[level], xr, xi);
+        for (int k = level - 1; k >= 1; k--) {
+            __syncthreads();
+            st_regs(o, Bre, Bim, n, xr, xi);
+            __syncthreads();
+            R qr[EPL], qi[EPL];
+            mul<false, false, false, Exact>(o, Are, Aim, Bre, Bim, n, (R)cinv[k], qr, qi);
+#pragma unroll
+            for (int q = 0; q < EPL; q++) {
+                xr[q] = (o.i[q] == o.j[q] ? R(1) : R(0)) + qr[q];
+                xi[q] = R(0) + qi[q];
+            }
+        }
+    }
+    __syncthreads();
+    st_regs(o, Bre, Bim, n, xr, xi);
+    ld_link<NT>(Are, Aim, Uin, l, n);
+    __syncthreads();
+    R ur[EPL], ui[EPL];
+    mul<false, false, false, Exact>(o, Bre, Bim, Are, Aim, n, R(1), ur, ui);
+    R *b = Uout + (size_t)l * 2 * n * n;
+#pragma unroll
+    for (int q = 0; q < EPL; q++)
+        if (o.ok[q]) {
+            const int e = o.i[q] * n + o.j[q];
+            b[2 * e] = ur[q];
+            b[2 * e + 1] = ui[q];
+        }
+}
+
+template <int NMAX, int NT, typename R, bool Exact, bool Drift>
+__global__ __launch_bounds__(NT) void k_kick(KickArgs a)
+{
+    using C = Cfg<NMAX, NT, R>;
+    constexpr int EPL = C::EPL;
+    __shared__ R Are[C::SLOT], Aim[C::SLOT], Bre[C::SLOT], Bim[C::SLOT];
+    const Geom &g = a.g;
+    const int n = g.N;
+    const long V = g.V;
+    const long l = blockIdx.x;
+    const int mu = (int)(l / V);
+    const long s = l - (long)mu * V;
+    const R *U = (const R *)a.U;
+    Own<EPL, NT> o;
+    o.init(n);
+    const long xp = g.nbr[(2 * mu) * V + s];
+
+    R wr[EPL], wi[EPL], tr[EPL], ti[EPL];
+#pragma unroll
+    for (int q = 0; q < EPL; q++) wr[q] = wi[q] = R(0);
+    for (int nu = 0; nu < g.D; nu++) {
+        if (nu == mu) continue;
+        const long xpn = g.nbr[(2 * nu) * V + s];
+        const long xmn = g.nbr[(2 * nu + 1) * V + s];
+        const long xpmn = g.nbr[(2 * nu + 1) * V + xp];
+        if constexpr (Exact) {
+            // ((U_mu(x) U_nu(x+mu)) U_mu(x+nu)^+) U_nu(x)^+
+            __syncthreads();
+            ld_link<NT>(Are, Aim, U, mu * V + s, n);
+            ld_link<NT>(Bre, Bim, U, nu * V + xp, n);
+            __syncthreads();
+            mul<false, false, false, true>(o, Are, Aim, Bre, Bim, n, R(1), tr, ti);
+            __syncthreads();
+            st_regs(o, Are, Aim, n, tr, ti);
+            ld_link<NT>(Bre, Bim, U, mu * V + xpn, n);
+            __syncthreads();
+            mul<false, true, false, true>(o, Are, Aim, Bre, Bim, n, R(1), tr, ti);
+            __syncthreads();
+            st_regs(o, Are, Aim, n, tr, ti);
+            ld_link<NT>(Bre, Bim, U, nu * V + s, n);
+            __syncthreads();
+            mul<false, true, false, true>(o, Are, Aim, Bre, Bim, n, R(1), tr, ti);
+            {
+#pragma clang fp contract(off)
+#pragma unroll
+                for (int q = 0; q < EPL; q++) {
+                    wr[q] = wr[q] + tr[q];
+                    wi[q] = wi[q] + ti[q];
+                }
+            }
+            // ((U_mu(x) U_nu(x+mu-nu)^+) U_mu(x-nu)^+) U_nu(x-nu)
+            __syncthreads();
+            ld_link<NT>(Are, Aim, U, mu * V + s, n);
+            ld_link<NT>(Bre, Bim, U, nu * V + xpmn, n);
+            __syncthreads();
+            mul<false, true, false, true>(o, Are, Aim, Bre, Bim, n, R(1), tr, ti);
+            __syncthreads();
+            st_regs(o, Are, Aim, n, tr, ti);
+            ld_link<NT>(Bre, Bim, U, mu * V + xmn, n);
+            __syncthreads();
+            mul<false, true, false, true>(o, Are, Aim, Bre, Bim, n, R(1), tr, ti);
+            __syncthreads();
+            st_regs(o, Are, Aim, n, tr, ti);
+            ld_link<NT>(Bre, Bim, U, nu * V + xmn, n);
+            __syncthreads();
+            mul<false, false, false, true>(o, Are, Aim, Bre, Bim, n, R(1), tr, ti);
+            {
+#pragma clang fp contract(off)
+#pragma unroll
+                for (int q = 0; q < EPL; q++) {
+                    wr[q] = wr[q] + tr[q];
+                    wi[q] = wi[q] + ti[q];
+                }
+            }
+        } else {
+            // S += U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+
+            __syncthreads();
+            ld_link<NT>(Are, Aim, U, nu * V + xp, n);
+            ld_link<NT>(Bre, Bim, U, mu * V + xpn, n);
+            __syncthreads();
+            mul<false, true, false, false>(o, Are, Aim, Bre, Bim, n, R(1), tr, ti);
+            __syncthreads();
+            st_regs(o, Are, Aim, n, tr, ti);
+            ld_link<NT>(Bre, Bim, U, nu * V + s, n);
+            __syncthreads();
+            mul<false, true, true, false>(o, Are, Aim, Bre, Bim, n, R(1), wr, wi);
+            // S += U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu)
+            __syncthreads();
+            ld_link<NT>(Are, Aim, U, nu * V + xpmn, n);
+            ld_link<NT>(Bre, Bim, U, mu * V + xmn, n);
+            __syncthreads();
+            mul<true, true, false, false>(o, Are, Aim, Bre, Bim, n, R(1), tr, ti);
+            __syncthreads();
+            st_regs(o, Are, Aim, n, tr, ti);
+            ld_link<NT>(Bre, Bim, U, nu * V + xmn, n);
+            __syncthreads();
+            mul<false, false, true, false>(o, Are, Aim, Bre, Bim, n, R(1), wr, wi);
+        }
+    }
+    if constexpr (!Exact) {
+        // W = U_mu(x) S
+        __syncthreads();
+        st_regs(o, Are, Aim, n, wr, wi);
+        ld_link<NT>(Bre, Bim, U, mu * V + s, n);
+        __syncthreads();
+        mul<false, false, false, false>(o, Bre, Bim, Are, Aim, n, R(1), tr, ti);
+#pragma unroll
+        for (int q = 0; q < EPL; q++) {
+            wr[q] = tr[q];
+            wi[q] = ti[q];
+        }
+    }
+    // TAH and kick on the owned elements (sunmat.h:355-371, sun.cpp:716)
+    __syncthreads();
+    st_regs(o, Are, Aim, n, wr, wi);
+    __syncthreads();
+    const R eps = (R)a.eps_kick, inv_n = (R)a.inv_n;
+    R *P = (R *)a.P;
+    double sum[3] = {0.0, 0.0, 0.0}, cmp[3] = {0.0, 0.0, 0.0};
+    R pr[EPL], pi[EPL], ppr[EPL], ppi[EPL];
+    {
+#pragma clang fp contract(off)
+        R tri = R(0);
+        for (int i = 0; i < n; i++) tri = tri + (Aim[i * n + i] + Aim[i * n + i]);
+        const R corr = tri * inv_n;
+#pragma unroll
+        for (int q = 0; q < EPL; q++) {
+            const int i = o.i[q], j = o.j[q];
+            R inr, ini;
+            mom_get(P, l, n, i, j, inr, ini);
+            ppr[q] = inr;
+            ppi[q] = ini;
+            if (i == j) {
+                const R mi = (Aim[i * n + i] + Aim[i * n + i]) - corr;
+                pr[q] = inr - (R(0.5) * mi) * eps;
+                pi[q] = R(0);
+            } else {
+                const int lo = i < j ? i : j, hi = i < j ? j : i;
+                const R dr = Are[lo * n + hi] - Are[hi * n + lo];
+                const R di = Aim[lo * n + hi] + Aim[hi * n + lo];
+                const R qr = (R(0.5) * di) * eps;
+                const R qi = (R(0.5) * dr) * eps;
+                // value of the (lo, hi) element; (hi, lo) is its conjugate
+                R ur, ui;
+                mom_get(P, l, n, lo, hi, ur, ui);
+                const R vr = ur - qr, vi = ui + qi;
+                pr[q] = vr;
+                pi[q] = i < j ? vi : -vi;
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < EPL; q++)
+        if (o.ok[q]) {
+            if (o.i[q] == o.j[q]) sum[0] += (double)wr[q];
+            sum[1] += (double)ppr[q] * (double)ppr[q] + (double)ppi[q] * (double)ppi[q];
+            sum[2] += (double)pr[q] * (double)pr[q] + (double)pi[q] * (double)pi[q];
+        }
+    // store the new momentum (compact: owners of i <= j); every thread has
+    // finished reading the old one
+    __syncthreads();
+    {
+        R *b = P + (size_t)l * n * n;
+        R *bp = (!Drift && a.Pprev) ? (R *)a.Pprev + (size_t)l * n * n : nullptr;
+#pragma unroll
+        for (int q = 0; q < EPL; q++) {
+            const int i = o.i[q], j = o.j[q];
+            if (!o.ok[q] || i > j) continue;
+            if (i == j) {
+                b[i] = pr[q];
+                if (bp) bp[i] = ppr[q];
+            } else {
+                const int h = hup(n, i, j);
+                b[h] = pr[q];
+                b[h + 1] = pi[q];
+                if (bp) {
+                    bp[h] = ppr[q];
+                    bp[h + 1] = ppi[q];
+                }
+            }
+        }
+    }
+    if constexpr (Drift) {
+        // rhs = i * (eps_d * P) into slot A
+        const R ed = (R)a.eps_drift;
+        __syncthreads();
+        {
+#pragma clang fp contract(off)
+#pragma unroll
+            for (int q = 0; q < EPL; q++)
+                if (o.ok[q]) {
+                    const int e = o.i[q] * n + o.j[q];
+                    Are[e] = -(pi[q] * ed);
+                    Aim[e] = pr[q] * ed;
+                }
+        }
+        __syncthreads();
+        drift_core<Exact>(o, Are, Aim, Bre, Bim, n, a.level, a.cinv, U, l, (R *)a.Uout);
+    }
+    block_reduce_store<3>(sum, cmp, a.part + 6 * (size_t)blockIdx.x);
+}
+
+template <int NMAX, int NT, typename R, bool Exact>
+__global__ __launch_bounds__(NT) void k_drift(Geom g, R *U, const R *P, double eps, int level, Cinv9 cinv)
+{
+    using C = Cfg<NMAX, NT, R>;
+    constexpr int EPL = C::EPL;
+    __shared__ R Are[C::SLOT], Aim[C::SLOT], Bre[C::SLOT], Bim[C::SLOT];
+    const int n = g.N;
+    const long l = blockIdx.x;
+    Own<EPL, NT> o;
+    o.init(n);
+    {
+#pragma clang fp contract(off)
+#pragma unroll
+        for (int q = 0; q < EPL; q++)
+            if (o.ok[q]) {
+                R re, im;
+                mom_get(P, l, n, o.i[q], o.j[q], re, im);
+                const int e = o.i[q] * n + o.j[q];
+                Are[e] = -(im * (R)eps);
+                Aim[e] = re * (R)eps;
+            }
+    }
+    __syncthreads();
+    drift_core<Exact>(o, Are, Aim, Bre, Bim, n, level, cinv.c, U, l, U);
+}
+
+template <int NMAX, int NT>
+hipError_t kick(const KickArgs &a, int mode, bool drift, hipStream_t st)
+{
+    const unsigned nb = (unsigned)(a.g.V * a.g.D);
+    switch (mode) {
+    case kModeF64:
+        if (drift) hipLaunchKernelGGL((k_kick<NMAX, NT, double, false, true>), dim3(nb), dim3(NT), 0, st, a);
+        else hipLaunchKernelGGL((k_kick<NMAX, NT, double, false, false>), dim3(nb), dim3(NT), 0, st, a);
+        break;
+    case kModeF64Exact:
+        if (drift) hipLaunchKernelGGL((k_kick<NMAX, NT, double, true, true>), dim3(nb), dim3(NT), 0, st, a);
+        else hipLaunchKernelGGL((k_kick<NMAX, NT, double, true, false>), dim3(nb), dim3(NT), 0, st, a);
+        break;
+    default:
+        if (drift) hipLaunchKernelGGL((k_kick<NMAX, NT, float, false, true>), dim3(nb), dim3(NT), 0, st, a);
+        else hipLaunchKernelGGL((k_kick<NMAX, NT, float, false, false>), dim3(nb), dim3(NT), 0, st, a);
+    }
+    return hipGetLastError();
+}
+
+template <int NMAX, int NT>
+hipError_t drift(const Geom &g, void *U, const void *P, double eps, int level, const double *c, int mode,
+                 hipStream_t st)
+{
+    const unsigned nb = (unsigned)(g.V * g.D);
+    Cinv9 ci;
+    for (int k = 0; k < 9; k++) ci.c[k] = c[k];
+    switch (mode) {
+    case kModeF64:
+        hipLaunchKernelGGL((k_drift<NMAX, NT, double, false>), dim3(nb), dim3(NT), 0, st, g, (double *)U,
+                           (const double *)P, eps, level, ci);
+        break;
+    case kModeF64Exact:
+        hipLaunchKernelGGL((k_drift<NMAX, NT, double, true>), dim3(nb), dim3(NT), 0, st, g, (double *)U,
+                           (const double *)P, eps, level, ci);
+        break;
+    default:
+        hipLaunchKernelGGL((k_drift<NMAX, NT, float, false>), dim3(nb), dim3(NT), 0, st, g, (float *)U,
+                           (const float *)P, eps, level, ci);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace team
+}  // namespace askja

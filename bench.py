@@ -1,0 +1,186 @@
+"""Benchmark of the HMC hot path on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): plain HMC for
+SU(3) in D=4 on an 8x32^3 lattice (N0 = 8 temporal), beta = 5.9, DT = 0.02,
+LEVEL = 6, hot (Haar-random) start, 50 MD steps per trajectory (tau = 1),
+counter-based momenta, FP64 (or --precision fp32: FP32 links and momenta with
+FP64 sums and Metropolis).  Data are synthetic: the links start Haar random
+from a fixed seed.
+
+One step = one HMC trajectory (momentum refresh, 50 fused kick+drift MD
+steps, the final half kick, H_old/H_new sums, Metropolis) plus the
+Polyakov-loop measurement that feeds the autocorrelation time.  --warmup
+trajectories thermalise the chain untimed; --steps trajectories are timed.
+
+metric value = (trajectories/s) / tau, with tau = 1 + 2 sum_t rho(t) the
+integrated autocorrelation time of |volume-averaged Polyakov loop| estimated
+over the timed trajectories by Wolff's automatic windowing (S = 1.5); tau = 1
+for uncorrelated data, so the value counts statistically independent
+configurations per second.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+NC, EXT, BETA, DT, LEVEL, NDRIFT = 3, [8, 32, 32, 32], 5.9, 0.02, 6, 50
+SEED = 120351616
+HBM_PEAK_GBS = 8000.0
+
+
+def tau_int_wolff(x, S=1.5):
+    """Integrated autocorrelation time (Wolff, Comput. Phys. Commun. 156
+    (2004) 143): automatic window W minimising the sum of statistical and
+    systematic errors.  Returns tau = 2 * tau_int(Wolff) = 1 + 2 sum rho."""
+    x = np.asarray(x, np.float64)
+    n = len(x)
+    if n < 8:
+        return float("nan"), 0
+    d = x - x.mean()
+    var = np.dot(d, d) / n
+    if var <= 0:
+        return 1.0, 0
+    f = np.fft.rfft(d, 2 * n)
+    acf = np.fft.irfft(f * np.conj(f))[:n] / (n - np.arange(n)) / var
+    tint = 0.5
+    for W in range(1, n // 2):
+        tint += acf[W]
+        tau = S / math.log((2 * tint + 1) / (2 * tint - 1)) if tint > 0.5 else 1e-9
+        g = math.exp(-W / tau) - tau / math.sqrt(W * n)
+        if g < 0:
+            return 2.0 * tint, W
+    return 2.0 * tint, n // 2
+
+
+def cpu_baseline(steps_sample):
+    """The oracle (FP64 C restatement of the reference, OpenMP over sites in
+    the kick and drift like sun.cpp:679,777) on the host cores, same lattice:
+    `steps_sample` MD steps timed and extrapolated to one trajectory
+    (NDRIFT steps + the two Hamiltonian evaluations)."""
+    from oracle import oracle as O
+
+    o = O.Oracle(NC, EXT, beta=BETA, level=LEVEL)
+    o.haar(SEED, 2)
+    o.mom_philox(SEED, 0)
+    t0 = time.perf_counter()
+    h = o.hamiltonian("pi")
+    t_h = time.perf_counter() - t0
+    o.updateE(DT / 2)
+    t0 = time.perf_counter()
+    for _ in range(steps_sample):
+        o.updateUandcopyDown(DT)
+        o.updateE(DT)
+    t_step = (time.perf_counter() - t0) / steps_sample
+    t_traj = NDRIFT * t_step + 2 * t_h
+    assert math.isfinite(h)
+    return {"value": 1.0 / t_traj, "unit": "traj/s", "cores": O.num_threads(), "kind": "port",
+            "sample": f"{steps_sample} FP64 MD steps (kick+drift) of the oracle on 8x32^3 SU(3), "
+                      f"{t_step * 1e3:.0f} ms/step, extrapolated to {NDRIFT} steps + 2 H evaluations"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=300)
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--roofline-launches", type=int, default=100)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU path is not built.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import askja_amd as A
+
+    prec = A.FP64 if args.precision == "fp64" else A.FP32
+    lat = A.Lattice(NC, EXT, prec)
+    lat.haar(SEED, 2)
+    t_w = time.perf_counter()
+    for t in range(args.warmup):
+        lat.hmc_trajectory(BETA, DT, NDRIFT, LEVEL, SEED, t, reunit=True)
+        if (t + 1) % 100 == 0:
+            print(f"[bench] warmup {t + 1}/{args.warmup} ({time.perf_counter() - t_w:.1f} s)", file=sys.stderr,
+                  flush=True)
+    lat.sync()
+    poly, dh, acc, plaq = [], [], [], []
+    t0 = time.perf_counter()
+    last = t0
+    for k in range(args.steps):
+        r = lat.hmc_trajectory(BETA, DT, NDRIFT, LEVEL, SEED, args.warmup + k, reunit=True)
+        poly.append(abs(lat.polyakov()))
+        dh.append(r.dh)
+        acc.append(r.accepted)
+        plaq.append(1.0 - r.action / lat.n_plaquettes())
+        now = time.perf_counter()
+        if now - last > 30:
+            print(f"[bench] timed {k + 1}/{args.steps}", file=sys.stderr, flush=True)
+            last = now
+    lat.sync()
+    T = time.perf_counter() - t0
+    traj_s = args.steps / T
+    tau, window = tau_int_wolff(poly)
+    value = traj_s / tau
+
+    # roofline of the dominant kernel: the fused kick+drift launch, timed
+    # with HIP events on the library's stream
+    nl = lat.V * lat.dim
+    esz = 8 if prec == A.FP64 else 4
+    bytes_per_link = 2 * (2 * NC * NC * esz) + 2 * (NC * NC * esz)  # read+write link, read+write momentum
+    flops_per_link = (4 * (lat.dim - 1) + 1 + LEVEL) * 8 * NC**3
+    lat.mom_refresh(SEED, 10**9, BETA)
+    for _ in range(5):
+        lat.kick_drift(DT, DT, LEVEL)
+    lat.timer_start()
+    for _ in range(args.roofline_launches):
+        lat.kick_drift(DT, DT, LEVEL)
+    ms = lat.timer_stop() / args.roofline_launches
+    achieved = bytes_per_link * nl / (ms * 1e-3) / 1e9
+    tflops = flops_per_link * nl / (ms * 1e-3) / 1e12
+
+    out = {
+        "metric": "indep. configs/s (traj/s / tau_int Polyakov), plain HMC",
+        "value": value,
+        "unit": "configs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1000.0 * T / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == A.FP64 else "f32",
+        "data": "synthetic (Haar-random SU(3) start, seed 120351616)",
+        "config": {"workload": "plain HMC, SU(3) D=4 8x32^3 beta=5.9 DT=0.02 LEVEL=6 hot start, "
+                               "50 MD steps/trajectory", "nc": NC, "extents": EXT, "beta": BETA, "dt": DT,
+                   "level": LEVEL, "md_steps_per_traj": NDRIFT, "precision": args.precision},
+        "traj_per_s": traj_s,
+        "md_steps_per_s": traj_s * NDRIFT,
+        "tau_int_polyakov": tau,
+        "tau_window": window,
+        "acceptance": float(np.mean(acc)),
+        "mean_plaquette": float(np.mean(plaq)),
+        "mean_dH": float(np.mean(dh)),
+        "exp_minus_dH": float(np.mean(np.exp(-np.asarray(dh)))),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "fused kick+drift", "ms_per_launch": ms, "bytes_per_link": bytes_per_link,
+                     "links_per_launch": nl, "tflops": tflops},
+    }
+    if not args.no_cpu:
+        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_steps)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

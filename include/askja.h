@@ -3,12 +3,12 @@
  *
  * The reference (paro8929/askja) has no library API: its hot path is a set of
  * void functions over global arrays in sun.cpp.  Each entry point below
- * replaces one of them (cited), operating on device-resident fields owned by a
- * context.  Plain pointers and sizes only; no C++ or torch types.
+ * replaces one of them (cited as file:line of the reference), operating on
+ * device-resident fields owned by a context.  Plain pointers and sizes only.
  *
  * Conventions
- *  - Every call returns 0 on success or a negative ASKJA_E* code; the message
- *    is kept per context (askja_last_error) and errors are sticky: after a
+ *  - Every call returns ASKJA_OK (0) or a negative ASKJA_E* code; the message
+ *    is kept per context (askja_last_error).  Errors are sticky: after a
  *    failure every further call on that context returns ASKJA_ESTICKY until
  *    askja_clear_error.
  *  - One host thread per context.  Kernels run on the context's HIP stream;
@@ -17,11 +17,12 @@
  *    boundary in the reference's U-*.conf layout: for each site in
  *    lexicographic order (x[D-1] fastest), for each direction mu = 0..D-1, an
  *    N x N complex matrix, row-major, (re, im) interleaved doubles
- *    (sun.cpp:289-302, structs.h:130-165).  Length V*D*N*N*2 doubles.
+ *    (sun.cpp:289-302, structs.h:130-165).  Length askja_field_doubles().
  *  - Device layout is private: links grouped by direction, sites in
- *    checkerboard order (even sites, then odd, each in lexicographic order),
- *    structure-of-arrays across 64-site tiles for N <= 3, matrix-contiguous
- *    for N >= 4; momenta stored as Hermitian halves (N^2 reals per link).
+ *    checkerboard order (even-parity sites, then odd, each lexicographic);
+ *    for N <= 3 the links are tiled in groups of 64 (one wavefront) with each
+ *    matrix element contiguous across the group, for N >= 4 every matrix is
+ *    contiguous.  Momenta are stored as compact Hermitian matrices (N^2 reals).
  */
 #ifndef ASKJA_H
 #define ASKJA_H
@@ -39,10 +40,10 @@ typedef struct askja_ctx askja_ctx;
 enum {
     ASKJA_FP64 = 0,       /* FP64 storage and arithmetic, FMA, factored staples */
     ASKJA_FP64_EXACT = 1, /* FP64 in the reference's operation order, no FMA:
-                             bit-identical to the reference's arithmetic */
-    ASKJA_FP32 = 2,       /* FP32 links/momenta/MD; FP64 sums, reunitarisation */
-    ASKJA_BF16X3 = 3      /* FP32 storage, link products as split BF16 (hi*hi +
-                             hi*lo + lo*hi) on MFMA with FP32 accumulation */
+                             links and momenta bit-identical to the reference's
+                             double arithmetic (sunmat.h:81-371, sun.cpp:672-801) */
+    ASKJA_FP32 = 2        /* FP32 links/momenta/MD; FP64 compensated sums,
+                             FP64 Metropolis and FP64 reunitarisation */
 };
 
 /* error codes */
@@ -56,21 +57,21 @@ enum {
     ASKJA_EIO = -6       /* file read/write failed */
 };
 
-/* Context: allocate_memory + generators + xygrid.init
- * (sun.cpp:96-125, 142-189, 1086-1097; structs.h:109-147).
- * extents[0..dim-1] = N0..N_{D-1}; extents of 1 are allowed.
- * Returns ASKJA_EUNSUP for nc > 64 or dim > 32. */
+/* ---- context ----------------------------------------------------------------- */
+/* allocate_memory + generators + xygrid.init (sun.cpp:96-125, 142-189,
+ * 1086-1097; structs.h:109-147).  extents[0..dim-1] = N0..N_{D-1}; extents of
+ * 1 are allowed.  1 <= nc <= 64, 1 <= dim <= 32.  Links start cold. */
 int askja_create(int nc, int dim, const int *extents, int precision, askja_ctx **out);
 /* free_memory (sun.cpp:128-139) */
 int askja_destroy(askja_ctx *ctx);
+/* Message of the last failure on ctx ("" if none); valid until the next call. */
 const char *askja_last_error(const askja_ctx *ctx);
 int askja_clear_error(askja_ctx *ctx);
-/* device ordinal, stream, sizes */
 int askja_info(const askja_ctx *ctx, long *volume, int *nc, int *dim, int *precision);
 /* Number of doubles in one host field buffer (V*D*N*N*2). */
 size_t askja_field_doubles(const askja_ctx *ctx);
 
-/* ---- links ---------------------------------------------------------------- */
+/* ---- links ------------------------------------------------------------------- */
 /* loadconf (sun.cpp:744-765), its in-memory half: links <- host buffer */
 int askja_links_upload(askja_ctx *ctx, const double *aos);
 /* storeconf (sun.cpp:289-302), its in-memory half: host buffer <- links */
@@ -78,101 +79,113 @@ int askja_links_download(askja_ctx *ctx, double *aos);
 /* initializeU (sun.cpp:194-207): cold start, all links = identity */
 int askja_links_cold(askja_ctx *ctx);
 /* Haar-random SU(N) start (no reference counterpart): Ginibre matrix from
- * Philox-4x32-10 (key = seed, stream selects the counter space), then the
- * SU(N) projection of askja_reunitarize. */
+ * Philox-4x32-10 (key = seed, counter = stream), then the projection of
+ * askja_reunitarize. */
 int askja_links_haar(askja_ctx *ctx, uint64_t seed, uint64_t stream);
 /* Project every link onto SU(N) in FP64 (no reference counterpart; the
  * reference only monitors unitarity, sun.cpp:441-471): modified Gram-Schmidt
- * on the rows, then multiply by exp(-i arg(det)/N). */
+ * on the rows, then multiplication by exp(-i arg(det)/N). */
 int askja_reunitarize(askja_ctx *ctx);
 /* takeconfiguration / loadconfiguration (sun.cpp:725-742):
  * op 0 = save links to the snapshot, 1 = restore links from it. */
 int askja_snapshot(askja_ctx *ctx, int op);
 
-/* ---- momenta --------------------------------------------------------------- */
-/* initializeE (sun.cpp:209-252), host-stream form: momenta <- host buffer
- * (Hermitian matrices, same layout as links).  With the reference's own
- * MT19937/polar stream generated on the host this reproduces its momenta
- * exactly.  Resets the pre-kick momentum to the same field. */
+/* ---- momenta ------------------------------------------------------------------ */
+/* momenta <- host buffer of Hermitian matrices (links layout); the momentum
+ * before the last kick is set to the same field. */
 int askja_mom_upload(askja_ctx *ctx, const double *aos);
 /* host buffer <- current momentum field (the reference's Pi) */
 int askja_mom_download(askja_ctx *ctx, double *aos);
-/* host buffer <- momentum before the last kick (the reference's pi) */
+/* host buffer <- momentum before the last askja_kick (the reference's pi) */
 int askja_mom_download_prev(askja_ctx *ctx, double *aos);
-/* initializeE, counter-based form: pi = sqrt(2N/beta) sum_a xi_a T^a with
- * xi from Philox-4x32-10 (key = seed, counter = traj_counter) and Box-Muller;
- * generator order as sun.cpp:142-189. */
-int askja_mom_refresh(askja_ctx *ctx, uint64_t seed, uint64_t traj_counter, double beta);
-/* Flip the sign of the momenta (for reversibility checks). */
+/* initializeE(seed) (sun.cpp:209-252) exactly as the reference computes it:
+ * GSL's mt19937 seeded with `seed`, Gaussian deviates by the polar method,
+ * order site -> direction -> generator, pi = sqrt(2N/beta) sum_a xi_a T^a
+ * with the reference's roundings.  Generated on the host, then uploaded. */
+int askja_mom_gsl(askja_ctx *ctx, unsigned long seed, double beta);
+/* initializeE, counter-based form (used for every trajectory after the
+ * first): xi from Philox-4x32-10 (key = seed, counter = counter) and
+ * Box-Muller, generated on the device. */
+int askja_mom_refresh(askja_ctx *ctx, uint64_t seed, uint64_t counter, double beta);
+/* Flip the sign of the momenta (reversibility checks). */
 int askja_mom_negate(askja_ctx *ctx);
 
-/* ---- molecular dynamics ------------------------------------------------------ */
-/* updateE (sun.cpp:672-723): Pi = pi - eps * TAH(U_mu(x) sum_nu staples).
- * Out of place like the reference: the momentum before the kick stays
- * available to the E_L and Gauss-law diagnostics. */
+/* ---- molecular dynamics -------------------------------------------------------- */
+/* updateE (sun.cpp:672-723): Pi = pi - eps * TAH(U_mu(x) sum_nu staples), with
+ * the momentum before the kick kept for askja_mom_download_prev and the E_L
+ * and Gauss-law diagnostics. */
 int askja_kick(askja_ctx *ctx, double eps);
 /* updateUandcopyDown (sun.cpp:774-790): U = T_L(i eps Pi) U with T_L the
  * order-`level` Taylor exp by Horner (sunmat.h:200-248; level outside 0..8
- * means 8), then pi = Pi. */
+ * means 8). */
 int askja_drift(askja_ctx *ctx, double eps, int level);
+/* A kick followed by the drift of the same links in one pass over the
+ * lattice (the pair updateE(eps_kick); updateUandcopyDown(eps_drift)). */
+int askja_kick_drift(askja_ctx *ctx, double eps_kick, double eps_drift, int level);
 /* The MD part of one trajectory (sun.cpp:1178-1218, 1249-1251):
  * kick(dt/2), ndrift x [drift(dt); kick(dt) except after the last drift],
- * kick(dt/2).  The reference's first trajectory has ndrift = UPDATE2-1. */
-int askja_md_trajectory(askja_ctx *ctx, double dt, int ndrift, int level);
+ * kick(dt/2).  The reference's first trajectory has ndrift = UPDATE2-1.
+ * energies (may be NULL) receives {S(start), sum Tr pi^2 (start), S(end),
+ * sum Tr Pi^2 (end)} with S = EnergyMag (sun.cpp:507-537). */
+int askja_md_trajectory(askja_ctx *ctx, double dt, int ndrift, int level, double *energies);
 
-/* ---- reductions (FP64, compensated) ------------------------------------------ */
+/* ---- reductions (FP64, compensated) ------------------------------------------- */
 /* EnergyMag (sun.cpp:507-537) split as PlaqExp (measurements4d.cpp:885-927):
  * s_t = sum_x sum_{k>=1} (1 - ReTr P_0k/N), s_s = sum over 1 <= j < k.
  * EnergyMag = s_t + s_s.  Either pointer may be NULL. */
 int askja_action(askja_ctx *ctx, double *s_t, double *s_s);
 /* simpleenergyEl (sun.cpp:540-548): sum Tr(Pi^2) of the current momenta */
 int askja_kinetic(askja_ctx *ctx, double *out);
-/* EnergyEl (sun.cpp:560-567): 1/4 sum tr2(pi + Pi) */
+/* EnergyEl (sun.cpp:560-567): 1/4 sum Tr (pi + Pi)^2 */
 int askja_energy_el(askja_ctx *ctx, double *out);
 /* GaussLaw (sun.cpp:415-438): tr2 of sum_{x,mu} [Pi_mu(x) - U^+ Pi U at x-mu] */
 int askja_gauss_law(askja_ctx *ctx, double *out);
-/* UViol (sun.cpp:441-471): sum ||U U^+ - 1||_F^2 */
+/* UViol (sun.cpp:441-471): sum over links of tr2(U U^+ - 1) */
 int askja_uviol(askja_ctx *ctx, double *out);
 
-/* ---- Wilson lines ------------------------------------------------------------- */
+/* ---- Wilson lines ----------------------------------------------------------------- */
 /* Traces Tr(prod_{t} U_dir(x + t dir))/N of every straight line in direction
  * dir that starts at x_dir = 0, ordered lexicographically by start site.
  * out holds 2*V/extent[dir] doubles (re, im).  dir = 0 gives the Polyakov
- * loops; WilsonLoop (sun.cpp:307-345) is entry 0, PolyakovLoop
+ * loops: WilsonLoop (sun.cpp:307-345) is |entry 0|, PolyakovLoop
  * (measurements4d.cpp:410-466) the mean of |.| over x1 at x2.. = 0, and
  * WilsonLoop(evs, dir) (measurements4d.cpp:311-355) the mean over x0. */
 int askja_wilson_lines(askja_ctx *ctx, int dir, double *out);
 
-/* ---- HMC -------------------------------------------------------------------- */
+/* ---- HMC -------------------------------------------------------------------------- */
 typedef struct {
     double beta;
     double dt;
     int ndrift;      /* drifts per trajectory (UPDATE2) */
     int level;       /* Taylor order */
-    int reunit;      /* project the chain state onto SU(N) before each trajectory */
+    int reunit;      /* project the chain state onto SU(N) before the trajectory */
     uint64_t seed;   /* Philox key for momenta and the Metropolis draw */
 } askja_hmc_params;
 
 typedef struct {
     double h_old, h_new, dh;
-    double uniform;    /* Metropolis draw (only used when dh > 0) */
+    double uniform;    /* Metropolis draw */
+    double action;     /* EnergyMag of the chain state after the decision */
     int accepted;
     int accept_case;   /* 1 = H decreased, 2 = uniform < exp(-dH), 0 = rejected */
 } askja_hmc_result;
 
 /* One HMC trajectory of the reference's loop (sun.cpp:1249-1318) with a
- * counter-based stream in place of the time(NULL) seeds: snapshot, momentum
- * refresh (counter = traj_counter), H_old, MD, H_new, accept if H_new <
- * H_old or u < exp(H_old - H_new) with u from Philox; on reject restore the
- * snapshot.  NaN energies reject, as in the reference. */
+ * counter-based stream in place of the time(NULL) seeds: momentum refresh
+ * (counter = traj_counter), H_old, MD, H_new, accept if H_new < H_old or
+ * u < exp(H_old - H_new) with u from Philox; on reject the links return to
+ * the state before the trajectory.  NaN energies reject, as in the reference. */
 int askja_hmc_trajectory(askja_ctx *ctx, const askja_hmc_params *p, uint64_t traj_counter,
                          askja_hmc_result *res);
 
-/* ---- raw-pointer entry points (device buffers owned by the caller) ----------- */
+/* ---- plumbing ------------------------------------------------------------------------ */
 /* Synchronise the context's stream. */
 int askja_sync(askja_ctx *ctx);
-/* Device pointers of the context's fields in the private layout (for callers
- * that share device memory, e.g. a torch extension). */
+/* Record a start event / a stop event on the context's stream; the stop call
+ * synchronises and returns the elapsed milliseconds between the two. */
+int askja_timer_start(askja_ctx *ctx);
+int askja_timer_stop(askja_ctx *ctx, double *ms);
+/* Device pointers of the current links and momenta (private layout). */
 int askja_device_fields(askja_ctx *ctx, void **links, void **mom, void **mom_prev);
 
 #ifdef __cplusplus

@@ -1,0 +1,269 @@
+"""HIP product path (libaskja.so through the C ABI) against the CPU oracle on
+the same seeded inputs.
+
+Bar: FP64_EXACT links and momenta bit-identical to the oracle (the
+reference's operation order); FP64 (FMA, factored staples) within 1e-13
+relative per kick/drift and 1e-10 absolute per trajectory; FP32 within 2e-5
+relative per kick/drift.  Reductions in FP64 within 1e-12 relative.
+"""
+import numpy as np
+import pytest
+
+import askja_amd as A
+from conftest import random_hermitian, random_su
+
+pytestmark = pytest.mark.gpu
+
+# (N, extents): register path N <= 3 (incl. odd extents, size-1 directions
+# and a volume that is not a multiple of the 64-link tile), workgroup path
+# N >= 4 for every size class.
+CASES = [
+    (2, [4, 3, 2, 2]),
+    (3, [4, 4, 4, 4]),
+    (3, [3, 5, 1, 2]),
+    (4, [3, 2, 2, 2]),
+    (8, [2, 2, 3, 2]),
+    (16, [2, 3, 2]),
+    (32, [2, 2]),
+    (64, [2, 2]),
+]
+IDS = [f"N{n}-{'x'.join(map(str, e))}" for n, e in CASES]
+
+
+def _state(oracle, n, ext, seed, beta=5.0, level=8):
+    rng = np.random.default_rng(seed)
+    o = oracle.Oracle(n, ext, beta=beta, level=level)
+    u = random_su(oracle, rng, (o.V, o.dim), n)
+    p = random_hermitian(rng, (o.V, o.dim), n, 0.7)
+    o.field("u")[...] = u
+    o.field("U")[...] = u
+    o.field("pi")[...] = p
+    o.field("Pi")[...] = p
+    return o, u, p
+
+
+def _gpu(n, ext, prec, u=None, p=None):
+    lat = A.Lattice(n, ext, prec)
+    if u is not None:
+        lat.set_links(u)
+    if p is not None:
+        lat.set_momenta(p)
+    return lat
+
+
+@pytest.mark.parametrize("n,ext", CASES, ids=IDS)
+def test_layout_roundtrip(oracle, n, ext):
+    o, u, p = _state(oracle, n, ext, 1)
+    lat = _gpu(n, ext, A.FP64, u, p)
+    assert np.array_equal(lat.links(), u)
+    assert np.array_equal(lat.momenta(), p)
+    assert np.array_equal(lat.momenta(prev=True), p)
+    lat32 = _gpu(n, ext, A.FP32, u, p)
+    assert np.array_equal(lat32.links(), u.astype(np.complex64).astype(np.complex128))
+
+
+@pytest.mark.parametrize("n,ext", CASES, ids=IDS)
+def test_kick_exact_bitwise(oracle, n, ext):
+    o, u, p = _state(oracle, n, ext, 2)
+    lat = _gpu(n, ext, A.FP64_EXACT, u, p)
+    o.updateE(0.037)
+    lat.updateE(0.037)
+    assert np.array_equal(lat.momenta(), o.field("Pi"))
+    assert np.array_equal(lat.momenta(prev=True), p)
+
+
+@pytest.mark.parametrize("level", [0, 1, 2, 6, 8, 11])
+@pytest.mark.parametrize("n,ext", CASES, ids=IDS)
+def test_drift_exact_bitwise(oracle, n, ext, level):
+    o, u, p = _state(oracle, n, ext, 3, level=level)
+    lat = _gpu(n, ext, A.FP64_EXACT, u, p)
+    o.updateUandcopyDown(0.05)
+    lat.updateUandcopyDown(0.05, level)
+    assert np.array_equal(lat.links(), o.field("U"))
+
+
+@pytest.mark.parametrize("n,ext", CASES, ids=IDS)
+def test_kick_drift_fast_and_fp32(oracle, n, ext):
+    o, u, p = _state(oracle, n, ext, 4)
+    o.updateE(0.05)
+    o.updateUandcopyDown(0.05)
+    for prec, tol in ((A.FP64, 1e-13), (A.FP32, 2e-5)):
+        lat = _gpu(n, ext, prec, u, p)
+        lat.updateE(0.05)
+        lat.updateUandcopyDown(0.05, 8)
+        scale_p = np.max(np.abs(o.field("pi")))
+        assert np.max(np.abs(lat.momenta() - o.field("pi"))) <= tol * scale_p * n
+        assert np.max(np.abs(lat.links() - o.field("u"))) <= tol * n
+
+
+@pytest.mark.parametrize("prec", [A.FP64_EXACT, A.FP64, A.FP32])
+@pytest.mark.parametrize("n,ext", CASES[:5], ids=IDS[:5])
+def test_fused_kick_drift_equals_separate(oracle, n, ext, prec):
+    o, u, p = _state(oracle, n, ext, 5)
+    a = _gpu(n, ext, prec, u, p)
+    b = _gpu(n, ext, prec, u, p)
+    a.updateE(0.03)
+    a.updateUandcopyDown(0.04, 6)
+    b.kick_drift(0.03, 0.04, 6)
+    if prec == A.FP64_EXACT:
+        assert np.array_equal(a.links(), b.links())
+        assert np.array_equal(a.momenta(), b.momenta())
+    else:
+        tol = 1e-14 if prec == A.FP64 else 1e-6
+        assert np.max(np.abs(a.links() - b.links())) <= tol * n
+        assert np.max(np.abs(a.momenta() - b.momenta())) <= tol * n * 4
+
+
+@pytest.mark.parametrize("n,ext", CASES[:6], ids=IDS[:6])
+def test_md_trajectory_exact_and_fast(oracle, n, ext):
+    o, u, p = _state(oracle, n, ext, 6, beta=4.0)
+    h0 = o.hamiltonian("pi")
+    o.md_trajectory(0.04, 6)
+    o.copyDown()
+    lat = _gpu(n, ext, A.FP64_EXACT, u, p)
+    e = lat.md_trajectory(0.04, 6, 8)
+    assert np.array_equal(lat.links(), o.field("u"))
+    assert np.array_equal(lat.momenta(), o.field("pi"))
+    beta, n2 = 4.0, 2.0 * n
+    assert abs(beta * (e[0] + e[1] / n2) - h0) < 1e-12 * abs(h0)
+    assert abs(beta * (e[2] + e[3] / n2) - o.hamiltonian("pi")) < 1e-12 * abs(h0)
+    fast = _gpu(n, ext, A.FP64, u, p)
+    fast.md_trajectory(0.04, 6, 8)
+    assert np.max(np.abs(fast.links() - o.field("u"))) < 1e-10
+
+
+@pytest.mark.parametrize("n,ext", CASES, ids=IDS)
+def test_reductions(oracle, n, ext):
+    o, u, p = _state(oracle, n, ext, 7)
+    o.updateE(0.02)
+    for prec in (A.FP64_EXACT, A.FP64, A.FP32):
+        tol = 1e-12 if prec != A.FP32 else 2e-5
+        lat = _gpu(n, ext, prec, u, p)
+        lat.updateE(0.02)
+        t, s = lat.plaq_split()
+        ot, os_ = o.plaq_exp()
+        assert abs(t - ot) <= tol * max(1.0, abs(ot)) * 10
+        assert abs(s - os_) <= tol * max(1.0, abs(os_)) * 10
+        assert abs(lat.simpleenergyEl() - o.simple_energy_el("Pi")) <= tol * o.simple_energy_el("Pi") * 10
+        assert abs(lat.EnergyEl() - o.energy_el()) <= tol * o.energy_el() * 10
+        g, og = lat.GaussLaw(), o.gauss_law()
+        assert abs(g - og) <= tol * 1e3 * max(1.0, abs(og))
+        pl = lat.wilson_lines(0)
+        assert np.max(np.abs(pl - o.polyakov_field())) <= tol * 10
+        if prec == A.FP64_EXACT:
+            assert abs(lat.WilsonLoop() - o.wilson_origin()) < 1e-14
+    # unitarity violation of perturbed links
+    o.field("u")[...] = u * 1.0003
+    lat = _gpu(n, ext, A.FP64, u * 1.0003, p)
+    assert abs(lat.UViol() - o.uviol()) < 1e-9 * o.uviol()
+
+
+@pytest.mark.parametrize("n,ext", CASES, ids=IDS)
+def test_wilson_lines_all_directions(oracle, n, ext):
+    o, u, p = _state(oracle, n, ext, 8)
+    lat = _gpu(n, ext, A.FP64, u, p)
+    V = o.V
+    for d in range(len(ext)):
+        got = lat.wilson_lines(d)
+        # brute force: lines with x_d = 0, lexicographic order of the start
+        starts = [i for i, x in enumerate(np.ndindex(*ext)) if x[d] == 0]
+        assert len(got) == len(starts) == V // ext[d]
+        for q, s in enumerate(starts):
+            w = np.eye(n, dtype=complex)
+            pos = s
+            for _ in range(ext[d]):
+                w = w @ u[pos, d]
+                pos = o.getplus(pos, d)
+            assert abs(got[q] - np.trace(w) / n) < 1e-12
+
+
+@pytest.mark.parametrize("n,ext", [(3, [2, 3, 2, 2]), (8, [2, 2, 2, 2]), (2, [3, 1, 2])])
+def test_mom_gsl_bitwise(oracle, n, ext):
+    beta, seed = 5.9, 120351616
+    o = oracle.Oracle(n, ext, beta=beta)
+    o.initializeE(seed)
+    for prec in (A.FP64, A.FP64_EXACT):
+        lat = _gpu(n, ext, prec)
+        lat.mom_gsl(seed, beta)
+        assert np.array_equal(lat.momenta(), o.field("pi"))
+
+
+@pytest.mark.parametrize("n,ext", CASES, ids=IDS)
+def test_mom_refresh_and_haar_vs_oracle(oracle, n, ext):
+    beta = 4.5
+    o = oracle.Oracle(n, ext, beta=beta)
+    o.mom_philox(11, 3)
+    o.haar(12, 1)
+    lat = _gpu(n, ext, A.FP64)
+    lat.mom_refresh(11, 3, beta)
+    lat.haar(12, 1)
+    assert np.max(np.abs(lat.momenta() - o.field("pi"))) < 1e-13
+    assert np.max(np.abs(lat.links() - o.field("u"))) < 1e-12
+    # reunitarisation of perturbed links
+    rng = np.random.default_rng(0)
+    u = o.field("u") + 1e-3 * (rng.standard_normal(o.field("u").shape) + 1j * rng.standard_normal(o.field("u").shape))
+    o.field("u")[...] = u
+    o.reunitarize()
+    lat.set_links(u)
+    lat.reunitarize()
+    assert np.max(np.abs(lat.links() - o.field("u"))) < 1e-12
+    assert lat.UViol() < 1e-24 * o.V * n
+
+
+def test_snapshot_restore(oracle):
+    n, ext = 3, [4, 4, 2, 2]
+    o, u, p = _state(oracle, n, ext, 9)
+    lat = _gpu(n, ext, A.FP64, u, p)
+    lat.takeconfiguration()
+    lat.kick_drift(0.1, 0.1, 8)
+    assert not np.array_equal(lat.links(), u)
+    lat.loadconfiguration()
+    assert np.array_equal(lat.links(), u)
+
+
+@pytest.mark.parametrize("prec,tol", [(A.FP64, 1e-11), (A.FP32, 5e-5)])
+def test_reversibility(oracle, prec, tol):
+    n, ext = 3, [4, 4, 4, 4]
+    lat = _gpu(n, ext, prec)
+    lat.haar(3, 0)
+    u0 = lat.links()
+    lat.mom_refresh(5, 0, 5.9)
+    p0 = lat.momenta()
+    lat.md_trajectory(0.02, 25, 8)
+    lat.mom_negate()
+    lat.md_trajectory(0.02, 25, 8)
+    assert np.max(np.abs(lat.links() - u0)) < tol
+    assert np.max(np.abs(lat.momenta() + p0)) < tol * 10
+
+
+def test_hmc_chain_vs_oracle(oracle):
+    n, ext, beta, dt, nd, seed = 3, [4, 4, 4, 4], 5.7, 0.05, 8, 17
+    o = oracle.Oracle(n, ext, beta=beta, level=8)
+    o.haar(21, 0)
+    rec = o.hmc(dt, nd, 6, seed, first_counter=0, reunit=True)
+    lat = _gpu(n, ext, A.FP64)
+    lat.haar(21, 0)
+    for t in range(6):
+        r = lat.hmc_trajectory(beta, dt, nd, 8, seed, t, reunit=True)
+        assert abs(r.dh - rec[t, 0]) < 1e-8, (t, r.dh, rec[t, 0])
+        assert r.accepted == int(rec[t, 1])
+        plaq = 1.0 - r.action / lat.n_plaquettes()
+        assert abs(plaq - rec[t, 2]) < 1e-12
+    assert np.max(np.abs(lat.links() - o.field("u"))) < 1e-10
+    pl = lat.polyakov()
+    assert abs(pl - complex(rec[-1, 3], rec[-1, 4])) < 1e-12
+
+
+def test_errors_are_reported(oracle):
+    with pytest.raises(A.AskjaError):
+        A.Lattice(65, [2, 2])
+    lat = _gpu(3, [2, 2, 2, 2], A.FP64)
+    buf = np.zeros(64)
+    rc = A.lib().askja_wilson_lines(lat._h, 7, buf.ctypes.data_as(A._DP))
+    assert rc == -1  # ASKJA_EINVAL
+    assert b"out of range" in A.lib().askja_last_error(lat._h)
+    # sticky until cleared
+    with pytest.raises(A.AskjaError, match="ESTICKY"):
+        lat.updateE(0.1)
+    A.lib().askja_clear_error(lat._h)
+    lat.updateE(0.1)
