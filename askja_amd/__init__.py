@@ -33,7 +33,7 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libaskja.so")
+LIB_PATH = os.environ.get("ASKJA_LIB") or os.path.join(_HERE, "libaskja.so")
 
 FP64, FP64_EXACT, FP32 = 0, 1, 2
 _ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ENOMEM", -4: "EUNSUP", -5: "ESTICKY", -6: "EIO"}
@@ -82,7 +82,9 @@ SIGNATURES = {
     "askja_gauss_law": (_I, [_P, _DP]),
     "askja_uviol": (_I, [_P, _DP]),
     "askja_wilson_lines": (_I, [_P, _I, _DP]),
+    "askja_line_matrix": (_I, [_P, _I, C.POINTER(C.c_int), _DP]),
     "askja_hmc_trajectory": (_I, [_P, C.POINTER(HmcParams), _U64, C.POINTER(HmcResult)]),
+    "askja_uniform": (_I, [_P, _U64, _U64, _DP]),
     "askja_sync": (_I, [_P]),
     "askja_timer_start": (_I, [_P]),
     "askja_timer_stop": (_I, [_P, _DP]),
@@ -248,6 +250,12 @@ class Lattice:
         out = np.zeros(2 * (self.V // self.ext[d]))
         self._ck(lib().askja_wilson_lines(self._h, d, _dptr(out)), "wilson_lines")
         return out.view(np.complex128)
+
+    def line_matrix(self, d, x):
+        out = np.zeros(2 * self.nc * self.nc)
+        xx = (C.c_int * self.dim)(*[int(v) for v in x])
+        self._ck(lib().askja_line_matrix(self._h, d, xx, _dptr(out)), "line_matrix")
+        return out.view(np.complex128).reshape(self.nc, self.nc)
 
     def WilsonLoop(self):
         """|Tr prod_t U_0(t, 0)|/N at the spatial origin (sun.cpp:307-345)."""

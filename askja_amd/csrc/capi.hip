@@ -5,6 +5,7 @@
 #include "kernels.hpp"
 #include "common.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -153,6 +154,14 @@ int do_kick(askja_ctx *c, const void *U, void *Uout, void *Pprev, double ek, dou
     return ASKJA_OK;
 }
 
+// Metropolis uniform of trajectory `counter`: Philox block 0 of link 0,
+// purpose 3, first word mapped to (0, 1).
+double metropolis_uniform(uint64_t seed, uint64_t counter)
+{
+    const U4 r = philox_block(seed, counter, kPurposeAccept, 0u, 0u);
+    return u01(r.x);
+}
+
 long n_plaquettes(const askja_ctx *c) { return c->V * (long)c->dim * (c->dim - 1) / 2; }
 
 // MD of one trajectory with the chain-state buffer `in0` left untouched;
@@ -215,32 +224,45 @@ int askja_create(int nc, int dim, const int *extents, int precision, askja_ctx *
     c->link_reals = ntiles * c->tile * 2 * nc * nc;
     c->mom_reals = ntiles * c->tile * nc * nc;
 
-    // checkerboard order: parity-0 sites then parity-1, each lexicographic
+    // Device site order: parity-0 sites then parity-1; within a parity the
+    // sites are sorted by (chunk of the longest direction, lexicographic
+    // index).  The longest direction is cut into min(8, extent) chunks, so a
+    // contiguous eighth of the sites (what one XCD processes) is a compact
+    // slab of the lattice rather than a thin slice of the slowest direction.
     c->lex2dev.resize(V);
     c->dev2lex.resize(V);
-    std::vector<int> x(dim);
+    int split = 0;
+    for (int d = 1; d < dim; d++)
+        if (extents[d] > extents[split]) split = d;
+    const int nchunk = extents[split] < 8 ? extents[split] : 8;
+    const int chunk = (extents[split] + nchunk - 1) / nchunk;
+    std::vector<long> key(V);
+    std::vector<int> par(V);
     long n0 = 0;
     for (long i = 0; i < V; i++) {
         long r = i;
-        int par = 0;
+        int p = 0, xs = 0;
         for (int d = 0; d < dim; d++) {
-            x[d] = (int)(r / c->helparr[d]);
+            const int xd = (int)(r / c->helparr[d]);
             r %= c->helparr[d];
-            par += x[d];
+            p += xd;
+            if (d == split) xs = xd;
         }
-        if ((par & 1) == 0) n0++;
+        par[i] = p & 1;
+        key[i] = (long)(xs / chunk) * V + i;
+        if (par[i] == 0) n0++;
     }
-    long k0 = 0, k1 = n0;
-    for (long i = 0; i < V; i++) {
-        long r = i;
-        int par = 0;
-        for (int d = 0; d < dim; d++) {
-            par += (int)(r / c->helparr[d]);
-            r %= c->helparr[d];
+    {
+        std::vector<long> order(V);
+        for (long i = 0; i < V; i++) order[i] = i;
+        std::sort(order.begin(), order.end(), [&](long a, long b) {
+            if (par[a] != par[b]) return par[a] < par[b];
+            return key[a] < key[b];
+        });
+        for (long s = 0; s < V; s++) {
+            c->dev2lex[s] = (int)order[s];
+            c->lex2dev[order[s]] = (int)s;
         }
-        const long s = (par & 1) == 0 ? k0++ : k1++;
-        c->lex2dev[i] = (int)s;
-        c->dev2lex[s] = (int)i;
     }
     // neighbour table in device order (periodic +-1 steps, structs.h:169-218)
     std::vector<int> nbr((size_t)2 * dim * V);
@@ -290,6 +312,7 @@ int askja_create(int nc, int dim, const int *extents, int precision, askja_ctx *
     g.nbr = c->d_nbr;
     g.dev2lex = c->d_dev2lex;
     g.lex2dev = c->d_lex2dev;
+    g.V0 = n0;
     for (int i = 0; i < dim; i++) {
         g.ext[i] = extents[i];
         g.helparr[i] = c->helparr[i];
@@ -699,6 +722,29 @@ int askja_wilson_lines(askja_ctx *c, int dir, double *out)
     return ASKJA_OK;
 }
 
+int askja_line_matrix(askja_ctx *c, int dir, const int *x, double *out)
+{
+    ENTER(c);
+    if (!x || !out) return fail(c, ASKJA_EINVAL, "line_matrix: null argument");
+    if (dir < 0 || dir >= c->dim) return fail(c, ASKJA_EINVAL, "line_matrix: dir %d out of range", dir);
+    long lex = 0;
+    for (int d = 0; d < c->dim; d++) {
+        if (x[d] < 0 || x[d] >= c->ext[d]) return fail(c, ASKJA_EINVAL, "line_matrix: coordinate %d out of range", d);
+        lex += x[d] * c->helparr[d];
+    }
+    int rc = ensure_scratch(c);
+    if (rc) return rc;
+    const size_t nr = (size_t)2 * c->nc * c->nc;
+    double *d = nullptr;
+    HIPCHK(c, hipMallocAsync((void **)&d, nr * sizeof(double), c->st));
+    hipError_t e = launch_line_matrix(c->geom, c->links[c->cur], dir, lex, c->mode, c->scratch, d, c->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d, nr * sizeof(double), hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipFreeAsync(d, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    if (e != hipSuccess) return fail(c, ASKJA_EHIP, "line_matrix: %s", hipGetErrorString(e));
+    return ASKJA_OK;
+}
+
 int askja_hmc_trajectory(askja_ctx *c, const askja_hmc_params *p, uint64_t traj_counter, askja_hmc_result *r)
 {
     ENTER(c);
@@ -717,19 +763,21 @@ int askja_hmc_trajectory(askja_ctx *c, const askja_hmc_params *p, uint64_t traj_
     r->h_old = (s0 + k0 / n2) * p->beta;
     r->h_new = (s1 + k1 / n2) * p->beta;
     r->dh = r->h_new - r->h_old;
-    U4 ctr;
-    ctr.x = 0u;
-    ctr.y = 0u;
-    ctr.z = (uint32_t)traj_counter;
-    ctr.w = (uint32_t)(traj_counter >> 32) ^ (kPurposeAccept << 24);
-    const U4 rr = philox4x32_10(ctr, (uint32_t)p->seed, (uint32_t)(p->seed >> 32));
-    r->uniform = u01(rr.x);
+    r->uniform = metropolis_uniform(p->seed, traj_counter);
     if (r->h_new < r->h_old) r->accept_case = 1;
     else if (r->uniform < std::exp(r->h_old - r->h_new)) r->accept_case = 2;
     else r->accept_case = 0;
     r->accepted = r->accept_case != 0;
     if (r->accepted) c->cur = fin;
     r->action = r->accepted ? s1 : s0;
+    return ASKJA_OK;
+}
+
+int askja_uniform(askja_ctx *c, uint64_t seed, uint64_t counter, double *out)
+{
+    ENTER(c);
+    if (!out) return fail(c, ASKJA_EINVAL, "uniform: null output");
+    *out = metropolis_uniform(seed, counter);
     return ASKJA_OK;
 }
 

@@ -79,6 +79,46 @@ __device__ inline void normals4(U4 r, double z[4])
     z[3] = rad * sin(ang);
 }
 
+// ------------------------------------------------------------ XCD mapping
+// The hardware hands consecutive workgroups of a launch to the 8 XCDs in
+// turn (workgroup i to XCD i % 8), each XCD with its own L2.  This maps the
+// launch index to a logical block so that every XCD walks one contiguous
+// range of logical blocks: spatial neighbours then share an L2.  Bijective
+// for any grid size.
+constexpr unsigned kNumXcd = 8;
+
+__device__ inline unsigned xcd_block(unsigned i, unsigned nblocks)
+{
+    const unsigned q = nblocks / kNumXcd, r = nblocks % kNumXcd;
+    const unsigned x = i % kNumXcd, k = i / kNumXcd;
+    return x * q + (x < r ? x : r) + k;
+}
+
+// Site block sb of a launch over both parities, blocks of `bs` sites:
+// blocks of even and odd sites alternate, so that a contiguous range of site
+// blocks covers one region of the lattice in both parities (the neighbours
+// of an even site are odd).  Returns the first device site of the block and
+// the number of valid sites in it.
+__device__ inline long parity_block(long sb, long V, long V0, int bs, int &count)
+{
+    const long V1 = V - V0;
+    const long n0 = (V0 + bs - 1) / bs, n1 = (V1 + bs - 1) / bs;
+    const long m = n0 < n1 ? n0 : n1;
+    long p, r;
+    if (sb < 2 * m) {
+        p = sb & 1;
+        r = sb >> 1;
+    } else {
+        p = n0 > n1 ? 0 : 1;
+        r = m + (sb - 2 * m);
+    }
+    const long base = p ? V0 : 0, len = p ? V1 : V0;
+    const long first = r * bs;
+    const long left = len - first;
+    count = left < bs ? (int)left : bs;
+    return base + first;
+}
+
 // ---------------------------------------------------------- reductions
 // Neumaier-compensated accumulation; a (s, c) pair represents s + c.
 __device__ inline void kahan_add(double &s, double &c, double x)

@@ -18,6 +18,7 @@ hipError_t small_uviol(const Geom &g, const void *U, int mode, double *part, lon
 hipError_t small_gauss(const Geom &g, const void *U, const void *P, int mode, double *part, long *np, double *out,
                        hipStream_t st);
 hipError_t small_wlines(const Geom &g, const void *U, int dir, int mode, double *out, hipStream_t st);
+hipError_t small_project(const Geom &g, void *U, int mode, int haar, uint64_t seed, uint64_t stream, hipStream_t st);
 #define ASKJA_TEAM_DECL(NM)                                                                                   \
     hipError_t team_kick_##NM(const KickArgs &a, int mode, bool drift, hipStream_t st);                       \
     hipError_t team_drift_##NM(const Geom &g, void *U, const void *P, double eps, int level, const double *c, \
@@ -524,6 +525,24 @@ __global__ void k_wlines_g(Geom g, const R *U, int dir, long q0, long nq, double
     out[2 * q + 1] = im / n;
 }
 
+// Full product matrix of one straight line (the matrix W of WilsonLoop,
+// sun.cpp:307-345), one thread, FP64 in the reference's product order.
+template <typename R>
+__global__ void k_line_matrix(Geom g, const R *U, int dir, long lex, double *scratch, double *out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int n = g.N, nr = 2 * n * n;
+    double *B = scratch, *T = B + nr;
+    long s = g.lex2dev[lex];
+    gload(U, g, dir * g.V + s, out);
+    for (int k = 1; k < g.ext[dir]; k++) {
+        s = g.nbr[(2 * dir) * g.V + s];
+        gload(U, g, dir * g.V + s, B);
+        gmul(out, false, B, false, T, n);
+        for (int e = 0; e < nr; e++) out[e] = T[e];
+    }
+}
+
 inline long threads_for(long work, long per_thread, long scratch_doubles)
 {
     long t = scratch_doubles / per_thread;
@@ -539,7 +558,7 @@ long generic_scratch_per_thread(int n) { return 12L * n * n; }
 // ---------------------------------------------------------------- dispatch
 long kick_partials(const Geom &g)
 {
-    if (g.N <= 3) return (long)nblk(g.V) * g.D;
+    if (g.N <= 3) return kick_site_blocks(g) * g.D;
     return g.V * g.D;
 }
 
@@ -658,6 +677,15 @@ hipError_t launch_wlines(const Geom &g, const void *U, int dir, int mode, double
     return hipSuccess;
 }
 
+hipError_t launch_line_matrix(const Geom &g, const void *U, int dir, long lex, int mode, double *scratch, double *out,
+                              hipStream_t st)
+{
+    if (mode == kModeF32)
+        hipLaunchKernelGGL(k_line_matrix<float>, dim3(1), dim3(64), 0, st, g, (const float *)U, dir, lex, scratch, out);
+    else hipLaunchKernelGGL(k_line_matrix<double>, dim3(1), dim3(64), 0, st, g, (const double *)U, dir, lex, scratch, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_links_from_aos(const Geom &g, const double *aos, void *U, int mode, hipStream_t st)
 {
     const long tot = g.V * g.D * 2L * g.N * g.N;
@@ -719,6 +747,7 @@ hipError_t launch_mom_scale(const Geom &g, void *P, double f, int mode, hipStrea
 hipError_t launch_project(const Geom &g, void *U, int mode, int haar, uint64_t seed, uint64_t stream, double *scratch,
                           long scratch_doubles, hipStream_t st)
 {
+    if (g.N <= 3) return small_project(g, U, mode, haar, seed, stream, st);
     const long nthr = threads_for(g.V * g.D, 4L * g.N * g.N, scratch_doubles);
     if (mode == kModeF32)
         hipLaunchKernelGGL(k_project<float>, dim3(nblk(nthr)), dim3(kB), 0, st, g, (float *)U, haar, seed, stream, scratch,

@@ -16,8 +16,9 @@ enum Mode : int {
 };
 
 // Geometry shared by every kernel.  Device site index s: all sites of even
-// parity (sum of coordinates mod 2) in lexicographic order, then all odd ones.
-// Link l = mu * V + s.  nbr[(2*d + 0) * V + s] = s + e_d, nbr[(2*d + 1) * V +
+// parity (sum of coordinates mod 2), then all odd ones; within a parity the
+// sites are ordered by slab of the longest direction (cut into at most 8
+// slabs), then lexicographically.  Link l = mu * V + s.  nbr[(2*d + 0) * V + s] = s + e_d, nbr[(2*d + 1) * V +
 // s] = s - e_d (periodic), dev2lex[s] = lexicographic index (x[D-1] fastest).
 struct Geom {
     long V;
@@ -27,6 +28,7 @@ struct Geom {
     const int *nbr;
     const int *dev2lex;
     const int *lex2dev;
+    long V0;        // number of even-parity sites (device sites 0 .. V0-1)
     int ext[32];
     long helparr[32];  // lexicographic strides, structs.h:130-136
 };
@@ -68,6 +70,12 @@ struct KickArgs {
 
 // Number of partial-sum blocks a kick launch writes (each 3 (s, c) pairs).
 long kick_partials(const Geom &g);
+// Site blocks of 256 of the register-path kick (even and odd sites blocked
+// separately).
+inline long kick_site_blocks(const Geom &g)
+{
+    return (g.V0 + 255) / 256 + (g.V - g.V0 + 255) / 256;
+}
 
 hipError_t launch_kick(const KickArgs &a, int mode, bool drift, hipStream_t st);
 hipError_t launch_drift(const Geom &g, void *U, const void *P, double eps, int level,
@@ -95,6 +103,11 @@ hipError_t launch_kinetic(const Geom &g, const void *P, const void *Q, int mode,
 // = Tr(prod U)/N of line q (lines in lexicographic order of their start).
 hipError_t launch_wlines(const Geom &g, const void *U, int dir, int mode, double *out,
                          double *scratch, long scratch_doubles, hipStream_t st);
+
+// Product matrix (2 N^2 doubles, device) of the straight line in direction
+// dir starting at lexicographic site lex; scratch >= 4 N^2 doubles.
+hipError_t launch_line_matrix(const Geom &g, const void *U, int dir, long lex, int mode, double *scratch, double *out,
+                              hipStream_t st);
 
 // ---- field set-up -----------------------------------------------------------
 // Host AoS (reference U-*.conf layout, doubles) <-> device layout.

@@ -92,11 +92,15 @@ __device__ inline void staple_w(const Geom &g, const R *__restrict__ U, int mu, 
 template <int N, typename R, bool Exact, bool Drift>
 __global__ __launch_bounds__(kBlock) void k_kick(KickArgs a)
 {
-    const long s = (long)blockIdx.x * kBlock + threadIdx.x;
-    const int mu = blockIdx.y;
     const Geom &g = a.g;
+    // blocks of the D directions of one run of sites are consecutive, and
+    // each XCD receives one contiguous range of them
+    const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+    const int mu = (int)(lb % (unsigned)g.D);
+    int cnt;
+    const long s = parity_block((long)(lb / (unsigned)g.D), g.V, g.V0, kBlock, cnt) + threadIdx.x;
     double sum[3] = {0.0, 0.0, 0.0}, cmp[3] = {0.0, 0.0, 0.0};
-    if (s < g.V) {
+    if ((int)threadIdx.x < cnt) {
         const R *U = (const R *)a.U;
         R *P = (R *)a.P;
         const long l = mu * g.V + s;
@@ -111,14 +115,15 @@ __global__ __launch_bounds__(kBlock) void k_kick(KickArgs a)
             const Cinv<R> ci = cinv_of<R>(a.cinv);
             CM<N, R> Ul, Un;
             load_link<N, R>(U, l, Ul);
-            taylor_drift<Exact>(Pout, (R)a.eps_drift, a.level, ci.c, Ul, Un);
+            if constexpr (Exact) taylor_drift<true>(Pout, (R)a.eps_drift, a.level, ci.c, Ul, Un);
+            else taylor_drift_fast(Pout, (R)a.eps_drift, a.level, ci.c, Ul, Un);
             store_link<N, R>((R *)a.Uout, l, Un);
         }
         sum[0] = (double)retr(W);
         sum[1] = herm_norm2(Pin);
         sum[2] = herm_norm2(Pout);
     }
-    block_reduce_store<3>(sum, cmp, a.part + 6 * ((size_t)blockIdx.y * gridDim.x + blockIdx.x));
+    block_reduce_store<3>(sum, cmp, a.part + 6 * (size_t)lb);
 }
 
 // Standalone drift, in place: U <- T_L(i eps P) U.
@@ -130,7 +135,8 @@ __global__ __launch_bounds__(kBlock) void k_drift(Geom g, R *U, const R *P, doub
     CM<N, R> Pl, Ul, Un;
     load_mom<N, R>(P, l, Pl);
     load_link<N, R>(U, l, Ul);
-    taylor_drift<Exact>(Pl, (R)eps, level, ci.c, Ul, Un);
+    if constexpr (Exact) taylor_drift<true>(Pl, (R)eps, level, ci.c, Ul, Un);
+    else taylor_drift_fast(Pl, (R)eps, level, ci.c, Ul, Un);
     store_link<N, R>(U, l, Un);
 }
 
@@ -248,10 +254,115 @@ __global__ __launch_bounds__(kBlock) void k_wlines(Geom g, const R *U, int dir, 
     out[2 * q + 1] = (double)imtr(W) / N;
 }
 
+// SU(N) projection of one link held in registers (N <= 3): modified
+// Gram-Schmidt on the rows, then exp(-i arg(det)/N); FP64 working copy.
+// haar: the link is first replaced by a Ginibre matrix from Philox (purpose
+// 2, counter = stream, link index = dir * V + lexicographic site).
+template <int N, typename R>
+__global__ __launch_bounds__(kBlock) void k_project(Geom g, R *U, int haar, uint64_t seed, uint64_t stream)
+{
+#pragma clang fp contract(off)
+    const long l = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (l >= g.V * g.D) return;
+    double ar[N][N], ai[N][N];
+    if (haar) {
+        const int d = (int)(l / g.V);
+        const uint32_t link = (uint32_t)(d * g.V + g.dev2lex[l - d * g.V]);
+        double z[4 * ((2 * N * N + 3) / 4)];
+#pragma unroll
+        for (int b = 0; b < (2 * N * N + 3) / 4; b++) normals4(philox_block(seed, stream, kPurposeHaar, link, (uint32_t)b), z + 4 * b);
+#pragma unroll
+        for (int i = 0; i < N; i++)
+#pragma unroll
+            for (int j = 0; j < N; j++) {
+                ar[i][j] = z[2 * (i * N + j)] * 0.7071067811865476;
+                ai[i][j] = z[2 * (i * N + j) + 1] * 0.7071067811865476;
+            }
+    } else {
+        CM<N, R> m;
+        load_link<N, R>(U, l, m);
+#pragma unroll
+        for (int i = 0; i < N; i++)
+#pragma unroll
+            for (int j = 0; j < N; j++) {
+                ar[i][j] = (double)m.re[i][j];
+                ai[i][j] = (double)m.im[i][j];
+            }
+    }
+#pragma unroll
+    for (int r = 0; r < N; r++) {
+#pragma unroll
+        for (int q = 0; q < r; q++) {
+            double dr = 0.0, di = 0.0;
+#pragma unroll
+            for (int k = 0; k < N; k++) {
+                dr += ar[q][k] * ar[r][k] + ai[q][k] * ai[r][k];
+                di += ar[q][k] * ai[r][k] - ai[q][k] * ar[r][k];
+            }
+#pragma unroll
+            for (int k = 0; k < N; k++) {
+                ar[r][k] -= dr * ar[q][k] - di * ai[q][k];
+                ai[r][k] -= dr * ai[q][k] + di * ar[q][k];
+            }
+        }
+        double nrm = 0.0;
+#pragma unroll
+        for (int k = 0; k < N; k++) nrm += ar[r][k] * ar[r][k] + ai[r][k] * ai[r][k];
+        nrm = 1.0 / sqrt(nrm);
+#pragma unroll
+        for (int k = 0; k < N; k++) {
+            ar[r][k] *= nrm;
+            ai[r][k] *= nrm;
+        }
+    }
+    // determinant by cofactors
+    double dr, di;
+    if constexpr (N == 1) {
+        dr = ar[0][0];
+        di = ai[0][0];
+    } else if constexpr (N == 2) {
+        dr = (ar[0][0] * ar[1][1] - ai[0][0] * ai[1][1]) - (ar[0][1] * ar[1][0] - ai[0][1] * ai[1][0]);
+        di = (ar[0][0] * ai[1][1] + ai[0][0] * ar[1][1]) - (ar[0][1] * ai[1][0] + ai[0][1] * ar[1][0]);
+    } else {
+        // row 0 times the cofactors of rows 1, 2
+        dr = 0.0;
+        di = 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const int a = (j + 1) % 3, b = (j + 2) % 3;
+            const double cr = (ar[1][a] * ar[2][b] - ai[1][a] * ai[2][b]) - (ar[1][b] * ar[2][a] - ai[1][b] * ai[2][a]);
+            const double ci = (ar[1][a] * ai[2][b] + ai[1][a] * ar[2][b]) - (ar[1][b] * ai[2][a] + ai[1][b] * ar[2][a]);
+            dr += ar[0][j] * cr - ai[0][j] * ci;
+            di += ar[0][j] * ci + ai[0][j] * cr;
+        }
+    }
+    const double ph = -atan2(di, dr) / N;
+    const double c = cos(ph), sn = sin(ph);
+    CM<N, R> out;
+#pragma unroll
+    for (int i = 0; i < N; i++)
+#pragma unroll
+        for (int j = 0; j < N; j++) {
+            out.re[i][j] = (R)(ar[i][j] * c - ai[i][j] * sn);
+            out.im[i][j] = (R)(ar[i][j] * sn + ai[i][j] * c);
+        }
+    store_link<N, R>(U, l, out);
+}
+
+template <int N>
+hipError_t project_mode(const Geom &g, void *U, int mode, int haar, uint64_t seed, uint64_t stream, hipStream_t st)
+{
+    const unsigned nb = (unsigned)((g.V * g.D + kBlock - 1) / kBlock);
+    if (mode == kModeF32)
+        hipLaunchKernelGGL((k_project<N, float>), dim3(nb), dim3(kBlock), 0, st, g, (float *)U, haar, seed, stream);
+    else hipLaunchKernelGGL((k_project<N, double>), dim3(nb), dim3(kBlock), 0, st, g, (double *)U, haar, seed, stream);
+    return hipGetLastError();
+}
+
 template <int N, typename R, bool Exact>
 hipError_t kick_n(const KickArgs &a, bool drift, hipStream_t st)
 {
-    dim3 grid((unsigned)((a.g.V + kBlock - 1) / kBlock), (unsigned)a.g.D);
+    dim3 grid((unsigned)kick_site_blocks(a.g) * (unsigned)a.g.D);
     if (drift) hipLaunchKernelGGL((k_kick<N, R, Exact, true>), grid, dim3(kBlock), 0, st, a);
     else hipLaunchKernelGGL((k_kick<N, R, Exact, false>), grid, dim3(kBlock), 0, st, a);
     return hipGetLastError();
@@ -392,6 +503,15 @@ hipError_t small_gauss(const Geom &g, const void *U, const void *P, int mode, do
     case 1: return gauss_mode<1>(g, U, P, mode, part, np, out, st);
     case 2: return gauss_mode<2>(g, U, P, mode, part, np, out, st);
     default: return gauss_mode<3>(g, U, P, mode, part, np, out, st);
+    }
+}
+
+hipError_t small_project(const Geom &g, void *U, int mode, int haar, uint64_t seed, uint64_t stream, hipStream_t st)
+{
+    switch (g.N) {
+    case 1: return project_mode<1>(g, U, mode, haar, seed, stream, st);
+    case 2: return project_mode<2>(g, U, mode, haar, seed, stream, st);
+    default: return project_mode<3>(g, U, mode, haar, seed, stream, st);
     }
 }
 

@@ -257,6 +257,41 @@ __device__ inline void taylor_drift(const CM<N, R> &P, R eps, int level, const R
     else mul_fast<false, false, false>(X, U, out);
 }
 
+// Fast Horner form of T_L(i eps P) U: with rhs = i eps P, every step
+// X <- 1 + (rhs/k) X is evaluated as Q = P X, X = 1 + i (eps/k) Q, so only
+// P, X and Q are live (FMA flavour; the exact flavour is taylor_drift).
+template <int N, typename R>
+__device__ inline void taylor_drift_fast(const CM<N, R> &P, R eps, int level, const R *cinv, const CM<N, R> &U,
+                                         CM<N, R> &out)
+{
+    if (level < 0 || level > 8) level = 8;
+    if (level == 0) {
+        out = U;
+        return;
+    }
+    CM<N, R> X, Q;
+    R f = eps * (level == 1 ? R(1) : cinv[level]);
+#pragma unroll
+    for (int i = 0; i < N; i++)
+#pragma unroll
+        for (int j = 0; j < N; j++) {
+            X.re[i][j] = (i == j ? R(1) : R(0)) - f * P.im[i][j];
+            X.im[i][j] = f * P.re[i][j];
+        }
+    for (int k = level - 1; k >= 1; k--) {
+        mul_fast<false, false, false>(P, X, Q);
+        f = eps * cinv[k];
+#pragma unroll
+        for (int i = 0; i < N; i++)
+#pragma unroll
+            for (int j = 0; j < N; j++) {
+                X.re[i][j] = (i == j ? R(1) : R(0)) - f * Q.im[i][j];
+                X.im[i][j] = f * Q.re[i][j];
+            }
+    }
+    mul_fast<false, false, false>(X, U, out);
+}
+
 // ReTr(A B^+) = sum_ij Re(A_ij conj(B_ij)) (FMA flavour).
 template <int N, typename R>
 __device__ inline R retr_abh(const CM<N, R> &A, const CM<N, R> &B)

@@ -184,9 +184,13 @@ __global__ __launch_bounds__(NT) void k_kick(KickArgs a)
     const Geom &g = a.g;
     const int n = g.N;
     const long V = g.V;
-    const long l = blockIdx.x;
-    const int mu = (int)(l / V);
-    const long s = l - (long)mu * V;
+    // the D links of one site are consecutive logical blocks, and each XCD
+    // takes one contiguous range of sites
+    const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+    const int mu = (int)(lb % (unsigned)g.D);
+    int cnt;
+    const long s = parity_block((long)(lb / (unsigned)g.D), V, g.V0, 1, cnt);
+    const long l = (long)mu * V + s;
     const R *U = (const R *)a.U;
     Own<EPL, NT> o;
     o.init(n);
@@ -374,7 +378,7 @@ __global__ __launch_bounds__(NT) void k_kick(KickArgs a)
         __syncthreads();
         drift_core<Exact>(o, Are, Aim, Bre, Bim, n, a.level, a.cinv, U, l, (R *)a.Uout);
     }
-    block_reduce_store<3>(sum, cmp, a.part + 6 * (size_t)blockIdx.x);
+    block_reduce_store<3>(sum, cmp, a.part + 6 * (size_t)lb);
 }
 
 template <int NMAX, int NT, typename R, bool Exact>

@@ -1,0 +1,541 @@
+// askja_main.cpp -- the `askja <params>` program on the MI355X library.
+//
+// Drop-in for the reference's driver (sun.cpp:1059-1373): the same positional
+// params file (load_parameters, sun.cpp:803-938), the same output directory
+// and file names, the same text rows in constraints-*.dat and Wilson-*.dat,
+// the same binary U-*.conf configurations and master-*.params checkpoints
+// (store_parameters, sun.cpp:940-1057), so the reference's measure programs
+// read the output unchanged.  The MD, energies and observables run on the GPU
+// through the C ABI of include/askja.h.
+//
+// Deliberate differences (the reference's randomness is seeded by time(NULL)
+// for every trajectory after the first, sun.cpp:1268-1311, which repeats
+// streams within one second and cannot be reproduced):
+//   - trajectory 1 of a fresh run with RANDOM 0 uses the reference's own
+//     momenta (mt19937 seed 120351616, polar Gaussians: askja_mom_gsl);
+//   - later momenta and the Metropolis uniforms come from Philox-4x32-10 with
+//     key = run seed and counter = trajectory number;
+//   - the run seed is appended to master-*.params after NTOT (both reference
+//     readers stop before it), so a resumed run continues the same stream;
+//   - the Wilson-*.dat phase columns, left uninitialised by the reference
+//     (sun.cpp:1119, 346-360), hold the sorted eigen-phases of the loop.
+// Extra options follow the params file: --precision fp64|fp64-exact|fp32,
+// --reunit (project the chain state onto SU(N) before every trajectory).
+#include "../../include/askja.h"
+
+#include <sys/stat.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+namespace {
+
+struct Params {
+    int NC = 2, DIM = 3;
+    std::vector<int> x;
+    double BETA = 1, dt = 0.1;
+    int STEPS = 100, UPDATE = 10, UPDATE2 = 10, CONFTAKE = 10, LEVEL = 5, VERBOSE = 1, RUNCONTINUE = 0, RANDOM = 0;
+    int START = 1, TAKEN = 0, NACC = 0, NAC1 = 0, NREC = 0, NTOT = 0;
+    bool has_seed = false;
+    unsigned long long seed = 0;
+};
+
+// load_parameters (sun.cpp:803-938): whitespace tokens, a header token, then
+// (name, value) pairs in a fixed order; names are not checked.
+bool load_parameters(const char *path, Params &p)
+{
+    std::ifstream in(path);
+    if (!in.is_open()) {
+        printf("\nERROR: params file %s could not be opened\n", path);
+        return false;
+    }
+    std::string name;
+    auto rint = [&](int &v) { in >> name >> v; };
+    auto rdbl = [&](double &v) { in >> name >> v; };
+    in >> name;  // header
+    rint(p.NC);
+    rint(p.DIM);
+    printf("DIM =%i\n", p.DIM);
+    if (p.DIM < 1 || p.DIM > 32) {
+        printf("\nERROR: DIM = %i not supported\n", p.DIM);
+        return false;
+    }
+    p.x.assign(p.DIM, 1);
+    for (int i = 0; i < p.DIM; i++) rint(p.x[i]);
+    rdbl(p.BETA);
+    printf("BETA =%f\n", p.BETA);
+    rdbl(p.dt);
+    printf("dt =%f\n", p.dt);
+    rint(p.STEPS);
+    printf("STEPS =%i\n", p.STEPS);
+    rint(p.UPDATE);
+    rint(p.UPDATE2);
+    rint(p.CONFTAKE);
+    printf("CONFTAKE =%i\n", p.CONFTAKE);
+    rint(p.LEVEL);
+    rint(p.VERBOSE);
+    rint(p.RUNCONTINUE);
+    rint(p.RANDOM);
+    if (!in) {
+        printf("\nERROR: params file %s is truncated or malformed\n", path);
+        return false;
+    }
+    if (p.RUNCONTINUE) {
+        rint(p.START);
+        printf("===> Info: Continuing run at t=%f\n", p.START * p.dt);
+        rint(p.TAKEN);
+        rint(p.NACC);
+        p.NACC = (p.NACC / p.CONFTAKE) * p.CONFTAKE;  // correct for non-stored parts
+        rint(p.NAC1);
+        rint(p.NREC);
+        rint(p.NTOT);
+        if (!in) {
+            printf("\nERROR: continuation fields missing in %s\n", path);
+            return false;
+        }
+        printf("INFO: Taken=%i Nacc=%i Nac1=%i Nrec=%i Ntot=%i\n", p.TAKEN, p.NACC, p.NAC1, p.NREC, p.NTOT);
+        unsigned long long s;
+        if (in >> name >> s && name == "SEED") {
+            p.has_seed = true;
+            p.seed = s;
+        }
+    }
+    if (p.NC < 1 || p.UPDATE < 1 || p.UPDATE2 < 1 || p.CONFTAKE < 1) {
+        printf("\nERROR: NC, UPDATE, UPDATE2 and CONFTAKE must be positive\n");
+        return false;
+    }
+    return true;
+}
+
+// store_parameters (sun.cpp:940-1057): re-reads the original params file
+// token by token and writes it with the run's counters appended.
+bool store_parameters(const char *infile, const char *outfile, const Params &p, int myt, int mytaken)
+{
+    // the reference copies infile to infile-tmp first (sun.cpp:947-949), so
+    // infile may be outfile itself (a resumed run rewrites its master file)
+    std::stringstream myin;
+    {
+        std::ifstream f(infile);
+        if (!f.is_open()) {
+            printf("\nERROR: one of the params files %s,%s could not be opened\n", infile, outfile);
+            return false;
+        }
+        myin << f.rdbuf();
+    }
+    std::ofstream myout(outfile);
+    if (!myout.is_open()) {
+        printf("\nERROR: one of the params files %s,%s could not be opened\n", infile, outfile);
+        return false;
+    }
+    std::string tok;
+    int iv;
+    double dv;
+    myin >> tok;
+    myout << tok << "\n";
+    myin >> tok;
+    myout << tok << "\t\t";
+    myin >> iv;
+    myout << iv << "\n";
+    myin >> tok;
+    myout << tok << "\t\t";
+    myin >> iv;
+    myout << iv << "\n";
+    for (int i = 0; i < p.DIM; i++) {
+        myin >> tok;
+        myout << tok << "\t\t";
+        myin >> iv;
+        myout << iv << "\n";
+    }
+    myin >> tok;
+    myout << tok << "\t\t";
+    myin >> dv;
+    myout << dv << "\n";
+    myin >> tok;
+    myout << tok << "\t\t";
+    myin >> dv;
+    myout << dv << "\n";
+    const int vals[] = {p.STEPS, p.UPDATE, p.UPDATE2, p.CONFTAKE, p.LEVEL, p.VERBOSE, 1, p.RANDOM};
+    const char *seps[] = {"\t\t", "\t\t", "\t\t", "\t\t", "\t", "\t\t", "\t", "\t\t"};
+    for (int k = 0; k < 8; k++) {
+        myin >> tok;
+        myout << tok << seps[k];
+        myin >> iv;
+        myout << vals[k] << "\n";
+    }
+    myout << "TIME-STEP\t" << myt << "\n";
+    myout << "TAKEN\t\t" << mytaken << "\n";
+    myout << "NACC\t\t" << p.NACC << "\n";
+    myout << "NAC1\t\t" << p.NAC1 << "\n";
+    myout << "NREC\t\t" << p.NREC << "\n";
+    myout << "NTOT\t\t" << p.NTOT << "\n";
+    myout << "SEED\t\t" << p.seed << "\n";
+    return (bool)myout;
+}
+
+// Eigen-phases of a unitary matrix W, ascending.  W is normal, so it shares
+// eigenvectors with the Hermitian H = (e^{i phi} W + e^{-i phi} W^+)/2 for any
+// phi; H is diagonalised by cyclic complex Jacobi rotations and each phase is
+// the argument of the Rayleigh quotient v^+ W v.  A fixed irrational phi
+// separates eigenvalues of H that would coincide for pairs theta, -theta.
+std::vector<double> unitary_phases(const std::vector<std::complex<double>> &W, int n)
+{
+    typedef std::complex<double> cd;
+    const cd e(std::cos(0.7390851332), std::sin(0.7390851332));
+    std::vector<cd> H(n * n), Vm(n * n, 0.0);
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) H[i * n + j] = 0.5 * (e * W[i * n + j] + std::conj(e * W[j * n + i]));
+    for (int i = 0; i < n; i++) Vm[i * n + i] = 1.0;
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0;
+        for (int i = 0; i < n; i++)
+            for (int j = i + 1; j < n; j++) off += std::norm(H[i * n + j]);
+        if (off < 1e-30) break;
+        for (int p = 0; p < n; p++)
+            for (int q = p + 1; q < n; q++) {
+                const cd hpq = H[p * n + q];
+                const double a = std::abs(hpq);
+                if (a < 1e-300) continue;
+                const double app = H[p * n + p].real(), aqq = H[q * n + q].real();
+                const double tau = (aqq - app) / (2 * a);
+                const double t = (tau >= 0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1 + tau * tau));
+                const double c = 1 / std::sqrt(1 + t * t), s = t * c;
+                const cd ph = hpq / a;  // rotation in the (p, q) plane: [c, s ph; -s conj(ph), c]
+                for (int k = 0; k < n; k++) {  // H <- H J
+                    const cd hkp = H[k * n + p], hkq = H[k * n + q];
+                    H[k * n + p] = c * hkp - s * std::conj(ph) * hkq;
+                    H[k * n + q] = s * ph * hkp + c * hkq;
+                }
+                for (int k = 0; k < n; k++) {  // H <- J^+ H
+                    const cd hpk = H[p * n + k], hqk = H[q * n + k];
+                    H[p * n + k] = c * hpk - s * ph * hqk;
+                    H[q * n + k] = s * std::conj(ph) * hpk + c * hqk;
+                }
+                for (int k = 0; k < n; k++) {  // V <- V J
+                    const cd vkp = Vm[k * n + p], vkq = Vm[k * n + q];
+                    Vm[k * n + p] = c * vkp - s * std::conj(ph) * vkq;
+                    Vm[k * n + q] = s * ph * vkp + c * vkq;
+                }
+            }
+    }
+    std::vector<double> out(n);
+    for (int k = 0; k < n; k++) {
+        cd r = 0;
+        for (int i = 0; i < n; i++) {
+            cd wv = 0;
+            for (int j = 0; j < n; j++) wv += W[i * n + j] * Vm[j * n + k];
+            r += std::conj(Vm[i * n + k]) * wv;
+        }
+        out[k] = std::arg(r);
+    }
+    std::sort(out.begin(), out.end());
+    return out;
+}
+
+int die(askja_ctx *c, int rc, const char *what)
+{
+    fprintf(stderr, "askja: %s failed (%d): %s\n", what, rc, c ? askja_last_error(c) : "");
+    return 1;
+}
+
+#define CK(expr)                                 \
+    do {                                         \
+        int rc_ = (expr);                        \
+        if (rc_ != ASKJA_OK) return die(ctx, rc_, #expr); \
+    } while (0)
+
+}  // namespace
+
+int main(int argc, char *argv[])
+{
+    if (argc < 2) {
+        printf("ERROR please specify a params file name");
+        printf("\nFinished with errors\n");
+        return 2;
+    }
+    const char *paramsfile = argv[1];
+    int precision = ASKJA_FP64;
+    bool reunit = false;
+    for (int a = 2; a < argc; a++) {
+        if (!strcmp(argv[a], "--precision") && a + 1 < argc) {
+            const char *v = argv[++a];
+            if (!strcmp(v, "fp64")) precision = ASKJA_FP64;
+            else if (!strcmp(v, "fp64-exact")) precision = ASKJA_FP64_EXACT;
+            else if (!strcmp(v, "fp32")) precision = ASKJA_FP32;
+            else {
+                fprintf(stderr, "askja: unknown precision %s\n", v);
+                return 2;
+            }
+        } else if (!strcmp(argv[a], "--reunit")) {
+            reunit = true;
+        } else {
+            fprintf(stderr, "askja: unknown option %s\n", argv[a]);
+            return 2;
+        }
+    }
+    Params P;
+    if (!load_parameters(paramsfile, P)) {
+        printf("===>Info: Statistics: acc=%i (%i) rec=%i, rate=%f\n", 0, 0, 0, 0.0 / 0.0);
+        printf("===>Took %i configurations\n", 0);
+        printf("Finished with errors\n");
+        return 1;
+    }
+    const int NC = P.NC, DIM = P.DIM;
+    const double BETA = P.BETA, dt = P.dt;
+    printf("SU(%i) simulation in %i space dimensions\n", NC, DIM);
+    for (int i = 0; i < DIM; i++) printf("Dimension %i: %i points\n", i, P.x[i]);
+    askja_ctx *ctx = nullptr;
+    CK(askja_create(NC, DIM, P.x.data(), precision, &ctx));
+
+    char outdir[255], buffer[1024];
+    snprintf(outdir, sizeof outdir, "output-DIM%i", DIM);
+    struct stat st;
+    if (stat(outdir, &st) != 0) {
+        printf("===> Info: directory %s not present, creating it\n", outdir);
+        mkdir(outdir, S_IRWXU | S_IRWXG | S_IROTH | S_IXOTH);
+    }
+    const int Nt = P.x[0], Ns = DIM > 1 ? P.x[1] : 1;
+    char tag[512];
+    snprintf(tag, sizeof tag, "N%i-Nt%i-Ns%i-BETA%.2f-DT%.4f-U%.4f-CT%i", NC, Nt, Ns, BETA, dt, P.UPDATE2 * dt,
+             P.CONFTAKE);
+    snprintf(buffer, sizeof buffer, "%s/constraints-%s.dat", outdir, tag);
+    FILE *constraints = fopen(buffer, P.RUNCONTINUE ? "a" : "w");
+    snprintf(buffer, sizeof buffer, "%s/Wilson-%s.dat", outdir, tag);
+    FILE *wilson = fopen(buffer, P.RUNCONTINUE ? "a" : "w");
+    if (!constraints || !wilson) {
+        fprintf(stderr, "askja: cannot open output files in %s\n", outdir);
+        return 1;
+    }
+    if (!P.RUNCONTINUE) {
+        fputs("#Time\t Gauss-law\t Total Energy\t E_L\t E_M\t Unitary-Violation\n", constraints);
+        fputs("#Time\t <W>\t Wilson-Line-Phases\n", wilson);
+    }
+
+    // ---- initial state (sun.cpp:1146-1179)
+    bool gsl_first = false;
+    if (!P.RUNCONTINUE) {
+        P.seed = P.RANDOM ? (unsigned long long)time(NULL) : 120351616ULL;
+        gsl_first = !P.RANDOM;
+        CK(askja_links_cold(ctx));
+    } else {
+        if (!P.has_seed) P.seed = (unsigned long long)time(NULL);
+        snprintf(buffer, sizeof buffer, "%s/U-%s-%i.conf", outdir, tag, P.TAKEN - 1);
+        printf("===> Info: Attempting to load configuration %s\n", buffer);
+        FILE *f = fopen(buffer, "rb");
+        if (!f) {
+            printf("Error loading fields file %s ABORTED!\n", buffer);
+            printf("Finished with errors\n");
+            return 1;
+        }
+        std::vector<double> u(askja_field_doubles(ctx));
+        const size_t got = fread(u.data(), sizeof(double), u.size(), f);
+        fclose(f);
+        if (got != u.size()) {
+            printf("Error loading fields file %s: %zu of %zu values ABORTED!\n", buffer, got, u.size());
+            printf("Finished with errors\n");
+            return 1;
+        }
+        CK(askja_links_upload(ctx, u.data()));
+    }
+    CK(askja_snapshot(ctx, 0));  // takeconfiguration
+
+    auto energy_mag = [&](double &em) -> int {
+        double t, s;
+        int rc = askja_action(ctx, &t, &s);
+        em = t + s;
+        return rc;
+    };
+    auto refresh = [&](uint64_t counter) -> int {
+        if (gsl_first) {
+            gsl_first = false;
+            return askja_mom_gsl(ctx, (unsigned long)P.seed, BETA);
+        }
+        return askja_mom_refresh(ctx, P.seed, counter, BETA);
+    };
+    // H = (EnergyMag + sum Tr p^2 / 2N) beta on the current momenta
+    auto hamiltonian = [&](double &h) -> int {
+        double em, k;
+        int rc = energy_mag(em);
+        if (rc) return rc;
+        if ((rc = askja_kinetic(ctx, &k))) return rc;
+        h = (em + k / (2 * NC)) * BETA;
+        return ASKJA_OK;
+    };
+    std::vector<double> wmat((size_t)2 * NC * NC);
+    std::vector<int> origin(DIM, 0);
+    // WilsonLoop (sun.cpp:307-362): |Tr W|/N at the origin and its phases
+    auto wilson_loop = [&](std::vector<double> &evs) -> int {
+        int rc = askja_line_matrix(ctx, 0, origin.data(), wmat.data());
+        if (rc) return rc;
+        std::vector<std::complex<double>> W((size_t)NC * NC);
+        std::complex<double> tr = 0;
+        for (int q = 0; q < NC * NC; q++) W[q] = std::complex<double>(wmat[2 * q], wmat[2 * q + 1]);
+        for (int i = 0; i < NC; i++) tr += W[i * NC + i];
+        evs.assign(NC + 1, 0.0);
+        evs[0] = std::abs(tr) / NC;
+        std::vector<double> ph = unitary_phases(W, NC);
+        for (int i = 0; i < NC; i++) evs[i + 1] = ph[i];
+        return ASKJA_OK;
+    };
+    auto write_wilson = [&](double t, const std::vector<double> &evs) {
+        fprintf(wilson, "%f\t", t);
+        for (int i = 0; i < NC + 1; i++) fprintf(wilson, "%g\t", evs[i]);
+        fprintf(wilson, "\n");
+        fflush(wilson);
+    };
+
+    uint64_t traj_counter = (uint64_t)P.NTOT;
+    CK(refresh(traj_counter));
+    double CRITENERGY;
+    CK(hamiltonian(CRITENERGY));
+    printf("\t INFO: Initial energy =%f\n", CRITENERGY);
+    // a kick is held back so that it can run fused with the next drift; it
+    // is flushed (standalone kick) whenever the state after it is observed
+    double pending = dt / 2;
+    auto flush = [&]() -> int {
+        if (pending == 0) return ASKJA_OK;
+        const double e = pending;
+        pending = 0;
+        return askja_kick(ctx, e);
+    };
+    auto drift = [&]() -> int {
+        if (pending != 0) {
+            const double e = pending;
+            pending = 0;
+            return askja_kick_drift(ctx, e, dt, P.LEVEL);
+        }
+        return askja_drift(ctx, dt, P.LEVEL);
+    };
+    CK(flush());
+    {
+        // t = 0 row (copyDown has made pi = Pi, so E_L = sum tr2(Pi))
+        double em, ee, gauss, uv;
+        std::vector<double> evs;
+        CK(energy_mag(em));
+        CK(askja_kinetic(ctx, &ee));
+        CK(askja_gauss_law(ctx, &gauss));
+        const double energy = 2 * NC * em + ee;
+        CK(wilson_loop(evs));
+        if (!P.RUNCONTINUE) {
+            if (P.VERBOSE)
+                printf("t=%f Gauss Law =%g Energy=%g Wx=%g EL=%g EM=%g\n", 0., gauss, energy / (2 * NC) * BETA, evs[0],
+                       ee / (2 * NC) * BETA, em * BETA);
+            CK(askja_uviol(ctx, &uv));
+            fprintf(constraints, "%f\t %g\t %g\t %g\t %f\t %g\n", 0., gauss, energy / (2 * NC) * BETA,
+                    ee / (2 * NC) * BETA, em * BETA, uv);
+            fflush(constraints);
+            write_wilson(0., evs);
+        }
+    }
+
+    const clock_t c0 = clock();
+    const time_t w0 = time(NULL);
+    int timekeep;
+    for (timekeep = P.START + 1; timekeep < P.STEPS + P.START + 1; timekeep++) {
+        CK(drift());
+        const bool traj_end = timekeep % P.UPDATE2 == 0;
+        if (!traj_end) pending = dt;
+        if (timekeep % P.UPDATE == 0) {
+            CK(flush());
+            double gauss, ee, em, uv;
+            std::vector<double> evs;
+            CK(askja_gauss_law(ctx, &gauss));
+            if (traj_end) CK(askja_kinetic(ctx, &ee));  // no kick this step: pi = Pi
+            else CK(askja_energy_el(ctx, &ee));
+            CK(energy_mag(em));
+            const double energy = 2 * NC * em + ee;
+            CK(askja_uviol(ctx, &uv));
+            CK(wilson_loop(evs));
+            if (P.VERBOSE)
+                printf("t=%f Gauss Law =%g Energy=%g Wx=%g EL=%g EM=%g U-Viol=%g \n", timekeep * dt, gauss,
+                       energy * BETA / (2 * NC), evs[0], ee * BETA / (2 * NC), em * BETA, uv);
+            fprintf(constraints, "%f\t %g\t %g\t %g\t %f\t %g\n", timekeep * dt, gauss, energy * BETA / (2 * NC),
+                    ee * BETA / (2 * NC), em * BETA, uv);
+            fflush(constraints);
+            write_wilson(timekeep * dt, evs);
+        }
+        if (traj_end) {
+            pending = dt / 2;
+            CK(flush());
+            double myenergy;
+            CK(hamiltonian(myenergy));
+            printf("\t INFO: Final energy =%f\n", myenergy);
+            P.NTOT++;
+            // Metropolis uniform: Philox (key = seed, purpose 3, counter)
+            bool accept;
+            if (myenergy < CRITENERGY) {
+                printf("\t ACCEPT case 1: %f<%f\n", myenergy, CRITENERGY);
+                P.NACC++;
+                P.NAC1++;
+                accept = true;
+            } else {
+                double nummi;
+                CK(askja_uniform(ctx, P.seed, traj_counter, &nummi));
+                const double bound = std::exp(-myenergy + CRITENERGY);
+                if (nummi < bound) {
+                    printf("\t ACCEPT case 2: %f<%f\n", nummi, bound);
+                    P.NACC++;
+                    accept = true;
+                } else {
+                    printf("\t REJECT %f>%f\n", nummi, bound);
+                    P.NREC++;
+                    accept = false;
+                }
+            }
+            if (accept) {
+                if (reunit) CK(askja_reunitarize(ctx));
+                CK(askja_snapshot(ctx, 0));
+            } else {
+                CK(askja_snapshot(ctx, 1));
+            }
+            if (P.VERBOSE) {
+                double em;
+                CK(energy_mag(em));
+                printf(accept ? "===> Info: accepted new configuration, em=%g\n"
+                              : "===> Info: loaded old configuration, em=%g\n",
+                       em * BETA);
+            }
+            traj_counter++;
+            CK(refresh(traj_counter));
+            CK(hamiltonian(CRITENERGY));
+            printf("\t INFO: initial energy =%f\n", CRITENERGY);
+            pending = dt / 2;
+            if (P.NTOT % P.CONFTAKE == 0) {
+                CK(flush());
+                snprintf(buffer, sizeof buffer, "%s/U-%s-%i.conf", outdir, tag, P.TAKEN);
+                printf("==>Info: storing configuration %s\n", buffer);
+                std::vector<double> u(askja_field_doubles(ctx));
+                CK(askja_links_download(ctx, u.data()));
+                FILE *f = fopen(buffer, "wb");
+                if (!f || fwrite(u.data(), sizeof(double), u.size(), f) != u.size()) {
+                    fprintf(stderr, "askja: writing %s failed\n", buffer);
+                    return 1;
+                }
+                fclose(f);
+                P.TAKEN++;
+                snprintf(buffer, sizeof buffer, "%s/master-%s.params", outdir, tag);
+                if (!store_parameters(paramsfile, buffer, P, timekeep, P.TAKEN)) return 1;
+            }
+        }
+    }
+    CK(flush());
+    CK(askja_sync(ctx));
+    fclose(constraints);
+    fclose(wilson);
+    const double wall = difftime(time(NULL), w0);
+    (void)c0;
+    askja_destroy(ctx);
+    printf("===>Info: Statistics: acc=%i (%i) rec=%i, rate=%f\n", P.NACC, P.NAC1, P.NREC, (P.NACC * 1.0) / (P.NTOT));
+    printf("===>Took %i configurations\n", P.TAKEN);
+    printf("===>Info: %d MD steps in %.0f s\n", P.STEPS, wall);
+    printf("Finished successfully\n");
+    return 0;
+}

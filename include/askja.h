@@ -19,7 +19,8 @@
  *    N x N complex matrix, row-major, (re, im) interleaved doubles
  *    (sun.cpp:289-302, structs.h:130-165).  Length askja_field_doubles().
  *  - Device layout is private: links grouped by direction, sites in
- *    checkerboard order (even-parity sites, then odd, each lexicographic);
+ *    checkerboard order (even-parity sites, then odd; within a parity sorted
+ *    by slab of the longest direction, then lexicographically);
  *    for N <= 3 the links are tiled in groups of 64 (one wavefront) with each
  *    matrix element contiguous across the group, for N >= 4 every matrix is
  *    contiguous.  Momenta are stored as compact Hermitian matrices (N^2 reals).
@@ -152,6 +153,12 @@ int askja_uviol(askja_ctx *ctx, double *out);
  * WilsonLoop(evs, dir) (measurements4d.cpp:311-355) the mean over x0. */
 int askja_wilson_lines(askja_ctx *ctx, int dir, double *out);
 
+/* The N x N product matrix (host buffer, 2 N^2 doubles) of the straight line
+ * in direction dir starting at coordinates x[0..D-1]: the matrix W of
+ * WilsonLoop (sun.cpp:307-345) for x = 0, dir = 0, whose eigen-phases fill
+ * the Wilson-*.dat columns. */
+int askja_line_matrix(askja_ctx *ctx, int dir, const int *x, double *out);
+
 /* ---- HMC -------------------------------------------------------------------------- */
 typedef struct {
     double beta;
@@ -177,6 +184,11 @@ typedef struct {
  * the state before the trajectory.  NaN energies reject, as in the reference. */
 int askja_hmc_trajectory(askja_ctx *ctx, const askja_hmc_params *p, uint64_t traj_counter,
                          askja_hmc_result *res);
+
+/* The Metropolis uniform in (0, 1) that askja_hmc_trajectory draws for
+ * trajectory `counter` (replaces gsl_rng_uniform of an mt19937 seeded with
+ * time(NULL), sun.cpp:1277-1287). */
+int askja_uniform(askja_ctx *ctx, uint64_t seed, uint64_t counter, double *out);
 
 /* ---- plumbing ------------------------------------------------------------------------ */
 /* Synchronise the context's stream. */

@@ -1,0 +1,163 @@
+"""The `askja <params>` program (askja_amd/driver) against the reference's
+file contract (sun.cpp:803-1373) and the oracle's first trajectory.
+
+CPU tests: argument and params-file errors end the program before any GPU
+call.  GPU tests: a short run writes the reference's output files with the
+reference's layouts, its first trajectory matches the oracle bit for bit
+(FP64_EXACT), and a resumed run continues the numbering.
+"""
+import glob
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+ASKJA = os.path.join(ROOT, "askja_amd", "driver", "askja")
+
+PARAMS = """#Parameters_file
+NC              3
+DIM             4
+N0              4
+N1              4
+N2              4
+N3              4
+BETA            5.9
+DT              0.05
+STEPS           {steps}
+UPDATE          10
+UPDATE2         20
+CONFTAKE        1
+LEVEL           6
+VERBOSE         0
+RUNCONTINUE     0
+RANDOM          0
+"""
+TAG = "N3-Nt4-Ns4-BETA5.90-DT0.0500-U1.0000-CT1"
+
+
+def _run(args, cwd, timeout=300):
+    if not os.path.exists(ASKJA):
+        pytest.fail("askja is not built (run __graft_entry__.build())")
+    return subprocess.run([ASKJA] + args, cwd=cwd, capture_output=True, text=True, timeout=timeout)
+
+
+def read_master(path, dim):
+    """Restatement of measurements4d.cpp:752-882 (load_parameters of the
+    measure programs): positional tokens, names ignored; with RUNCONTINUE 1
+    it reads TIME-STEP, TAKEN, NACC, NAC1, NREC."""
+    tok = open(path).read().split()
+    it = iter(tok)
+    next(it)
+    vals = {}
+    next(it); vals["NC"] = int(next(it))
+    next(it); vals["DIM"] = int(next(it))
+    assert vals["DIM"] == dim
+    vals["x"] = []
+    for _ in range(dim):
+        next(it); vals["x"].append(int(next(it)))
+    for k, f in (("BETA", float), ("DT", float), ("STEPS", int), ("UPDATE", int), ("UPDATE2", int),
+                 ("CONFTAKE", int), ("LEVEL", int), ("VERBOSE", int), ("RUNCONTINUE", int), ("RANDOM", int)):
+        next(it); vals[k] = f(next(it))
+    if vals["RUNCONTINUE"]:
+        for k in ("TIME-STEP", "TAKEN", "NACC", "NAC1", "NREC"):
+            name = next(it)
+            assert name == k
+            vals[k] = int(next(it))
+    return vals
+
+
+def test_missing_and_truncated_params(tmp_path):
+    r = _run([str(tmp_path / "nope.txt")], tmp_path)
+    assert r.returncode != 0
+    assert "could not be opened" in r.stdout and "Finished with errors" in r.stdout
+    (tmp_path / "short.txt").write_text("#Parameters_file\nNC 3\nDIM 4\nN0 4\n")
+    r = _run([str(tmp_path / "short.txt")], tmp_path)
+    assert r.returncode != 0 and "Finished with errors" in r.stdout
+    assert not (tmp_path / "output-DIM4").exists()
+    r = _run([], tmp_path)
+    assert r.returncode != 0
+
+
+@pytest.mark.gpu
+def test_run_files_and_first_trajectory(tmp_path, oracle):
+    import askja_amd as A
+
+    (tmp_path / "p.txt").write_text(PARAMS.format(steps=60))
+    r = _run(["p.txt", "--precision", "fp64-exact"], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Finished successfully" in r.stdout
+    out = tmp_path / "output-DIM4"
+    cons = (out / f"constraints-{TAG}.dat").read_text().splitlines()
+    assert cons[0] == "#Time\t Gauss-law\t Total Energy\t E_L\t E_M\t Unitary-Violation"
+    rows = [list(map(float, l.split("\t"))) for l in cons[1:]]
+    assert [round(r_[0], 6) for r_ in rows] == [0.0, 0.5, 1.0, 1.5, 2.0, 2.5, 3.0]
+    assert all(len(r_) == 6 for r_ in rows)
+    wil = (out / f"Wilson-{TAG}.dat").read_text().splitlines()
+    assert wil[0] == "#Time\t <W>\t Wilson-Line-Phases"
+    for l in wil[1:]:
+        f = l.rstrip("\t").split("\t")
+        assert len(f) == 1 + 3 + 1
+        ph = list(map(float, f[2:]))
+        # phases of an SU(3) matrix sum to 0 mod 2 pi (printed with %g)
+        assert ph == sorted(ph)
+        assert min(abs(sum(ph)), abs(abs(sum(ph)) - 2 * np.pi)) < 1e-4
+    confs = sorted(glob.glob(str(out / f"U-{TAG}-*.conf")))
+    assert len(confs) == 3
+    V, D, N = 256, 4, 3
+    for c in confs:
+        assert os.path.getsize(c) == V * D * N * N * 16
+    m = read_master(out / f"master-{TAG}.params", 4)
+    assert m["RUNCONTINUE"] == 1 and m["TAKEN"] == 3 and m["TIME-STEP"] == 60
+    assert m["NACC"] + m["NREC"] == 3 and m["BETA"] == 5.9 and m["x"] == [4, 4, 4, 4]
+
+    # trajectory 1 on the oracle: cold start, reference momenta, 19 drifts
+    o = oracle.Oracle(N, [4, 4, 4, 4], beta=5.9, level=6)
+    o.initializeU()
+    o.initializeE(120351616)
+    h0 = o.hamiltonian("pi")
+    t0_em = o.energy_mag()
+    o.md_trajectory(0.05, 19)
+    h1 = o.hamiltonian("Pi")
+    lat = A.Lattice(N, [4, 4, 4, 4], A.FP64)
+    u_draw = ctypes_uniform(A, lat, 120351616, 0)
+    accepted = h1 < h0 or u_draw < np.exp(h0 - h1)
+    conf0 = np.fromfile(confs[0], dtype=np.complex128).reshape(V, D, N, N)
+    if accepted:
+        assert np.array_equal(conf0, o.field("u"))
+    else:
+        assert np.array_equal(conf0, np.broadcast_to(np.eye(N), conf0.shape))
+    assert ("ACCEPT" if accepted else "REJECT") in r.stdout.split("Final energy")[1].splitlines()[1]
+    # t = 0 row: E_M = beta * EnergyMag of the cold start = 0, U-violation 0
+    assert rows[0][4] == 0.0 and t0_em == 0.0 and rows[0][5] == 0.0
+
+
+def ctypes_uniform(A, lat, seed, counter):
+    import ctypes as C
+    v = C.c_double()
+    assert A.lib().askja_uniform(lat._h, seed, counter, C.byref(v)) == 0
+    return v.value
+
+
+@pytest.mark.gpu
+def test_resume_continues_numbering(tmp_path):
+    (tmp_path / "p.txt").write_text(PARAMS.format(steps=40))
+    r = _run(["p.txt"], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = tmp_path / "output-DIM4"
+    master = out / f"master-{TAG}.params"
+    assert read_master(master, 4)["TAKEN"] == 2
+    r = _run([str(master)], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Continuing run at t=2.000000" in r.stdout
+    confs = sorted(glob.glob(str(out / f"U-{TAG}-*.conf")))
+    assert [os.path.basename(c).rsplit("-", 1)[1] for c in confs] == ["0.conf", "1.conf", "2.conf", "3.conf"]
+    m = read_master(master, 4)
+    assert m["TAKEN"] == 4 and m["TIME-STEP"] == 80
+    cons = (out / f"constraints-{TAG}.dat").read_text().splitlines()
+    times = [round(float(l.split("\t")[0]), 6) for l in cons[1:]]
+    assert times == [0.0, 0.5, 1.0, 1.5, 2.0, 2.5, 3.0, 3.5, 4.0]
+    # the seed travels after NTOT, where neither reference reader looks
+    assert open(master).read().split()[-2] == "SEED"
