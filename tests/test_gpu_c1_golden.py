@@ -1,0 +1,52 @@
+"""One fixed-momentum trajectory of the reference's example workload at
+DIM 4 (NC 8, 10^4, BETA 20, DT 0.01, LEVEL 8, cold start, mt19937 seed
+120351616 momenta, 999 drifts) against the oracle's result committed in
+tests/golden/c1_trajectory1.json (written by make_c1_trajectory.py).
+
+FP64_EXACT must reproduce the oracle's links and momenta bit for bit (equal
+SHA-256); FP64 (FMA, factored staples) must agree link by link to 1e-10.
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+import askja_amd as A
+from conftest import ROOT
+
+pytestmark = pytest.mark.gpu
+
+GOLD = json.load(open(os.path.join(ROOT, "tests", "golden", "c1_trajectory1.json")))
+
+
+def _run(prec):
+    lat = A.Lattice(GOLD["nc"], GOLD["extents"], prec)
+    lat.cold()
+    lat.mom_gsl(GOLD["seed"], GOLD["beta"])
+    e = lat.md_trajectory(GOLD["dt"], GOLD["ndrift"], GOLD["level"])
+    return lat, e
+
+
+def test_c1_trajectory_exact_bitwise():
+    lat, e = _run(A.FP64_EXACT)
+    u = lat.links()
+    assert hashlib.sha256(np.ascontiguousarray(u).tobytes()).hexdigest() == GOLD["links_sha256"]
+    p = lat.momenta()
+    assert hashlib.sha256(np.ascontiguousarray(p).tobytes()).hexdigest() == GOLD["momenta_sha256"]
+    for s in GOLD["samples"]:
+        want = np.array(s["re"]) + 1j * np.array(s["im"])
+        assert np.array_equal(u[s["site"], s["dir"]].ravel(), want)
+    beta, n2 = GOLD["beta"], 2.0 * GOLD["nc"]
+    assert abs(beta * (e[0] + e[1] / n2) - GOLD["h_old"]) < 1e-9 * GOLD["h_old"]
+    assert abs(beta * (e[2] + e[3] / n2) - GOLD["h_new"]) < 1e-9 * GOLD["h_old"]
+
+
+def test_c1_trajectory_fast_link_by_link():
+    ex, _ = _run(A.FP64_EXACT)
+    fast, e = _run(A.FP64)
+    d = np.max(np.abs(fast.links() - ex.links()))
+    assert d <= 1e-10, d
+    beta, n2 = GOLD["beta"], 2.0 * GOLD["nc"]
+    assert abs(beta * (e[2] + e[3] / n2) - GOLD["h_new"]) < 1e-6
