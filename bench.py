@@ -85,6 +85,26 @@ def cpu_baseline(steps_sample):
                       f"{t_step * 1e3:.0f} ms/step, extrapolated to {NDRIFT} steps + 2 H evaluations"}
 
 
+def copy_probe(gib=1, reps=5):
+    """A plain device-to-device copy of a known size (the runtime's copy
+    kernel), for calibrating the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    n = gib << 30
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 0, C.c_size_t(n)) == 0
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+        assert hip.hipDeviceSynchronize() == 0
+        dt = time.perf_counter() - t0
+        print(json.dumps({"copy_bytes": n, "ms": dt * 1e3, "GBps_read_plus_write": 2 * n / dt / 1e9}))
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,7 +114,11 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--roofline-launches", type=int, default=100)
+    ap.add_argument("--copy-probe", action="store_true", help="only run a 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU path is not built.",
               file=sys.stderr)
