@@ -85,6 +85,16 @@ SIGNATURES = {
     "askja_line_matrix": (_I, [_P, _I, C.POINTER(C.c_int), _DP]),
     "askja_hmc_trajectory": (_I, [_P, C.POINTER(HmcParams), _U64, C.POINTER(HmcResult)]),
     "askja_uniform": (_I, [_P, _U64, _U64, _DP]),
+    "askja_flow_define": (_I, [_P, _I, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), _I, _D, _D]),
+    "askja_flow_block_size": (_I, [_P, C.POINTER(C.c_int)]),
+    "askja_flow_set_weights": (_I, [_P, _P, _I]),
+    "askja_flow_apply": (_I, [_P, _DP]),
+    "askja_flow_download": (_I, [_P, _DP]),
+    "askja_flow_gradient": (_I, [_P, _D, _I, _DP, _DP]),
+    "askja_flow_gradient_download": (_I, [_P, _DP]),
+    "askja_flow_weight_gradient": (_I, [_P, _DP]),
+    "askja_flow_hmc_trajectory": (_I, [_P, C.POINTER(HmcParams), _U64, C.POINTER(HmcResult)]),
+    "askja_flow_wilson_lines": (_I, [_P, _I, _DP]),
     "askja_sync": (_I, [_P]),
     "askja_timer_start": (_I, [_P]),
     "askja_timer_stop": (_I, [_P, _DP]),
@@ -271,6 +281,63 @@ class Lattice:
         r = HmcResult()
         self._ck(lib().askja_hmc_trajectory(self._h, C.byref(p), counter, C.byref(r)), "hmc_trajectory")
         return r
+
+    # -- learned field transformation ------------------------------------------
+    def flow_define(self, mu, parity, block, nblocks, rho_max=0.3, delta=1e-3):
+        n = len(mu)
+        arr = lambda v: (C.c_int * n)(*[int(x) for x in v])
+        self._ck(lib().askja_flow_define(self._h, n, arr(mu), arr(parity), arr(block), int(nblocks), rho_max, delta),
+                 "flow_define")
+        b = C.c_int()
+        self._ck(lib().askja_flow_block_size(self._h, C.byref(b)), "flow_block_size")
+        self.flow_bsize, self.flow_nblocks = b.value, int(nblocks)
+        return self.flow_bsize
+
+    def flow_set_weights(self, w):
+        """w: numpy float32 array (host) or an object with data_ptr() (device)."""
+        if hasattr(w, "data_ptr"):
+            self._ck(lib().askja_flow_set_weights(self._h, C.c_void_p(w.data_ptr()), 1), "flow_set_weights")
+        else:
+            w = np.ascontiguousarray(np.asarray(w, np.float32))
+            assert w.size == self.flow_bsize * self.flow_nblocks
+            self._ck(lib().askja_flow_set_weights(self._h, w.ctypes.data_as(C.c_void_p), 0), "flow_set_weights")
+
+    def flow_apply(self):
+        lj = C.c_double()
+        self._ck(lib().askja_flow_apply(self._h, C.byref(lj)), "flow_apply")
+        return lj.value
+
+    def flow_output(self):
+        out = self._empty()
+        self._ck(lib().askja_flow_download(self._h, _dptr(out.view(np.float64))), "flow_download")
+        return out
+
+    def flow_gradient(self, beta, wgrad=False):
+        s, lj = C.c_double(), C.c_double()
+        self._ck(lib().askja_flow_gradient(self._h, beta, 1 if wgrad else 0, C.byref(s), C.byref(lj)),
+                 "flow_gradient")
+        return s.value, lj.value
+
+    def flow_link_gradient(self):
+        out = self._empty()
+        self._ck(lib().askja_flow_gradient_download(self._h, _dptr(out.view(np.float64))), "flow_gradient_download")
+        return out
+
+    def flow_weight_gradient(self):
+        out = np.zeros(self.flow_bsize * self.flow_nblocks)
+        self._ck(lib().askja_flow_weight_gradient(self._h, _dptr(out)), "flow_weight_gradient")
+        return out
+
+    def flow_hmc_trajectory(self, beta, dt, ndrift, level, seed, counter, reunit=True):
+        p = HmcParams(beta, dt, ndrift, level, 1 if reunit else 0, seed)
+        r = HmcResult()
+        self._ck(lib().askja_flow_hmc_trajectory(self._h, C.byref(p), counter, C.byref(r)), "flow_hmc_trajectory")
+        return r
+
+    def flow_polyakov(self):
+        out = np.zeros(2 * (self.V // self.ext[0]))
+        self._ck(lib().askja_flow_wilson_lines(self._h, 0, _dptr(out)), "flow_wilson_lines")
+        return complex(out.view(np.complex128).mean())
 
     def sync(self):
         self._ck(lib().askja_sync(self._h), "sync")

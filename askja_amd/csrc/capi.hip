@@ -42,6 +42,16 @@ struct askja_ctx {
     int sticky = 0;
     std::string err;
     Geom geom{};
+    // learned field transformation
+    int flow_nsub = 0, flow_nblocks = 0, flow_bsize = 0, flow_seg = 8;
+    std::vector<FlowSub> flow_subs;
+    std::vector<int> flow_block;
+    float *flow_w = nullptr;     // nblocks * bsize MLP weights
+    double *flow_gw = nullptr;   // their gradient
+    double *flow_g = nullptr;    // 9 doubles per link: Lie gradient
+    double *flow_lj = nullptr;   // per sub-layer log J (s, c)
+    void *flow_out = nullptr;    // U = F(V)
+    std::vector<void *> flow_ck, flow_work;
 };
 
 namespace {
@@ -332,8 +342,13 @@ int askja_destroy(askja_ctx *c)
     for (int b = 0; b < 3; b++)
         if (c->links[b]) (void)hipFree(c->links[b]);
     void *bufs[] = {c->snap, c->mom, c->mom_prev, c->part, c->res, c->scratch, c->staging,
-                    c->d_nbr, c->d_dev2lex, c->d_lex2dev};
+                    c->d_nbr, c->d_dev2lex, c->d_lex2dev, c->flow_w, c->flow_gw, c->flow_g, c->flow_lj,
+                    c->flow_out};
     for (void *p : bufs)
+        if (p) (void)hipFree(p);
+    for (void *p : c->flow_ck)
+        if (p) (void)hipFree(p);
+    for (void *p : c->flow_work)
         if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -779,6 +794,303 @@ int askja_uniform(askja_ctx *c, uint64_t seed, uint64_t counter, double *out)
     if (!out) return fail(c, ASKJA_EINVAL, "uniform: null output");
     *out = metropolis_uniform(seed, counter);
     return ASKJA_OK;
+}
+
+// ------------------------------------------------------------------ flow
+namespace {
+
+int flow_ready(askja_ctx *c)
+{
+    if (c->flow_nsub <= 0) return fail(c, ASKJA_EINVAL, "flow: askja_flow_define has not been called");
+    return ASKJA_OK;
+}
+
+size_t link_bytes(const askja_ctx *c) { return c->link_reals * elem_bytes(c); }
+
+// U = F(V) into flow_out; with store, the inputs of sub-layers 0, K, 2K, ...
+// are kept in flow_ck.  Per sub-layer log J into flow_lj.
+int flow_forward(askja_ctx *c, bool store)
+{
+    HIPCHK(c, hipMemcpyAsync(c->flow_out, c->links[c->cur], link_bytes(c), hipMemcpyDeviceToDevice, c->st));
+    const long np = flow_forward_partials(c->geom, 0) > flow_forward_partials(c->geom, 1)
+                        ? flow_forward_partials(c->geom, 0)
+                        : flow_forward_partials(c->geom, 1);
+    int rc = ensure_part(c, (size_t)np * 2);
+    if (rc) return rc;
+    for (int k = 0; k < c->flow_nsub; k++) {
+        if (store && k % c->flow_seg == 0)
+            HIPCHK(c, hipMemcpyAsync(c->flow_ck[k / c->flow_seg], c->flow_out, link_bytes(c), hipMemcpyDeviceToDevice,
+                                     c->st));
+        const FlowSub &f = c->flow_subs[k];
+        HIPCHK(c, launch_flow_forward(c->geom, c->flow_out, c->flow_out, f, c->mode, c->part, c->st));
+        HIPCHK(c, launch_finalize(c->part, flow_forward_partials(c->geom, f.par), 1, c->flow_lj + 2 * k, c->st));
+    }
+    return ASKJA_OK;
+}
+
+int flow_logj(askja_ctx *c, double *logj)
+{
+    std::vector<double> h((size_t)2 * c->flow_nsub);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->flow_lj, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    double s = 0.0;
+    for (int k = 0; k < c->flow_nsub; k++) s += h[2 * k] + h[2 * k + 1];
+    *logj = s;
+    return ASKJA_OK;
+}
+
+// Gradient of Phi = beta S_W(F(V)) - log J(V) with respect to V into
+// flow_g (and the MLP weights into flow_gw when wgrad); returns S_W(F(V)).
+int flow_gradient(askja_ctx *c, double beta, bool wgrad, double *action)
+{
+    int rc = flow_forward(c, true);
+    if (rc) return rc;
+    const long nl = c->NL;
+    if ((rc = ensure_part(c, (size_t)((nl + 255) / 256) * 2))) return rc;
+    long np = 0;
+    HIPCHK(c, launch_wilson_grad(c->geom, c->flow_out, beta / (2.0 * c->nc), c->mode, c->flow_g, c->part, &np, c->st));
+    HIPCHK(c, launch_finalize(c->part, np, 1, c->res, c->st));
+    if (wgrad)
+        HIPCHK(c, hipMemsetAsync(c->flow_gw, 0, (size_t)c->flow_nblocks * c->flow_bsize * sizeof(double), c->st));
+    const int K = c->flow_seg;
+    const int nseg = (c->flow_nsub + K - 1) / K;
+    for (int j = nseg - 1; j >= 0; j--) {
+        const int k0 = j * K, k1 = std::min(c->flow_nsub, k0 + K);
+        HIPCHK(c, hipMemcpyAsync(c->flow_work[0], c->flow_ck[j], link_bytes(c), hipMemcpyDeviceToDevice, c->st));
+        for (int k = k0; k + 1 < k1; k++) {
+            void *in = c->flow_work[k - k0], *out = c->flow_work[k - k0 + 1];
+            HIPCHK(c, hipMemcpyAsync(out, in, link_bytes(c), hipMemcpyDeviceToDevice, c->st));
+            HIPCHK(c, launch_flow_forward(c->geom, out, out, c->flow_subs[k], c->mode, c->part, c->st));
+        }
+        for (int k = k1 - 1; k >= k0; k--) {
+            double *gw = wgrad ? c->flow_gw + (size_t)c->flow_block[k] * c->flow_bsize : nullptr;
+            HIPCHK(c, launch_flow_backward(c->geom, c->flow_work[k - k0], c->flow_subs[k], c->mode, c->flow_g, -1.0, gw,
+                                           c->st));
+        }
+    }
+    double h[2];
+    if ((rc = read_res(c, h, 2))) return rc;
+    *action = (double)n_plaquettes(c) - pair_val(h, 0) / (4.0 * c->nc);
+    return ASKJA_OK;
+}
+
+int kinetic_now(askja_ctx *c, double *k)
+{
+    long np = 0;
+    int rc = ensure_part(c, (size_t)((c->NL + 255) / 256) * 4);
+    if (rc) return rc;
+    HIPCHK(c, launch_kinetic(c->geom, c->mom, nullptr, c->mode, c->part, &np, c->st));
+    HIPCHK(c, launch_finalize(c->part, np, 2, c->res, c->st));
+    double h[4];
+    if ((rc = read_res(c, h, 4))) return rc;
+    *k = pair_val(h, 0);
+    return ASKJA_OK;
+}
+
+}  // namespace
+
+int askja_flow_define(askja_ctx *c, int nsub, const int *mu, const int *parity, const int *block, int nblocks,
+                      double rho_max, double delta)
+{
+    ENTER(c);
+    if (c->nc != 3) return fail(c, ASKJA_EUNSUP, "flow: only SU(3) is supported");
+    if (c->dim < 3) return fail(c, ASKJA_EUNSUP, "flow: needs D >= 3 (planes orthogonal to every direction)");
+    if (nsub < 1 || nsub > 4096 || nblocks < 1 || !mu || !parity || !block)
+        return fail(c, ASKJA_EINVAL, "flow_define: bad sizes or null arrays");
+    if (!(rho_max > 0.0) || rho_max * 2.0 * (c->dim - 1) >= 2.0)
+        return fail(c, ASKJA_EINVAL, "flow_define: rho_max must be in (0, 1/(D-1)) for invertibility");
+    for (int d = 0; d < c->dim; d++)
+        if (c->ext[d] % 2) return fail(c, ASKJA_EUNSUP, "flow: every extent must be even (checkerboard masks)");
+    for (int k = 0; k < nsub; k++)
+        if (mu[k] < 0 || mu[k] >= c->dim || (parity[k] != 0 && parity[k] != 1) || block[k] < 0 || block[k] >= nblocks)
+            return fail(c, ASKJA_EINVAL, "flow_define: sub-layer %d out of range", k);
+    const int nf = (c->dim - 1) * (c->dim - 2) / 2;
+    if (nf > 28) return fail(c, ASKJA_EUNSUP, "flow: at most 28 features (D <= 9)");
+    // (re)allocate
+    void *old[] = {c->flow_w, c->flow_gw, c->flow_g, c->flow_lj, c->flow_out};
+    for (void *p : old)
+        if (p) (void)hipFree(p);
+    for (void *p : c->flow_ck) (void)hipFree(p);
+    for (void *p : c->flow_work) (void)hipFree(p);
+    c->flow_ck.clear();
+    c->flow_work.clear();
+    c->flow_nsub = 0;
+    c->flow_bsize = 32 * nf + 2 * 32 + 1;
+    c->flow_nblocks = nblocks;
+    HIPCHK(c, hipMalloc(&c->flow_w, (size_t)nblocks * c->flow_bsize * sizeof(float)));
+    HIPCHK(c, hipMemset(c->flow_w, 0, (size_t)nblocks * c->flow_bsize * sizeof(float)));
+    HIPCHK(c, hipMalloc(&c->flow_gw, (size_t)nblocks * c->flow_bsize * sizeof(double)));
+    HIPCHK(c, hipMalloc(&c->flow_g, (size_t)c->NL * 9 * sizeof(double)));
+    HIPCHK(c, hipMalloc(&c->flow_lj, (size_t)2 * nsub * sizeof(double)));
+    HIPCHK(c, hipMalloc(&c->flow_out, link_bytes(c)));
+    const int nseg = (nsub + c->flow_seg - 1) / c->flow_seg;
+    for (int j = 0; j < nseg; j++) {
+        void *p = nullptr;
+        HIPCHK(c, hipMalloc(&p, link_bytes(c)));
+        c->flow_ck.push_back(p);
+    }
+    for (int j = 0; j < std::min(nsub, c->flow_seg); j++) {
+        void *p = nullptr;
+        HIPCHK(c, hipMalloc(&p, link_bytes(c)));
+        c->flow_work.push_back(p);
+    }
+    c->flow_subs.resize(nsub);
+    c->flow_block.assign(block, block + nsub);
+    for (int k = 0; k < nsub; k++)
+        c->flow_subs[k] = FlowSub{mu[k], parity[k], c->flow_w + (size_t)block[k] * c->flow_bsize, nf, rho_max, delta};
+    c->flow_nsub = nsub;
+    return ASKJA_OK;
+}
+
+int askja_flow_block_size(askja_ctx *c, int *bsize)
+{
+    ENTER(c);
+    int rc = flow_ready(c);
+    if (rc) return rc;
+    if (bsize) *bsize = c->flow_bsize;
+    return ASKJA_OK;
+}
+
+int askja_flow_set_weights(askja_ctx *c, const float *w, int on_device)
+{
+    ENTER(c);
+    int rc = flow_ready(c);
+    if (rc) return rc;
+    if (!w) return fail(c, ASKJA_EINVAL, "flow_set_weights: null buffer");
+    HIPCHK(c, hipMemcpyAsync(c->flow_w, w, (size_t)c->flow_nblocks * c->flow_bsize * sizeof(float),
+                             on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return ASKJA_OK;
+}
+
+int askja_flow_apply(askja_ctx *c, double *logj)
+{
+    ENTER(c);
+    int rc = flow_ready(c);
+    if (rc) return rc;
+    if ((rc = flow_forward(c, false))) return rc;
+    double lj;
+    if ((rc = flow_logj(c, &lj))) return rc;
+    if (logj) *logj = lj;
+    return ASKJA_OK;
+}
+
+int askja_flow_download(askja_ctx *c, double *aos)
+{
+    ENTER(c);
+    int rc = flow_ready(c);
+    if (rc) return rc;
+    if (!aos) return fail(c, ASKJA_EINVAL, "flow_download: null buffer");
+    if ((rc = ensure_staging(c))) return rc;
+    HIPCHK(c, launch_links_to_aos(c->geom, c->flow_out, c->staging, c->mode, c->st));
+    HIPCHK(c, hipMemcpyAsync(aos, c->staging, c->staging_doubles * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return ASKJA_OK;
+}
+
+int askja_flow_gradient(askja_ctx *c, double beta, int wgrad, double *action, double *logj)
+{
+    ENTER(c);
+    int rc = flow_ready(c);
+    if (rc) return rc;
+    if (!(beta > 0)) return fail(c, ASKJA_EINVAL, "flow_gradient: beta must be positive");
+    double s, lj;
+    if ((rc = flow_gradient(c, beta, wgrad != 0, &s))) return rc;
+    if ((rc = flow_logj(c, &lj))) return rc;
+    if (action) *action = s;
+    if (logj) *logj = lj;
+    return ASKJA_OK;
+}
+
+int askja_flow_gradient_download(askja_ctx *c, double *aos)
+{
+    ENTER(c);
+    int rc = flow_ready(c);
+    if (rc) return rc;
+    if (!aos) return fail(c, ASKJA_EINVAL, "flow_gradient_download: null buffer");
+    std::vector<double> g((size_t)c->NL * 9);
+    HIPCHK(c, hipMemcpyAsync(g.data(), c->flow_g, g.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    const int n = 3;
+    for (long i = 0; i < c->V; i++)
+        for (int d = 0; d < c->dim; d++) {
+            const double *b = g.data() + 9 * (size_t)(d * c->V + c->lex2dev[i]);
+            double *m = aos + ((size_t)i * c->dim + d) * 2 * n * n;
+            const double re[3][3] = {{b[0], b[3], b[5]}, {b[3], b[1], b[7]}, {b[5], b[7], b[2]}};
+            const double im[3][3] = {{0, b[4], b[6]}, {-b[4], 0, b[8]}, {-b[6], -b[8], 0}};
+            for (int p = 0; p < n; p++)
+                for (int q = 0; q < n; q++) {
+                    m[2 * (p * n + q)] = re[p][q];
+                    m[2 * (p * n + q) + 1] = im[p][q];
+                }
+        }
+    return ASKJA_OK;
+}
+
+int askja_flow_weight_gradient(askja_ctx *c, double *out)
+{
+    ENTER(c);
+    int rc = flow_ready(c);
+    if (rc) return rc;
+    if (!out) return fail(c, ASKJA_EINVAL, "flow_weight_gradient: null buffer");
+    HIPCHK(c, hipMemcpyAsync(out, c->flow_gw, (size_t)c->flow_nblocks * c->flow_bsize * sizeof(double),
+                             hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return ASKJA_OK;
+}
+
+int askja_flow_hmc_trajectory(askja_ctx *c, const askja_hmc_params *p, uint64_t traj_counter, askja_hmc_result *r)
+{
+    ENTER(c);
+    int rc = flow_ready(c);
+    if (rc) return rc;
+    if (!p || !r) return fail(c, ASKJA_EINVAL, "flow_hmc_trajectory: null argument");
+    if (!(p->beta > 0) || !(p->dt > 0) || p->ndrift < 1)
+        return fail(c, ASKJA_EINVAL, "flow_hmc_trajectory: beta, dt and ndrift must be positive");
+    if (p->reunit && (rc = askja_reunitarize(c))) return rc;
+    if ((rc = askja_snapshot(c, 0))) return rc;
+    if ((rc = askja_mom_refresh(c, p->seed, traj_counter, p->beta))) return rc;
+    const double cf = 2.0 * c->nc / p->beta;
+    double s0, lj0, k0, s1, lj1, k1;
+    if ((rc = kinetic_now(c, &k0))) return rc;
+    if ((rc = flow_gradient(c, p->beta, false, &s0))) return rc;
+    if ((rc = flow_logj(c, &lj0))) return rc;
+    HIPCHK(c, launch_kick_g(c->geom, c->mom, c->flow_g, 0.5 * p->dt * cf, c->mode, c->st));
+    double ci[9];
+    fill_cinv(ci);
+    s1 = s0;
+    lj1 = lj0;
+    for (int t = 1; t <= p->ndrift; t++) {
+        HIPCHK(c, launch_drift(c->geom, c->links[c->cur], c->mom, p->dt, p->level, ci, c->mode, c->st));
+        if ((rc = flow_gradient(c, p->beta, false, &s1))) return rc;
+        HIPCHK(c, launch_kick_g(c->geom, c->mom, c->flow_g, (t < p->ndrift ? 1.0 : 0.5) * p->dt * cf, c->mode, c->st));
+    }
+    if ((rc = flow_logj(c, &lj1))) return rc;
+    if ((rc = kinetic_now(c, &k1))) return rc;
+    r->h_old = p->beta * s0 - lj0 + k0 / (2.0 * c->nc) * p->beta;
+    r->h_new = p->beta * s1 - lj1 + k1 / (2.0 * c->nc) * p->beta;
+    r->dh = r->h_new - r->h_old;
+    r->uniform = metropolis_uniform(p->seed, traj_counter);
+    if (r->h_new < r->h_old) r->accept_case = 1;
+    else if (r->uniform < std::exp(r->h_old - r->h_new)) r->accept_case = 2;
+    else r->accept_case = 0;
+    r->accepted = r->accept_case != 0;
+    if (!r->accepted && (rc = askja_snapshot(c, 1))) return rc;
+    r->action = r->accepted ? s1 : s0;
+    return ASKJA_OK;
+}
+
+int askja_flow_wilson_lines(askja_ctx *c, int dir, double *out)
+{
+    ENTER(c);
+    int rc = flow_ready(c);
+    if (rc) return rc;
+    if ((rc = flow_forward(c, false))) return rc;
+    void *save = c->links[c->cur];
+    c->links[c->cur] = c->flow_out;
+    rc = askja_wilson_lines(c, dir, out);
+    c->links[c->cur] = save;
+    return rc;
 }
 
 int askja_sync(askja_ctx *c)

@@ -1,0 +1,570 @@
+// flow.hip -- the learned field transformation U = F(V) used as an HMC
+// proposal map (field-transformation HMC), for SU(3).
+//
+// F is a stack of masked sub-layers.  Sub-layer k updates the links of one
+// direction mu_k on the sites of one parity p_k; every other link is frozen.
+// For an updated link l = (x, mu):
+//   S      = sum_{nu != mu} [U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+
+//                            + U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu)]  (frozen staples)
+//   W      = U_l S                                  (the staple loop through l)
+//   P_j    = plaquettes at x in the planes not containing mu (frozen)
+//   Lam    = TAH(sum_j P_j),  Lhat = Lam / sqrt(2 Tr Lam^2 + delta^2)
+//   rho    = rho_max tanh(MLP(ReTr P_j / 3))        (2 layers, hidden 32)
+//   theta  = rho Im Tr(Lhat W)
+//   U'_l   = exp(i theta Lhat) U_l
+// Lhat, rho and the staples depend on frozen links only, so the Jacobian of
+// the sub-layer is the rank-one update of a left translation along Lhat:
+//   det J = 1 + d theta / dt (e^{i t Lhat} U_l) = 1 + rho ReTr(Lhat^2 W),
+// positive for |rho| < 1/3 (Tr Lhat^2 <= 1/2, ||W|| <= 2(D-1) = 6 at D=4), and
+// the sub-layer is a diffeomorphism (on every orbit t -> t + theta(t) is
+// monotone).  Gauge equivariance: Lhat and W transform covariantly at x,
+// theta and the features are invariant.
+//
+// The backward kernel propagates Lie-algebra gradients g_l = sum_a D^a Phi
+// T^a (Hermitian traceless, D^a = d/dt at e^{i t T^a} U_l) of
+// Phi = beta S_W(F(V)) - log J(V) from the output links to the input links,
+// recomputing every sub-layer's forward quantities from its stored input.
+#include "kernels.hpp"
+#include "common.hpp"
+
+namespace askja {
+namespace flow {
+
+constexpr int kB = 256;
+constexpr int kHid = 32;
+
+struct C3 {
+    double r[3][3], i[3][3];
+};
+
+__device__ inline void zero(C3 &a)
+{
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int q = 0; q < 3; q++) a.r[p][q] = a.i[p][q] = 0.0;
+}
+
+__device__ inline void eye(C3 &a)
+{
+    zero(a);
+#pragma unroll
+    for (int p = 0; p < 3; p++) a.r[p][p] = 1.0;
+}
+
+// C = op(A) op(B)
+template <bool DA, bool DB>
+__device__ inline C3 mm(const C3 &A, const C3 &B)
+{
+    C3 C;
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            double cr = 0.0, ci = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const double ar = DA ? A.r[k][p] : A.r[p][k], ai = DA ? -A.i[k][p] : A.i[p][k];
+                const double br = DB ? B.r[q][k] : B.r[k][q], bi = DB ? -B.i[q][k] : B.i[k][q];
+                cr = fma(ar, br, cr);
+                cr = fma(-ai, bi, cr);
+                ci = fma(ar, bi, ci);
+                ci = fma(ai, br, ci);
+            }
+            C.r[p][q] = cr;
+            C.i[p][q] = ci;
+        }
+    return C;
+}
+
+__device__ inline void addto(C3 &a, const C3 &b, double s = 1.0)
+{
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            a.r[p][q] += s * b.r[p][q];
+            a.i[p][q] += s * b.i[p][q];
+        }
+}
+
+// multiply by a complex scalar (cr + i ci)
+__device__ inline C3 cscale(const C3 &a, double cr, double ci)
+{
+    C3 b;
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            b.r[p][q] = a.r[p][q] * cr - a.i[p][q] * ci;
+            b.i[p][q] = a.r[p][q] * ci + a.i[p][q] * cr;
+        }
+    return b;
+}
+
+__device__ inline double retr(const C3 &a) { return a.r[0][0] + a.r[1][1] + a.r[2][2]; }
+__device__ inline double imtr(const C3 &a) { return a.i[0][0] + a.i[1][1] + a.i[2][2]; }
+
+// traceless Hermitian part of (Z + Z^+)/2
+__device__ inline C3 proj(const C3 &z)
+{
+    C3 h;
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            h.r[p][q] = 0.5 * (z.r[p][q] + z.r[q][p]);
+            h.i[p][q] = 0.5 * (z.i[p][q] - z.i[q][p]);
+        }
+    const double t = (h.r[0][0] + h.r[1][1] + h.r[2][2]) / 3.0;
+#pragma unroll
+    for (int p = 0; p < 3; p++) h.r[p][p] -= t;
+    return h;
+}
+
+// TAH(C) = -i/2 [(C - C^+) - Tr(C - C^+)/3]
+__device__ inline C3 tah(const C3 &c)
+{
+    C3 d;
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            d.r[p][q] = c.r[p][q] - c.r[q][p];
+            d.i[p][q] = c.i[p][q] + c.i[q][p];
+        }
+    const double ti = (d.i[0][0] + d.i[1][1] + d.i[2][2]) / 3.0;
+#pragma unroll
+    for (int p = 0; p < 3; p++) d.i[p][p] -= ti;
+    return cscale(d, 0.0, -0.5);
+}
+
+__device__ inline double fnorm2(const C3 &a)
+{
+    double s = 0.0;
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int q = 0; q < 3; q++) s += a.r[p][q] * a.r[p][q] + a.i[p][q] * a.i[p][q];
+    return s;
+}
+
+// exp(i X) for Hermitian X by Taylor series to convergence (||X|| < ~3)
+__device__ inline C3 expi(const C3 &X)
+{
+    C3 E, T;
+    eye(E);
+    eye(T);
+    for (int k = 1; k < 48; k++) {
+        T = mm<false, false>(T, X);
+        T = cscale(T, 0.0, 1.0 / k);  // T <- T (iX) / k
+        addto(E, T);
+        if (fnorm2(T) < 1e-36) break;
+    }
+    return E;
+}
+
+// sum_k (-i)^k/(k+1)! ad_X^k(G) = int_0^1 e^{-isX} G e^{isX} ds
+__device__ inline C3 avg_ad(const C3 &X, const C3 &G)
+{
+    C3 S = G, T = G;
+    double fac = 1.0;
+    for (int k = 1; k < 48; k++) {
+        C3 c = mm<false, false>(X, T);
+        addto(c, mm<false, false>(T, X), -1.0);
+        T = cscale(c, 0.0, -1.0);  // T <- -i [X, T]
+        fac /= (k + 1);
+        addto(S, T, fac);
+        if (fnorm2(T) * fac * fac < 1e-36) break;
+    }
+    return S;
+}
+
+// ---- field access (N = 3, links tiled by 64) -----------------------------
+template <typename R>
+__device__ inline C3 ld(const R *U, long l)
+{
+    const R *b = U + (size_t)(l >> 6) * (18 * 64) + (l & 63);
+    C3 m;
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            m.r[p][q] = (double)b[((p * 3 + q) * 2) * 64];
+            m.i[p][q] = (double)b[((p * 3 + q) * 2 + 1) * 64];
+        }
+    return m;
+}
+
+template <typename R>
+__device__ inline void st(R *U, long l, const C3 &m)
+{
+    R *b = U + (size_t)(l >> 6) * (18 * 64) + (l & 63);
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            b[((p * 3 + q) * 2) * 64] = (R)m.r[p][q];
+            b[((p * 3 + q) * 2 + 1) * 64] = (R)m.i[p][q];
+        }
+}
+
+// gradient buffer: 9 doubles per link (compact Hermitian: diag, then
+// (0,1), (0,2), (1,2) as re, im)
+__device__ inline void gadd(double *g, long l, const C3 &h)
+{
+    double *b = g + 9 * (size_t)l;
+    atomicAdd(b + 0, h.r[0][0]);
+    atomicAdd(b + 1, h.r[1][1]);
+    atomicAdd(b + 2, h.r[2][2]);
+    atomicAdd(b + 3, h.r[0][1]);
+    atomicAdd(b + 4, h.i[0][1]);
+    atomicAdd(b + 5, h.r[0][2]);
+    atomicAdd(b + 6, h.i[0][2]);
+    atomicAdd(b + 7, h.r[1][2]);
+    atomicAdd(b + 8, h.i[1][2]);
+}
+
+__device__ inline C3 gget(const double *g, long l)
+{
+    const double *b = g + 9 * (size_t)l;
+    C3 h;
+    h.r[0][0] = b[0];
+    h.r[1][1] = b[1];
+    h.r[2][2] = b[2];
+    h.i[0][0] = h.i[1][1] = h.i[2][2] = 0.0;
+    h.r[0][1] = h.r[1][0] = b[3];
+    h.i[0][1] = b[4];
+    h.i[1][0] = -b[4];
+    h.r[0][2] = h.r[2][0] = b[5];
+    h.i[0][2] = b[6];
+    h.i[2][0] = -b[6];
+    h.r[1][2] = h.r[2][1] = b[7];
+    h.i[1][2] = b[8];
+    h.i[2][1] = -b[8];
+    return h;
+}
+
+__device__ inline void gput(double *g, long l, const C3 &h)
+{
+    double *b = g + 9 * (size_t)l;
+    b[0] = h.r[0][0];
+    b[1] = h.r[1][1];
+    b[2] = h.r[2][2];
+    b[3] = h.r[0][1];
+    b[4] = h.i[0][1];
+    b[5] = h.r[0][2];
+    b[6] = h.i[0][2];
+    b[7] = h.r[1][2];
+    b[8] = h.i[1][2];
+}
+
+struct Sub {
+    int mu, par;
+    const float *w;  // MLP block: W1[kHid][nf], b1[kHid], w2[kHid], b2
+    int nf;          // number of features = planes not containing mu
+    double rho_max, delta;
+};
+
+// every forward quantity of one updated link
+struct Fwd {
+    C3 U, S, W, Lam, Lh, E;
+    double f[28], h[kHid];
+    double n, z, rho, theta, K;
+};
+
+template <typename R>
+__device__ inline void forward_link(const Geom &g, const R *U, const Sub &sb, long s, Fwd &F)
+{
+    const long V = g.V;
+    const int mu = sb.mu;
+    const long xp = g.nbr[(2 * mu) * V + s];
+    F.U = ld(U, mu * V + s);
+    zero(F.S);
+    C3 C;
+    zero(C);
+    int j = 0;
+    for (int nu = 0; nu < g.D; nu++) {
+        if (nu == mu) continue;
+        const long xpn = g.nbr[(2 * nu) * V + s];
+        const long xmn = g.nbr[(2 * nu + 1) * V + s];
+        const long xpmn = g.nbr[(2 * nu + 1) * V + xp];
+        const C3 a = ld(U, nu * V + xp), b = ld(U, mu * V + xpn), c = ld(U, nu * V + s);
+        addto(F.S, mm<false, true>(mm<false, true>(a, b), c));
+        const C3 d = ld(U, nu * V + xpmn), e = ld(U, mu * V + xmn), f = ld(U, nu * V + xmn);
+        addto(F.S, mm<false, false>(mm<true, true>(d, e), f));
+        for (int la = nu + 1; la < g.D; la++) {
+            if (la == mu) continue;
+            const long xnu = xpn, xla = g.nbr[(2 * la) * V + s];
+            const C3 P = mm<false, true>(mm<false, true>(mm<false, false>(c, ld(U, la * V + xnu)), ld(U, nu * V + xla)),
+                                         ld(U, la * V + s));
+            F.f[j++] = retr(P) / 3.0;
+            addto(C, P);
+        }
+    }
+    F.W = mm<false, false>(F.U, F.S);
+    F.Lam = tah(C);
+    F.n = sqrt(2.0 * retr(mm<false, false>(F.Lam, F.Lam)) + sb.delta * sb.delta);
+    F.Lh = cscale(F.Lam, 1.0 / F.n, 0.0);
+    double z = (double)sb.w[kHid * sb.nf + 2 * kHid];
+    for (int k = 0; k < kHid; k++) {
+        double a = (double)sb.w[kHid * sb.nf + k];
+        for (int q = 0; q < sb.nf; q++) a += (double)sb.w[k * sb.nf + q] * F.f[q];
+        F.h[k] = tanh(a);
+        z += (double)sb.w[kHid * sb.nf + kHid + k] * F.h[k];
+    }
+    F.z = z;
+    F.rho = sb.rho_max * tanh(z);
+    F.theta = F.rho * imtr(mm<false, false>(F.Lh, F.W));
+    F.K = retr(mm<false, false>(mm<false, false>(F.Lh, F.Lh), F.W));
+    F.E = expi(cscale(F.Lh, F.theta, 0.0));
+}
+
+// sites of parity p: device sites [0, V0) or [V0, V)
+__device__ inline bool site_of(const Geom &g, int par, long t, long &s)
+{
+    const long base = par ? g.V0 : 0, len = par ? g.V - g.V0 : g.V0;
+    s = base + t;
+    return t < len;
+}
+
+template <typename R>
+__global__ __launch_bounds__(kB) void k_forward(Geom g, const R *Uin, R *Uout, Sub sb, double *part)
+{
+    const long t = (long)blockIdx.x * kB + threadIdx.x;
+    double sum[1] = {0.0}, cmp[1] = {0.0};
+    long s;
+    if (site_of(g, sb.par, t, s)) {
+        Fwd F;
+        forward_link(g, Uin, sb, s, F);
+        st(Uout, sb.mu * g.V + s, mm<false, false>(F.E, F.U));
+        sum[0] = log1p(F.rho * F.K);
+    }
+    block_reduce_store<1>(sum, cmp, part + 2 * (size_t)blockIdx.x);
+}
+
+// Backward of one sub-layer.  g holds the gradient with respect to the
+// sub-layer's output on entry and with respect to its input on exit.
+// c_logj = dPhi/dlogJ (-1 for Phi = beta S - log J).  gw (may be null)
+// accumulates dPhi/dweights of the sub-layer's MLP block.
+template <typename R>
+__global__ __launch_bounds__(kB) void k_backward(Geom g, const R *U, Sub sb, double *gr, double c_logj, double *gw)
+{
+    const long t = (long)blockIdx.x * kB + threadIdx.x;
+    long s;
+    if (!site_of(g, sb.par, t, s)) return;
+    const long V = g.V;
+    const int mu = sb.mu;
+    const long l = mu * V + s;
+    Fwd F;
+    forward_link(g, U, sb, s, F);
+    const C3 gp = gget(gr, l);
+    // U' = E U
+    C3 gU = proj(mm<true, false>(F.E, mm<false, false>(gp, F.E)));
+    const double dtheta = 2.0 * retr(mm<false, false>(gp, F.Lh));
+    const C3 X = cscale(F.Lh, F.theta, 0.0);
+    C3 gLh = cscale(avg_ad(X, gp), F.theta, 0.0);
+    // theta = rho Im Tr(Lhat W)
+    double drho = dtheta * imtr(mm<false, false>(F.Lh, F.W));
+    C3 WmWd;
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            WmWd.r[p][q] = F.W.r[p][q] - F.W.r[q][p];
+            WmWd.i[p][q] = F.W.i[p][q] + F.W.i[q][p];
+        }
+    addto(gLh, cscale(WmWd, 0.0, -0.25 * dtheta * F.rho));  // dtheta rho (W - W^+)/(4i)
+    C3 GW = cscale(F.Lh, 0.0, -0.5 * dtheta * F.rho);     // dtheta rho (-i Lhat)/2
+    // log J = log(1 + rho K), K = ReTr(Lhat^2 W)
+    const double a = c_logj / (1.0 + F.rho * F.K);
+    drho += a * F.K;
+    C3 M = mm<false, false>(F.Lh, F.W);
+    addto(M, mm<false, false>(F.W, F.Lh));
+    C3 Md;
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            Md.r[p][q] = M.r[p][q] + M.r[q][p];
+            Md.i[p][q] = M.i[p][q] - M.i[q][p];
+        }
+    addto(gLh, Md, 0.25 * a * F.rho);
+    addto(GW, mm<false, false>(F.Lh, F.Lh), 0.5 * a * F.rho);
+    // rho = rho_max tanh(z), MLP
+    const double th = F.rho / sb.rho_max;
+    const double dz = drho * sb.rho_max * (1.0 - th * th);
+    double df[28];
+    for (int q = 0; q < sb.nf; q++) df[q] = 0.0;
+    const int nf = sb.nf;
+    for (int k = 0; k < kHid; k++) {
+        const double dh = dz * (double)sb.w[kHid * nf + kHid + k];
+        const double da = dh * (1.0 - F.h[k] * F.h[k]);
+        for (int q = 0; q < nf; q++) df[q] += da * (double)sb.w[k * nf + q];
+        if (gw) {
+            atomicAdd(gw + kHid * nf + kHid + k, dz * F.h[k]);
+            atomicAdd(gw + kHid * nf + k, da);
+            for (int q = 0; q < nf; q++) atomicAdd(gw + k * nf + q, da * F.f[q]);
+        }
+    }
+    if (gw) atomicAdd(gw + kHid * nf + 2 * kHid, dz);
+    // Lhat = Lam / n
+    const double tgl = 2.0 * retr(mm<false, false>(gLh, F.Lam));
+    C3 gLam = cscale(gLh, 1.0 / F.n, 0.0);
+    addto(gLam, F.Lam, -tgl / (F.n * F.n * F.n));
+    gLam = proj(gLam);
+    const C3 GC = cscale(gLam, 0.0, -1.0);  // Lam = TAH(C)
+    // own link through W = U S
+    addto(gU, proj(cscale(mm<false, false>(F.W, GW), 0.0, 1.0)));
+    gput(gr, l, gU);
+    // staple links: G_S = G_W U
+    const C3 GS = mm<false, false>(GW, F.U);
+    const long xp = g.nbr[(2 * mu) * V + s];
+    int j = 0;
+    for (int nu = 0; nu < g.D; nu++) {
+        if (nu == mu) continue;
+        const long xpn = g.nbr[(2 * nu) * V + s];
+        const long xmn = g.nbr[(2 * nu + 1) * V + s];
+        const long xpmn = g.nbr[(2 * nu + 1) * V + xp];
+        const C3 ca = ld(U, nu * V + xp), cb = ld(U, mu * V + xpn), cc = ld(U, nu * V + s);
+        // t1 = a b^+ c^+
+        const C3 abd = mm<false, true>(ca, cb);
+        const C3 t1 = mm<false, true>(abd, cc);
+        gadd(gr, nu * V + xp, proj(cscale(mm<false, false>(t1, GS), 0.0, 1.0)));
+        gadd(gr, mu * V + xpn, proj(cscale(mm<true, false>(cc, mm<false, false>(GS, abd)), 0.0, -1.0)));
+        gadd(gr, nu * V + s, proj(cscale(mm<false, false>(GS, t1), 0.0, -1.0)));
+        const C3 cd = ld(U, nu * V + xpmn), ce = ld(U, mu * V + xmn), cf = ld(U, nu * V + xmn);
+        // t2 = d^+ e^+ f
+        const C3 de = mm<true, true>(cd, ce);
+        const C3 fGS = mm<false, false>(cf, GS);
+        gadd(gr, nu * V + xpmn, proj(cscale(mm<false, true>(mm<true, false>(ce, fGS), cd), 0.0, -1.0)));
+        gadd(gr, mu * V + xmn, proj(cscale(mm<false, false>(fGS, de), 0.0, -1.0)));
+        gadd(gr, nu * V + xmn, proj(cscale(mm<false, false>(fGS, de), 0.0, 1.0)));
+        // plaquettes P = p1 p2 p3^+ p4^+ in planes (nu, la) not containing mu
+        for (int la = nu + 1; la < g.D; la++) {
+            if (la == mu) continue;
+            const long xla = g.nbr[(2 * la) * V + s];
+            const C3 p1 = cc, p2 = ld(U, la * V + xpn), p3 = ld(U, nu * V + xla), p4 = ld(U, la * V + s);
+            C3 GP = GC;
+            const double cf_ = df[j++] / 6.0;
+#pragma unroll
+            for (int p = 0; p < 3; p++) GP.r[p][p] += cf_;
+            const C3 p12 = mm<false, false>(p1, p2);
+            const C3 p123 = mm<false, true>(p12, p3);
+            const C3 P = mm<false, true>(p123, p4);
+            gadd(gr, nu * V + s, proj(cscale(mm<false, false>(P, GP), 0.0, 1.0)));
+            gadd(gr, la * V + xpn, proj(cscale(mm<false, false>(mm<false, true>(mm<false, true>(p2, p3), p4),
+                                                             mm<false, false>(GP, p1)),
+                                               0.0, 1.0)));
+            gadd(gr, nu * V + xla, proj(cscale(mm<true, false>(p4, mm<false, false>(GP, p123)), 0.0, -1.0)));
+            gadd(gr, la * V + s, proj(cscale(mm<false, false>(GP, P), 0.0, -1.0)));
+        }
+    }
+}
+
+// g_l = (beta / 2N) TAH(U_l sum staples): gradient of beta S_W
+template <typename R>
+__global__ __launch_bounds__(kB) void k_wilson_grad(Geom g, const R *U, double coef, double *gr, double *part)
+{
+    const long t = (long)blockIdx.x * kB + threadIdx.x;
+    double sum[1] = {0.0}, cmp[1] = {0.0};
+    if (t < g.V * g.D) {
+        const long V = g.V;
+        const int mu = (int)(t / V);
+        const long s = t - mu * V;
+        const long xp = g.nbr[(2 * mu) * V + s];
+        C3 S;
+        zero(S);
+        for (int nu = 0; nu < g.D; nu++) {
+            if (nu == mu) continue;
+            const long xpn = g.nbr[(2 * nu) * V + s];
+            const long xmn = g.nbr[(2 * nu + 1) * V + s];
+            const long xpmn = g.nbr[(2 * nu + 1) * V + xp];
+            addto(S, mm<false, true>(mm<false, true>(ld(U, nu * V + xp), ld(U, mu * V + xpn)), ld(U, nu * V + s)));
+            addto(S, mm<false, false>(mm<true, true>(ld(U, nu * V + xpmn), ld(U, mu * V + xmn)), ld(U, nu * V + xmn)));
+        }
+        const C3 W = mm<false, false>(ld(U, t), S);
+        gput(gr, t, cscale(tah(W), coef, 0.0));
+        sum[0] = retr(W);
+    }
+    block_reduce_store<1>(sum, cmp, part + 2 * (size_t)blockIdx.x);
+}
+
+// P <- P - eps c g (compact momenta, tile 64)
+template <typename R>
+__global__ void k_kick_g(Geom g, R *P, const double *gr, double eps_c)
+{
+    const long l = (long)blockIdx.x * kB + threadIdx.x;
+    if (l >= g.V * g.D) return;
+    const double *b = gr + 9 * (size_t)l;
+    R *p = P + (size_t)(l >> 6) * (9 * 64) + (l & 63);
+    // momentum compact order: diag 0..2, (0,1), (0,2), (1,2) as (re, im): same
+#pragma unroll
+    for (int c = 0; c < 9; c++) p[c * 64] = (R)((double)p[c * 64] - eps_c * b[c]);
+}
+
+}  // namespace flow
+
+// ---------------------------------------------------------------- launchers
+static flow::Sub to_sub(const FlowSub &f) { return flow::Sub{f.mu, f.par, f.w, f.nf, f.rho_max, f.delta}; }
+
+static unsigned par_blocks(const Geom &g, int par)
+{
+    const long len = par ? g.V - g.V0 : g.V0;
+    return (unsigned)((len + flow::kB - 1) / flow::kB);
+}
+
+long flow_forward_partials(const Geom &g, int par) { return par_blocks(g, par); }
+
+hipError_t launch_flow_forward(const Geom &g, const void *Uin, void *Uout, const FlowSub &f, int mode, double *part,
+                               hipStream_t st)
+{
+    const unsigned nb = par_blocks(g, f.par);
+    if (mode == kModeF32)
+        hipLaunchKernelGGL(flow::k_forward<float>, dim3(nb), dim3(flow::kB), 0, st, g, (const float *)Uin, (float *)Uout,
+                           to_sub(f), part);
+    else
+        hipLaunchKernelGGL(flow::k_forward<double>, dim3(nb), dim3(flow::kB), 0, st, g, (const double *)Uin,
+                           (double *)Uout, to_sub(f), part);
+    return hipGetLastError();
+}
+
+hipError_t launch_flow_backward(const Geom &g, const void *U, const FlowSub &f, int mode, double *gr, double c_logj,
+                                double *gw, hipStream_t st)
+{
+    const unsigned nb = par_blocks(g, f.par);
+    if (mode == kModeF32)
+        hipLaunchKernelGGL(flow::k_backward<float>, dim3(nb), dim3(flow::kB), 0, st, g, (const float *)U, to_sub(f), gr,
+                           c_logj, gw);
+    else
+        hipLaunchKernelGGL(flow::k_backward<double>, dim3(nb), dim3(flow::kB), 0, st, g, (const double *)U, to_sub(f),
+                           gr, c_logj, gw);
+    return hipGetLastError();
+}
+
+hipError_t launch_wilson_grad(const Geom &g, const void *U, double coef, int mode, double *gr, double *part,
+                              long *np, hipStream_t st)
+{
+    const long nl = g.V * g.D;
+    const unsigned nb = (unsigned)((nl + flow::kB - 1) / flow::kB);
+    *np = nb;
+    if (mode == kModeF32)
+        hipLaunchKernelGGL(flow::k_wilson_grad<float>, dim3(nb), dim3(flow::kB), 0, st, g, (const float *)U, coef, gr,
+                           part);
+    else
+        hipLaunchKernelGGL(flow::k_wilson_grad<double>, dim3(nb), dim3(flow::kB), 0, st, g, (const double *)U, coef, gr,
+                           part);
+    return hipGetLastError();
+}
+
+hipError_t launch_kick_g(const Geom &g, void *P, const double *gr, double eps_c, int mode, hipStream_t st)
+{
+    const long nl = g.V * g.D;
+    const unsigned nb = (unsigned)((nl + flow::kB - 1) / flow::kB);
+    if (mode == kModeF32)
+        hipLaunchKernelGGL(flow::k_kick_g<float>, dim3(nb), dim3(flow::kB), 0, st, g, (float *)P, gr, eps_c);
+    else hipLaunchKernelGGL(flow::k_kick_g<double>, dim3(nb), dim3(flow::kB), 0, st, g, (double *)P, gr, eps_c);
+    return hipGetLastError();
+}
+
+}  // namespace askja

@@ -128,4 +128,22 @@ hipError_t launch_mom_scale(const Geom &g, void *P, double f, int mode, hipStrea
 hipError_t launch_project(const Geom &g, void *U, int mode, int haar, uint64_t seed, uint64_t stream,
                           double *scratch, long scratch_doubles, hipStream_t st);
 
+// ---- learned field transformation (flow.hip, N = 3) -------------------------
+struct FlowSub {
+    int mu, par;     // updated links: direction mu on sites of parity par
+    const float *w;  // device MLP block of the sub-layer
+    int nf;          // features = planes not containing mu
+    double rho_max, delta;
+};
+long flow_forward_partials(const Geom &g, int par);
+hipError_t launch_flow_forward(const Geom &g, const void *Uin, void *Uout, const FlowSub &f, int mode, double *part,
+                               hipStream_t st);
+hipError_t launch_flow_backward(const Geom &g, const void *U, const FlowSub &f, int mode, double *gr, double c_logj,
+                                double *gw, hipStream_t st);
+// gr = coef * TAH(U sum staples) (9 doubles per link), part: ReTr W per block
+hipError_t launch_wilson_grad(const Geom &g, const void *U, double coef, int mode, double *gr, double *part,
+                              long *np, hipStream_t st);
+// P <- P - eps_c * gr
+hipError_t launch_kick_g(const Geom &g, void *P, const double *gr, double eps_c, int mode, hipStream_t st);
+
 }  // namespace askja

@@ -1,0 +1,214 @@
+"""Learned proposal: training of the gauge-equivariant field transformation
+U = F(V) (askja_amd/csrc/flow.hip) and field-transformation HMC (FTHMC) runs.
+
+The transformation's forward, log-Jacobian and backward are the library's HIP
+kernels; ``FlowAction`` wraps them in a ``torch.autograd.Function`` so that
+torch's optimiser can train the MLP weights (which live in a CUDA tensor and
+are handed to the kernels by device pointer).
+
+Training objective.  With q = the distribution of plain-HMC configurations V
+at a coupling beta_q and p the target at beta,
+    KL(q || p~) = E_q[beta S_W(F(V)) - log J(V)] - beta_q E_q[S_W(V)] + const,
+where p~(V) ~ exp(-beta S_W(F(V))) J(V) is the distribution FTHMC samples in
+V.  Minimising the first term makes the V-chain at beta behave like plain HMC
+at beta_q, while U = F(V) is distributed exactly as exp(-beta S_W(U)) (the
+Metropolis test includes log J).
+"""
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+if __package__ in (None, ""):
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+import askja_amd as A  # noqa: E402
+
+
+def sweep_masks(dim, layers):
+    """Sub-layers of `layers` full sweeps over (direction, parity); every
+    sweep shares one MLP weight block."""
+    mu, par, blk = [], [], []
+    for layer in range(layers):
+        for m in range(dim):
+            for p in (0, 1):
+                mu.append(m)
+                par.append(p)
+                blk.append(layer)
+    return mu, par, blk
+
+
+def init_weights(nblocks, bsize, nf, seed=0, scale=0.1):
+    """Small random W1, b1; w2 and b2 zero, so the untrained map is F = id."""
+    rng = np.random.default_rng(seed)
+    w = np.zeros(nblocks * bsize, np.float32)
+    for b in range(nblocks):
+        o = b * bsize
+        w[o: o + 32 * nf] = rng.standard_normal(32 * nf) * scale
+        w[o + 32 * nf: o + 32 * nf + 32] = rng.standard_normal(32) * scale
+    return w
+
+
+def flow_action_function():
+    import torch
+
+    class FlowAction(torch.autograd.Function):
+        """Phi(w) = beta S_W(F_w(V)) - log J_w(V) for the configuration V held by
+        `lat`; forward and backward run the library's HIP kernels."""
+
+        @staticmethod
+        def forward(ctx, w, lat, beta):
+            torch.cuda.synchronize()
+            lat.flow_set_weights(w.detach().contiguous())
+            s, lj = lat.flow_gradient(beta, wgrad=True)
+            g = torch.from_numpy(lat.flow_weight_gradient()).to(device=w.device, dtype=w.dtype)
+            ctx.save_for_backward(g)
+            return torch.tensor(beta * s - lj, dtype=torch.float64, device=w.device)
+
+        @staticmethod
+        def backward(ctx, go):
+            (g,) = ctx.saved_tensors
+            return go.to(g.dtype) * g, None, None
+
+    return FlowAction
+
+
+def plain_chain(lat, beta, dt, ndrift, level, seed, first, n, measure=True):
+    """n plain HMC trajectories; returns (|P| series, acceptance, seconds)."""
+    series, acc = [], 0
+    lat.sync()
+    t0 = time.perf_counter()
+    for k in range(n):
+        r = lat.hmc_trajectory(beta, dt, ndrift, level, seed, first + k, reunit=True)
+        acc += r.accepted
+        if measure:
+            series.append(abs(lat.polyakov()))
+    lat.sync()
+    return np.array(series), acc / max(n, 1), time.perf_counter() - t0
+
+
+def flow_chain(lat, beta, dt, ndrift, level, seed, first, n):
+    """n FTHMC trajectories; |P| measured on U = F(V)."""
+    series, acc = [], 0
+    lat.sync()
+    t0 = time.perf_counter()
+    for k in range(n):
+        r = lat.flow_hmc_trajectory(beta, dt, ndrift, level, seed, first + k, reunit=True)
+        acc += r.accepted
+        series.append(abs(lat.flow_polyakov()))
+    lat.sync()
+    return np.array(series), acc / max(n, 1), time.perf_counter() - t0
+
+
+def tau_int(x, S=1.5):
+    """tau = 1 + 2 sum rho(t) with Wolff's automatic window."""
+    x = np.asarray(x, np.float64)
+    n = len(x)
+    d = x - x.mean()
+    var = np.dot(d, d) / n
+    if n < 8 or var <= 0:
+        return float("nan")
+    f = np.fft.rfft(d, 2 * n)
+    acf = np.fft.irfft(f * np.conj(f))[:n] / (n - np.arange(n)) / var
+    tint = 0.5
+    for W in range(1, n // 2):
+        tint += acf[W]
+        tau = S / math.log((2 * tint + 1) / (2 * tint - 1)) if tint > 0.5 else 1e-9
+        if math.exp(-W / tau) - tau / math.sqrt(W * n) < 0:
+            return 2.0 * tint
+    return 2.0 * tint
+
+
+def train(lat, configs, beta, w0, epochs=2, lr=2e-3, log=None):
+    """Adam on E_V[beta S_W(F(V)) - log J(V)] over the training configurations."""
+    import torch
+
+    fa = flow_action_function()
+    w = torch.tensor(w0, device="cuda", dtype=torch.float32, requires_grad=True)
+    opt = torch.optim.Adam([w], lr=lr)
+    losses = []
+    for ep in range(epochs):
+        tot = 0.0
+        for V in configs:
+            lat.set_links(V)
+            opt.zero_grad()
+            loss = fa.apply(w, lat, beta)
+            loss.backward()
+            opt.step()
+            tot += float(loss)
+        losses.append(tot / len(configs))
+        if log:
+            log(f"[learned] epoch {ep + 1}/{epochs} loss {losses[-1]:.3f}")
+    return w.detach().cpu().numpy(), losses
+
+
+def main():
+    import argparse
+
+    ap = argparse.ArgumentParser(description="learned-proposal HMC vs plain HMC (SU(3), D=4)")
+    ap.add_argument("--extent", type=int, default=16)
+    ap.add_argument("--beta", type=float, default=6.0)
+    ap.add_argument("--beta-train", type=float, default=5.7)
+    ap.add_argument("--layers", type=int, default=1)
+    ap.add_argument("--ntrain", type=int, default=500)
+    ap.add_argument("--epochs", type=int, default=2)
+    ap.add_argument("--ntraj", type=int, default=400)
+    ap.add_argument("--therm", type=int, default=200)
+    ap.add_argument("--dt", type=float, default=0.05)
+    ap.add_argument("--ndrift", type=int, default=20)
+    ap.add_argument("--level", type=int, default=8)
+    ap.add_argument("--seed", type=int, default=11)
+    args = ap.parse_args()
+    # torch first: the library then binds to torch's HIP runtime, so both
+    # share one runtime in this process
+    import torch
+
+    torch.cuda.init()
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    ext = [args.extent] * 4
+    lat = A.Lattice(3, ext, A.FP64)
+    # training set: plain HMC at beta_train
+    lat.haar(args.seed, 3)
+    plain_chain(lat, args.beta_train, args.dt, args.ndrift, args.level, args.seed, 0, args.therm, measure=False)
+    configs = []
+    for k in range(args.ntrain):
+        plain_chain(lat, args.beta_train, args.dt, args.ndrift, args.level, args.seed, 10**6 + 2 * k, 2, measure=False)
+        configs.append(lat.links())
+    log(f"[learned] {len(configs)} training configurations at beta {args.beta_train}")
+    mu, par, blk = sweep_masks(4, args.layers)
+    bsize = lat.flow_define(mu, par, blk, args.layers, 0.3, 1e-3)
+    nf = 3
+    w0 = init_weights(args.layers, bsize, nf, seed=args.seed)
+    t0 = time.perf_counter()
+    w, losses = train(lat, configs, args.beta, w0, epochs=args.epochs, log=log)
+    t_train = time.perf_counter() - t0
+    # chains at beta from a thermalised start
+    lat.set_links(configs[-1])
+    plain_chain(lat, args.beta, args.dt, args.ndrift, args.level, args.seed, 2 * 10**6, args.therm, measure=False)
+    start = lat.links()
+    res = {}
+    pl, acc_p, t_p = plain_chain(lat, args.beta, args.dt, args.ndrift, args.level, args.seed, 3 * 10**6, args.ntraj)
+    res["plain"] = {"traj_per_s": args.ntraj / t_p, "acceptance": acc_p, "tau": tau_int(pl),
+                    "mean_abs_polyakov": float(pl.mean())}
+    for name, weights in (("flow_untrained", w0), ("flow_trained", w)):
+        lat.set_links(start)
+        lat.flow_set_weights(weights)
+        fl, acc_f, t_f = flow_chain(lat, args.beta, args.dt, args.ndrift, args.level, args.seed, 4 * 10**6, args.ntraj)
+        res[name] = {"traj_per_s": args.ntraj / t_f, "acceptance": acc_f, "tau": tau_int(fl),
+                     "mean_abs_polyakov": float(fl.mean())}
+    for k, v in res.items():
+        v["indep_configs_per_s"] = v["traj_per_s"] / v["tau"]
+    out = {"workload": f"SU(3) D=4 {args.extent}^4 beta={args.beta}, FTHMC {args.layers} sweep layer(s) "
+                       f"({len(mu)} masked sub-layers), trained on {args.ntrain} plain-HMC configs at "
+                       f"beta={args.beta_train}, dt={args.dt}, {args.ndrift} MD steps",
+           "train_seconds": t_train, "loss": losses, "results": res,
+           "gain_learned_vs_plain": res["flow_trained"]["indep_configs_per_s"] / res["plain"]["indep_configs_per_s"],
+           "gain_trained_vs_untrained": res["flow_trained"]["indep_configs_per_s"]
+           / res["flow_untrained"]["indep_configs_per_s"]}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -190,6 +190,45 @@ int askja_hmc_trajectory(askja_ctx *ctx, const askja_hmc_params *p, uint64_t tra
  * time(NULL), sun.cpp:1277-1287). */
 int askja_uniform(askja_ctx *ctx, uint64_t seed, uint64_t counter, double *out);
 
+/* ---- learned field transformation (no reference counterpart; SU(3), D >= 3) ----
+ * U = F(V): nsub masked sub-layers; sub-layer k updates the links of
+ * direction mu[k] on the sites of parity parity[k] as
+ *   U'_l = exp(i theta Lhat) U_l,  theta = rho Im Tr(Lhat U_l S_l),
+ * with S_l the staple sum, Lhat the normalised traceless anti-Hermitian part of
+ * the plaquettes at x in the planes without mu (both frozen in the sub-layer),
+ * rho = rho_max tanh(MLP(ReTr of those plaquettes / 3)); the MLP (hidden 32,
+ * FP32) of sub-layer k is weight block block[k].  log det J = sum over updated
+ * links of log(1 + rho ReTr(Lhat^2 U_l S_l)).  Requires even extents and
+ * 0 < rho_max < 1/(D-1) (invertibility).  Block layout: W1[32][nf], b1[32],
+ * w2[32], b2 with nf = (D-1)(D-2)/2. */
+int askja_flow_define(askja_ctx *ctx, int nsub, const int *mu, const int *parity, const int *block, int nblocks,
+                      double rho_max, double delta);
+/* Floats per weight block. */
+int askja_flow_block_size(askja_ctx *ctx, int *bsize);
+/* All nblocks * bsize weights, from host memory or (on_device != 0) device memory. */
+int askja_flow_set_weights(askja_ctx *ctx, const float *w, int on_device);
+/* U = F(V) of the current links V (kept in the context), log det J. */
+int askja_flow_apply(askja_ctx *ctx, double *logj);
+/* host AoS buffer <- U = F(V) of the last apply / gradient call */
+int askja_flow_download(askja_ctx *ctx, double *aos);
+/* Phi = beta S_W(F(V)) - log J(V): its Lie gradient with respect to V (kept
+ * for askja_flow_gradient_download) by the hand-written backward kernels, with
+ * segment checkpointing (every 8th sub-layer input is stored, the rest
+ * recomputed), and with wgrad != 0 the gradient with respect to the weights.
+ * action = S_W(F(V)) (EnergyMag normalisation), logj = log det J(V). */
+int askja_flow_gradient(askja_ctx *ctx, double beta, int wgrad, double *action, double *logj);
+/* host AoS buffer <- Hermitian traceless g_l = sum_a T^a d/dt Phi(e^{i t T^a} V_l) */
+int askja_flow_gradient_download(askja_ctx *ctx, double *aos);
+/* host buffer <- dPhi/dweights (nblocks * bsize doubles) of the last gradient call with wgrad */
+int askja_flow_weight_gradient(askja_ctx *ctx, double *out);
+/* One field-transformation HMC trajectory in V: H = beta S_W(F(V)) - log J(V)
+ * + beta/(2N) sum Tr P^2, kick-drift-kick with the flow force, exact
+ * Metropolis; on reject V returns to its value before the trajectory. */
+int askja_flow_hmc_trajectory(askja_ctx *ctx, const askja_hmc_params *p, uint64_t traj_counter,
+                              askja_hmc_result *res);
+/* askja_wilson_lines of U = F(V). */
+int askja_flow_wilson_lines(askja_ctx *ctx, int dir, double *out);
+
 /* ---- plumbing ------------------------------------------------------------------------ */
 /* Synchronise the context's stream. */
 int askja_sync(askja_ctx *ctx);

@@ -1,0 +1,105 @@
+"""CPU restatement (numpy, FP64) of the learned field transformation F(V).
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP kernels in
+askja_amd/csrc/flow.hip.  The reference has no counterpart (BASELINE.json's
+learned-proposal addition); the definitions follow the sub-layer stated there:
+
+  for every link l = (x, mu) of the sub-layer's mask (direction mu, sites of
+  one parity), with every other link frozen:
+    S     = sum_{nu != mu} [U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+
+                            + U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu)]
+    W     = U_l S
+    P_j   = plaquettes at x in the planes (nu < la) without mu
+    Lam   = TAH(sum_j P_j), Lhat = Lam / sqrt(2 Tr Lam^2 + delta^2)
+    rho   = rho_max tanh(b2 + w2 . tanh(W1 f + b1)),  f_j = ReTr P_j / 3
+    theta = rho Im Tr(Lhat W)
+    U'_l  = expm(i theta Lhat) U_l,   log J += log(1 + rho ReTr(Lhat^2 W))
+
+Weight block layout: W1[32][nf], b1[32], w2[32], b2.  Fields are the AoS
+arrays [V, D, 3, 3] complex128 of the reference's U-*.conf order; geometry
+from the Oracle (structs.h:169-218 restated).
+"""
+import numpy as np
+import scipy.linalg
+
+HID = 32
+
+
+def tah(c):
+    d = c - c.conj().T
+    d = d - np.trace(d) / 3 * np.eye(3)
+    return -0.5j * d
+
+
+def parity(ext, i):
+    return sum(np.unravel_index(i, ext)) % 2
+
+
+def sub_layer(o, U, mu, par, w, nf, rho_max, delta):
+    """One masked sub-layer applied to U (copied); returns (U', log J)."""
+    D, V = o.dim, o.V
+    out = U.copy()
+    logj = 0.0
+    W1 = w[: HID * nf].reshape(HID, nf).astype(np.float64)
+    b1 = w[HID * nf: HID * nf + HID].astype(np.float64)
+    w2 = w[HID * nf + HID: HID * nf + 2 * HID].astype(np.float64)
+    b2 = float(w[HID * nf + 2 * HID])
+    for x in range(V):
+        if parity(o.ext, x) != par:
+            continue
+        xp = o.getplus(x, mu)
+        S = np.zeros((3, 3), complex)
+        C = np.zeros((3, 3), complex)
+        f = []
+        for nu in range(D):
+            if nu == mu:
+                continue
+            xpn, xmn = o.getplus(x, nu), o.getminus(x, nu)
+            xpmn = o.getminus(xp, nu)
+            S += U[xp, nu] @ U[xpn, mu].conj().T @ U[x, nu].conj().T
+            S += U[xpmn, nu].conj().T @ U[xmn, mu].conj().T @ U[xmn, nu]
+            for la in range(nu + 1, D):
+                if la == mu:
+                    continue
+                xla = o.getplus(x, la)
+                P = U[x, nu] @ U[xpn, la] @ U[xla, nu].conj().T @ U[x, la].conj().T
+                f.append(np.trace(P).real / 3)
+                C += P
+        Wl = U[x, mu] @ S
+        Lam = tah(C)
+        n = np.sqrt(2 * np.trace(Lam @ Lam).real + delta * delta)
+        Lh = Lam / n
+        z = b2 + w2 @ np.tanh(W1 @ np.array(f) + b1)
+        rho = rho_max * np.tanh(z)
+        theta = rho * np.trace(Lh @ Wl).imag
+        K = np.trace(Lh @ Lh @ Wl).real
+        out[x, mu] = scipy.linalg.expm(1j * theta * Lh) @ U[x, mu]
+        logj += np.log1p(rho * K)
+    return out, logj
+
+
+def forward(o, V, subs, weights, bsize, nf, rho_max, delta):
+    """subs: list of (mu, parity, block).  Returns (U = F(V), log J)."""
+    U = V.copy()
+    lj = 0.0
+    for mu, par, blk in subs:
+        U, l = sub_layer(o, U, mu, par, weights[blk * bsize:(blk + 1) * bsize], nf, rho_max, delta)
+        lj += l
+    return U, lj
+
+
+def energy_mag(o, U):
+    """EnergyMag (sun.cpp:507-537) of an AoS field."""
+    s = 0.0
+    for x in range(o.V):
+        for j in range(o.dim):
+            for k in range(j + 1, o.dim):
+                P = U[x, j] @ U[o.getplus(x, j), k] @ U[o.getplus(x, k), j].conj().T @ U[x, k].conj().T
+                s += 1 - np.trace(P).real / 3
+    return s
+
+
+def phi(o, V, subs, weights, bsize, nf, rho_max, delta, beta):
+    """Phi = beta S_W(F(V)) - log J(V)."""
+    U, lj = forward(o, V, subs, weights, bsize, nf, rho_max, delta)
+    return beta * energy_mag(o, U) - lj

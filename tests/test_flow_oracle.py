@@ -1,0 +1,73 @@
+"""Pins the CPU restatement of the learned field transformation
+(oracle/flow_oracle.py) by mathematics: its log-Jacobian against a
+brute-force dense Jacobian of every updated link, gauge equivariance, and the
+rho = 0 identity.  (The reference has no counterpart to compare with.)"""
+import numpy as np
+import pytest
+import scipy.linalg
+
+from conftest import random_su
+
+
+@pytest.fixture(scope="module")
+def fo():
+    from oracle import flow_oracle
+    return flow_oracle
+
+
+def _setup(oracle, ext, seed, nblocks=1):
+    o = oracle.Oracle(3, ext)
+    rng = np.random.default_rng(seed)
+    V = random_su(oracle, rng, (o.V, o.dim), 3)
+    nf = (o.dim - 1) * (o.dim - 2) // 2
+    bsize = 32 * nf + 65
+    w = (rng.standard_normal(nblocks * bsize) * 0.5).astype(np.float32)
+    return o, rng, V, nf, bsize, w
+
+
+def test_logj_matches_dense_jacobian(oracle, fo):
+    o, rng, V, nf, bsize, w = _setup(oracle, [4, 2, 2, 2], 1)
+    gens = oracle.generators(3)
+    mu, par = 1, 0
+    Up, lj = fo.sub_layer(o, V, mu, par, w, nf, 0.3, 1e-3)
+    eps = 1e-5
+    total = 0.0
+    for x in range(o.V):
+        if fo.parity(o.ext, x) != par:
+            continue
+        J = np.zeros((8, 8))
+        for a in range(8):
+            outs = []
+            for sgn in (1, -1):
+                W = V.copy()
+                W[x, mu] = scipy.linalg.expm(1j * sgn * eps * gens[a]) @ V[x, mu]
+                outs.append(fo.sub_layer(o, W, mu, par, w, nf, 0.3, 1e-3)[0][x, mu])
+            L = scipy.linalg.logm(outs[0] @ np.linalg.inv(outs[1])) / (2j * eps)
+            for c in range(8):
+                J[c, a] = 2 * np.trace(gens[c] @ L).real
+        total += np.log(np.linalg.det(J))
+    assert abs(total - lj) < 1e-6 * max(1.0, abs(lj)), (total, lj)
+    assert abs(lj) > 1e-3  # the test is not vacuous
+
+
+def test_gauge_equivariance(oracle, fo):
+    o, rng, V, nf, bsize, w = _setup(oracle, [4, 2, 2, 4], 2, nblocks=2)
+    subs = [(0, 0, 0), (2, 1, 1), (3, 0, 0)]
+    U, lj = fo.forward(o, V, subs, w, bsize, nf, 0.3, 1e-3)
+    g = random_su(oracle, rng, (o.V,), 3)
+    Vg = np.empty_like(V)
+    for x in range(o.V):
+        for d in range(o.dim):
+            Vg[x, d] = g[x] @ V[x, d] @ g[o.getplus(x, d)].conj().T
+    Ug, ljg = fo.forward(o, Vg, subs, w, bsize, nf, 0.3, 1e-3)
+    for x in range(o.V):
+        for d in range(o.dim):
+            assert np.allclose(Ug[x, d], g[x] @ U[x, d] @ g[o.getplus(x, d)].conj().T, atol=1e-12)
+    assert abs(ljg - lj) < 1e-11
+
+
+def test_zero_coefficients_are_identity(oracle, fo):
+    o, rng, V, nf, bsize, w = _setup(oracle, [2, 2, 2, 2], 3)
+    w[:] = 0
+    U, lj = fo.forward(o, V, [(0, 0, 0), (1, 1, 0)], w, bsize, nf, 0.3, 1e-3)
+    assert np.array_equal(U, V) and lj == 0.0
