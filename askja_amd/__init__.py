@@ -35,7 +35,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("ASKJA_LIB") or os.path.join(_HERE, "libaskja.so")
 
-FP64, FP64_EXACT, FP32 = 0, 1, 2
+FP64, FP64_EXACT, FP32, BF16X3 = 0, 1, 2, 3
 _ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ENOMEM", -4: "EUNSUP", -5: "ESTICKY", -6: "EIO"}
 
 # every exported entry point: name -> (restype, argtypes)

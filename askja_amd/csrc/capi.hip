@@ -82,7 +82,7 @@ int fail(askja_ctx *c, int code, const char *fmt, ...)
         if ((c)->sticky) return ASKJA_ESTICKY;                                 \
     } while (0)
 
-size_t elem_bytes(const askja_ctx *c) { return c->mode == kModeF32 ? 4 : 8; }
+size_t elem_bytes(const askja_ctx *c) { return f32_mode(c->mode) ? 4 : 8; }
 
 int ensure_part(askja_ctx *c, size_t nd)
 {
@@ -210,7 +210,7 @@ int askja_create(int nc, int dim, const int *extents, int precision, askja_ctx *
     if (!out || !extents) return ASKJA_EINVAL;
     *out = nullptr;
     if (nc < 1 || nc > 64 || dim < 1 || dim > 32) return ASKJA_EUNSUP;
-    if (precision < ASKJA_FP64 || precision > ASKJA_FP32) return ASKJA_EUNSUP;
+    if (precision < ASKJA_FP64 || precision > ASKJA_BF16X3) return ASKJA_EUNSUP;
     long V = 1;
     for (int i = 0; i < dim; i++) {
         if (extents[i] < 1) return ASKJA_EINVAL;
@@ -221,7 +221,10 @@ int askja_create(int nc, int dim, const int *extents, int precision, askja_ctx *
     c->nc = nc;
     c->dim = dim;
     c->prec = precision;
-    c->mode = precision == ASKJA_FP64 ? kModeF64 : precision == ASKJA_FP64_EXACT ? kModeF64Exact : kModeF32;
+    c->mode = precision == ASKJA_FP64         ? kModeF64
+              : precision == ASKJA_FP64_EXACT ? kModeF64Exact
+              : precision == ASKJA_FP32       ? kModeF32
+                                              : kModeBF16x3;
     c->V = V;
     c->NL = V * dim;
     for (int i = 0; i < dim; i++) c->ext[i] = extents[i];
@@ -843,6 +846,7 @@ int flow_logj(askja_ctx *c, double *logj)
 // flow_g (and the MLP weights into flow_gw when wgrad); returns S_W(F(V)).
 int flow_gradient(askja_ctx *c, double beta, bool wgrad, double *action)
 {
+    // action == nullptr: no host synchronisation (inside an MD trajectory)
     int rc = flow_forward(c, true);
     if (rc) return rc;
     const long nl = c->NL;
@@ -868,9 +872,11 @@ int flow_gradient(askja_ctx *c, double beta, bool wgrad, double *action)
                                            c->st));
         }
     }
-    double h[2];
-    if ((rc = read_res(c, h, 2))) return rc;
-    *action = (double)n_plaquettes(c) - pair_val(h, 0) / (4.0 * c->nc);
+    if (action) {
+        double h[2];
+        if ((rc = read_res(c, h, 2))) return rc;
+        *action = (double)n_plaquettes(c) - pair_val(h, 0) / (4.0 * c->nc);
+    }
     return ASKJA_OK;
 }
 
@@ -1062,7 +1068,7 @@ int askja_flow_hmc_trajectory(askja_ctx *c, const askja_hmc_params *p, uint64_t 
     lj1 = lj0;
     for (int t = 1; t <= p->ndrift; t++) {
         HIPCHK(c, launch_drift(c->geom, c->links[c->cur], c->mom, p->dt, p->level, ci, c->mode, c->st));
-        if ((rc = flow_gradient(c, p->beta, false, &s1))) return rc;
+        if ((rc = flow_gradient(c, p->beta, false, t == p->ndrift ? &s1 : nullptr))) return rc;
         HIPCHK(c, launch_kick_g(c->geom, c->mom, c->flow_g, (t < p->ndrift ? 1.0 : 0.5) * p->dt * cf, c->mode, c->st));
     }
     if ((rc = flow_logj(c, &lj1))) return rc;

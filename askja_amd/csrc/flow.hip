@@ -520,7 +520,7 @@ hipError_t launch_flow_forward(const Geom &g, const void *Uin, void *Uout, const
                                hipStream_t st)
 {
     const unsigned nb = par_blocks(g, f.par);
-    if (mode == kModeF32)
+    if (f32_mode(mode))
         hipLaunchKernelGGL(flow::k_forward<float>, dim3(nb), dim3(flow::kB), 0, st, g, (const float *)Uin, (float *)Uout,
                            to_sub(f), part);
     else
@@ -533,7 +533,7 @@ hipError_t launch_flow_backward(const Geom &g, const void *U, const FlowSub &f, 
                                 double *gw, hipStream_t st)
 {
     const unsigned nb = par_blocks(g, f.par);
-    if (mode == kModeF32)
+    if (f32_mode(mode))
         hipLaunchKernelGGL(flow::k_backward<float>, dim3(nb), dim3(flow::kB), 0, st, g, (const float *)U, to_sub(f), gr,
                            c_logj, gw);
     else
@@ -548,7 +548,7 @@ hipError_t launch_wilson_grad(const Geom &g, const void *U, double coef, int mod
     const long nl = g.V * g.D;
     const unsigned nb = (unsigned)((nl + flow::kB - 1) / flow::kB);
     *np = nb;
-    if (mode == kModeF32)
+    if (f32_mode(mode))
         hipLaunchKernelGGL(flow::k_wilson_grad<float>, dim3(nb), dim3(flow::kB), 0, st, g, (const float *)U, coef, gr,
                            part);
     else
@@ -561,7 +561,7 @@ hipError_t launch_kick_g(const Geom &g, void *P, const double *gr, double eps_c,
 {
     const long nl = g.V * g.D;
     const unsigned nb = (unsigned)((nl + flow::kB - 1) / flow::kB);
-    if (mode == kModeF32)
+    if (f32_mode(mode))
         hipLaunchKernelGGL(flow::k_kick_g<float>, dim3(nb), dim3(flow::kB), 0, st, g, (float *)P, gr, eps_c);
     else hipLaunchKernelGGL(flow::k_kick_g<double>, dim3(nb), dim3(flow::kB), 0, st, g, (double *)P, gr, eps_c);
     return hipGetLastError();

@@ -28,6 +28,9 @@ ASKJA_TEAM_DECL(16)
 ASKJA_TEAM_DECL(32)
 ASKJA_TEAM_DECL(64)
 #undef ASKJA_TEAM_DECL
+hipError_t mfma_kick_16(const KickArgs &a, bool drift, hipStream_t st);
+hipError_t mfma_kick_32(const KickArgs &a, bool drift, hipStream_t st);
+hipError_t mfma_kick_64(const KickArgs &a, bool drift, hipStream_t st);
 
 namespace {
 
@@ -565,6 +568,12 @@ long kick_partials(const Geom &g)
 hipError_t launch_kick(const KickArgs &a, int mode, bool drift, hipStream_t st)
 {
     const int n = a.g.N;
+    if (mode == kModeBF16x3 && n > 8) {
+        if (n <= 16) return mfma_kick_16(a, drift, st);
+        if (n <= 32) return mfma_kick_32(a, drift, st);
+        return mfma_kick_64(a, drift, st);
+    }
+    if (mode == kModeBF16x3) mode = kModeF32;
     if (n <= 3) return small_kick(a, mode, drift, st);
     if (n <= 8) return team_kick_8(a, mode, drift, st);
     if (n <= 16) return team_kick_16(a, mode, drift, st);
@@ -576,6 +585,7 @@ hipError_t launch_drift(const Geom &g, void *U, const void *P, double eps, int l
                         hipStream_t st)
 {
     const int n = g.N;
+    if (mode == kModeBF16x3) mode = kModeF32;
     if (n <= 3) return small_drift(g, U, P, eps, level, cinv, mode, st);
     if (n <= 8) return team_drift_8(g, U, P, eps, level, cinv, mode, st);
     if (n <= 16) return team_drift_16(g, U, P, eps, level, cinv, mode, st);
@@ -600,7 +610,7 @@ hipError_t launch_plaq(const Geom &g, const void *U, int mode, double inv_n, dou
     if (g.N <= 3) return small_plaq(g, U, mode, inv_n, part, np, st);
     const long nthr = threads_for(g.V, 4L * 2 * g.N * g.N, scratch_doubles);
     *np = nblk(nthr);
-    if (mode == kModeF32)
+    if (f32_mode(mode))
         hipLaunchKernelGGL(k_plaq_g<float>, dim3(nblk(nthr)), dim3(kB), 0, st, g, (const float *)U, inv_n, scratch, nthr, part);
     else
         hipLaunchKernelGGL(k_plaq_g<double>, dim3(nblk(nthr)), dim3(kB), 0, st, g, (const double *)U, inv_n, scratch, nthr,
@@ -614,7 +624,7 @@ hipError_t launch_uviol(const Geom &g, const void *U, int mode, double *part, lo
     if (g.N <= 3) return small_uviol(g, U, mode, part, np, st);
     const long nthr = threads_for(g.V * g.D, 4L * 2 * g.N * g.N, scratch_doubles);
     *np = nblk(nthr);
-    if (mode == kModeF32)
+    if (f32_mode(mode))
         hipLaunchKernelGGL(k_uviol_g<float>, dim3(nblk(nthr)), dim3(kB), 0, st, g, (const float *)U, scratch, nthr, part);
     else
         hipLaunchKernelGGL(k_uviol_g<double>, dim3(nblk(nthr)), dim3(kB), 0, st, g, (const double *)U, scratch, nthr, part);
@@ -628,7 +638,7 @@ hipError_t launch_gauss(const Geom &g, const void *U, const void *P, int mode, d
     const int nr = 2 * g.N * g.N;
     const long nthr = threads_for(g.V * g.D, 6L * nr, scratch_doubles);
     *np = nthr;
-    if (mode == kModeF32)
+    if (f32_mode(mode))
         hipLaunchKernelGGL(k_gauss_g<float>, dim3(nblk(nthr)), dim3(kB), 0, st, g, (const float *)U, (const float *)P,
                            scratch, nthr);
     else
@@ -649,7 +659,7 @@ hipError_t launch_kinetic(const Geom &g, const void *P, const void *Q, int mode,
 {
     const long nl = g.V * g.D;
     *np = nblk(nl);
-    if (mode == kModeF32)
+    if (f32_mode(mode))
         hipLaunchKernelGGL(k_kinetic<float>, dim3(nblk(nl)), dim3(kB), 0, st, g, (const float *)P, (const float *)Q, part);
     else
         hipLaunchKernelGGL(k_kinetic<double>, dim3(nblk(nl)), dim3(kB), 0, st, g, (const double *)P, (const double *)Q,
@@ -665,7 +675,7 @@ hipError_t launch_wlines(const Geom &g, const void *U, int dir, int mode, double
     const long chunk = threads_for(nlines, 3L * 2 * g.N * g.N, scratch_doubles);
     for (long q0 = 0; q0 < nlines; q0 += chunk) {
         const long nq = nlines - q0 < chunk ? nlines - q0 : chunk;
-        if (mode == kModeF32)
+        if (f32_mode(mode))
             hipLaunchKernelGGL(k_wlines_g<float>, dim3(nblk(nq)), dim3(kB), 0, st, g, (const float *)U, dir, q0, nq,
                                scratch, out);
         else
@@ -680,7 +690,7 @@ hipError_t launch_wlines(const Geom &g, const void *U, int dir, int mode, double
 hipError_t launch_line_matrix(const Geom &g, const void *U, int dir, long lex, int mode, double *scratch, double *out,
                               hipStream_t st)
 {
-    if (mode == kModeF32)
+    if (f32_mode(mode))
         hipLaunchKernelGGL(k_line_matrix<float>, dim3(1), dim3(64), 0, st, g, (const float *)U, dir, lex, scratch, out);
     else hipLaunchKernelGGL(k_line_matrix<double>, dim3(1), dim3(64), 0, st, g, (const double *)U, dir, lex, scratch, out);
     return hipGetLastError();
@@ -689,7 +699,7 @@ hipError_t launch_line_matrix(const Geom &g, const void *U, int dir, long lex, i
 hipError_t launch_links_from_aos(const Geom &g, const double *aos, void *U, int mode, hipStream_t st)
 {
     const long tot = g.V * g.D * 2L * g.N * g.N;
-    if (mode == kModeF32) hipLaunchKernelGGL(k_links_from_aos<float>, dim3(nblk(tot)), dim3(kB), 0, st, g, aos, (float *)U);
+    if (f32_mode(mode)) hipLaunchKernelGGL(k_links_from_aos<float>, dim3(nblk(tot)), dim3(kB), 0, st, g, aos, (float *)U);
     else hipLaunchKernelGGL(k_links_from_aos<double>, dim3(nblk(tot)), dim3(kB), 0, st, g, aos, (double *)U);
     return hipGetLastError();
 }
@@ -697,7 +707,7 @@ hipError_t launch_links_from_aos(const Geom &g, const double *aos, void *U, int 
 hipError_t launch_links_to_aos(const Geom &g, const void *U, double *aos, int mode, hipStream_t st)
 {
     const long tot = g.V * g.D * 2L * g.N * g.N;
-    if (mode == kModeF32) hipLaunchKernelGGL(k_links_to_aos<float>, dim3(nblk(tot)), dim3(kB), 0, st, g, (const float *)U, aos);
+    if (f32_mode(mode)) hipLaunchKernelGGL(k_links_to_aos<float>, dim3(nblk(tot)), dim3(kB), 0, st, g, (const float *)U, aos);
     else hipLaunchKernelGGL(k_links_to_aos<double>, dim3(nblk(tot)), dim3(kB), 0, st, g, (const double *)U, aos);
     return hipGetLastError();
 }
@@ -705,7 +715,7 @@ hipError_t launch_links_to_aos(const Geom &g, const void *U, double *aos, int mo
 hipError_t launch_mom_from_aos(const Geom &g, const double *aos, void *P, int mode, hipStream_t st)
 {
     const long tot = g.V * g.D * (long)g.N * g.N;
-    if (mode == kModeF32) hipLaunchKernelGGL(k_mom_from_aos<float>, dim3(nblk(tot)), dim3(kB), 0, st, g, aos, (float *)P);
+    if (f32_mode(mode)) hipLaunchKernelGGL(k_mom_from_aos<float>, dim3(nblk(tot)), dim3(kB), 0, st, g, aos, (float *)P);
     else hipLaunchKernelGGL(k_mom_from_aos<double>, dim3(nblk(tot)), dim3(kB), 0, st, g, aos, (double *)P);
     return hipGetLastError();
 }
@@ -713,7 +723,7 @@ hipError_t launch_mom_from_aos(const Geom &g, const double *aos, void *P, int mo
 hipError_t launch_mom_to_aos(const Geom &g, const void *P, double *aos, int mode, hipStream_t st)
 {
     const long tot = g.V * g.D * (long)g.N * g.N;
-    if (mode == kModeF32) hipLaunchKernelGGL(k_mom_to_aos<float>, dim3(nblk(tot)), dim3(kB), 0, st, g, (const float *)P, aos);
+    if (f32_mode(mode)) hipLaunchKernelGGL(k_mom_to_aos<float>, dim3(nblk(tot)), dim3(kB), 0, st, g, (const float *)P, aos);
     else hipLaunchKernelGGL(k_mom_to_aos<double>, dim3(nblk(tot)), dim3(kB), 0, st, g, (const double *)P, aos);
     return hipGetLastError();
 }
@@ -721,7 +731,7 @@ hipError_t launch_mom_to_aos(const Geom &g, const void *P, double *aos, int mode
 hipError_t launch_links_cold(const Geom &g, void *U, int mode, hipStream_t st)
 {
     const long tot = g.V * g.D * 2L * g.N * g.N;
-    if (mode == kModeF32) hipLaunchKernelGGL(k_cold<float>, dim3(nblk(tot)), dim3(kB), 0, st, g, (float *)U);
+    if (f32_mode(mode)) hipLaunchKernelGGL(k_cold<float>, dim3(nblk(tot)), dim3(kB), 0, st, g, (float *)U);
     else hipLaunchKernelGGL(k_cold<double>, dim3(nblk(tot)), dim3(kB), 0, st, g, (double *)U);
     return hipGetLastError();
 }
@@ -730,7 +740,7 @@ hipError_t launch_mom_refresh(const Geom &g, void *P, uint64_t seed, uint64_t co
                               hipStream_t st)
 {
     const long nl = g.V * g.D;
-    if (mode == kModeF32)
+    if (f32_mode(mode))
         hipLaunchKernelGGL(k_mom_refresh<float>, dim3(nblk(nl)), dim3(kB), 0, st, g, (float *)P, seed, counter, c);
     else hipLaunchKernelGGL(k_mom_refresh<double>, dim3(nblk(nl)), dim3(kB), 0, st, g, (double *)P, seed, counter, c);
     return hipGetLastError();
@@ -739,7 +749,7 @@ hipError_t launch_mom_refresh(const Geom &g, void *P, uint64_t seed, uint64_t co
 hipError_t launch_mom_scale(const Geom &g, void *P, double f, int mode, hipStream_t st)
 {
     const long tot = g.V * g.D * (long)g.N * g.N;
-    if (mode == kModeF32) hipLaunchKernelGGL(k_mom_scale<float>, dim3(nblk(tot)), dim3(kB), 0, st, g, (float *)P, f);
+    if (f32_mode(mode)) hipLaunchKernelGGL(k_mom_scale<float>, dim3(nblk(tot)), dim3(kB), 0, st, g, (float *)P, f);
     else hipLaunchKernelGGL(k_mom_scale<double>, dim3(nblk(tot)), dim3(kB), 0, st, g, (double *)P, f);
     return hipGetLastError();
 }
@@ -749,7 +759,7 @@ hipError_t launch_project(const Geom &g, void *U, int mode, int haar, uint64_t s
 {
     if (g.N <= 3) return small_project(g, U, mode, haar, seed, stream, st);
     const long nthr = threads_for(g.V * g.D, 4L * g.N * g.N, scratch_doubles);
-    if (mode == kModeF32)
+    if (f32_mode(mode))
         hipLaunchKernelGGL(k_project<float>, dim3(nblk(nthr)), dim3(kB), 0, st, g, (float *)U, haar, seed, stream, scratch,
                            nthr);
     else

@@ -13,7 +13,11 @@ enum Mode : int {
     kModeF64 = 0,       // FP64, FMA, factored staples
     kModeF64Exact = 1,  // FP64 in the reference's operation order, no FMA
     kModeF32 = 2,       // FP32 storage and arithmetic, FP64 sums
+    kModeBF16x3 = 3,    // FP32 storage; N >= 9 link products as split BF16 on MFMA
 };
+
+// FP32 storage (every mode but the FP64 ones)
+__host__ __device__ inline bool f32_mode(int mode) { return mode == kModeF32 || mode == kModeBF16x3; }
 
 // Geometry shared by every kernel.  Device site index s: all sites of even
 // parity (sum of coordinates mod 2), then all odd ones; within a parity the

@@ -353,7 +353,7 @@ template <int N>
 hipError_t project_mode(const Geom &g, void *U, int mode, int haar, uint64_t seed, uint64_t stream, hipStream_t st)
 {
     const unsigned nb = (unsigned)((g.V * g.D + kBlock - 1) / kBlock);
-    if (mode == kModeF32)
+    if (f32_mode(mode))
         hipLaunchKernelGGL((k_project<N, float>), dim3(nb), dim3(kBlock), 0, st, g, (float *)U, haar, seed, stream);
     else hipLaunchKernelGGL((k_project<N, double>), dim3(nb), dim3(kBlock), 0, st, g, (double *)U, haar, seed, stream);
     return hipGetLastError();
@@ -423,7 +423,7 @@ hipError_t uviol_mode(const Geom &g, const void *U, int mode, double *part, long
 {
     const unsigned nb = (unsigned)((g.V * g.D + kBlock - 1) / kBlock);
     *np = nb;
-    if (mode == kModeF32) hipLaunchKernelGGL((k_uviol<N, float>), dim3(nb), dim3(kBlock), 0, st, g, (const float *)U, part);
+    if (f32_mode(mode)) hipLaunchKernelGGL((k_uviol<N, float>), dim3(nb), dim3(kBlock), 0, st, g, (const float *)U, part);
     else hipLaunchKernelGGL((k_uviol<N, double>), dim3(nb), dim3(kBlock), 0, st, g, (const double *)U, part);
     return hipGetLastError();
 }
@@ -434,7 +434,7 @@ hipError_t gauss_mode(const Geom &g, const void *U, const void *P, int mode, dou
 {
     const unsigned nb = (unsigned)((g.V * g.D + kBlock - 1) / kBlock);
     *np = nb;
-    if (mode == kModeF32)
+    if (f32_mode(mode))
         hipLaunchKernelGGL((k_gauss<N, float>), dim3(nb), dim3(kBlock), 0, st, g, (const float *)U, (const float *)P, part);
     else
         hipLaunchKernelGGL((k_gauss<N, double>), dim3(nb), dim3(kBlock), 0, st, g, (const double *)U, (const double *)P,
@@ -449,7 +449,7 @@ hipError_t wlines_mode(const Geom &g, const void *U, int dir, int mode, double *
 {
     const long nlines = g.V / g.ext[dir];
     const unsigned nb = (unsigned)((nlines + kBlock - 1) / kBlock);
-    if (mode == kModeF32)
+    if (f32_mode(mode))
         hipLaunchKernelGGL((k_wlines<N, float>), dim3(nb), dim3(kBlock), 0, st, g, (const float *)U, dir, nlines, out);
     else
         hipLaunchKernelGGL((k_wlines<N, double>), dim3(nb), dim3(kBlock), 0, st, g, (const double *)U, dir, nlines, out);

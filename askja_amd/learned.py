@@ -137,7 +137,7 @@ def train(lat, configs, beta, w0, epochs=2, lr=2e-3, log=None):
             loss = fa.apply(w, lat, beta)
             loss.backward()
             opt.step()
-            tot += float(loss)
+            tot += float(loss.detach())
         losses.append(tot / len(configs))
         if log:
             log(f"[learned] epoch {ep + 1}/{epochs} loss {losses[-1]:.3f}")

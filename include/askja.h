@@ -43,8 +43,11 @@ enum {
     ASKJA_FP64_EXACT = 1, /* FP64 in the reference's operation order, no FMA:
                              links and momenta bit-identical to the reference's
                              double arithmetic (sunmat.h:81-371, sun.cpp:672-801) */
-    ASKJA_FP32 = 2        /* FP32 links/momenta/MD; FP64 compensated sums,
+    ASKJA_FP32 = 2,       /* FP32 links/momenta/MD; FP64 compensated sums,
                              FP64 Metropolis and FP64 reunitarisation */
+    ASKJA_BF16X3 = 3      /* as FP32, but for N >= 9 the kick/drift link products
+                             run on MFMA as split BF16 (hi*hi + hi*lo + lo*hi)
+                             with FP32 accumulation */
 };
 
 /* error codes */

@@ -267,3 +267,34 @@ def test_errors_are_reported(oracle):
         lat.updateE(0.1)
     A.lib().askja_clear_error(lat._h)
     lat.updateE(0.1)
+
+
+@pytest.mark.parametrize("n,ext", [(12, [2, 2, 2]), (16, [2, 3, 2]), (24, [2, 2]), (64, [2, 2])],
+                         ids=["N12", "N16", "N24", "N64"])
+def test_bf16x3_kick_drift(oracle, n, ext):
+    """Split-precision BF16x3 MFMA link products (FP32 accumulation): about
+    16 significant bits per product, so the kick and drift agree with the FP64
+    oracle to ~1e-4 relative, and with the FP32 VALU path as closely."""
+    o, u, p = _state(oracle, n, ext, 10)
+    o.updateE(0.05)
+    o.updateUandcopyDown(0.05)
+    for prec in (A.BF16X3, A.FP32):
+        lat = _gpu(n, ext, prec, u, p)
+        lat.updateE(0.05)
+        lat.updateUandcopyDown(0.05, 8)
+        scale_p = np.max(np.abs(o.field("pi")))
+        dp = np.max(np.abs(lat.momenta() - o.field("pi"))) / scale_p
+        du = np.max(np.abs(lat.links() - o.field("u")))
+        assert dp < 2e-4 * n and du < 2e-4 * n, (prec, dp, du)
+    # the fused kick+drift on MFMA equals kick then drift to rounding
+    a = _gpu(n, ext, A.BF16X3, u, p)
+    b = _gpu(n, ext, A.BF16X3, u, p)
+    a.updateE(0.03)
+    a.updateUandcopyDown(0.04, 6)
+    b.kick_drift(0.03, 0.04, 6)
+    assert np.max(np.abs(a.links() - b.links())) < 2e-4 * n
+    # HMC with BF16x3 MD stays exact through the FP64 Metropolis test
+    c = _gpu(n, ext, A.BF16X3)
+    c.haar(1, 0)
+    r = c.hmc_trajectory(n * n / 2.0, 0.01, 5, 8, 2, 0)
+    assert np.isfinite(r.dh) and abs(r.dh) < 5.0

@@ -1,0 +1,87 @@
+"""Statistical parity of the GPU HMC chain: ensemble averages against the
+oracle's chain (tests/golden/ensemble_su3_4x4.json, written by
+make_ensemble.py) and the plaquette of 2D SU(3) against the exact
+single-plaquette result (2D plaquettes are independent up to an exponentially
+small finite-volume correction)."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+import scipy.special
+
+import askja_amd as A
+from conftest import ROOT
+
+pytestmark = pytest.mark.gpu
+
+
+def tau_int(x, S=1.5):
+    x = np.asarray(x, np.float64)
+    n = len(x)
+    d = x - x.mean()
+    var = np.dot(d, d) / n
+    f = np.fft.rfft(d, 2 * n)
+    acf = np.fft.irfft(f * np.conj(f))[:n] / (n - np.arange(n)) / var
+    tint = 0.5
+    for W in range(1, n // 2):
+        tint += acf[W]
+        tau = S / math.log((2 * tint + 1) / (2 * tint - 1)) if tint > 0.5 else 1e-9
+        if math.exp(-W / tau) - tau / math.sqrt(W * n) < 0:
+            return 2 * tint
+    return 2 * tint
+
+
+def mean_err(x):
+    x = np.asarray(x)
+    return x.mean(), x.std() * math.sqrt(tau_int(x) / len(x))
+
+
+def test_ensemble_matches_oracle_chain():
+    gold = json.load(open(os.path.join(ROOT, "tests", "golden", "ensemble_su3_4x4.json")))
+    lat = A.Lattice(3, [4, 4, 4, 4], A.FP64)
+    lat.haar(4242, 0)
+    for t in range(200):
+        lat.hmc_trajectory(5.7, 0.05, 10, 8, 99, t)
+    plaq, pol, acc = [], [], 0
+    for t in range(4000):
+        r = lat.hmc_trajectory(5.7, 0.05, 10, 8, 99, 1000 + t)
+        acc += r.accepted
+        plaq.append(1.0 - r.action / lat.n_plaquettes())
+        pol.append(abs(lat.polyakov()))
+    for name, series in (("plaquette", plaq), ("abs_polyakov", pol)):
+        m, e = mean_err(series)
+        g = gold[name]
+        z = abs(m - g["mean"]) / math.hypot(e, g["err"])
+        assert z < 4.0, (name, m, e, g)
+    assert abs(acc / 4000 - gold["acceptance"]) < 0.05
+
+
+def exact_2d_plaquette(n, beta):
+    """<ReTr U / N> of the single-plaquette SU(N) integral with weight
+    exp((beta/N) ReTr U): Z(x) = sum_k det[I_{k+i-j}(x)], x = beta/N."""
+    def logz(x):
+        z = 0.0
+        for k in range(-30, 31):
+            m = np.array([[scipy.special.iv(k + i - j, x) for j in range(n)] for i in range(n)])
+            z += np.linalg.det(m)
+        return math.log(z)
+    x, h = beta / n, 1e-5
+    return (logz(x + h) - logz(x - h)) / (2 * h) / n
+
+
+@pytest.mark.parametrize("prec", [A.FP64, A.FP32])
+def test_two_dimensional_exact_plaquette(prec):
+    n, beta = 3, 6.0
+    want = exact_2d_plaquette(n, beta)
+    lat = A.Lattice(n, [64, 64], prec)
+    lat.haar(5, 0)
+    for t in range(100):
+        lat.hmc_trajectory(beta, 0.1, 10, 8, 3, t)
+    plaq = []
+    for t in range(1500):
+        r = lat.hmc_trajectory(beta, 0.1, 10, 8, 3, 500 + t)
+        plaq.append(1.0 - r.action / lat.n_plaquettes())
+    m, e = mean_err(plaq)
+    assert abs(m - want) < 4 * e + 1e-4, (m, e, want)
