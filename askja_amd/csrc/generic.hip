@@ -22,15 +22,17 @@ hipError_t small_project(const Geom &g, void *U, int mode, int haar, uint64_t se
 #define ASKJA_TEAM_DECL(NM)                                                                                   \
     hipError_t team_kick_##NM(const KickArgs &a, int mode, bool drift, hipStream_t st);                       \
     hipError_t team_drift_##NM(const Geom &g, void *U, const void *P, double eps, int level, const double *c, \
-                               int mode, hipStream_t st);
+                               int mode, hipStream_t st);                                                     \
+    hipError_t team_project_##NM(const Geom &g, void *U, int mode, int haar, uint64_t seed, uint64_t stream,    \
+                                 hipStream_t st);
 ASKJA_TEAM_DECL(8)
 ASKJA_TEAM_DECL(16)
 ASKJA_TEAM_DECL(32)
 ASKJA_TEAM_DECL(64)
 #undef ASKJA_TEAM_DECL
-hipError_t mfma_kick_16(const KickArgs &a, bool drift, hipStream_t st);
-hipError_t mfma_kick_32(const KickArgs &a, bool drift, hipStream_t st);
-hipError_t mfma_kick_64(const KickArgs &a, bool drift, hipStream_t st);
+hipError_t mfma_kick_16(const KickArgs &a, int mode, bool drift, hipStream_t st);
+hipError_t mfma_kick_32(const KickArgs &a, int mode, bool drift, hipStream_t st);
+hipError_t mfma_kick_64(const KickArgs &a, int mode, bool drift, hipStream_t st);
 
 namespace {
 
@@ -568,10 +570,12 @@ long kick_partials(const Geom &g)
 hipError_t launch_kick(const KickArgs &a, int mode, bool drift, hipStream_t st)
 {
     const int n = a.g.N;
-    if (mode == kModeBF16x3 && n > 8) {
-        if (n <= 16) return mfma_kick_16(a, drift, st);
-        if (n <= 32) return mfma_kick_32(a, drift, st);
-        return mfma_kick_64(a, drift, st);
+    // N >= 9: complex link products on MFMA, except the exact mode, which
+    // keeps the reference's operation order on the VALU
+    if (n > 8 && mode != kModeF64Exact) {
+        if (n <= 16) return mfma_kick_16(a, mode, drift, st);
+        if (n <= 32) return mfma_kick_32(a, mode, drift, st);
+        return mfma_kick_64(a, mode, drift, st);
     }
     if (mode == kModeBF16x3) mode = kModeF32;
     if (n <= 3) return small_kick(a, mode, drift, st);
@@ -758,6 +762,10 @@ hipError_t launch_project(const Geom &g, void *U, int mode, int haar, uint64_t s
                           long scratch_doubles, hipStream_t st)
 {
     if (g.N <= 3) return small_project(g, U, mode, haar, seed, stream, st);
+    if (g.N <= 8) return team_project_8(g, U, mode, haar, seed, stream, st);
+    if (g.N <= 16) return team_project_16(g, U, mode, haar, seed, stream, st);
+    if (g.N <= 32) return team_project_32(g, U, mode, haar, seed, stream, st);
+    if (g.N <= 64) return team_project_64(g, U, mode, haar, seed, stream, st);
     const long nthr = threads_for(g.V * g.D, 4L * g.N * g.N, scratch_doubles);
     if (f32_mode(mode))
         hipLaunchKernelGGL(k_project<float>, dim3(nblk(nthr)), dim3(kB), 0, st, g, (float *)U, haar, seed, stream, scratch,

@@ -1,8 +1,15 @@
-// mfma16.hip -- instantiates the BF16x3 MFMA kick for N <= 16.
+// mfma16.hip -- instantiates the MFMA kick for N <= 16 (FP64, FP32, BF16x3).
 #include "mfma_impl.hpp"
 
 namespace askja {
 
-hipError_t mfma_kick_16(const KickArgs &a, bool drift, hipStream_t st) { return mfma::kick<16>(a, drift, st); }
+hipError_t mfma_kick_16(const KickArgs &a, int mode, bool drift, hipStream_t st)
+{
+    switch (mode) {
+    case kModeF64: return mfma::kick<mfma::kF64, 16>(a, drift, st);
+    case kModeF32: return mfma::kick<mfma::kF32, 16>(a, drift, st);
+    default: return mfma::kick<mfma::kBF16X3, 16>(a, drift, st);
+    }
+}
 
 }  // namespace askja

@@ -1,15 +1,22 @@
-// mfma_impl.hpp -- kick (+ drift) for 9 <= N <= 64 in split-precision
-// BF16x3 on the MFMA units: FP32 links and momenta; every complex link
-// product C = A B is computed as two real products of the stacked form
+// mfma_impl.hpp -- kick (+ drift) for 9 <= N <= 64 on the MFMA units.
+// Every complex link product C = A B is computed as two real products of
+// the stacked form
 //   Re C = [Ar | Ai] [Br ; -Bi],   Im C = [Ar | Ai] [Bi ; Br]   (K = 2 NMAX)
-// with each FP32 operand split as x = hi + lo (hi = bf16(x), lo = bf16(x - hi))
-// and hi*hi + hi*lo + lo*hi accumulated in FP32 by v_mfma_f32_16x16x32_bf16.
-//
-// One workgroup per link, one wavefront per 16-row block of the (zero
-// padded) NMAX x NMAX matrices; lane l of a wavefront holds, for every
-// 16-column tile, the outputs at rows 4*(l/16)+i (i < 4), column l%16 of the
-// tile (the MFMA accumulator layout).  Operands are staged in two LDS slots
-// (planar re/im, row stride NMAX).
+// in one of three arithmetic policies:
+//   F64    : FP64 links, v_mfma_f64_16x16x4_f64;
+//   F32    : FP32 links, v_mfma_f32_16x16x4_f32;
+//   BF16X3 : FP32 links split as x = hi + lo (hi = bf16(x), lo = bf16(x - hi)),
+//            hi*hi + hi*lo + lo*hi accumulated in FP32 by
+//            v_mfma_f32_16x16x32_bf16.
+// One workgroup per link; the (zero padded) NMAX x NMAX matrices are cut in
+// 16-row blocks, each handled by WPR wavefronts that split its 16-column
+// tiles.  Lane l holds, for every tile of its wavefront, four outputs in
+// column l%16 of the tile, at rows 4*(l/16)+q (f32 accumulators) or
+// 4*q+l/16 (f64 accumulators), q < 4: the MFMA accumulator layouts, both
+// checked on gfx950 with one-hot operands.  The A operand of lane l is
+// A[l%16][k0 + l/16] and the B operand B[k0 + l/16][l%16] for both 16x16x4
+// forms.
+// Operands are staged in two LDS slots (planar re/im, row stride NMAX).
 #pragma once
 #include "kernels.hpp"
 #include "common.hpp"
@@ -19,27 +26,60 @@ namespace mfma {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
 
-template <int NMAX>
+enum Policy { kF64 = 0, kF32 = 1, kBF16X3 = 2 };
+
+template <int P>
+struct Pol;
+template <>
+struct Pol<kF64> {
+    typedef double R;
+    typedef f64x4 V4;
+};
+template <>
+struct Pol<kF32> {
+    typedef float R;
+    typedef f32x4 V4;
+};
+template <>
+struct Pol<kBF16X3> {
+    typedef float R;
+    typedef f32x4 V4;
+};
+
+template <int P, int NMAX>
 struct Cfg {
-    static constexpr int NT = 64 * (NMAX / 16);  // one wavefront per row block
-    static constexpr int NCT = NMAX / 16;        // column tiles
+    static constexpr int WPR = (P == kF64 && NMAX == 64) ? 2 : 1;  // wavefronts per 16-row block
+    static constexpr int NT = 64 * (NMAX / 16) * WPR;
+    static constexpr int NCT = NMAX / 16 / WPR;  // column tiles per wavefront
     static constexpr int SLOT = NMAX * NMAX;
 };
 
-template <int NMAX>
+template <int P, int NMAX>
 struct Acc {
-    f32x4 r[NMAX / 16], i[NMAX / 16];
+    typename Pol<P>::V4 r[Cfg<P, NMAX>::NCT], i[Cfg<P, NMAX>::NCT];
 };
 
-template <int NMAX>
-__device__ inline void acc_zero(Acc<NMAX> &a)
+template <int P, int NMAX>
+__device__ inline void acc_zero(Acc<P, NMAX> &a)
 {
 #pragma unroll
-    for (int t = 0; t < NMAX / 16; t++) {
-        a.r[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-        a.i[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-    }
+    for (int t = 0; t < Cfg<P, NMAX>::NCT; t++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) a.r[t][q] = a.i[t][q] = 0;
+}
+
+// wavefront geometry: 16-row block and first column tile of this wavefront
+template <int P, int NMAX>
+__device__ inline int wave_rowblock()
+{
+    return (threadIdx.x >> 6) / Cfg<P, NMAX>::WPR;
+}
+template <int P, int NMAX>
+__device__ inline int wave_tile0()
+{
+    return ((threadIdx.x >> 6) % Cfg<P, NMAX>::WPR) * Cfg<P, NMAX>::NCT;
 }
 
 __device__ inline void split8(const float (&x)[8], bf16x8 &hi, bf16x8 &lo)
@@ -52,91 +92,127 @@ __device__ inline void split8(const float (&x)[8], bf16x8 &hi, bf16x8 &lo)
     }
 }
 
-// acc (+)= op(A) op(B) on the wavefront's 16-row block; A, B in LDS slots.
-template <int NMAX, bool DA, bool DB>
-__device__ inline void cmul(const float *Ar, const float *Ai, const float *Br, const float *Bi, Acc<NMAX> &acc)
+// acc (+)= op(A) op(B) on the wavefront's rows and column tiles; A, B in LDS.
+template <int P, int NMAX, bool DA, bool DB, typename R>
+__device__ inline void cmul(const R *Ar, const R *Ai, const R *Br, const R *Bi, Acc<P, NMAX> &acc)
 {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int row = 16 * w + (lane & 15);
-    const int kq = 8 * (lane >> 4);
+    constexpr int NCT = Cfg<P, NMAX>::NCT;
+    const int lane = threadIdx.x & 63;
+    const int row = 16 * wave_rowblock<P, NMAX>() + (lane & 15);
+    const int t0 = wave_tile0<P, NMAX>();
+    if constexpr (P == kBF16X3) {
+        const int kq = 8 * (lane >> 4);
 #pragma unroll
-    for (int ks = 0; ks < (2 * NMAX) / 32; ks++) {
-        const int k0 = 32 * ks + kq;           // first of this lane's 8 stacked k
-        const bool imag = k0 >= NMAX;          // all 8 in the same half
-        const int kb = imag ? k0 - NMAX : k0;  // column of A / row of B
-        float av[8];
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const int k = kb + j;
-            // op(A)[row][k]
-            const float re = DA ? Ar[k * NMAX + row] : Ar[row * NMAX + k];
-            const float im = DA ? -Ai[k * NMAX + row] : Ai[row * NMAX + k];
-            av[j] = imag ? im : re;
-        }
-        bf16x8 ah, al;
-        split8(av, ah, al);
-#pragma unroll
-        for (int t = 0; t < NMAX / 16; t++) {
-            const int col = 16 * t + (lane & 15);
-            float brv[8], biv[8];
+        for (int ks = 0; ks < (2 * NMAX) / 32; ks++) {
+            const int k0 = 32 * ks + kq;           // first of this lane's 8 stacked k
+            const bool imag = k0 >= NMAX;          // all 8 in the same half
+            const int kb = imag ? k0 - NMAX : k0;  // column of A / row of B
+            float av[8];
 #pragma unroll
             for (int j = 0; j < 8; j++) {
                 const int k = kb + j;
-                // op(B)[k][col]
-                const float re = DB ? Br[col * NMAX + k] : Br[k * NMAX + col];
-                const float im = DB ? -Bi[col * NMAX + k] : Bi[k * NMAX + col];
-                brv[j] = imag ? -im : re;  // [Br ; -Bi]
-                biv[j] = imag ? re : im;   // [Bi ; Br]
+                const float re = DA ? Ar[k * NMAX + row] : Ar[row * NMAX + k];
+                const float im = DA ? -Ai[k * NMAX + row] : Ai[row * NMAX + k];
+                av[j] = imag ? im : re;
             }
-            bf16x8 bh, bl, ch, cl;
-            split8(brv, bh, bl);
-            split8(biv, ch, cl);
-            acc.r[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc.r[t], 0, 0, 0);
-            acc.r[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc.r[t], 0, 0, 0);
-            acc.r[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc.r[t], 0, 0, 0);
-            acc.i[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, ch, acc.i[t], 0, 0, 0);
-            acc.i[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, cl, acc.i[t], 0, 0, 0);
-            acc.i[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, ch, acc.i[t], 0, 0, 0);
+            bf16x8 ah, al;
+            split8(av, ah, al);
+#pragma unroll
+            for (int t = 0; t < NCT; t++) {
+                const int col = 16 * (t0 + t) + (lane & 15);
+                float brv[8], biv[8];
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    const int k = kb + j;
+                    const float re = DB ? Br[col * NMAX + k] : Br[k * NMAX + col];
+                    const float im = DB ? -Bi[col * NMAX + k] : Bi[k * NMAX + col];
+                    brv[j] = imag ? -im : re;  // [Br ; -Bi]
+                    biv[j] = imag ? re : im;   // [Bi ; Br]
+                }
+                bf16x8 bh, bl, ch, cl;
+                split8(brv, bh, bl);
+                split8(biv, ch, cl);
+                acc.r[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc.r[t], 0, 0, 0);
+                acc.r[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc.r[t], 0, 0, 0);
+                acc.r[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc.r[t], 0, 0, 0);
+                acc.i[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, ch, acc.i[t], 0, 0, 0);
+                acc.i[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, cl, acc.i[t], 0, 0, 0);
+                acc.i[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, ch, acc.i[t], 0, 0, 0);
+            }
+        }
+    } else {
+        // 16x16x4: lane l holds A[row][k0] and B[k0][col], k0 = 4 ks + l/16
+#pragma unroll 4
+        for (int ks = 0; ks < (2 * NMAX) / 4; ks++) {
+            const int k0 = 4 * ks + (lane >> 4);
+            const bool imag = k0 >= NMAX;
+            const int k = imag ? k0 - NMAX : k0;
+            const R are = DA ? Ar[k * NMAX + row] : Ar[row * NMAX + k];
+            const R aim = DA ? -Ai[k * NMAX + row] : Ai[row * NMAX + k];
+            const R av = imag ? aim : are;
+#pragma unroll
+            for (int t = 0; t < NCT; t++) {
+                const int col = 16 * (t0 + t) + (lane & 15);
+                const R bre = DB ? Br[col * NMAX + k] : Br[k * NMAX + col];
+                const R bim = DB ? -Bi[col * NMAX + k] : Bi[k * NMAX + col];
+                const R br = imag ? -bim : bre, bi = imag ? bre : bim;
+                if constexpr (P == kF64) {
+                    acc.r[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, br, acc.r[t], 0, 0, 0);
+                    acc.i[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bi, acc.i[t], 0, 0, 0);
+                } else {
+                    acc.r[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, br, acc.r[t], 0, 0, 0);
+                    acc.i[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bi, acc.i[t], 0, 0, 0);
+                }
+            }
         }
     }
 }
 
 // row / column of accumulator entry (t, q) of this lane
-__device__ inline int acc_row(int q) { return 16 * (threadIdx.x >> 6) + 4 * ((threadIdx.x & 63) >> 4) + q; }
-__device__ inline int acc_col(int t) { return 16 * t + (threadIdx.x & 15); }
+template <int P, int NMAX>
+__device__ inline int acc_row(int q)
+{
+    const int lr = (threadIdx.x & 63) >> 4;
+    return 16 * wave_rowblock<P, NMAX>() + (P == kF64 ? 4 * q + lr : 4 * lr + q);
+}
+template <int P, int NMAX>
+__device__ inline int acc_col(int t)
+{
+    return 16 * (wave_tile0<P, NMAX>() + t) + (threadIdx.x & 15);
+}
 
-template <int NMAX>
-__device__ inline void acc_store(const Acc<NMAX> &a, float *Sr, float *Si)
+template <int P, int NMAX, typename R>
+__device__ inline void acc_store(const Acc<P, NMAX> &a, R *Sr, R *Si)
 {
 #pragma unroll
-    for (int t = 0; t < NMAX / 16; t++)
+    for (int t = 0; t < Cfg<P, NMAX>::NCT; t++)
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            Sr[acc_row(q) * NMAX + acc_col(t)] = a.r[t][q];
-            Si[acc_row(q) * NMAX + acc_col(t)] = a.i[t][q];
+            Sr[acc_row<P, NMAX>(q) * NMAX + acc_col<P, NMAX>(t)] = a.r[t][q];
+            Si[acc_row<P, NMAX>(q) * NMAX + acc_col<P, NMAX>(t)] = a.i[t][q];
         }
 }
 
 // global link (matrix contiguous, n x n complex) -> zero-padded slot
-template <int NMAX>
-__device__ inline void ld_link(float *Sr, float *Si, const float *__restrict__ f, long l, int n)
+template <int P, int NMAX, typename R>
+__device__ inline void ld_link(R *Sr, R *Si, const R *__restrict__ f, long l, int n)
 {
-    constexpr int NT = Cfg<NMAX>::NT;
-    const float *b = f + (size_t)l * 2 * n * n;
+    constexpr int NT = Cfg<P, NMAX>::NT;
+    const R *b = f + (size_t)l * 2 * n * n;
     for (int e = threadIdx.x; e < NMAX * NMAX; e += NT) {
         const int r = e / NMAX, c = e % NMAX;
         const bool in = r < n && c < n;
-        Sr[e] = in ? b[2 * (r * n + c)] : 0.f;
-        Si[e] = in ? b[2 * (r * n + c) + 1] : 0.f;
+        Sr[e] = in ? b[2 * (r * n + c)] : R(0);
+        Si[e] = in ? b[2 * (r * n + c) + 1] : R(0);
     }
 }
 
-template <int NMAX>
-__device__ inline float mom_el(const float *__restrict__ P, long l, int n, int i, int j, float &im)
+template <typename R>
+__device__ inline R mom_el(const R *__restrict__ P, long l, int n, int i, int j, R &im)
 {
-    const float *b = P + (size_t)l * n * n;
+    const R *b = P + (size_t)l * n * n;
     if (i == j) {
-        im = 0.f;
+        im = R(0);
         return b[i];
     }
     const int lo = i < j ? i : j, hi = i < j ? j : i;
@@ -145,11 +221,12 @@ __device__ inline float mom_el(const float *__restrict__ P, long l, int n, int i
     return b[h];
 }
 
-template <int NMAX, bool Drift>
-__global__ __launch_bounds__(Cfg<NMAX>::NT) void k_kick(KickArgs a)
+template <int P, int NMAX, bool Drift>
+__global__ __launch_bounds__((Cfg<P, NMAX>::NT)) void k_kick(KickArgs a)
 {
-    constexpr int NCT = Cfg<NMAX>::NCT;
-    __shared__ float Are[Cfg<NMAX>::SLOT], Aim[Cfg<NMAX>::SLOT], Bre[Cfg<NMAX>::SLOT], Bim[Cfg<NMAX>::SLOT];
+    typedef typename Pol<P>::R R;
+    constexpr int NCT = Cfg<P, NMAX>::NCT;
+    __shared__ R Are[Cfg<P, NMAX>::SLOT], Aim[Cfg<P, NMAX>::SLOT], Bre[Cfg<P, NMAX>::SLOT], Bim[Cfg<P, NMAX>::SLOT];
     const Geom &g = a.g;
     const int n = g.N;
     const long V = g.V;
@@ -158,10 +235,10 @@ __global__ __launch_bounds__(Cfg<NMAX>::NT) void k_kick(KickArgs a)
     int cnt;
     const long s = parity_block((long)(lb / (unsigned)g.D), V, g.V0, 1, cnt);
     const long l = (long)mu * V + s;
-    const float *U = (const float *)a.U;
+    const R *U = (const R *)a.U;
     const long xp = g.nbr[(2 * mu) * V + s];
 
-    Acc<NMAX> S, T;
+    Acc<P, NMAX> S, T;
     acc_zero(S);
     for (int nu = 0; nu < g.D; nu++) {
         if (nu == mu) continue;
@@ -170,69 +247,69 @@ __global__ __launch_bounds__(Cfg<NMAX>::NT) void k_kick(KickArgs a)
         const long xpmn = g.nbr[(2 * nu + 1) * V + xp];
         // S += U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+
         __syncthreads();
-        ld_link<NMAX>(Are, Aim, U, nu * V + xp, n);
-        ld_link<NMAX>(Bre, Bim, U, mu * V + xpn, n);
+        ld_link<P, NMAX>(Are, Aim, U, nu * V + xp, n);
+        ld_link<P, NMAX>(Bre, Bim, U, mu * V + xpn, n);
         __syncthreads();
         acc_zero(T);
-        cmul<NMAX, false, true>(Are, Aim, Bre, Bim, T);
+        cmul<P, NMAX, false, true>(Are, Aim, Bre, Bim, T);
         __syncthreads();
         acc_store(T, Are, Aim);
-        ld_link<NMAX>(Bre, Bim, U, nu * V + s, n);
+        ld_link<P, NMAX>(Bre, Bim, U, nu * V + s, n);
         __syncthreads();
-        cmul<NMAX, false, true>(Are, Aim, Bre, Bim, S);
+        cmul<P, NMAX, false, true>(Are, Aim, Bre, Bim, S);
         // S += U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu)
         __syncthreads();
-        ld_link<NMAX>(Are, Aim, U, nu * V + xpmn, n);
-        ld_link<NMAX>(Bre, Bim, U, mu * V + xmn, n);
+        ld_link<P, NMAX>(Are, Aim, U, nu * V + xpmn, n);
+        ld_link<P, NMAX>(Bre, Bim, U, mu * V + xmn, n);
         __syncthreads();
         acc_zero(T);
-        cmul<NMAX, true, true>(Are, Aim, Bre, Bim, T);
+        cmul<P, NMAX, true, true>(Are, Aim, Bre, Bim, T);
         __syncthreads();
         acc_store(T, Are, Aim);
-        ld_link<NMAX>(Bre, Bim, U, nu * V + xmn, n);
+        ld_link<P, NMAX>(Bre, Bim, U, nu * V + xmn, n);
         __syncthreads();
-        cmul<NMAX, false, false>(Are, Aim, Bre, Bim, S);
+        cmul<P, NMAX, false, false>(Are, Aim, Bre, Bim, S);
     }
     // W = U_mu(x) S
     __syncthreads();
     acc_store(S, Are, Aim);
-    ld_link<NMAX>(Bre, Bim, U, mu * V + s, n);
+    ld_link<P, NMAX>(Bre, Bim, U, mu * V + s, n);
     __syncthreads();
     acc_zero(T);
-    cmul<NMAX, false, false>(Bre, Bim, Are, Aim, T);
+    cmul<P, NMAX, false, false>(Bre, Bim, Are, Aim, T);
     __syncthreads();
     acc_store(T, Are, Aim);
     __syncthreads();
-    // TAH and kick on the lane's entries
-    const float eps = (float)a.eps_kick, inv_n = (float)a.inv_n;
-    float *P = (float *)a.P;
-    float tri = 0.f;
+    // TAH and kick on the lane's entries (sunmat.h:355-371, sun.cpp:716)
+    const R eps = (R)a.eps_kick, inv_n = (R)a.inv_n;
+    R *Pm = (R *)a.P;
+    R tri = R(0);
     for (int i = 0; i < n; i++) tri += Aim[i * NMAX + i] + Aim[i * NMAX + i];
-    const float corr = tri * inv_n;
+    const R corr = tri * inv_n;
     double sum[3] = {0.0, 0.0, 0.0}, cmp[3] = {0.0, 0.0, 0.0};
-    float pr[NCT][4], pim[NCT][4], opr[NCT][4], opi[NCT][4];
+    R pr[NCT][4], pim[NCT][4], opr[NCT][4], opi[NCT][4];
 #pragma unroll
     for (int t = 0; t < NCT; t++)
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const int i = acc_row(q), j = acc_col(t);
-            pr[t][q] = pim[t][q] = opr[t][q] = opi[t][q] = 0.f;
+            const int i = acc_row<P, NMAX>(q), j = acc_col<P, NMAX>(t);
+            pr[t][q] = pim[t][q] = opr[t][q] = opi[t][q] = R(0);
             if (i >= n || j >= n) continue;
-            float ii;
-            const float ir = mom_el<NMAX>(P, l, n, i, j, ii);
+            R ii;
+            const R ir = mom_el(Pm, l, n, i, j, ii);
             opr[t][q] = ir;
             opi[t][q] = ii;
             if (i == j) {
-                const float mi = (Aim[i * NMAX + i] + Aim[i * NMAX + i]) - corr;
-                pr[t][q] = ir - (0.5f * mi) * eps;
+                const R mi = (Aim[i * NMAX + i] + Aim[i * NMAX + i]) - corr;
+                pr[t][q] = ir - (R(0.5) * mi) * eps;
                 sum[0] += (double)Are[i * NMAX + i];
             } else {
                 const int lo = i < j ? i : j, hi = i < j ? j : i;
-                const float dr = Are[lo * NMAX + hi] - Are[hi * NMAX + lo];
-                const float di = Aim[lo * NMAX + hi] + Aim[hi * NMAX + lo];
-                float ui;
-                const float ur = mom_el<NMAX>(P, l, n, lo, hi, ui);
-                const float vr = ur - (0.5f * di) * eps, vi = ui + (0.5f * dr) * eps;
+                const R dr = Are[lo * NMAX + hi] - Are[hi * NMAX + lo];
+                const R di = Aim[lo * NMAX + hi] + Aim[hi * NMAX + lo];
+                R ui;
+                const R ur = mom_el(Pm, l, n, lo, hi, ui);
+                const R vr = ur - (R(0.5) * di) * eps, vi = ui + (R(0.5) * dr) * eps;
                 pr[t][q] = vr;
                 pim[t][q] = i < j ? vi : -vi;
             }
@@ -241,13 +318,13 @@ __global__ __launch_bounds__(Cfg<NMAX>::NT) void k_kick(KickArgs a)
         }
     __syncthreads();  // every lane has read the old momentum
     {
-        float *b = P + (size_t)l * n * n;
-        float *bp = (!Drift && a.Pprev) ? (float *)a.Pprev + (size_t)l * n * n : nullptr;
+        R *b = Pm + (size_t)l * n * n;
+        R *bp = (!Drift && a.Pprev) ? (R *)a.Pprev + (size_t)l * n * n : nullptr;
 #pragma unroll
         for (int t = 0; t < NCT; t++)
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const int i = acc_row(q), j = acc_col(t);
+                const int i = acc_row<P, NMAX>(q), j = acc_col<P, NMAX>(t);
                 if (i >= n || j >= n || i > j) continue;
                 if (i == j) {
                     b[i] = pr[t][q];
@@ -264,8 +341,9 @@ __global__ __launch_bounds__(Cfg<NMAX>::NT) void k_kick(KickArgs a)
             }
     }
     if constexpr (Drift) {
-        // rhs = i eps_d P into slot A; X = 1 + rhs c_L in slot B (lane entries)
-        const float ed = (float)a.eps_drift;
+        // rhs = i eps_d P into slot A; X = 1 + c_L rhs in registers; Horner
+        // steps X = 1 + c_k (rhs X) (sunmat.h:200-248)
+        const R ed = (R)a.eps_drift;
         int level = a.level;
         if (level < 0 || level > 8) level = 8;
         __syncthreads();
@@ -273,51 +351,51 @@ __global__ __launch_bounds__(Cfg<NMAX>::NT) void k_kick(KickArgs a)
         for (int t = 0; t < NCT; t++)
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const int i = acc_row(q), j = acc_col(t);
+                const int i = acc_row<P, NMAX>(q), j = acc_col<P, NMAX>(t);
                 Are[i * NMAX + j] = -(pim[t][q] * ed);
                 Aim[i * NMAX + j] = pr[t][q] * ed;
             }
-        Acc<NMAX> X;
-        const float cl = level <= 1 ? 1.f : (float)a.cinv[level];
+        Acc<P, NMAX> X;
+        const R cl = level <= 1 ? R(1) : (R)a.cinv[level];
 #pragma unroll
         for (int t = 0; t < NCT; t++)
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const int i = acc_row(q), j = acc_col(t);
-                const bool d = i == j && i < n;
-                X.r[t][q] = level == 0 ? (d ? 1.f : 0.f) : (d ? 1.f : 0.f) - cl * (pim[t][q] * ed);
-                X.i[t][q] = level == 0 ? 0.f : cl * (pr[t][q] * ed);
+                const int i = acc_row<P, NMAX>(q), j = acc_col<P, NMAX>(t);
+                const R d = (i == j && i < n) ? R(1) : R(0);
+                X.r[t][q] = level == 0 ? d : d - cl * (pim[t][q] * ed);
+                X.i[t][q] = level == 0 ? R(0) : cl * (pr[t][q] * ed);
             }
         for (int k = level - 1; k >= 1; k--) {
             __syncthreads();
             acc_store(X, Bre, Bim);
             __syncthreads();
-            Acc<NMAX> Q;
+            Acc<P, NMAX> Q;
             acc_zero(Q);
-            cmul<NMAX, false, false>(Are, Aim, Bre, Bim, Q);
-            const float ck = (float)a.cinv[k];
+            cmul<P, NMAX, false, false>(Are, Aim, Bre, Bim, Q);
+            const R ck = (R)a.cinv[k];
 #pragma unroll
             for (int t = 0; t < NCT; t++)
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
-                    const int i = acc_row(q), j = acc_col(t);
-                    X.r[t][q] = (i == j && i < n ? 1.f : 0.f) + ck * Q.r[t][q];
+                    const int i = acc_row<P, NMAX>(q), j = acc_col<P, NMAX>(t);
+                    X.r[t][q] = ((i == j && i < n) ? R(1) : R(0)) + ck * Q.r[t][q];
                     X.i[t][q] = ck * Q.i[t][q];
                 }
         }
         __syncthreads();
         acc_store(X, Bre, Bim);
-        ld_link<NMAX>(Are, Aim, U, l, n);
+        ld_link<P, NMAX>(Are, Aim, U, l, n);
         __syncthreads();
-        Acc<NMAX> Un;
+        Acc<P, NMAX> Un;
         acc_zero(Un);
-        cmul<NMAX, false, false>(Bre, Bim, Are, Aim, Un);
-        float *ob = (float *)a.Uout + (size_t)l * 2 * n * n;
+        cmul<P, NMAX, false, false>(Bre, Bim, Are, Aim, Un);
+        R *ob = (R *)a.Uout + (size_t)l * 2 * n * n;
 #pragma unroll
         for (int t = 0; t < NCT; t++)
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const int i = acc_row(q), j = acc_col(t);
+                const int i = acc_row<P, NMAX>(q), j = acc_col<P, NMAX>(t);
                 if (i < n && j < n) {
                     ob[2 * (i * n + j)] = Un.r[t][q];
                     ob[2 * (i * n + j) + 1] = Un.i[t][q];
@@ -327,13 +405,13 @@ __global__ __launch_bounds__(Cfg<NMAX>::NT) void k_kick(KickArgs a)
     block_reduce_store<3>(sum, cmp, a.part + 6 * (size_t)lb);
 }
 
-template <int NMAX>
+template <int P, int NMAX>
 hipError_t kick(const KickArgs &a, bool drift, hipStream_t st)
 {
     const unsigned nb = (unsigned)(a.g.V * a.g.D);
-    constexpr int NT = Cfg<NMAX>::NT;
-    if (drift) hipLaunchKernelGGL((k_kick<NMAX, true>), dim3(nb), dim3(NT), 0, st, a);
-    else hipLaunchKernelGGL((k_kick<NMAX, false>), dim3(nb), dim3(NT), 0, st, a);
+    constexpr int NT = Cfg<P, NMAX>::NT;
+    if (drift) hipLaunchKernelGGL((k_kick<P, NMAX, true>), dim3(nb), dim3(NT), 0, st, a);
+    else hipLaunchKernelGGL((k_kick<P, NMAX, false>), dim3(nb), dim3(NT), 0, st, a);
     return hipGetLastError();
 }
 

@@ -15,4 +15,9 @@ hipError_t team_drift_64(const Geom &g, void *U, const void *P, double eps, int 
     return team::drift<64, 256>(g, U, P, eps, level, c, mode, st);
 }
 
+hipError_t team_project_64(const Geom &g, void *U, int mode, int haar, uint64_t seed, uint64_t stream, hipStream_t st)
+{
+    return team::project<64, 256>(g, U, mode, haar, seed, stream, st);
+}
+
 }  // namespace askja

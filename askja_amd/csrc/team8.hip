@@ -15,4 +15,9 @@ hipError_t team_drift_8(const Geom &g, void *U, const void *P, double eps, int l
     return team::drift<8, 64>(g, U, P, eps, level, c, mode, st);
 }
 
+hipError_t team_project_8(const Geom &g, void *U, int mode, int haar, uint64_t seed, uint64_t stream, hipStream_t st)
+{
+    return team::project<8, 64>(g, U, mode, haar, seed, stream, st);
+}
+
 }  // namespace askja

@@ -407,6 +407,155 @@ __global__ __launch_bounds__(NT) void k_drift(Geom g, R *U, const R *P, double e
     drift_core<Exact>(o, Are, Aim, Bre, Bim, n, level, cinv.c, U, l, U);
 }
 
+// SU(N) projection of one link per workgroup (N >= 4), FP64 in LDS:
+// classical Gram-Schmidt with one re-orthogonalisation pass on the rows
+// (all projections of a row computed in parallel), then the determinant by LU
+// with partial pivoting on a copy and multiplication by exp(-i arg(det)/N).
+// haar: the link is first filled with a Ginibre matrix from Philox (purpose
+// 2, counter = stream, link index = dir * V + lexicographic site; normal 2e
+// and 2e+1 are the real and imaginary parts of element e, scaled 1/sqrt 2).
+template <int NMAX, int NT, typename R>
+__global__ __launch_bounds__(NT) void k_project(Geom g, R *U, int haar, uint64_t seed, uint64_t stream)
+{
+    __shared__ double ar[NMAX * NMAX], ai[NMAX * NMAX], mr[NMAX * NMAX], mi[NMAX * NMAX];
+    __shared__ double dr[NMAX], di[NMAX], red[NT];
+    __shared__ int piv;
+    const int n = g.N, tid = threadIdx.x;
+    const long l = blockIdx.x;
+    R *b = U + (size_t)l * 2 * n * n;
+    if (haar) {
+        const int d = (int)(l / g.V);
+        const uint32_t link = (uint32_t)(d * g.V + g.dev2lex[l - d * g.V]);
+        for (int blk = tid; 4 * blk < 2 * n * n; blk += NT) {
+            double z[4];
+            normals4(philox_block(seed, stream, kPurposeHaar, link, (uint32_t)blk), z);
+            for (int q = 0; q < 4 && 4 * blk + q < 2 * n * n; q++) {
+                const int a = 4 * blk + q;
+                if (a & 1) ai[a >> 1] = z[q] * 0.7071067811865476;
+                else ar[a >> 1] = z[q] * 0.7071067811865476;
+            }
+        }
+    } else {
+        for (int e = tid; e < n * n; e += NT) {
+            ar[e] = (double)b[2 * e];
+            ai[e] = (double)b[2 * e + 1];
+        }
+    }
+    __syncthreads();
+    for (int r = 0; r < n; r++) {
+        for (int pass = 0; pass < 2; pass++) {
+            // d_q = <row q, row r> = sum_k conj(a_qk) a_rk for q < r
+            for (int q = tid; q < r; q += NT) {
+                double sr = 0.0, si = 0.0;
+                for (int k = 0; k < n; k++) {
+                    const double qr = ar[q * n + k], qi = ai[q * n + k], xr = ar[r * n + k], xi = ai[r * n + k];
+                    sr += qr * xr + qi * xi;
+                    si += qr * xi - qi * xr;
+                }
+                dr[q] = sr;
+                di[q] = si;
+            }
+            __syncthreads();
+            for (int k = tid; k < n; k += NT) {
+                double xr = ar[r * n + k], xi = ai[r * n + k];
+                for (int q = 0; q < r; q++) {
+                    const double qr = ar[q * n + k], qi = ai[q * n + k];
+                    xr -= dr[q] * qr - di[q] * qi;
+                    xi -= dr[q] * qi + di[q] * qr;
+                }
+                ar[r * n + k] = xr;
+                ai[r * n + k] = xi;
+            }
+            __syncthreads();
+        }
+        double part = 0.0;
+        for (int k = tid; k < n; k += NT) part += ar[r * n + k] * ar[r * n + k] + ai[r * n + k] * ai[r * n + k];
+        red[tid] = part;
+        __syncthreads();
+        for (int off = NT / 2; off > 0; off >>= 1) {
+            if (tid < off) red[tid] += red[tid + off];
+            __syncthreads();
+        }
+        const double inv = 1.0 / sqrt(red[0]);
+        __syncthreads();
+        for (int k = tid; k < n; k += NT) {
+            ar[r * n + k] *= inv;
+            ai[r * n + k] *= inv;
+        }
+        __syncthreads();
+    }
+    // determinant of the orthonormalised matrix by LU on a copy
+    for (int e = tid; e < n * n; e += NT) {
+        mr[e] = ar[e];
+        mi[e] = ai[e];
+    }
+    __syncthreads();
+    double detr = 1.0, deti = 0.0;  // kept identical in every thread
+    for (int c = 0; c < n; c++) {
+        if (tid == 0) {
+            int p = c;
+            double best = -1.0;
+            for (int r = c; r < n; r++) {
+                const double v = mr[r * n + c] * mr[r * n + c] + mi[r * n + c] * mi[r * n + c];
+                if (v > best) {
+                    best = v;
+                    p = r;
+                }
+            }
+            piv = p;
+        }
+        __syncthreads();
+        const int p = piv;
+        if (p != c) {
+            for (int k = tid; k < n; k += NT) {
+                double t = mr[c * n + k];
+                mr[c * n + k] = mr[p * n + k];
+                mr[p * n + k] = t;
+                t = mi[c * n + k];
+                mi[c * n + k] = mi[p * n + k];
+                mi[p * n + k] = t;
+            }
+            detr = -detr;
+            deti = -deti;
+        }
+        __syncthreads();
+        const double pr = mr[c * n + c], pim = mi[c * n + c];
+        const double nr = detr * pr - deti * pim, ni = detr * pim + deti * pr;
+        detr = nr;
+        deti = ni;
+        const double den = pr * pr + pim * pim;
+        if (den == 0.0) break;
+        // eliminate below the pivot: element (r, k), r > c, k > c
+        const int m = n - c - 1;
+        for (int e = tid; e < m * m; e += NT) {
+            const int r = c + 1 + e / m, k = c + 1 + e % m;
+            const double xr = mr[r * n + c], xi = mi[r * n + c];
+            const double fr = (xr * pr + xi * pim) / den, fi = (xi * pr - xr * pim) / den;
+            const double br = mr[c * n + k], bi = mi[c * n + k];
+            mr[r * n + k] -= fr * br - fi * bi;
+            mi[r * n + k] -= fr * bi + fi * br;
+        }
+        __syncthreads();
+    }
+    const double ph = -atan2(deti, detr) / n;
+    const double cr = cos(ph), ci = sin(ph);
+    for (int e = tid; e < n * n; e += NT) {
+        b[2 * e] = (R)(ar[e] * cr - ai[e] * ci);
+        b[2 * e + 1] = (R)(ar[e] * ci + ai[e] * cr);
+    }
+}
+
+template <int NMAX, int NT>
+hipError_t project(const Geom &g, void *U, int mode, int haar, uint64_t seed, uint64_t stream, hipStream_t st)
+{
+    const unsigned nb = (unsigned)(g.V * g.D);
+    if (f32_mode(mode))
+        hipLaunchKernelGGL((k_project<NMAX, NT, float>), dim3(nb), dim3(NT), 0, st, g, (float *)U, haar, seed, stream);
+    else
+        hipLaunchKernelGGL((k_project<NMAX, NT, double>), dim3(nb), dim3(NT), 0, st, g, (double *)U, haar, seed, stream);
+    return hipGetLastError();
+}
+
 template <int NMAX, int NT>
 hipError_t kick(const KickArgs &a, int mode, bool drift, hipStream_t st)
 {

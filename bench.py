@@ -105,6 +105,77 @@ def copy_probe(gib=1, reps=5):
     hip.hipFree(b)
 
 
+STUDY_CONFIGS = {
+    # name: (N, extents, beta, dt, LEVEL, MD steps per trajectory, precisions)
+    "C2": (3, [8, 32, 32, 32], 5.9, 0.02, 6, 50, ("fp64", "fp32")),
+    "C3": (16, [6, 24, 24, 24], 4 * 16 * 16 / 10.0, 0.01, 8, 20, ("fp64", "fp32", "bf16x3")),
+    "C5": (64, [128, 128], 64.0 * 64.0, 0.005, 8, 20, ("fp64", "fp32", "bf16x3")),
+}
+
+
+def precision_study(names):
+    """Delta-H and plaquette error of FP32 and BF16x3 against FP64: the same
+    thermalised-from-Haar configuration and the same momenta, one trajectory
+    each (the MD steps per trajectory are stated per config)."""
+    import askja_amd as A
+
+    pm = {"fp64": A.FP64, "fp32": A.FP32, "bf16x3": A.BF16X3}
+    for name in names:
+        n, ext, beta, dt, level, nd, precs = STUDY_CONFIGS[name]
+        ref = A.Lattice(n, ext, A.FP64)
+        ref.haar(SEED, 5)
+        for t in range(3):
+            ref.hmc_trajectory(beta, dt, nd, level, SEED, t, reunit=True)
+        u0 = ref.links()
+        rows = {}
+        for pn in precs:
+            lat = A.Lattice(n, ext, pm[pn])
+            lat.set_links(u0)
+            lat.mom_refresh(SEED, 1000, beta)
+            t0 = time.perf_counter()
+            e = lat.md_trajectory(dt, nd, level)
+            T = time.perf_counter() - t0
+            h0 = beta * (e[0] + e[1] / (2 * n))
+            h1 = beta * (e[2] + e[3] / (2 * n))
+            rows[pn] = {"dH": h1 - h0, "H_old": h0, "seconds": T,
+                        "plaquette_end": 1.0 - e[2] / lat.n_plaquettes()}
+            lat.close()
+        f64 = rows["fp64"]
+        for pn, v in rows.items():
+            v["dH_err_vs_fp64"] = v["dH"] - f64["dH"]
+            v["H_old_err_vs_fp64"] = v["H_old"] - f64["H_old"]
+            v["plaquette_err_vs_fp64"] = v["plaquette_end"] - f64["plaquette_end"]
+        print(json.dumps({"study": "precision", "config": name, "nc": n, "extents": ext, "beta": beta, "dt": dt,
+                          "level": level, "md_steps": nd, "results": rows}), flush=True)
+        ref.close()
+
+
+def c3_scan(ntraj, precs=("bf16x3", "fp32")):
+    """beta = 4 N^2/10 x {0.8, 0.9, 1.0, 1.1, 1.2} for SU(16), 6x24^3: mean
+    plaquette and acceptance over ntraj trajectories from a Haar start."""
+    import askja_amd as A
+
+    pm = {"fp32": A.FP32, "bf16x3": A.BF16X3, "fp64": A.FP64}
+    n, ext, beta0, dt, level, nd, _ = STUDY_CONFIGS["C3"]
+    for pn in precs:
+        lat = A.Lattice(n, ext, pm[pn])
+        for f in (0.8, 0.9, 1.0, 1.1, 1.2):
+            beta = beta0 * f
+            lat.haar(SEED, 6)
+            plaq, acc = [], 0
+            t0 = time.perf_counter()
+            for t in range(ntraj):
+                r = lat.hmc_trajectory(beta, dt, nd, level, SEED, t, reunit=False)
+                acc += r.accepted
+                plaq.append(1.0 - r.action / lat.n_plaquettes())
+            lat.sync()
+            T = time.perf_counter() - t0
+            print(json.dumps({"study": "c3_scan", "precision": pn, "beta": beta, "ntraj": ntraj,
+                              "plaquette_last_half": float(np.mean(plaq[ntraj // 2:])), "acceptance": acc / ntraj,
+                              "traj_per_s": ntraj / T}), flush=True)
+        lat.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,9 +186,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--roofline-launches", type=int, default=100)
     ap.add_argument("--copy-probe", action="store_true", help="only run a 1 GiB device copy (counter calibration)")
+    ap.add_argument("--study", choices=["precision", "c3scan"], help="print a precision or beta-scan study instead")
+    ap.add_argument("--study-configs", default="C2,C3,C5")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
+        return
+    if args.study == "precision":
+        precision_study(args.study_configs.split(","))
+        return
+    if args.study == "c3scan":
+        c3_scan(max(args.steps, 2))
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU path is not built.",

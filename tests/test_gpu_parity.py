@@ -96,6 +96,23 @@ def test_kick_drift_fast_and_fp32(oracle, n, ext):
         assert np.max(np.abs(lat.links() - o.field("u"))) <= tol * n
 
 
+@pytest.mark.parametrize("n,ext", [(9, [2, 2, 2]), (12, [2, 3, 2]), (24, [2, 2]), (40, [2, 2])],
+                         ids=["N9", "N12", "N24", "N40"])
+def test_mfma_kick_drift_padded_sizes(oracle, n, ext):
+    """N >= 9 runs the link products on MFMA (FP64 16x16x4 / FP32 16x16x4
+    tiles) with the matrix zero-padded to a multiple of 16; the padding must
+    not leak into the result."""
+    o, u, p = _state(oracle, n, ext, 12)
+    o.updateE(0.05)
+    o.updateUandcopyDown(0.05)
+    for prec, tol in ((A.FP64, 1e-13), (A.FP32, 2e-5)):
+        lat = _gpu(n, ext, prec, u, p)
+        lat.kick_drift(0.05, 0.05, 8)
+        scale_p = np.max(np.abs(o.field("pi")))
+        assert np.max(np.abs(lat.momenta() - o.field("pi"))) <= tol * scale_p * n, prec
+        assert np.max(np.abs(lat.links() - o.field("u"))) <= tol * n, prec
+
+
 @pytest.mark.parametrize("prec", [A.FP64_EXACT, A.FP64, A.FP32])
 @pytest.mark.parametrize("n,ext", CASES[:5], ids=IDS[:5])
 def test_fused_kick_drift_equals_separate(oracle, n, ext, prec):
