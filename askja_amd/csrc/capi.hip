@@ -43,7 +43,7 @@ struct askja_ctx {
     std::string err;
     Geom geom{};
     // learned field transformation
-    int flow_nsub = 0, flow_nblocks = 0, flow_bsize = 0, flow_seg = 8;
+    int flow_nsub = 0, flow_nblocks = 0, flow_bsize = 0;
     std::vector<FlowSub> flow_subs;
     std::vector<int> flow_block;
     float *flow_w = nullptr;     // nblocks * bsize MLP weights
@@ -51,7 +51,9 @@ struct askja_ctx {
     double *flow_g = nullptr;    // 9 doubles per link: Lie gradient
     double *flow_lj = nullptr;   // per sub-layer log J (s, c)
     void *flow_out = nullptr;    // U = F(V)
-    std::vector<void *> flow_ck, flow_work;
+    void *flow_work = nullptr;   // backward working field
+    void *flow_log = nullptr;    // input links of every sub-layer
+    double *flow_contrib = nullptr;  // backward neighbour contributions
 };
 
 namespace {
@@ -265,6 +267,38 @@ int askja_create(int nc, int dim, const int *extents, int precision, askja_ctx *
         key[i] = (long)(xs / chunk) * V + i;
         if (par[i] == 0) n0++;
     }
+    if (const char *ord = getenv("ASKJA_SITE_ORDER")) {
+        // experimental: mixed-radix key over items "d/b" (block index of
+        // x_d), "d%b" (x_d within its block) or "d" (x_d)
+        struct Item { int d, b, kind; };
+        std::vector<Item> items;
+        for (const char *q = ord; *q;) {
+            Item it{(int)strtol(q, (char **)&q, 10), 0, 0};
+            if (*q == '/' || *q == '%') {
+                it.kind = *q == '/' ? 1 : 2;
+                it.b = (int)strtol(q + 1, (char **)&q, 10);
+            }
+            items.push_back(it);
+            if (*q == ',') q++;
+        }
+        for (long i = 0; i < V; i++) {
+            long k = 0;
+            for (const Item &it : items) {
+                const int xd = (int)((i / c->helparr[it.d]) % extents[it.d]);
+                const int e = extents[it.d];
+                if (it.kind == 1) k = k * ((e + it.b - 1) / it.b) + xd / it.b;
+                else if (it.kind == 2) k = k * it.b + xd % it.b;
+                else k = k * e + xd;
+            }
+            key[i] = k;
+        }
+        std::vector<long> chk(key);
+        std::sort(chk.begin(), chk.end());
+        if (std::adjacent_find(chk.begin(), chk.end()) != chk.end()) {
+            delete c;
+            return ASKJA_EINVAL;
+        }
+    }
     {
         std::vector<long> order(V);
         for (long i = 0; i < V; i++) order[i] = i;
@@ -346,12 +380,8 @@ int askja_destroy(askja_ctx *c)
         if (c->links[b]) (void)hipFree(c->links[b]);
     void *bufs[] = {c->snap, c->mom, c->mom_prev, c->part, c->res, c->scratch, c->staging,
                     c->d_nbr, c->d_dev2lex, c->d_lex2dev, c->flow_w, c->flow_gw, c->flow_g, c->flow_lj,
-                    c->flow_out};
+                    c->flow_out, c->flow_work, c->flow_log, c->flow_contrib};
     for (void *p : bufs)
-        if (p) (void)hipFree(p);
-    for (void *p : c->flow_ck)
-        if (p) (void)hipFree(p);
-    for (void *p : c->flow_work)
         if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -810,8 +840,13 @@ int flow_ready(askja_ctx *c)
 
 size_t link_bytes(const askja_ctx *c) { return c->link_reals * elem_bytes(c); }
 
-// U = F(V) into flow_out; with store, the inputs of sub-layers 0, K, 2K, ...
-// are kept in flow_ck.  Per sub-layer log J into flow_lj.
+void *flow_log_of(askja_ctx *c, int k)
+{
+    return (char *)c->flow_log + (size_t)k * flow_log_reals(c->geom) * elem_bytes(c);
+}
+
+// U = F(V) into flow_out; with store, the input links of every sub-layer
+// go to flow_log.  Per sub-layer log J into flow_lj.
 int flow_forward(askja_ctx *c, bool store)
 {
     HIPCHK(c, hipMemcpyAsync(c->flow_out, c->links[c->cur], link_bytes(c), hipMemcpyDeviceToDevice, c->st));
@@ -821,11 +856,9 @@ int flow_forward(askja_ctx *c, bool store)
     int rc = ensure_part(c, (size_t)np * 2);
     if (rc) return rc;
     for (int k = 0; k < c->flow_nsub; k++) {
-        if (store && k % c->flow_seg == 0)
-            HIPCHK(c, hipMemcpyAsync(c->flow_ck[k / c->flow_seg], c->flow_out, link_bytes(c), hipMemcpyDeviceToDevice,
-                                     c->st));
         const FlowSub &f = c->flow_subs[k];
-        HIPCHK(c, launch_flow_forward(c->geom, c->flow_out, c->flow_out, f, c->mode, c->part, c->st));
+        HIPCHK(c, launch_flow_forward(c->geom, c->flow_out, c->flow_out, f, c->mode, c->part,
+                                      store ? flow_log_of(c, k) : nullptr, c->st));
         HIPCHK(c, launch_finalize(c->part, flow_forward_partials(c->geom, f.par), 1, c->flow_lj + 2 * k, c->st));
     }
     return ASKJA_OK;
@@ -856,21 +889,13 @@ int flow_gradient(askja_ctx *c, double beta, bool wgrad, double *action)
     HIPCHK(c, launch_finalize(c->part, np, 1, c->res, c->st));
     if (wgrad)
         HIPCHK(c, hipMemsetAsync(c->flow_gw, 0, (size_t)c->flow_nblocks * c->flow_bsize * sizeof(double), c->st));
-    const int K = c->flow_seg;
-    const int nseg = (c->flow_nsub + K - 1) / K;
-    for (int j = nseg - 1; j >= 0; j--) {
-        const int k0 = j * K, k1 = std::min(c->flow_nsub, k0 + K);
-        HIPCHK(c, hipMemcpyAsync(c->flow_work[0], c->flow_ck[j], link_bytes(c), hipMemcpyDeviceToDevice, c->st));
-        for (int k = k0; k + 1 < k1; k++) {
-            void *in = c->flow_work[k - k0], *out = c->flow_work[k - k0 + 1];
-            HIPCHK(c, hipMemcpyAsync(out, in, link_bytes(c), hipMemcpyDeviceToDevice, c->st));
-            HIPCHK(c, launch_flow_forward(c->geom, out, out, c->flow_subs[k], c->mode, c->part, c->st));
-        }
-        for (int k = k1 - 1; k >= k0; k--) {
-            double *gw = wgrad ? c->flow_gw + (size_t)c->flow_block[k] * c->flow_bsize : nullptr;
-            HIPCHK(c, launch_flow_backward(c->geom, c->flow_work[k - k0], c->flow_subs[k], c->mode, c->flow_g, -1.0, gw,
-                                           c->st));
-        }
+    // backward through the sub-layers in reverse on a copy of F(V); each
+    // backward launch restores its sub-layer's input links from the log
+    HIPCHK(c, hipMemcpyAsync(c->flow_work, c->flow_out, link_bytes(c), hipMemcpyDeviceToDevice, c->st));
+    for (int k = c->flow_nsub - 1; k >= 0; k--) {
+        double *gw = wgrad ? c->flow_gw + (size_t)c->flow_block[k] * c->flow_bsize : nullptr;
+        HIPCHK(c, launch_flow_backward(c->geom, c->flow_work, c->flow_subs[k], c->mode, c->flow_g, -1.0, gw,
+                                       flow_log_of(c, k), c->flow_contrib, c->st));
     }
     if (action) {
         double h[2];
@@ -900,7 +925,8 @@ int askja_flow_define(askja_ctx *c, int nsub, const int *mu, const int *parity, 
 {
     ENTER(c);
     if (c->nc != 3) return fail(c, ASKJA_EUNSUP, "flow: only SU(3) is supported");
-    if (c->dim < 3) return fail(c, ASKJA_EUNSUP, "flow: needs D >= 3 (planes orthogonal to every direction)");
+    if (c->dim < 3 || c->dim > 4)
+        return fail(c, ASKJA_EUNSUP, "flow: D = 3 or 4 (D >= 3: planes orthogonal to every direction)");
     if (nsub < 1 || nsub > 4096 || nblocks < 1 || !mu || !parity || !block)
         return fail(c, ASKJA_EINVAL, "flow_define: bad sizes or null arrays");
     if (!(rho_max > 0.0) || rho_max * 2.0 * (c->dim - 1) >= 2.0)
@@ -911,15 +937,14 @@ int askja_flow_define(askja_ctx *c, int nsub, const int *mu, const int *parity, 
         if (mu[k] < 0 || mu[k] >= c->dim || (parity[k] != 0 && parity[k] != 1) || block[k] < 0 || block[k] >= nblocks)
             return fail(c, ASKJA_EINVAL, "flow_define: sub-layer %d out of range", k);
     const int nf = (c->dim - 1) * (c->dim - 2) / 2;
-    if (nf > 28) return fail(c, ASKJA_EUNSUP, "flow: at most 28 features (D <= 9)");
     // (re)allocate
-    void *old[] = {c->flow_w, c->flow_gw, c->flow_g, c->flow_lj, c->flow_out};
+    void *old[] = {c->flow_w, c->flow_gw, c->flow_g, c->flow_lj, c->flow_out, c->flow_work, c->flow_log, c->flow_contrib};
     for (void *p : old)
         if (p) (void)hipFree(p);
-    for (void *p : c->flow_ck) (void)hipFree(p);
-    for (void *p : c->flow_work) (void)hipFree(p);
-    c->flow_ck.clear();
-    c->flow_work.clear();
+    c->flow_w = nullptr;
+    c->flow_gw = c->flow_g = c->flow_lj = nullptr;
+    c->flow_out = c->flow_work = c->flow_log = nullptr;
+    c->flow_contrib = nullptr;
     c->flow_nsub = 0;
     c->flow_bsize = 32 * nf + 2 * 32 + 1;
     c->flow_nblocks = nblocks;
@@ -929,17 +954,9 @@ int askja_flow_define(askja_ctx *c, int nsub, const int *mu, const int *parity, 
     HIPCHK(c, hipMalloc(&c->flow_g, (size_t)c->NL * 9 * sizeof(double)));
     HIPCHK(c, hipMalloc(&c->flow_lj, (size_t)2 * nsub * sizeof(double)));
     HIPCHK(c, hipMalloc(&c->flow_out, link_bytes(c)));
-    const int nseg = (nsub + c->flow_seg - 1) / c->flow_seg;
-    for (int j = 0; j < nseg; j++) {
-        void *p = nullptr;
-        HIPCHK(c, hipMalloc(&p, link_bytes(c)));
-        c->flow_ck.push_back(p);
-    }
-    for (int j = 0; j < std::min(nsub, c->flow_seg); j++) {
-        void *p = nullptr;
-        HIPCHK(c, hipMalloc(&p, link_bytes(c)));
-        c->flow_work.push_back(p);
-    }
+    HIPCHK(c, hipMalloc(&c->flow_work, link_bytes(c)));
+    HIPCHK(c, hipMalloc(&c->flow_log, (size_t)nsub * flow_log_reals(c->geom) * elem_bytes(c)));
+    HIPCHK(c, hipMalloc((void **)&c->flow_contrib, flow_contrib_doubles(c->geom) * sizeof(double)));
     c->flow_subs.resize(nsub);
     c->flow_block.assign(block, block + nsub);
     for (int k = 0; k < nsub; k++)

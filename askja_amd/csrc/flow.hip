@@ -259,6 +259,36 @@ __device__ inline void gput(double *g, long l, const C3 &h)
     b[8] = h.i[1][2];
 }
 
+// Contribution slots of the backward (no atomics): the backward thread of
+// updated link t writes what it adds to each neighbouring link into its own
+// slot, slot-major planes of 9 doubles: c[(slot * 9 + e) * T + t]; the
+// gather kernel then sums, for every link, the slots that name it.
+// Slots of link (x, MU): staple direction i (nu != MU, i = nu - (nu > MU))
+// roles 0..5 = targets (x+MU, nu), (x+nu, MU), (x, nu), (x+MU-nu, nu),
+// (x-nu, MU), (x-nu, nu) -> slot 6 i + role; plaquette plane j (nu < la,
+// neither MU, in loop order) roles 0..3 = targets (x, nu), (x+nu, la),
+// (x+la, nu), (x, la) -> slot 6 (D-1) + 4 j + role.
+__device__ inline void cput(double *c, int slot, long t, long T, const C3 &h)
+{
+    double *b = c + (size_t)slot * 9 * T + t;
+    b[0 * T] = h.r[0][0];
+    b[1 * T] = h.r[1][1];
+    b[2 * T] = h.r[2][2];
+    b[3 * T] = h.r[0][1];
+    b[4 * T] = h.i[0][1];
+    b[5 * T] = h.r[0][2];
+    b[6 * T] = h.i[0][2];
+    b[7 * T] = h.r[1][2];
+    b[8 * T] = h.i[1][2];
+}
+
+__device__ inline void cacc(double (&a)[9], const double *c, int slot, long t, long T)
+{
+    const double *b = c + (size_t)slot * 9 * T + t;
+#pragma unroll
+    for (int e = 0; e < 9; e++) a[e] += b[e * T];
+}
+
 struct Sub {
     int mu, par;
     const float *w;  // MLP block: W1[kHid][nf], b1[kHid], w2[kHid], b2
@@ -266,58 +296,85 @@ struct Sub {
     double rho_max, delta;
 };
 
-// every forward quantity of one updated link
+constexpr int nfeat(int D) { return (D - 1) * (D - 2) / 2; }
+
+// the forward quantities of one updated link that the backward reuses
+template <int D>
 struct Fwd {
-    C3 U, S, W, Lam, Lh, E;
-    double f[28], h[kHid];
-    double n, z, rho, theta, K;
+    C3 U, W, Lh;  // own link (sub-layer input), U S, normalised TAH(C)
+    double n, rho, theta, K;
+    double f[nfeat(D)];
 };
 
-template <typename R>
-__device__ inline void forward_link(const Geom &g, const R *U, const Sub &sb, long s, Fwd &F)
+// f[j] = v for a run-time j, keeping f in registers
+template <int NF>
+__device__ inline void set_feature(double (&f)[NF], int j, double v)
 {
+#pragma unroll
+    for (int q = 0; q < NF; q++)
+        if (q == j) f[q] = v;
+}
+
+// Plaquette features, MLP, staples and rotation angle of link (x = s, MU).
+// F.U must hold the own link on entry.  The plaquettes come first (C,
+// features, Lhat), then the staples (S, W = U S), so that only one
+// accumulator is live at a time; the loops stay rolled (a fully unrolled
+// stencil hoists every neighbour load and spills).
+template <typename R, int D, int MU>
+__device__ inline void forward_link(const Geom &g, const R *U, const Sub &sb, long s, Fwd<D> &F)
+{
+    constexpr int nf = nfeat(D);
     const long V = g.V;
-    const int mu = sb.mu;
-    const long xp = g.nbr[(2 * mu) * V + s];
-    F.U = ld(U, mu * V + s);
-    zero(F.S);
-    C3 C;
-    zero(C);
-    int j = 0;
-    for (int nu = 0; nu < g.D; nu++) {
-        if (nu == mu) continue;
-        const long xpn = g.nbr[(2 * nu) * V + s];
-        const long xmn = g.nbr[(2 * nu + 1) * V + s];
-        const long xpmn = g.nbr[(2 * nu + 1) * V + xp];
-        const C3 a = ld(U, nu * V + xp), b = ld(U, mu * V + xpn), c = ld(U, nu * V + s);
-        addto(F.S, mm<false, true>(mm<false, true>(a, b), c));
-        const C3 d = ld(U, nu * V + xpmn), e = ld(U, mu * V + xmn), f = ld(U, nu * V + xmn);
-        addto(F.S, mm<false, false>(mm<true, true>(d, e), f));
-        for (int la = nu + 1; la < g.D; la++) {
-            if (la == mu) continue;
-            const long xnu = xpn, xla = g.nbr[(2 * la) * V + s];
-            const C3 P = mm<false, true>(mm<false, true>(mm<false, false>(c, ld(U, la * V + xnu)), ld(U, nu * V + xla)),
-                                         ld(U, la * V + s));
-            F.f[j++] = retr(P) / 3.0;
-            addto(C, P);
+    {
+        C3 C;
+        zero(C);
+        int j = 0;
+#pragma unroll 1
+        for (int nu = 0; nu < D; nu++) {
+            if (nu == MU) continue;
+            const long xpn = g.nbr[(2 * nu) * V + s];
+#pragma unroll 1
+            for (int la = nu + 1; la < D; la++) {
+                if (la == MU) continue;
+                const long xla = g.nbr[(2 * la) * V + s];
+                const C3 P = mm<false, true>(
+                    mm<false, true>(mm<false, false>(ld(U, nu * V + s), ld(U, la * V + xpn)), ld(U, nu * V + xla)),
+                    ld(U, la * V + s));
+                set_feature(F.f, j++, retr(P) / 3.0);
+                addto(C, P);
+            }
         }
+        const C3 Lam = tah(C);
+        F.n = sqrt(2.0 * retr(mm<false, false>(Lam, Lam)) + sb.delta * sb.delta);
+        F.Lh = cscale(Lam, 1.0 / F.n, 0.0);
     }
-    F.W = mm<false, false>(F.U, F.S);
-    F.Lam = tah(C);
-    F.n = sqrt(2.0 * retr(mm<false, false>(F.Lam, F.Lam)) + sb.delta * sb.delta);
-    F.Lh = cscale(F.Lam, 1.0 / F.n, 0.0);
-    double z = (double)sb.w[kHid * sb.nf + 2 * kHid];
+    const float *w = sb.w;
+    double z = (double)w[kHid * nf + 2 * kHid];
+#pragma unroll 1
     for (int k = 0; k < kHid; k++) {
-        double a = (double)sb.w[kHid * sb.nf + k];
-        for (int q = 0; q < sb.nf; q++) a += (double)sb.w[k * sb.nf + q] * F.f[q];
-        F.h[k] = tanh(a);
-        z += (double)sb.w[kHid * sb.nf + kHid + k] * F.h[k];
+        double a = (double)w[kHid * nf + k];
+#pragma unroll
+        for (int q = 0; q < nf; q++) a += (double)w[k * nf + q] * F.f[q];
+        z += (double)w[kHid * nf + kHid + k] * tanh(a);
     }
-    F.z = z;
     F.rho = sb.rho_max * tanh(z);
+    {
+        const long xp = g.nbr[(2 * MU) * V + s];
+        C3 S;
+        zero(S);
+#pragma unroll 1
+        for (int nu = 0; nu < D; nu++) {
+            if (nu == MU) continue;
+            const long xpn = g.nbr[(2 * nu) * V + s];
+            const long xmn = g.nbr[(2 * nu + 1) * V + s];
+            const long xpmn = g.nbr[(2 * nu + 1) * V + xp];
+            addto(S, mm<false, true>(mm<false, true>(ld(U, nu * V + xp), ld(U, MU * V + xpn)), ld(U, nu * V + s)));
+            addto(S, mm<false, false>(mm<true, true>(ld(U, nu * V + xpmn), ld(U, MU * V + xmn)), ld(U, nu * V + xmn)));
+        }
+        F.W = mm<false, false>(F.U, S);
+    }
     F.theta = F.rho * imtr(mm<false, false>(F.Lh, F.W));
     F.K = retr(mm<false, false>(mm<false, false>(F.Lh, F.Lh), F.W));
-    F.E = expi(cscale(F.Lh, F.theta, 0.0));
 }
 
 // sites of parity p: device sites [0, V0) or [V0, V)
@@ -328,138 +385,276 @@ __device__ inline bool site_of(const Geom &g, int par, long t, long &s)
     return t < len;
 }
 
-template <typename R>
-__global__ __launch_bounds__(kB) void k_forward(Geom g, const R *Uin, R *Uout, Sub sb, double *part)
+// One sub-layer in place (Uin may equal Uout: no thread reads another
+// thread's updated link).  With log, the sub-layer's input links are kept,
+// link of parity-site t at index t (64-link tiles), for the backward.
+template <typename R, int D, int MU>
+__global__ __launch_bounds__(kB) void k_forward(Geom g, const R *Uin, R *Uout, Sub sb, double *part, R *log)
 {
     const long t = (long)blockIdx.x * kB + threadIdx.x;
     double sum[1] = {0.0}, cmp[1] = {0.0};
     long s;
     if (site_of(g, sb.par, t, s)) {
-        Fwd F;
-        forward_link(g, Uin, sb, s, F);
-        st(Uout, sb.mu * g.V + s, mm<false, false>(F.E, F.U));
+        Fwd<D> F;
+        F.U = ld(Uin, MU * g.V + s);
+        if (log) st(log, t, F.U);
+        forward_link<R, D, MU>(g, Uin, sb, s, F);
+        const C3 E = expi(cscale(F.Lh, F.theta, 0.0));
+        st(Uout, MU * g.V + s, mm<false, false>(E, F.U));
         sum[0] = log1p(F.rho * F.K);
     }
     block_reduce_store<1>(sum, cmp, part + 2 * (size_t)blockIdx.x);
 }
 
-// Backward of one sub-layer.  g holds the gradient with respect to the
-// sub-layer's output on entry and with respect to its input on exit.
-// c_logj = dPhi/dlogJ (-1 for Phi = beta S - log J).  gw (may be null)
-// accumulates dPhi/dweights of the sub-layer's MLP block.
-template <typename R>
-__global__ __launch_bounds__(kB) void k_backward(Geom g, const R *U, Sub sb, double *gr, double c_logj, double *gw)
+__device__ inline double wave_sum(double v)
 {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Backward of one sub-layer.  U holds the sub-layer's output field on entry;
+// the kernel restores its input (the updated links from log) in place.  gr
+// holds the gradient with respect to the sub-layer's output on entry and
+// with respect to its input on exit.  c_logj = dPhi/dlogJ (-1 for
+// Phi = beta S - log J).  gw (may be null) accumulates dPhi/dweights of the
+// sub-layer's MLP block.  Every forward quantity is recomputed from the
+// sub-layer input (layer recomputation; nothing but the input is stored).
+template <typename R, int D, int MU>
+__global__ __launch_bounds__(kB) void k_backward(Geom g, R *U, Sub sb, double *gr, double c_logj, double *gw,
+                                                 const R *log, double *contrib, long T)
+{
+    constexpr int nf = nfeat(D);
     const long t = (long)blockIdx.x * kB + threadIdx.x;
     long s;
-    if (!site_of(g, sb.par, t, s)) return;
+    const bool ok = site_of(g, sb.par, t, s);
     const long V = g.V;
-    const int mu = sb.mu;
-    const long l = mu * V + s;
-    Fwd F;
-    forward_link(g, U, sb, s, F);
-    const C3 gp = gget(gr, l);
-    // U' = E U
-    C3 gU = proj(mm<true, false>(F.E, mm<false, false>(gp, F.E)));
-    const double dtheta = 2.0 * retr(mm<false, false>(gp, F.Lh));
-    const C3 X = cscale(F.Lh, F.theta, 0.0);
-    C3 gLh = cscale(avg_ad(X, gp), F.theta, 0.0);
-    // theta = rho Im Tr(Lhat W)
-    double drho = dtheta * imtr(mm<false, false>(F.Lh, F.W));
-    C3 WmWd;
+    Fwd<D> F;
 #pragma unroll
-    for (int p = 0; p < 3; p++)
-#pragma unroll
-        for (int q = 0; q < 3; q++) {
-            WmWd.r[p][q] = F.W.r[p][q] - F.W.r[q][p];
-            WmWd.i[p][q] = F.W.i[p][q] + F.W.i[q][p];
+    for (int q = 0; q < nf; q++) F.f[q] = 0.0;
+    double dz = 0.0;
+    C3 GC;
+    if (ok) {
+        const long l = MU * V + s;
+        F.U = ld(log, t);
+        st(U, l, F.U);
+        forward_link<R, D, MU>(g, U, sb, s, F);
+        const C3 gp = gget(gr, l);
+        // U' = E U,  E = exp(i theta Lhat)
+        C3 gU;
+        {
+            const C3 E = expi(cscale(F.Lh, F.theta, 0.0));
+            gU = proj(mm<true, false>(E, mm<false, false>(gp, E)));
         }
-    addto(gLh, cscale(WmWd, 0.0, -0.25 * dtheta * F.rho));  // dtheta rho (W - W^+)/(4i)
-    C3 GW = cscale(F.Lh, 0.0, -0.5 * dtheta * F.rho);     // dtheta rho (-i Lhat)/2
-    // log J = log(1 + rho K), K = ReTr(Lhat^2 W)
-    const double a = c_logj / (1.0 + F.rho * F.K);
-    drho += a * F.K;
-    C3 M = mm<false, false>(F.Lh, F.W);
-    addto(M, mm<false, false>(F.W, F.Lh));
-    C3 Md;
+        const double dtheta = 2.0 * retr(mm<false, false>(gp, F.Lh));
+        C3 gLh = cscale(avg_ad(cscale(F.Lh, F.theta, 0.0), gp), F.theta, 0.0);
+        // theta = rho Im Tr(Lhat W)
+        double drho = dtheta * imtr(mm<false, false>(F.Lh, F.W));
+        {
+            C3 WmWd;
 #pragma unroll
-    for (int p = 0; p < 3; p++)
+            for (int p = 0; p < 3; p++)
 #pragma unroll
-        for (int q = 0; q < 3; q++) {
-            Md.r[p][q] = M.r[p][q] + M.r[q][p];
-            Md.i[p][q] = M.i[p][q] - M.i[q][p];
+                for (int q = 0; q < 3; q++) {
+                    WmWd.r[p][q] = F.W.r[p][q] - F.W.r[q][p];
+                    WmWd.i[p][q] = F.W.i[p][q] + F.W.i[q][p];
+                }
+            addto(gLh, cscale(WmWd, 0.0, -0.25 * dtheta * F.rho));  // dtheta rho (W - W^+)/(4i)
         }
-    addto(gLh, Md, 0.25 * a * F.rho);
-    addto(GW, mm<false, false>(F.Lh, F.Lh), 0.5 * a * F.rho);
-    // rho = rho_max tanh(z), MLP
-    const double th = F.rho / sb.rho_max;
-    const double dz = drho * sb.rho_max * (1.0 - th * th);
-    double df[28];
-    for (int q = 0; q < sb.nf; q++) df[q] = 0.0;
-    const int nf = sb.nf;
-    for (int k = 0; k < kHid; k++) {
-        const double dh = dz * (double)sb.w[kHid * nf + kHid + k];
-        const double da = dh * (1.0 - F.h[k] * F.h[k]);
-        for (int q = 0; q < nf; q++) df[q] += da * (double)sb.w[k * nf + q];
-        if (gw) {
-            atomicAdd(gw + kHid * nf + kHid + k, dz * F.h[k]);
-            atomicAdd(gw + kHid * nf + k, da);
-            for (int q = 0; q < nf; q++) atomicAdd(gw + k * nf + q, da * F.f[q]);
+        C3 GW = cscale(F.Lh, 0.0, -0.5 * dtheta * F.rho);  // dtheta rho (-i Lhat)/2
+        // log J = log(1 + rho K), K = ReTr(Lhat^2 W)
+        const double a = c_logj / (1.0 + F.rho * F.K);
+        drho += a * F.K;
+        {
+            C3 M = mm<false, false>(F.Lh, F.W);
+            addto(M, mm<false, false>(F.W, F.Lh));
+            C3 Md;
+#pragma unroll
+            for (int p = 0; p < 3; p++)
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    Md.r[p][q] = M.r[p][q] + M.r[q][p];
+                    Md.i[p][q] = M.i[p][q] - M.i[q][p];
+                }
+            addto(gLh, Md, 0.25 * a * F.rho);
+        }
+        addto(GW, mm<false, false>(F.Lh, F.Lh), 0.5 * a * F.rho);
+        // rho = rho_max tanh(z)
+        const double th = F.rho / sb.rho_max;
+        dz = drho * sb.rho_max * (1.0 - th * th);
+        // Lhat = Lam / n, Lam = n Lhat
+        {
+            const double tgl = 2.0 * F.n * retr(mm<false, false>(gLh, F.Lh));
+            C3 gLam = cscale(gLh, 1.0 / F.n, 0.0);
+            addto(gLam, F.Lh, -tgl / (F.n * F.n));
+            GC = cscale(proj(gLam), 0.0, -1.0);  // Lam = TAH(C)
+        }
+        // own link through W = U S
+        addto(gU, proj(cscale(mm<false, false>(F.W, GW), 0.0, 1.0)));
+        gput(gr, l, gU);
+        // staple links: G_S = G_W U
+        const C3 GS = mm<false, false>(GW, F.U);
+        const long xp = g.nbr[(2 * MU) * V + s];
+#pragma unroll 1
+        for (int nu = 0; nu < D; nu++) {
+            if (nu == MU) continue;
+            const long xpn = g.nbr[(2 * nu) * V + s];
+            const long xmn = g.nbr[(2 * nu + 1) * V + s];
+            const long xpmn = g.nbr[(2 * nu + 1) * V + xp];
+            const int sl = 6 * (nu - (nu > MU));
+            {
+                const C3 ca = ld(U, nu * V + xp), cb = ld(U, MU * V + xpn), cc = ld(U, nu * V + s);
+                // t1 = a b^+ c^+
+                const C3 abd = mm<false, true>(ca, cb);
+                const C3 t1 = mm<false, true>(abd, cc);
+                cput(contrib, sl + 0, t, T, proj(cscale(mm<false, false>(t1, GS), 0.0, 1.0)));
+                cput(contrib, sl + 1, t, T, proj(cscale(mm<true, false>(cc, mm<false, false>(GS, abd)), 0.0, -1.0)));
+                cput(contrib, sl + 2, t, T, proj(cscale(mm<false, false>(GS, t1), 0.0, -1.0)));
+            }
+            {
+                const C3 cd = ld(U, nu * V + xpmn), ce = ld(U, MU * V + xmn), cf = ld(U, nu * V + xmn);
+                // t2 = d^+ e^+ f
+                const C3 de = mm<true, true>(cd, ce);
+                const C3 fGS = mm<false, false>(cf, GS);
+                cput(contrib, sl + 3, t, T, proj(cscale(mm<false, true>(mm<true, false>(ce, fGS), cd), 0.0, -1.0)));
+                const C3 fd = mm<false, false>(fGS, de);
+                cput(contrib, sl + 4, t, T, proj(cscale(fd, 0.0, -1.0)));
+                cput(contrib, sl + 5, t, T, proj(cscale(fd, 0.0, 1.0)));
+            }
         }
     }
-    if (gw) atomicAdd(gw + kHid * nf + 2 * kHid, dz);
-    // Lhat = Lam / n
-    const double tgl = 2.0 * retr(mm<false, false>(gLh, F.Lam));
-    C3 gLam = cscale(gLh, 1.0 / F.n, 0.0);
-    addto(gLam, F.Lam, -tgl / (F.n * F.n * F.n));
-    gLam = proj(gLam);
-    const C3 GC = cscale(gLam, 0.0, -1.0);  // Lam = TAH(C)
-    // own link through W = U S
-    addto(gU, proj(cscale(mm<false, false>(F.W, GW), 0.0, 1.0)));
-    gput(gr, l, gU);
-    // staple links: G_S = G_W U
-    const C3 GS = mm<false, false>(GW, F.U);
-    const long xp = g.nbr[(2 * mu) * V + s];
+    // MLP backward, every lane (inactive lanes carry dz = 0, f = 0) so that
+    // the weight gradient can be summed over the wavefront first
+    const float *w = sb.w;
+    double df[nf > 0 ? nf : 1];
+#pragma unroll
+    for (int q = 0; q < nf; q++) df[q] = 0.0;
+    const bool lead = (threadIdx.x & 63) == 0;
+#pragma unroll 1
+    for (int k = 0; k < kHid; k++) {
+        double a = (double)w[kHid * nf + k];
+#pragma unroll
+        for (int q = 0; q < nf; q++) a += (double)w[k * nf + q] * F.f[q];
+        const double h = tanh(a);
+        const double dh = dz * (double)w[kHid * nf + kHid + k];
+        const double da = dh * (1.0 - h * h);
+#pragma unroll
+        for (int q = 0; q < nf; q++) df[q] += da * (double)w[k * nf + q];
+        if (gw) {
+            const double s_w2 = wave_sum(dz * h), s_b1 = wave_sum(da);
+            if (lead) {
+                atomicAdd(gw + kHid * nf + kHid + k, s_w2);
+                atomicAdd(gw + kHid * nf + k, s_b1);
+            }
+#pragma unroll
+            for (int q = 0; q < nf; q++) {
+                const double s_w1 = wave_sum(da * F.f[q]);
+                if (lead) atomicAdd(gw + k * nf + q, s_w1);
+            }
+        }
+    }
+    if (gw) {
+        const double s_b2 = wave_sum(dz);
+        if (lead) atomicAdd(gw + kHid * nf + 2 * kHid, s_b2);
+    }
+    if (!ok) return;
+    // plaquettes P = p1 p2 p3^+ p4^+ in the planes (nu, la) without MU:
+    // dPhi/dLam through C = sum P, and dPhi/df through ReTr P / 3
     int j = 0;
-    for (int nu = 0; nu < g.D; nu++) {
-        if (nu == mu) continue;
+#pragma unroll 1
+    for (int nu = 0; nu < D; nu++) {
+        if (nu == MU) continue;
         const long xpn = g.nbr[(2 * nu) * V + s];
-        const long xmn = g.nbr[(2 * nu + 1) * V + s];
-        const long xpmn = g.nbr[(2 * nu + 1) * V + xp];
-        const C3 ca = ld(U, nu * V + xp), cb = ld(U, mu * V + xpn), cc = ld(U, nu * V + s);
-        // t1 = a b^+ c^+
-        const C3 abd = mm<false, true>(ca, cb);
-        const C3 t1 = mm<false, true>(abd, cc);
-        gadd(gr, nu * V + xp, proj(cscale(mm<false, false>(t1, GS), 0.0, 1.0)));
-        gadd(gr, mu * V + xpn, proj(cscale(mm<true, false>(cc, mm<false, false>(GS, abd)), 0.0, -1.0)));
-        gadd(gr, nu * V + s, proj(cscale(mm<false, false>(GS, t1), 0.0, -1.0)));
-        const C3 cd = ld(U, nu * V + xpmn), ce = ld(U, mu * V + xmn), cf = ld(U, nu * V + xmn);
-        // t2 = d^+ e^+ f
-        const C3 de = mm<true, true>(cd, ce);
-        const C3 fGS = mm<false, false>(cf, GS);
-        gadd(gr, nu * V + xpmn, proj(cscale(mm<false, true>(mm<true, false>(ce, fGS), cd), 0.0, -1.0)));
-        gadd(gr, mu * V + xmn, proj(cscale(mm<false, false>(fGS, de), 0.0, -1.0)));
-        gadd(gr, nu * V + xmn, proj(cscale(mm<false, false>(fGS, de), 0.0, 1.0)));
-        // plaquettes P = p1 p2 p3^+ p4^+ in planes (nu, la) not containing mu
-        for (int la = nu + 1; la < g.D; la++) {
-            if (la == mu) continue;
+#pragma unroll 1
+        for (int la = nu + 1; la < D; la++) {
+            if (la == MU) continue;
             const long xla = g.nbr[(2 * la) * V + s];
-            const C3 p1 = cc, p2 = ld(U, la * V + xpn), p3 = ld(U, nu * V + xla), p4 = ld(U, la * V + s);
+            const C3 p1 = ld(U, nu * V + s), p2 = ld(U, la * V + xpn), p3 = ld(U, nu * V + xla), p4 = ld(U, la * V + s);
             C3 GP = GC;
-            const double cf_ = df[j++] / 6.0;
+            double dfj = 0.0;
+#pragma unroll
+            for (int q = 0; q < nf; q++)
+                if (q == j) dfj = df[q];
+            j++;
+            const double cf_ = dfj / 6.0;
 #pragma unroll
             for (int p = 0; p < 3; p++) GP.r[p][p] += cf_;
             const C3 p12 = mm<false, false>(p1, p2);
             const C3 p123 = mm<false, true>(p12, p3);
             const C3 P = mm<false, true>(p123, p4);
-            gadd(gr, nu * V + s, proj(cscale(mm<false, false>(P, GP), 0.0, 1.0)));
-            gadd(gr, la * V + xpn, proj(cscale(mm<false, false>(mm<false, true>(mm<false, true>(p2, p3), p4),
-                                                             mm<false, false>(GP, p1)),
-                                               0.0, 1.0)));
-            gadd(gr, nu * V + xla, proj(cscale(mm<true, false>(p4, mm<false, false>(GP, p123)), 0.0, -1.0)));
-            gadd(gr, la * V + s, proj(cscale(mm<false, false>(GP, P), 0.0, -1.0)));
+            const int sl = 6 * (D - 1) + 4 * (j - 1);
+            cput(contrib, sl + 0, t, T, proj(cscale(mm<false, false>(P, GP), 0.0, 1.0)));
+            cput(contrib, sl + 1, t, T,
+                 proj(cscale(mm<false, false>(mm<false, true>(mm<false, true>(p2, p3), p4), mm<false, false>(GP, p1)),
+                             0.0, 1.0)));
+            cput(contrib, sl + 2, t, T, proj(cscale(mm<true, false>(p4, mm<false, false>(GP, p123)), 0.0, -1.0)));
+            cput(contrib, sl + 3, t, T, proj(cscale(mm<false, false>(GP, P), 0.0, -1.0)));
         }
     }
+}
+
+// index of plane (a, b), a < b, neither MU, in the backward's loop order
+template <int D, int MU>
+__device__ inline int plane_index(int a, int b)
+{
+    int j = 0;
+    for (int nu = 0; nu < D; nu++) {
+        if (nu == MU) continue;
+        for (int la = nu + 1; la < D; la++) {
+            if (la == MU) continue;
+            if (nu == a && la == b) return j;
+            j++;
+        }
+    }
+    return -1;
+}
+
+// gr[y, nu] += the contribution slots of the sub-layer's updated links (x,
+// MU) of parity p that name link (y, nu) (the inverse of the slot map).
+template <typename R, int D, int MU>
+__global__ __launch_bounds__(kB) void k_gather(Geom g, double *gr, const double *contrib, int par, long T)
+{
+    const long L = (long)blockIdx.x * kB + threadIdx.x;
+    const long V = g.V;
+    if (L >= V * D) return;
+    const int nu = (int)(L / V);
+    const long y = L - nu * V;
+    const int py = y >= g.V0;
+    const long base = par ? g.V0 : 0;
+    double a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (nu == MU) {
+        if (py == par) return;  // updated link: its gradient is final
+        for (int n2 = 0; n2 < D; n2++) {
+            if (n2 == MU) continue;
+            const int sl = 6 * (n2 - (n2 > MU));
+            cacc(a, contrib, sl + 1, g.nbr[(2 * n2 + 1) * V + y] - base, T);  // x = y - n2
+            cacc(a, contrib, sl + 4, g.nbr[(2 * n2) * V + y] - base, T);      // x = y + n2
+        }
+    } else {
+        const int sl = 6 * (nu - (nu > MU));
+        if (py == par) {
+            cacc(a, contrib, sl + 2, y - base, T);  // x = y
+            const long x = g.nbr[(2 * nu) * V + g.nbr[(2 * MU + 1) * V + y]];
+            cacc(a, contrib, sl + 3, x - base, T);  // x = y - MU + nu
+            for (int lam = 0; lam < D; lam++) {
+                if (lam == nu || lam == MU) continue;
+                const int sp = 6 * (D - 1) + 4 * plane_index<D, MU>(nu < lam ? nu : lam, nu < lam ? lam : nu);
+                cacc(a, contrib, sp + (nu < lam ? 0 : 3), y - base, T);  // x = y: p1 or p4
+            }
+        } else {
+            cacc(a, contrib, sl + 0, g.nbr[(2 * MU + 1) * V + y] - base, T);  // x = y - MU
+            cacc(a, contrib, sl + 5, g.nbr[(2 * nu) * V + y] - base, T);      // x = y + nu
+            for (int lam = 0; lam < D; lam++) {
+                if (lam == nu || lam == MU) continue;
+                const int sp = 6 * (D - 1) + 4 * plane_index<D, MU>(nu < lam ? nu : lam, nu < lam ? lam : nu);
+                // x = y - lam: p3 (nu < lam, U_nu(x + lam)) or p2 (lam < nu, U_nu(x + lam))
+                cacc(a, contrib, sp + (nu < lam ? 2 : 1), g.nbr[(2 * lam + 1) * V + y] - base, T);
+            }
+        }
+    }
+    double *b = gr + 9 * (size_t)L;
+#pragma unroll
+    for (int e = 0; e < 9; e++) b[e] += a[e];
 }
 
 // g_l = (beta / 2N) TAH(U_l sum staples): gradient of beta S_W
@@ -516,30 +711,76 @@ static unsigned par_blocks(const Geom &g, int par)
 
 long flow_forward_partials(const Geom &g, int par) { return par_blocks(g, par); }
 
-hipError_t launch_flow_forward(const Geom &g, const void *Uin, void *Uout, const FlowSub &f, int mode, double *part,
-                               hipStream_t st)
+size_t flow_log_reals(const Geom &g)
 {
-    const unsigned nb = par_blocks(g, f.par);
-    if (f32_mode(mode))
-        hipLaunchKernelGGL(flow::k_forward<float>, dim3(nb), dim3(flow::kB), 0, st, g, (const float *)Uin, (float *)Uout,
-                           to_sub(f), part);
-    else
-        hipLaunchKernelGGL(flow::k_forward<double>, dim3(nb), dim3(flow::kB), 0, st, g, (const double *)Uin,
-                           (double *)Uout, to_sub(f), part);
-    return hipGetLastError();
+    const long n = g.V0 > g.V - g.V0 ? g.V0 : g.V - g.V0;
+    return (size_t)((n + 63) / 64) * 64 * 18;
 }
 
-hipError_t launch_flow_backward(const Geom &g, const void *U, const FlowSub &f, int mode, double *gr, double c_logj,
-                                double *gw, hipStream_t st)
+// (D, mu) dispatch for the flow kernels: the plane enumeration is unrolled
+// at compile time
+template <template <typename, int, int> class K, typename R, typename... Args>
+static hipError_t flow_dispatch(int D, int mu, unsigned nb, hipStream_t st, Args... args)
+{
+#define ASKJA_FLOW_CASE(DD, MM)                                                              \
+    if (D == DD && mu == MM) {                                                               \
+        hipLaunchKernelGGL((K<R, DD, MM>::fn), dim3(nb), dim3(flow::kB), 0, st, args...);   \
+        return hipGetLastError();                                                            \
+    }
+    ASKJA_FLOW_CASE(3, 0) ASKJA_FLOW_CASE(3, 1) ASKJA_FLOW_CASE(3, 2)
+    ASKJA_FLOW_CASE(4, 0) ASKJA_FLOW_CASE(4, 1) ASKJA_FLOW_CASE(4, 2) ASKJA_FLOW_CASE(4, 3)
+#undef ASKJA_FLOW_CASE
+    return hipErrorInvalidValue;
+}
+
+template <typename R, int D, int MU>
+struct FwdK {
+    static constexpr auto fn = flow::k_forward<R, D, MU>;
+};
+template <typename R, int D, int MU>
+struct BwdK {
+    static constexpr auto fn = flow::k_backward<R, D, MU>;
+};
+template <typename R, int D, int MU>
+struct GatK {
+    static constexpr auto fn = flow::k_gather<R, D, MU>;
+};
+
+hipError_t launch_flow_forward(const Geom &g, const void *Uin, void *Uout, const FlowSub &f, int mode, double *part,
+                               void *log, hipStream_t st)
 {
     const unsigned nb = par_blocks(g, f.par);
     if (f32_mode(mode))
-        hipLaunchKernelGGL(flow::k_backward<float>, dim3(nb), dim3(flow::kB), 0, st, g, (const float *)U, to_sub(f), gr,
-                           c_logj, gw);
-    else
-        hipLaunchKernelGGL(flow::k_backward<double>, dim3(nb), dim3(flow::kB), 0, st, g, (const double *)U, to_sub(f),
-                           gr, c_logj, gw);
-    return hipGetLastError();
+        return flow_dispatch<FwdK, float>(g.D, f.mu, nb, st, g, (const float *)Uin, (float *)Uout, to_sub(f), part,
+                                          (float *)log);
+    return flow_dispatch<FwdK, double>(g.D, f.mu, nb, st, g, (const double *)Uin, (double *)Uout, to_sub(f), part,
+                                       (double *)log);
+}
+
+hipError_t launch_flow_backward(const Geom &g, void *U, const FlowSub &f, int mode, double *gr, double c_logj,
+                                double *gw, const void *log, double *contrib, hipStream_t st)
+{
+    const unsigned nb = par_blocks(g, f.par);
+    const long T = (long)(flow_log_reals(g) / 18);
+    const unsigned nl = (unsigned)((g.V * g.D + flow::kB - 1) / flow::kB);
+    hipError_t e;
+    if (f32_mode(mode)) {
+        e = flow_dispatch<BwdK, float>(g.D, f.mu, nb, st, g, (float *)U, to_sub(f), gr, c_logj, gw, (const float *)log,
+                                       contrib, T);
+        if (e == hipSuccess) e = flow_dispatch<GatK, float>(g.D, f.mu, nl, st, g, gr, (const double *)contrib, f.par, T);
+    } else {
+        e = flow_dispatch<BwdK, double>(g.D, f.mu, nb, st, g, (double *)U, to_sub(f), gr, c_logj, gw,
+                                        (const double *)log, contrib, T);
+        if (e == hipSuccess)
+            e = flow_dispatch<GatK, double>(g.D, f.mu, nl, st, g, gr, (const double *)contrib, f.par, T);
+    }
+    return e;
+}
+
+size_t flow_contrib_doubles(const Geom &g)
+{
+    const int nf = flow::nfeat(g.D);
+    return (size_t)(6 * (g.D - 1) + 4 * nf) * 9 * (flow_log_reals(g) / 18);
 }
 
 hipError_t launch_wilson_grad(const Geom &g, const void *U, double coef, int mode, double *gr, double *part,

@@ -140,10 +140,16 @@ struct FlowSub {
     double rho_max, delta;
 };
 long flow_forward_partials(const Geom &g, int par);
+// one sub-layer in place; log (may be null) receives its input links
 hipError_t launch_flow_forward(const Geom &g, const void *Uin, void *Uout, const FlowSub &f, int mode, double *part,
-                               hipStream_t st);
-hipError_t launch_flow_backward(const Geom &g, const void *U, const FlowSub &f, int mode, double *gr, double c_logj,
-                                double *gw, hipStream_t st);
+                               void *log, hipStream_t st);
+// backward of one sub-layer; restores its input links from log into U
+// (contrib: flow_contrib_doubles scratch for the neighbour contributions)
+hipError_t launch_flow_backward(const Geom &g, void *U, const FlowSub &f, int mode, double *gr, double c_logj,
+                                double *gw, const void *log, double *contrib, hipStream_t st);
+size_t flow_contrib_doubles(const Geom &g);
+// link reals of one sub-layer's input log (sites of one parity, 64-link tiles)
+size_t flow_log_reals(const Geom &g);
 // gr = coef * TAH(U sum staples) (9 doubles per link), part: ReTr W per block
 hipError_t launch_wilson_grad(const Geom &g, const void *U, double coef, int mode, double *gr, double *part,
                               long *np, hipStream_t st);
