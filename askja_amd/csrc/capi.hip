@@ -176,6 +176,37 @@ double metropolis_uniform(uint64_t seed, uint64_t counter)
 
 long n_plaquettes(const askja_ctx *c) { return c->V * (long)c->dim * (c->dim - 1) / 2; }
 
+// S = sum_p (1 - ReTr P / N) of the links in U in FP64 arithmetic.  The
+// FP32 and BF16x3 modes take the Metropolis energies from this rather than
+// from the MD's reduced-precision products, so that H is the exact FP64
+// energy of the stored (FP32) links.  Overwrites res.
+int exact_action(askja_ctx *c, const void *U, double *s)
+{
+    int rc;
+    if (c->nc > 8) {
+        const long np = kick_partials(c->geom);
+        if ((rc = ensure_part(c, (size_t)np * 6))) return rc;
+        const KickArgs a = kick_args(c, U, nullptr, nullptr, 0.0, 0.0, 0);
+        HIPCHK(c, launch_action_f64(a, c->st));
+        HIPCHK(c, launch_finalize(c->part, np, 3, c->res, c->st));
+        double h[6];
+        if ((rc = read_res(c, h, 6))) return rc;
+        *s = (double)n_plaquettes(c) - pair_val(h, 0) / (4.0 * c->nc);
+        return ASKJA_OK;
+    }
+    if ((rc = ensure_scratch(c))) return rc;
+    long np = 0;
+    const long cap = (c->V + 255) / 256 + 1024;
+    if ((rc = ensure_part(c, (size_t)cap * 4))) return rc;
+    HIPCHK(c, launch_plaq(c->geom, U, c->mode, 1. / (double)c->nc, c->part, &np, c->scratch, (long)c->scratch_doubles,
+                          c->st));
+    HIPCHK(c, launch_finalize(c->part, np, 2, c->res, c->st));
+    double h[4];
+    if ((rc = read_res(c, h, 4))) return rc;
+    *s = pair_val(h, 0) + pair_val(h, 1);
+    return ASKJA_OK;
+}
+
 // MD of one trajectory with the chain-state buffer `in0` left untouched;
 // returns the buffer index that holds the final links.  Energies of the
 // first and last kick go to res[0..5] and res[6..11].
@@ -632,6 +663,7 @@ int askja_kick_drift(askja_ctx *c, double eps_kick, double eps_drift, int level)
 int askja_md_trajectory(askja_ctx *c, double dt, int ndrift, int level, double *energies)
 {
     ENTER(c);
+    const int start = c->cur;
     int fin = c->cur;
     int rc = md_pingpong(c, dt, ndrift, level, &fin);
     if (rc) return rc;
@@ -644,6 +676,10 @@ int askja_md_trajectory(askja_ctx *c, double dt, int ndrift, int level, double *
         energies[1] = pair_val(h, 1);
         energies[2] = np - pair_val(h + 6, 0) / n4;
         energies[3] = pair_val(h + 6, 2);
+        if (f32_mode(c->mode)) {
+            if ((rc = exact_action(c, c->links[start], &energies[0]))) return rc;
+            if ((rc = exact_action(c, c->links[fin], &energies[2]))) return rc;
+        }
     }
     return ASKJA_OK;
 }
@@ -806,8 +842,12 @@ int askja_hmc_trajectory(askja_ctx *c, const askja_hmc_params *p, uint64_t traj_
     double h[12];
     if ((rc = read_res(c, h, 12))) return rc;
     const double np = (double)n_plaquettes(c), n4 = 4.0 * c->nc, n2 = 2.0 * c->nc;
-    const double s0 = np - pair_val(h, 0) / n4, k0 = pair_val(h, 1);
-    const double s1 = np - pair_val(h + 6, 0) / n4, k1 = pair_val(h + 6, 2);
+    double s0 = np - pair_val(h, 0) / n4, s1 = np - pair_val(h + 6, 0) / n4;
+    const double k0 = pair_val(h, 1), k1 = pair_val(h + 6, 2);
+    if (f32_mode(c->mode)) {
+        if ((rc = exact_action(c, c->links[c->cur], &s0))) return rc;
+        if ((rc = exact_action(c, c->links[fin], &s1))) return rc;
+    }
     r->h_old = (s0 + k0 / n2) * p->beta;
     r->h_new = (s1 + k1 / n2) * p->beta;
     r->dh = r->h_new - r->h_old;

@@ -33,6 +33,9 @@ ASKJA_TEAM_DECL(64)
 hipError_t mfma_kick_16(const KickArgs &a, int mode, bool drift, hipStream_t st);
 hipError_t mfma_kick_32(const KickArgs &a, int mode, bool drift, hipStream_t st);
 hipError_t mfma_kick_64(const KickArgs &a, int mode, bool drift, hipStream_t st);
+hipError_t mfma_action_16(const KickArgs &a, hipStream_t st);
+hipError_t mfma_action_32(const KickArgs &a, hipStream_t st);
+hipError_t mfma_action_64(const KickArgs &a, hipStream_t st);
 
 namespace {
 
@@ -583,6 +586,15 @@ hipError_t launch_kick(const KickArgs &a, int mode, bool drift, hipStream_t st)
     if (n <= 16) return team_kick_16(a, mode, drift, st);
     if (n <= 32) return team_kick_32(a, mode, drift, st);
     return team_kick_64(a, mode, drift, st);
+}
+
+hipError_t launch_action_f64(const KickArgs &a, hipStream_t st)
+{
+    const int n = a.g.N;
+    if (n <= 8) return hipErrorInvalidValue;
+    if (n <= 16) return mfma_action_16(a, st);
+    if (n <= 32) return mfma_action_32(a, st);
+    return mfma_action_64(a, st);
 }
 
 hipError_t launch_drift(const Geom &g, void *U, const void *P, double eps, int level, const double *cinv, int mode,

@@ -12,4 +12,6 @@ hipError_t mfma_kick_64(const KickArgs &a, int mode, bool drift, hipStream_t st)
     }
 }
 
+hipError_t mfma_action_64(const KickArgs &a, hipStream_t st) { return mfma::action_f64<64>(a, st); }
+
 }  // namespace askja

@@ -194,16 +194,16 @@ __device__ inline void acc_store(const Acc<P, NMAX> &a, R *Sr, R *Si)
 }
 
 // global link (matrix contiguous, n x n complex) -> zero-padded slot
-template <int P, int NMAX, typename R>
-__device__ inline void ld_link(R *Sr, R *Si, const R *__restrict__ f, long l, int n)
+template <int P, int NMAX, typename R, typename S>
+__device__ inline void ld_link(R *Sr, R *Si, const S *__restrict__ f, long l, int n)
 {
     constexpr int NT = Cfg<P, NMAX>::NT;
-    const R *b = f + (size_t)l * 2 * n * n;
+    const S *b = f + (size_t)l * 2 * n * n;
     for (int e = threadIdx.x; e < NMAX * NMAX; e += NT) {
         const int r = e / NMAX, c = e % NMAX;
         const bool in = r < n && c < n;
-        Sr[e] = in ? b[2 * (r * n + c)] : R(0);
-        Si[e] = in ? b[2 * (r * n + c) + 1] : R(0);
+        Sr[e] = in ? (R)b[2 * (r * n + c)] : R(0);
+        Si[e] = in ? (R)b[2 * (r * n + c) + 1] : R(0);
     }
 }
 
@@ -221,7 +221,10 @@ __device__ inline R mom_el(const R *__restrict__ P, long l, int n, int i, int j,
     return b[h];
 }
 
-template <int P, int NMAX, bool Drift>
+// LS: link storage type (FP32 links with FP64 arithmetic for the exact
+// action of the reduced-precision modes).  ActionOnly: only sum ReTr W
+// (= 4 sum of the plaquettes touching the link) into the partials.
+template <int P, int NMAX, bool Drift, typename LS = typename Pol<P>::R, bool ActionOnly = false>
 __global__ __launch_bounds__((Cfg<P, NMAX>::NT)) void k_kick(KickArgs a)
 {
     typedef typename Pol<P>::R R;
@@ -235,7 +238,7 @@ __global__ __launch_bounds__((Cfg<P, NMAX>::NT)) void k_kick(KickArgs a)
     int cnt;
     const long s = parity_block((long)(lb / (unsigned)g.D), V, g.V0, 1, cnt);
     const long l = (long)mu * V + s;
-    const R *U = (const R *)a.U;
+    const LS *U = (const LS *)a.U;
     const long xp = g.nbr[(2 * mu) * V + s];
 
     Acc<P, NMAX> S, T;
@@ -280,6 +283,18 @@ __global__ __launch_bounds__((Cfg<P, NMAX>::NT)) void k_kick(KickArgs a)
     __syncthreads();
     acc_store(T, Are, Aim);
     __syncthreads();
+    if constexpr (ActionOnly) {
+        double sum[3] = {0.0, 0.0, 0.0}, cmp[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int t = 0; t < NCT; t++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int i = acc_row<P, NMAX>(q), j = acc_col<P, NMAX>(t);
+                if (i == j && i < n) sum[0] += (double)Are[i * NMAX + i];
+            }
+        block_reduce_store<3>(sum, cmp, a.part + 6 * (size_t)lb);
+        return;
+    }
     // TAH and kick on the lane's entries (sunmat.h:355-371, sun.cpp:716)
     const R eps = (R)a.eps_kick, inv_n = (R)a.inv_n;
     R *Pm = (R *)a.P;
@@ -403,6 +418,16 @@ __global__ __launch_bounds__((Cfg<P, NMAX>::NT)) void k_kick(KickArgs a)
             }
     }
     block_reduce_store<3>(sum, cmp, a.part + 6 * (size_t)lb);
+}
+
+// sum ReTr W in FP64 arithmetic over FP32 links (partials as the kick's)
+template <int NMAX>
+hipError_t action_f64(const KickArgs &a, hipStream_t st)
+{
+    const unsigned nb = (unsigned)(a.g.V * a.g.D);
+    constexpr int NT = Cfg<kF64, NMAX>::NT;
+    hipLaunchKernelGGL((k_kick<kF64, NMAX, false, float, true>), dim3(nb), dim3(NT), 0, st, a);
+    return hipGetLastError();
 }
 
 template <int P, int NMAX>

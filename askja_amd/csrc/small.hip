@@ -143,7 +143,7 @@ __global__ __launch_bounds__(kBlock) void k_drift(Geom g, R *U, const R *P, doub
 // Plaquette sums per site: temporal planes (0, k) and spatial planes (j, k),
 // 1 <= j < k, of 1 - ReTr P_jk(x)/N (EnergyMag sun.cpp:507-537, PlaqExp
 // measurements4d.cpp:885-927).
-template <int N, typename R, bool Exact>
+template <int N, typename R, bool Exact, typename C = R>
 __global__ __launch_bounds__(kBlock) void k_plaq(Geom g, const R *U, double inv_n, double *part)
 {
     const long s = (long)blockIdx.x * kBlock + threadIdx.x;
@@ -155,7 +155,7 @@ __global__ __launch_bounds__(kBlock) void k_plaq(Geom g, const R *U, double inv_
             for (int k = j + 1; k < g.D; k++) {
                 const long xk = g.nbr[(2 * k) * V + s];
                 double tr;
-                CM<N, R> A, B, T1, T2;
+                CM<N, C> A, B, T1, T2;
                 if constexpr (Exact) {
                     load_link<N, R>(U, j * V + s, A);
                     load_link<N, R>(U, k * V + xj, B);
@@ -412,8 +412,9 @@ hipError_t plaq_mode(const Geom &g, const void *U, int mode, double inv_n, doubl
     case kModeF64Exact:
         hipLaunchKernelGGL((k_plaq<N, double, true>), dim3(nb), dim3(kBlock), 0, st, g, (const double *)U, inv_n, part);
         break;
-    default:
-        hipLaunchKernelGGL((k_plaq<N, float, false>), dim3(nb), dim3(kBlock), 0, st, g, (const float *)U, inv_n, part);
+    default:  // FP32 links, FP64 arithmetic: the exact action of the stored links
+        hipLaunchKernelGGL((k_plaq<N, float, false, double>), dim3(nb), dim3(kBlock), 0, st, g, (const float *)U, inv_n,
+                           part);
     }
     return hipGetLastError();
 }

@@ -30,10 +30,12 @@ __device__ inline void cm_zero(CM<N, R> &m)
 }
 
 // ---- loads/stores in the 64-site tiled SoA layout -------------------------
-template <int N, typename R>
-__device__ inline void load_link(const R *__restrict__ f, long l, CM<N, R> &m)
+// storage type S, arithmetic type R (S = R except for FP64 arithmetic on FP32
+// links)
+template <int N, typename S, typename R>
+__device__ inline void load_link(const S *__restrict__ f, long l, CM<N, R> &m)
 {
-    const R *b = f + (size_t)(l >> 6) * (2 * N * N * 64) + (l & 63);
+    const S *b = f + (size_t)(l >> 6) * (2 * N * N * 64) + (l & 63);
 #pragma unroll
     for (int i = 0; i < N; i++)
 #pragma unroll

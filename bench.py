@@ -106,38 +106,44 @@ def copy_probe(gib=1, reps=5):
 
 
 STUDY_CONFIGS = {
-    # name: (N, extents, beta, dt, LEVEL, MD steps per trajectory, precisions)
-    "C2": (3, [8, 32, 32, 32], 5.9, 0.02, 6, 50, ("fp64", "fp32")),
-    "C3": (16, [6, 24, 24, 24], 4 * 16 * 16 / 10.0, 0.01, 8, 20, ("fp64", "fp32", "bf16x3")),
-    "C5": (64, [128, 128], 64.0 * 64.0, 0.005, 8, 20, ("fp64", "fp32", "bf16x3")),
+    # name: (N, extents, beta, dt, LEVEL, MD steps per trajectory (length 1),
+    #        precisions, thermalising trajectories)
+    "C2": (3, [8, 32, 32, 32], 5.9, 0.02, 6, 50, ("fp64", "fp32"), 100),
+    "C3": (16, [6, 24, 24, 24], 4 * 16 * 16 / 10.0, 0.01, 8, 100, ("fp64", "fp32", "bf16x3"), 30),
+    "C5": (64, [128, 128], 64.0 * 64.0, 0.005, 8, 200, ("fp64", "fp32", "bf16x3"), 20),
 }
 
 
 def precision_study(names):
-    """Delta-H and plaquette error of FP32 and BF16x3 against FP64: the same
-    thermalised-from-Haar configuration and the same momenta, one trajectory
-    each (the MD steps per trajectory are stated per config)."""
+    """Delta-H and plaquette error of FP32 and BF16x3 against FP64 on one
+    thermalised configuration (Haar start, thermalised by exact HMC with the
+    fastest MD precision) with the same momenta: one trajectory of length 1
+    in each precision."""
     import askja_amd as A
 
     pm = {"fp64": A.FP64, "fp32": A.FP32, "bf16x3": A.BF16X3}
     for name in names:
-        n, ext, beta, dt, level, nd, precs = STUDY_CONFIGS[name]
-        ref = A.Lattice(n, ext, A.FP64)
+        n, ext, beta, dt, level, nd, precs, therm = STUDY_CONFIGS[name]
+        ref = A.Lattice(n, ext, A.BF16X3 if n > 8 else A.FP32)
         ref.haar(SEED, 5)
-        for t in range(3):
+        for t in range(therm):
             ref.hmc_trajectory(beta, dt, nd, level, SEED, t, reunit=True)
+        print(f"[study] {name} thermalised ({therm} trajectories)", file=sys.stderr, flush=True)
         u0 = ref.links()
         rows = {}
         for pn in precs:
             lat = A.Lattice(n, ext, pm[pn])
             lat.set_links(u0)
+            lat.reunitarize()
             lat.mom_refresh(SEED, 1000, beta)
+            lat.sync()
             t0 = time.perf_counter()
             e = lat.md_trajectory(dt, nd, level)
             T = time.perf_counter() - t0
             h0 = beta * (e[0] + e[1] / (2 * n))
             h1 = beta * (e[2] + e[3] / (2 * n))
             rows[pn] = {"dH": h1 - h0, "H_old": h0, "seconds": T,
+                        "plaquette_start": 1.0 - e[0] / lat.n_plaquettes(),
                         "plaquette_end": 1.0 - e[2] / lat.n_plaquettes()}
             lat.close()
         f64 = rows["fp64"]
@@ -146,7 +152,7 @@ def precision_study(names):
             v["H_old_err_vs_fp64"] = v["H_old"] - f64["H_old"]
             v["plaquette_err_vs_fp64"] = v["plaquette_end"] - f64["plaquette_end"]
         print(json.dumps({"study": "precision", "config": name, "nc": n, "extents": ext, "beta": beta, "dt": dt,
-                          "level": level, "md_steps": nd, "results": rows}), flush=True)
+                          "level": level, "md_steps": nd, "therm": therm, "results": rows}), flush=True)
         ref.close()
 
 
@@ -156,7 +162,7 @@ def c3_scan(ntraj, precs=("bf16x3", "fp32")):
     import askja_amd as A
 
     pm = {"fp32": A.FP32, "bf16x3": A.BF16X3, "fp64": A.FP64}
-    n, ext, beta0, dt, level, nd, _ = STUDY_CONFIGS["C3"]
+    n, ext, beta0, dt, level, nd, _, _ = STUDY_CONFIGS["C3"]
     for pn in precs:
         lat = A.Lattice(n, ext, pm[pn])
         for f in (0.8, 0.9, 1.0, 1.1, 1.2):
@@ -176,6 +182,46 @@ def c3_scan(ntraj, precs=("bf16x3", "fp32")):
         lat.close()
 
 
+def c5_exact(ntraj, therm=20, precs=("bf16x3", "fp64")):
+    """The 2D SU(64) workload (128^2, beta = N^2 = 4096, DT 0.005, LEVEL 8,
+    200 MD steps: trajectory length 1) in FP64 and in BF16x3 with the FP64
+    Metropolis test, Haar start: mean plaquette over ntraj trajectories
+    after therm, against the exact single-plaquette value of 2D SU(64)
+    (tests/golden/exact_2d_plaquette.json)."""
+    import askja_amd as A
+
+    gold = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden",
+                                       "exact_2d_plaquette.json")))
+    want = [p["plaquette"] for p in gold["points"] if p["nc"] == 64 and p["beta"] == 4096.0][0]
+    pm = {"bf16x3": A.BF16X3, "fp32": A.FP32, "fp64": A.FP64}
+    for pn in precs:
+        lat = A.Lattice(64, [128, 128], pm[pn])
+        lat.haar(SEED, 7)
+        plaq, acc, dh = [], 0, []
+        t0 = time.perf_counter()
+        for t in range(therm + ntraj):
+            if t == therm:
+                lat.sync()
+                t0 = time.perf_counter()
+            r = lat.hmc_trajectory(4096.0, 0.005, 200, 8, SEED, t, reunit=True)
+            if t >= therm:
+                acc += r.accepted
+                dh.append(r.dh)
+                plaq.append(1.0 - r.action / lat.n_plaquettes())
+            print(f"[c5] {pn} trajectory {t + 1}/{therm + ntraj} plaquette {1.0 - r.action / lat.n_plaquettes():.6f}",
+                  file=sys.stderr, flush=True)
+        lat.sync()
+        T = time.perf_counter() - t0
+        tau, _ = tau_int_wolff(plaq) if len(plaq) >= 8 else (1.0, 0)
+        m = float(np.mean(plaq))
+        e = float(np.std(plaq) * math.sqrt(max(tau, 1.0) / len(plaq)))
+        print(json.dumps({"study": "c5_exact_2d", "precision": pn, "ntraj": ntraj, "therm": therm,
+                          "plaquette": m, "err": e, "exact": want, "z": (m - want) / e if e > 0 else None,
+                          "acceptance": acc / ntraj, "mean_abs_dH": float(np.mean(np.abs(dh))),
+                          "traj_per_s": ntraj / T}), flush=True)
+        lat.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -186,8 +232,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--roofline-launches", type=int, default=100)
     ap.add_argument("--copy-probe", action="store_true", help="only run a 1 GiB device copy (counter calibration)")
-    ap.add_argument("--study", choices=["precision", "c3scan"], help="print a precision or beta-scan study instead")
+    ap.add_argument("--study", choices=["precision", "c3scan", "c5exact"],
+                    help="print a precision, beta-scan or exact-2D study instead")
     ap.add_argument("--study-configs", default="C2,C3,C5")
+    ap.add_argument("--study-therm", type=int, default=20, help="thermalising trajectories (c5exact)")
+    ap.add_argument("--study-precisions", default="bf16x3,fp64", help="precisions (c5exact)")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
@@ -197,6 +246,9 @@ def main():
         return
     if args.study == "c3scan":
         c3_scan(max(args.steps, 2))
+        return
+    if args.study == "c5exact":
+        c5_exact(max(args.steps, 2), args.study_therm, tuple(args.study_precisions.split(",")))
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU path is not built.",

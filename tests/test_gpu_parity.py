@@ -315,3 +315,24 @@ def test_bf16x3_kick_drift(oracle, n, ext):
     c.haar(1, 0)
     r = c.hmc_trajectory(n * n / 2.0, 0.01, 5, 8, 2, 0)
     assert np.isfinite(r.dh) and abs(r.dh) < 5.0
+
+
+@pytest.mark.parametrize("n,ext,prec", [(3, [4, 4, 4, 4], A.FP32), (5, [2, 2, 2, 2], A.FP32),
+                                        (12, [2, 2, 2], A.BF16X3), (16, [2, 3, 2], A.FP32)],
+                         ids=["N3-fp32", "N5-fp32", "N12-bf16x3", "N16-fp32"])
+def test_reduced_precision_metropolis_energy_is_fp64(n, ext, prec):
+    """In the FP32 and BF16x3 modes the MD runs in reduced precision, but the
+    action in H_old / H_new is the FP64-arithmetic action of the stored links
+    (not the MD's reduced-precision ReTr W), so the Metropolis test is exact
+    for the stored chain state."""
+    beta = 2.0 * n
+    lat = _gpu(n, ext, prec)
+    lat.haar(3, 0)
+    for t in range(3):
+        r = lat.hmc_trajectory(beta, 0.01, 5, 8, 2, t)
+        ref = _gpu(n, ext, A.FP64, lat.links())  # the stored FP32 links, exactly
+        s_ref = ref.EnergyMag()
+        assert abs(r.action - s_ref) < 1e-12 * lat.n_plaquettes(), (t, r.action, s_ref)
+    e = lat.md_trajectory(0.01, 3, 8)
+    ref = _gpu(n, ext, A.FP64, lat.links())
+    assert abs(e[2] - ref.EnergyMag()) < 1e-12 * lat.n_plaquettes()

@@ -85,3 +85,25 @@ def test_two_dimensional_exact_plaquette(prec):
         plaq.append(1.0 - r.action / lat.n_plaquettes())
     m, e = mean_err(plaq)
     assert abs(m - want) < 4 * e + 1e-4, (m, e, want)
+
+
+@pytest.mark.parametrize("prec", [A.FP64, A.BF16X3])
+def test_two_dimensional_su64_exact_plaquette(prec):
+    """SU(64) at beta = N^2 (the Gross-Witten point) on a 16^2 lattice:
+    MFMA link products (FP64, or BF16x3 with the FP64 Metropolis test), the
+    chain's plaquette against the exact Bessel-determinant value
+    (tests/golden/exact_2d_plaquette.json, oracle/exact2d.py)."""
+    gold = json.load(open(os.path.join(ROOT, "tests", "golden", "exact_2d_plaquette.json")))
+    want = [p["plaquette"] for p in gold["points"] if p["nc"] == 64 and p["beta"] == 4096.0][0]
+    lat = A.Lattice(64, [16, 16], prec)
+    lat.haar(7, 0)
+    for t in range(40):
+        lat.hmc_trajectory(4096.0, 0.025, 40, 8, 9, t)
+    plaq, acc = [], 0
+    for t in range(160):
+        r = lat.hmc_trajectory(4096.0, 0.025, 40, 8, 9, 100 + t)
+        acc += r.accepted
+        plaq.append(1.0 - r.action / lat.n_plaquettes())
+    m, e = mean_err(plaq)
+    assert acc / 160 > 0.5
+    assert abs(m - want) < 4 * e + 2e-4, (m, e, want)
