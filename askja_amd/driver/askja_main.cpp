@@ -19,7 +19,7 @@
 //     readers stop before it), so a resumed run continues the same stream;
 //   - the Wilson-*.dat phase columns, left uninitialised by the reference
 //     (sun.cpp:1119, 346-360), hold the sorted eigen-phases of the loop.
-// Extra options follow the params file: --precision fp64|fp64-exact|fp32,
+// Extra options follow the params file: --precision fp64|fp64-exact|fp32|bf16x3,
 // --reunit (project the chain state onto SU(N) before every trajectory).
 #include "../../include/askja.h"
 
@@ -270,6 +270,7 @@ int main(int argc, char *argv[])
             if (!strcmp(v, "fp64")) precision = ASKJA_FP64;
             else if (!strcmp(v, "fp64-exact")) precision = ASKJA_FP64_EXACT;
             else if (!strcmp(v, "fp32")) precision = ASKJA_FP32;
+            else if (!strcmp(v, "bf16x3")) precision = ASKJA_BF16X3;
             else {
                 fprintf(stderr, "askja: unknown precision %s\n", v);
                 return 2;

@@ -89,7 +89,7 @@ def plain_chain(lat, beta, dt, ndrift, level, seed, first, n, measure=True):
     return np.array(series), acc / max(n, 1), time.perf_counter() - t0
 
 
-def flow_chain(lat, beta, dt, ndrift, level, seed, first, n):
+def flow_chain(lat, beta, dt, ndrift, level, seed, first, n, log=None):
     """n FTHMC trajectories; |P| measured on U = F(V)."""
     series, acc = [], 0
     lat.sync()
@@ -98,6 +98,8 @@ def flow_chain(lat, beta, dt, ndrift, level, seed, first, n):
         r = lat.flow_hmc_trajectory(beta, dt, ndrift, level, seed, first + k, reunit=True)
         acc += r.accepted
         series.append(abs(lat.flow_polyakov()))
+        if log and (k + 1) % 50 == 0:
+            log(f"[learned] flow trajectory {k + 1}/{n} ({time.perf_counter() - t0:.1f} s)")
     lat.sync()
     return np.array(series), acc / max(n, 1), time.perf_counter() - t0
 
@@ -151,13 +153,13 @@ def main():
     ap.add_argument("--extent", type=int, default=16)
     ap.add_argument("--beta", type=float, default=6.0)
     ap.add_argument("--beta-train", type=float, default=5.7)
-    ap.add_argument("--layers", type=int, default=1)
+    ap.add_argument("--layers", type=int, default=4)
     ap.add_argument("--ntrain", type=int, default=500)
     ap.add_argument("--epochs", type=int, default=2)
     ap.add_argument("--ntraj", type=int, default=400)
     ap.add_argument("--therm", type=int, default=200)
-    ap.add_argument("--dt", type=float, default=0.05)
-    ap.add_argument("--ndrift", type=int, default=20)
+    ap.add_argument("--dt", type=float, default=0.02)
+    ap.add_argument("--ndrift", type=int, default=50)
     ap.add_argument("--level", type=int, default=8)
     ap.add_argument("--seed", type=int, default=11)
     args = ap.parse_args()
@@ -195,7 +197,9 @@ def main():
     for name, weights in (("flow_untrained", w0), ("flow_trained", w)):
         lat.set_links(start)
         lat.flow_set_weights(weights)
-        fl, acc_f, t_f = flow_chain(lat, args.beta, args.dt, args.ndrift, args.level, args.seed, 4 * 10**6, args.ntraj)
+        log(f"[learned] {name} chain")
+        fl, acc_f, t_f = flow_chain(lat, args.beta, args.dt, args.ndrift, args.level, args.seed, 4 * 10**6, args.ntraj,
+                                    log=log)
         res[name] = {"traj_per_s": args.ntraj / t_f, "acceptance": acc_f, "tau": tau_int(fl),
                      "mean_abs_polyakov": float(fl.mean())}
     for k, v in res.items():

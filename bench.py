@@ -35,6 +35,13 @@ SEED = 120351616
 HBM_PEAK_GBS = 8000.0
 
 
+# HBM bytes per fused kick+drift launch of this workload, measured with two
+# rocprofv3 --pmc passes (FETCH_SIZE scaled by 2.0, WRITE_SIZE by 1.0, from a
+# 1 GiB copy's readings): reads 772.7 MB + writes 226.8 MB (FP64),
+# 264.3 MB + 113.5 MB (FP32)
+PMC_TRAFFIC_BYTES = {"fp64": 999.5e6, "fp32": 377.8e6}
+
+
 def tau_int_wolff(x, S=1.5):
     """Integrated autocorrelation time (Wolff, Comput. Phys. Commun. 156
     (2004) 143): automatic window W minimising the sum of statistical and
@@ -222,6 +229,39 @@ def c5_exact(ntraj, therm=20, precs=("bf16x3", "fp64")):
         lat.close()
 
 
+def kernel_study(reps=None):
+    """Per-config time of one fused kick+drift launch (HIP events on the
+    library's stream) and of one reunitarisation, every precision; flops per
+    link (4(D-1) + 1 + LEVEL) 8 N^3."""
+    import askja_amd as A
+
+    cfgs = [("C2", 3, [8, 32, 32, 32], 6), ("C4", 3, [16, 16, 16, 16], 8), ("C1", 8, [10, 10, 10, 10], 8),
+            ("C3", 16, [6, 24, 24, 24], 8), ("C5", 64, [128, 128], 8)]
+    pm = (("fp64", A.FP64), ("fp64-exact", A.FP64_EXACT), ("fp32", A.FP32), ("bf16x3", A.BF16X3))
+    for name, n, ext, level in cfgs:
+        for pn, prec in pm:
+            if pn == "bf16x3" and n <= 8:
+                continue  # N <= 8 runs BF16x3 as FP32
+            lat = A.Lattice(n, ext, prec)
+            lat.haar(SEED, 2)
+            lat.mom_refresh(SEED, 0, 5.9)
+            lat.kick_drift(0.01, 0.01, level)
+            lat.sync()
+            r = reps or (20 if n <= 8 else 3)
+            lat.timer_start()
+            for _ in range(r):
+                lat.kick_drift(0.01, 0.01, level)
+            ms = lat.timer_stop() / r
+            lat.timer_start()
+            lat.reunitarize()
+            ms_re = lat.timer_stop()
+            flops = (4 * (lat.dim - 1) + 1 + level) * 8 * n ** 3 * lat.V * lat.dim
+            print(json.dumps({"study": "kernels", "config": name, "nc": n, "extents": ext, "precision": pn,
+                              "ms_kick_drift": round(ms, 4), "tflops": round(flops / ms / 1e9, 2),
+                              "ms_reunitarize": round(ms_re, 3)}), flush=True)
+            lat.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -232,8 +272,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--roofline-launches", type=int, default=100)
     ap.add_argument("--copy-probe", action="store_true", help="only run a 1 GiB device copy (counter calibration)")
-    ap.add_argument("--study", choices=["precision", "c3scan", "c5exact"],
-                    help="print a precision, beta-scan or exact-2D study instead")
+    ap.add_argument("--study", choices=["precision", "c3scan", "c5exact", "kernels"],
+                    help="print a precision, beta-scan, exact-2D or per-config kernel study instead")
     ap.add_argument("--study-configs", default="C2,C3,C5")
     ap.add_argument("--study-therm", type=int, default=20, help="thermalising trajectories (c5exact)")
     ap.add_argument("--study-precisions", default="bf16x3,fp64", help="precisions (c5exact)")
@@ -246,6 +286,9 @@ def main():
         return
     if args.study == "c3scan":
         c3_scan(max(args.steps, 2))
+        return
+    if args.study == "kernels":
+        kernel_study()
         return
     if args.study == "c5exact":
         c5_exact(max(args.steps, 2), args.study_therm, tuple(args.study_precisions.split(",")))
@@ -327,7 +370,9 @@ def main():
         "mean_dH": float(np.mean(dh)),
         "exp_minus_dH": float(np.mean(np.exp(-np.asarray(dh)))),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": PMC_TRAFFIC_BYTES[args.precision],
+                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) of this launch, "
+                                       "profiles/kick_drift_su3_8x32c_pmc.md",
                      "kernel": "fused kick+drift", "ms_per_launch": ms, "bytes_per_link": bytes_per_link,
                      "links_per_launch": nl, "tflops": tflops},
     }

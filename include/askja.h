@@ -44,7 +44,9 @@ enum {
                              links and momenta bit-identical to the reference's
                              double arithmetic (sunmat.h:81-371, sun.cpp:672-801) */
     ASKJA_FP32 = 2,       /* FP32 links/momenta/MD; FP64 compensated sums,
-                             FP64 Metropolis and FP64 reunitarisation */
+                             FP64 reunitarisation, and an FP64 Metropolis test
+                             whose energies are the FP64-arithmetic action and
+                             kinetic energy of the stored FP32 fields */
     ASKJA_BF16X3 = 3      /* as FP32, but for N >= 9 the kick/drift link products
                              run on MFMA as split BF16 (hi*hi + hi*lo + lo*hi)
                              with FP32 accumulation */
@@ -130,7 +132,8 @@ int askja_kick_drift(askja_ctx *ctx, double eps_kick, double eps_drift, int leve
  * kick(dt/2), ndrift x [drift(dt); kick(dt) except after the last drift],
  * kick(dt/2).  The reference's first trajectory has ndrift = UPDATE2-1.
  * energies (may be NULL) receives {S(start), sum Tr pi^2 (start), S(end),
- * sum Tr Pi^2 (end)} with S = EnergyMag (sun.cpp:507-537). */
+ * sum Tr Pi^2 (end)} with S = EnergyMag (sun.cpp:507-537), in the FP32 and
+ * BF16x3 modes the FP64-arithmetic action of the stored links. */
 int askja_md_trajectory(askja_ctx *ctx, double dt, int ndrift, int level, double *energies);
 
 /* ---- reductions (FP64, compensated) ------------------------------------------- */
@@ -184,7 +187,10 @@ typedef struct {
  * counter-based stream in place of the time(NULL) seeds: momentum refresh
  * (counter = traj_counter), H_old, MD, H_new, accept if H_new < H_old or
  * u < exp(H_old - H_new) with u from Philox; on reject the links return to
- * the state before the trajectory.  NaN energies reject, as in the reference. */
+ * the state before the trajectory.  NaN energies reject, as in the reference.
+ * In the FP32 and BF16x3 modes H_old and H_new are the FP64-arithmetic
+ * energies of the stored FP32 fields (an extra action pass at each end), so
+ * the chain is exact for them whatever the MD precision. */
 int askja_hmc_trajectory(askja_ctx *ctx, const askja_hmc_params *p, uint64_t traj_counter,
                          askja_hmc_result *res);
 
@@ -193,7 +199,7 @@ int askja_hmc_trajectory(askja_ctx *ctx, const askja_hmc_params *p, uint64_t tra
  * time(NULL), sun.cpp:1277-1287). */
 int askja_uniform(askja_ctx *ctx, uint64_t seed, uint64_t counter, double *out);
 
-/* ---- learned field transformation (no reference counterpart; SU(3), D >= 3) ----
+/* ---- learned field transformation (no reference counterpart; SU(3), D = 3 or 4) ----
  * U = F(V): nsub masked sub-layers; sub-layer k updates the links of
  * direction mu[k] on the sites of parity parity[k] as
  *   U'_l = exp(i theta Lhat) U_l,  theta = rho Im Tr(Lhat U_l S_l),
@@ -215,10 +221,12 @@ int askja_flow_apply(askja_ctx *ctx, double *logj);
 /* host AoS buffer <- U = F(V) of the last apply / gradient call */
 int askja_flow_download(askja_ctx *ctx, double *aos);
 /* Phi = beta S_W(F(V)) - log J(V): its Lie gradient with respect to V (kept
- * for askja_flow_gradient_download) by the hand-written backward kernels, with
- * segment checkpointing (every 8th sub-layer input is stored, the rest
- * recomputed), and with wgrad != 0 the gradient with respect to the weights.
- * action = S_W(F(V)) (EnergyMag normalisation), logj = log det J(V). */
+ * for askja_flow_gradient_download) by the hand-written backward kernels, and
+ * with wgrad != 0 the gradient with respect to the weights.  The forward pass
+ * keeps only each sub-layer's input links (its masked links, 1/(2D) of a
+ * field per sub-layer); the backward recomputes every sub-layer's
+ * intermediates from that input.  Deterministic (no atomics on the link
+ * gradient).  action = S_W(F(V)) (EnergyMag normalisation), logj = log det J(V). */
 int askja_flow_gradient(askja_ctx *ctx, double beta, int wgrad, double *action, double *logj);
 /* host AoS buffer <- Hermitian traceless g_l = sum_a T^a d/dt Phi(e^{i t T^a} V_l) */
 int askja_flow_gradient_download(askja_ctx *ctx, double *aos);

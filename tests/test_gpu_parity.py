@@ -336,3 +336,24 @@ def test_reduced_precision_metropolis_energy_is_fp64(n, ext, prec):
     e = lat.md_trajectory(0.01, 3, 8)
     ref = _gpu(n, ext, A.FP64, lat.links())
     assert abs(e[2] - ref.EnergyMag()) < 1e-12 * lat.n_plaquettes()
+
+
+def test_example_params_regime_dim10(oracle):
+    """The reference's own example (params_N8S10T10B20k.txt): NC 8, DIM 10,
+    10 x 10 x 1^8, beta 20000, LEVEL 8.  Eight size-1 directions make every
+    neighbour step in them a self-reference; the exact mode must still be
+    bit-identical to the oracle, and the fast mode within the FP64 bar."""
+    n, ext = 8, [10, 10] + [1] * 8
+    o, u, p = _state(oracle, n, ext, 21, beta=20000.0, level=8)
+    lat = _gpu(n, ext, A.FP64_EXACT, u, p)
+    for _ in range(3):
+        o.updateE(0.01)
+        o.updateUandcopyDown(0.01)
+        lat.updateE(0.01)
+        lat.updateUandcopyDown(0.01, 8)
+    assert np.array_equal(lat.momenta(), o.field("pi"))
+    assert np.array_equal(lat.links(), o.field("u"))
+    fast = _gpu(n, ext, A.FP64, u, p)
+    for _ in range(3):
+        fast.kick_drift(0.01, 0.01, 8)
+    assert np.max(np.abs(fast.links() - o.field("u"))) < 1e-12
