@@ -976,7 +976,7 @@ int askja_flow_define(askja_ctx *c, int nsub, const int *mu, const int *parity, 
     for (int k = 0; k < nsub; k++)
         if (mu[k] < 0 || mu[k] >= c->dim || (parity[k] != 0 && parity[k] != 1) || block[k] < 0 || block[k] >= nblocks)
             return fail(c, ASKJA_EINVAL, "flow_define: sub-layer %d out of range", k);
-    const int nf = (c->dim - 1) * (c->dim - 2) / 2;
+    const int nf = 3 * (c->dim - 1) * (c->dim - 2) / 2;  // plaquette and two rectangles per plane
     // (re)allocate
     void *old[] = {c->flow_w, c->flow_gw, c->flow_g, c->flow_lj, c->flow_out, c->flow_work, c->flow_log, c->flow_contrib};
     for (void *p : old)

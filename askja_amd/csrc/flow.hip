@@ -8,8 +8,9 @@
 //                            + U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu)]  (frozen staples)
 //   W      = U_l S                                  (the staple loop through l)
 //   P_j    = plaquettes at x in the planes not containing mu (frozen)
+//   R_j    = the 2x1 and 1x2 rectangles at x in those planes (frozen)
 //   Lam    = TAH(sum_j P_j),  Lhat = Lam / sqrt(2 Tr Lam^2 + delta^2)
-//   rho    = rho_max tanh(MLP(ReTr P_j / 3))        (2 layers, hidden 32)
+//   rho    = rho_max tanh(MLP(ReTr P_j / 3, ReTr R_j / 3))   (2 layers, hidden 32)
 //   theta  = rho Im Tr(Lhat W)
 //   U'_l   = exp(i theta Lhat) U_l
 // Lhat, rho and the staples depend on frozen links only, so the Jacobian of
@@ -296,7 +297,34 @@ struct Sub {
     double rho_max, delta;
 };
 
-constexpr int nfeat(int D) { return (D - 1) * (D - 2) / 2; }
+// planes (nu < la) without a given direction, and MLP features: the
+// plaquette of every plane, then the 2x1 and 1x2 rectangles plane by plane
+constexpr int nplanes(int D) { return (D - 1) * (D - 2) / 2; }
+constexpr int nfeat(int D) { return 3 * nplanes(D); }
+
+// The two rectangles at x in plane (nu, la), as six links A_1 ... A_6 with
+// A_1..3 = X_1..3 and A_4..6 = X_4..6^+:
+//   R1: X = U_nu(x), U_nu(x+nu), U_la(x+2nu), U_nu(x+nu+la), U_nu(x+la), U_la(x)
+//   R2: X = U_nu(x), U_la(x+nu), U_la(x+nu+la), U_nu(x+2la), U_la(x+la), U_la(x)
+struct Rect {
+    long l[6];
+};
+__device__ inline void rectangles(const Geom &g, long s, int nu, int la, Rect &r1, Rect &r2)
+{
+    const long V = g.V;
+    const long xpn = g.nbr[(2 * nu) * V + s], xla = g.nbr[(2 * la) * V + s];
+    const long x2n = g.nbr[(2 * nu) * V + xpn], xnl = g.nbr[(2 * la) * V + xpn], x2l = g.nbr[(2 * la) * V + xla];
+    r1 = Rect{{nu * V + s, nu * V + xpn, la * V + x2n, nu * V + xnl, nu * V + xla, la * V + s}};
+    r2 = Rect{{nu * V + s, la * V + xpn, la * V + xnl, nu * V + x2l, la * V + xla, la * V + s}};
+}
+
+template <typename R>
+__device__ inline C3 rect_loop(const R *U, const Rect &r)
+{
+    C3 M = mm<false, false>(mm<false, false>(ld(U, r.l[0]), ld(U, r.l[1])), ld(U, r.l[2]));
+    M = mm<false, true>(mm<false, true>(mm<false, true>(M, ld(U, r.l[3])), ld(U, r.l[4])), ld(U, r.l[5]));
+    return M;
+}
 
 // the forward quantities of one updated link that the backward reuses
 template <int D>
@@ -340,8 +368,13 @@ __device__ inline void forward_link(const Geom &g, const R *U, const Sub &sb, lo
                 const C3 P = mm<false, true>(
                     mm<false, true>(mm<false, false>(ld(U, nu * V + s), ld(U, la * V + xpn)), ld(U, nu * V + xla)),
                     ld(U, la * V + s));
-                set_feature(F.f, j++, retr(P) / 3.0);
+                set_feature(F.f, j, retr(P) / 3.0);
                 addto(C, P);
+                Rect r1, r2;
+                rectangles(g, s, nu, la, r1, r2);
+                set_feature(F.f, nplanes(D) + 2 * j, retr(rect_loop(U, r1)) / 3.0);
+                set_feature(F.f, nplanes(D) + 2 * j + 1, retr(rect_loop(U, r2)) / 3.0);
+                j++;
             }
         }
         const C3 Lam = tah(C);
@@ -420,6 +453,36 @@ __device__ inline double wave_sum(double v)
 // Phi = beta S - log J).  gw (may be null) accumulates dPhi/dweights of the
 // sub-layer's MLP block.  Every forward quantity is recomputed from the
 // sub-layer input (layer recomputation; nothing but the input is stored).
+// Lie-gradient contributions of c ReTr R (c = dPhi/df / 6 with
+// f = ReTr R / 3) to the six links of rectangle R = A_1 ... A_6 (A_1..3 = X,
+// A_4..6 = X^+): with the cyclic rotations M_k = A_k ... A_6 A_1 ... A_{k-1},
+// link k gets proj(+i c M_k) for k <= 3 and proj(-i c M_{k+1}) for k >= 4
+// (M_7 = M_1).  M_{k+1} = A_k^{-1} M_k A_k, with A^{-1} = A^+ (SU(3) links).
+template <typename R>
+__device__ inline void rect_grad(const R *U, const Rect &r, double c, double *contrib, int sl, long t, long T)
+{
+    const C3 M1 = rect_loop(U, r);
+    cput(contrib, sl + 0, t, T, proj(cscale(M1, 0.0, c)));
+    cput(contrib, sl + 5, t, T, proj(cscale(M1, 0.0, -c)));
+    C3 M = M1;
+#pragma unroll
+    for (int k = 0; k < 2; k++) {  // M_2, M_3
+        const C3 X = ld(U, r.l[k]);
+        M = mm<true, false>(X, mm<false, false>(M, X));
+        cput(contrib, sl + 1 + k, t, T, proj(cscale(M, 0.0, c)));
+    }
+    {  // M_4 = X_3^+ M_3 X_3
+        const C3 X = ld(U, r.l[2]);
+        M = mm<true, false>(X, mm<false, false>(M, X));
+    }
+#pragma unroll
+    for (int k = 3; k < 5; k++) {  // M_5, M_6 (A_k = X_k^+)
+        const C3 X = ld(U, r.l[k]);
+        M = mm<false, true>(mm<false, false>(X, M), X);
+        cput(contrib, sl + k, t, T, proj(cscale(M, 0.0, -c)));
+    }
+}
+
 template <typename R, int D, int MU>
 __global__ __launch_bounds__(kB) void k_backward(Geom g, R *U, Sub sb, double *gr, double c_logj, double *gw,
                                                  const R *log, double *contrib, long T)
@@ -582,32 +645,34 @@ __global__ __launch_bounds__(kB) void k_backward(Geom g, R *U, Sub sb, double *g
             const C3 p12 = mm<false, false>(p1, p2);
             const C3 p123 = mm<false, true>(p12, p3);
             const C3 P = mm<false, true>(p123, p4);
-            const int sl = 6 * (D - 1) + 4 * (j - 1);
+            const int sl = 6 * (D - 1) + 16 * (j - 1);
             cput(contrib, sl + 0, t, T, proj(cscale(mm<false, false>(P, GP), 0.0, 1.0)));
             cput(contrib, sl + 1, t, T,
                  proj(cscale(mm<false, false>(mm<false, true>(mm<false, true>(p2, p3), p4), mm<false, false>(GP, p1)),
                              0.0, 1.0)));
             cput(contrib, sl + 2, t, T, proj(cscale(mm<true, false>(p4, mm<false, false>(GP, p123)), 0.0, -1.0)));
             cput(contrib, sl + 3, t, T, proj(cscale(mm<false, false>(GP, P), 0.0, -1.0)));
+            // rectangles: dPhi/df (f = ReTr R / 3) only
+            Rect r1, r2;
+            rectangles(g, s, nu, la, r1, r2);
+            double df1 = 0.0, df2 = 0.0;
+#pragma unroll
+            for (int e = 0; e < nf; e++) {
+                if (e == nplanes(D) + 2 * (j - 1)) df1 = df[e];
+                if (e == nplanes(D) + 2 * (j - 1) + 1) df2 = df[e];
+            }
+            rect_grad(U, r1, df1 / 6.0, contrib, sl + 4, t, T);
+            rect_grad(U, r2, df2 / 6.0, contrib, sl + 10, t, T);
         }
     }
 }
 
-// index of plane (a, b), a < b, neither MU, in the backward's loop order
-template <int D, int MU>
-__device__ inline int plane_index(int a, int b)
-{
-    int j = 0;
-    for (int nu = 0; nu < D; nu++) {
-        if (nu == MU) continue;
-        for (int la = nu + 1; la < D; la++) {
-            if (la == MU) continue;
-            if (nu == a && la == b) return j;
-            j++;
-        }
-    }
-    return -1;
-}
+// Plane-loop roles of the contribution slots (per plane (p0, p1), 16 slots):
+// plaquette links p1..p4, then rectangle R1 A_1..A_6, then R2 A_1..A_6.  Role
+// r names link (x + kRoleA[r] p0 + kRoleB[r] p1, p0 if kRoleDir[r] == 0 else p1).
+__constant__ int kRoleDir[16] = {0, 1, 0, 1, 0, 0, 1, 0, 0, 1, 0, 1, 1, 0, 1, 1};
+__constant__ int kRoleA[16] = {0, 1, 0, 0, 0, 1, 2, 1, 0, 0, 0, 1, 1, 0, 0, 0};
+__constant__ int kRoleB[16] = {0, 0, 1, 0, 0, 0, 0, 1, 1, 0, 0, 0, 1, 2, 1, 0};
 
 // gr[y, nu] += the contribution slots of the sub-layer's updated links (x,
 // MU) of parity p that name link (y, nu) (the inverse of the slot map).
@@ -636,19 +701,30 @@ __global__ __launch_bounds__(kB) void k_gather(Geom g, double *gr, const double 
             cacc(a, contrib, sl + 2, y - base, T);  // x = y
             const long x = g.nbr[(2 * nu) * V + g.nbr[(2 * MU + 1) * V + y]];
             cacc(a, contrib, sl + 3, x - base, T);  // x = y - MU + nu
-            for (int lam = 0; lam < D; lam++) {
-                if (lam == nu || lam == MU) continue;
-                const int sp = 6 * (D - 1) + 4 * plane_index<D, MU>(nu < lam ? nu : lam, nu < lam ? lam : nu);
-                cacc(a, contrib, sp + (nu < lam ? 0 : 3), y - base, T);  // x = y: p1 or p4
-            }
         } else {
             cacc(a, contrib, sl + 0, g.nbr[(2 * MU + 1) * V + y] - base, T);  // x = y - MU
             cacc(a, contrib, sl + 5, g.nbr[(2 * nu) * V + y] - base, T);      // x = y + nu
-            for (int lam = 0; lam < D; lam++) {
-                if (lam == nu || lam == MU) continue;
-                const int sp = 6 * (D - 1) + 4 * plane_index<D, MU>(nu < lam ? nu : lam, nu < lam ? lam : nu);
-                // x = y - lam: p3 (nu < lam, U_nu(x + lam)) or p2 (lam < nu, U_nu(x + lam))
-                cacc(a, contrib, sp + (nu < lam ? 2 : 1), g.nbr[(2 * lam + 1) * V + y] - base, T);
+        }
+        // plane loops (plaquette and rectangles) through link (y, nu): role r
+        // of plane (p0, p1) names link (x + a p0 + b p1, dir); x = y - a p0 - b p1
+        int j = 0;
+        for (int p0 = 0; p0 < D; p0++) {
+            if (p0 == MU) continue;
+            for (int p1 = p0 + 1; p1 < D; p1++) {
+                if (p1 == MU) continue;
+                if (nu == p0 || nu == p1) {
+                    const int side = nu == p0 ? 0 : 1;
+                    for (int r = 0; r < 16; r++) {
+                        if (kRoleDir[r] != side) continue;
+                        const int ra = kRoleA[r], rb = kRoleB[r];
+                        if (((py + ra + rb) & 1) != par) continue;
+                        long x = y;
+                        for (int q = 0; q < ra; q++) x = g.nbr[(2 * p0 + 1) * V + x];
+                        for (int q = 0; q < rb; q++) x = g.nbr[(2 * p1 + 1) * V + x];
+                        cacc(a, contrib, 6 * (D - 1) + 16 * j + r, x - base, T);
+                    }
+                }
+                j++;
             }
         }
     }
@@ -779,8 +855,7 @@ hipError_t launch_flow_backward(const Geom &g, void *U, const FlowSub &f, int mo
 
 size_t flow_contrib_doubles(const Geom &g)
 {
-    const int nf = flow::nfeat(g.D);
-    return (size_t)(6 * (g.D - 1) + 4 * nf) * 9 * (flow_log_reals(g) / 18);
+    return (size_t)(6 * (g.D - 1) + 16 * flow::nplanes(g.D)) * 9 * (flow_log_reals(g) / 18);
 }
 
 hipError_t launch_wilson_grad(const Geom &g, const void *U, double coef, int mode, double *gr, double *part,

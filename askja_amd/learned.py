@@ -41,7 +41,8 @@ def sweep_masks(dim, layers):
 
 
 def init_weights(nblocks, bsize, nf, seed=0, scale=0.1):
-    """Small random W1, b1; w2 and b2 zero, so the untrained map is F = id."""
+    """Small random W1, b1; w2 and b2 zero, so the untrained map is F = id.
+    Block layout W1[32][nf], b1[32], w2[32], b2 (bsize = 32 nf + 65)."""
     rng = np.random.default_rng(seed)
     w = np.zeros(nblocks * bsize, np.float32)
     for b in range(nblocks):
@@ -181,7 +182,7 @@ def main():
     log(f"[learned] {len(configs)} training configurations at beta {args.beta_train}")
     mu, par, blk = sweep_masks(4, args.layers)
     bsize = lat.flow_define(mu, par, blk, args.layers, 0.3, 1e-3)
-    nf = 3
+    nf = (bsize - 65) // 32  # features: plaquette and two rectangles per plane
     w0 = init_weights(args.layers, bsize, nf, seed=args.seed)
     t0 = time.perf_counter()
     w, losses = train(lat, configs, args.beta, w0, epochs=args.epochs, log=log)

@@ -205,11 +205,12 @@ int askja_uniform(askja_ctx *ctx, uint64_t seed, uint64_t counter, double *out);
  *   U'_l = exp(i theta Lhat) U_l,  theta = rho Im Tr(Lhat U_l S_l),
  * with S_l the staple sum, Lhat the normalised traceless anti-Hermitian part of
  * the plaquettes at x in the planes without mu (both frozen in the sub-layer),
- * rho = rho_max tanh(MLP(ReTr of those plaquettes / 3)); the MLP (hidden 32,
- * FP32) of sub-layer k is weight block block[k].  log det J = sum over updated
- * links of log(1 + rho ReTr(Lhat^2 U_l S_l)).  Requires even extents and
- * 0 < rho_max < 1/(D-1) (invertibility).  Block layout: W1[32][nf], b1[32],
- * w2[32], b2 with nf = (D-1)(D-2)/2. */
+ * rho = rho_max tanh(MLP(f)) with features f = ReTr / 3 of those plaquettes,
+ * then of the 2x1 and 1x2 rectangles at x in the same planes, plane by plane;
+ * the MLP (hidden 32, FP32) of sub-layer k is weight block block[k].
+ * log det J = sum over updated links of log(1 + rho ReTr(Lhat^2 U_l S_l)).
+ * Requires even extents and 0 < rho_max < 1/(D-1) (invertibility).  Block
+ * layout: W1[32][nf], b1[32], w2[32], b2 with nf = 3 (D-1)(D-2)/2. */
 int askja_flow_define(askja_ctx *ctx, int nsub, const int *mu, const int *parity, const int *block, int nblocks,
                       double rho_max, double delta);
 /* Floats per weight block. */

@@ -10,8 +10,12 @@ learned-proposal addition); the definitions follow the sub-layer stated there:
                             + U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu)]
     W     = U_l S
     P_j   = plaquettes at x in the planes (nu < la) without mu
+    R_j   = the two rectangles at x in plane j: 2 x 1 (two steps along nu)
+            and 1 x 2 (two steps along la), same orientation as P_j
     Lam   = TAH(sum_j P_j), Lhat = Lam / sqrt(2 Tr Lam^2 + delta^2)
-    rho   = rho_max tanh(b2 + w2 . tanh(W1 f + b1)),  f_j = ReTr P_j / 3
+    rho   = rho_max tanh(b2 + w2 . tanh(W1 f + b1)), features
+            f = (ReTr P_j / 3 for every plane, then ReTr R1_j / 3, ReTr R2_j / 3
+            plane by plane): nf = 3 x (number of planes)
     theta = rho Im Tr(Lhat W)
     U'_l  = expm(i theta Lhat) U_l,   log J += log(1 + rho ReTr(Lhat^2 W))
 
@@ -35,9 +39,40 @@ def parity(ext, i):
     return sum(np.unravel_index(i, ext)) % 2
 
 
+def link_inputs(o, U, x, mu):
+    """Frozen inputs of link (x, mu): staple sum S, plaquette sum C and the
+    feature vector (plaquettes, then the two rectangles of every plane)."""
+    D = o.dim
+    xp = o.getplus(x, mu)
+    S = np.zeros((3, 3), complex)
+    C = np.zeros((3, 3), complex)
+    f, fr = [], []
+    for nu in range(D):
+        if nu == mu:
+            continue
+        xpn, xmn = o.getplus(x, nu), o.getminus(x, nu)
+        xpmn = o.getminus(xp, nu)
+        S += U[xp, nu] @ U[xpn, mu].conj().T @ U[x, nu].conj().T
+        S += U[xpmn, nu].conj().T @ U[xmn, mu].conj().T @ U[xmn, nu]
+        for la in range(nu + 1, D):
+            if la == mu:
+                continue
+            xla = o.getplus(x, la)
+            P = U[x, nu] @ U[xpn, la] @ U[xla, nu].conj().T @ U[x, la].conj().T
+            f.append(np.trace(P).real / 3)
+            C += P
+            x2n, xnl, x2l = o.getplus(xpn, nu), o.getplus(xpn, la), o.getplus(xla, la)
+            R1 = (U[x, nu] @ U[xpn, nu] @ U[x2n, la] @ U[xnl, nu].conj().T @ U[xla, nu].conj().T
+                  @ U[x, la].conj().T)
+            R2 = (U[x, nu] @ U[xpn, la] @ U[xnl, la] @ U[x2l, nu].conj().T @ U[xla, la].conj().T
+                  @ U[x, la].conj().T)
+            fr += [np.trace(R1).real / 3, np.trace(R2).real / 3]
+    return S, C, np.array(f + fr)
+
+
 def sub_layer(o, U, mu, par, w, nf, rho_max, delta):
     """One masked sub-layer applied to U (copied); returns (U', log J)."""
-    D, V = o.dim, o.V
+    V = o.V
     out = U.copy()
     logj = 0.0
     W1 = w[: HID * nf].reshape(HID, nf).astype(np.float64)
@@ -47,29 +82,12 @@ def sub_layer(o, U, mu, par, w, nf, rho_max, delta):
     for x in range(V):
         if parity(o.ext, x) != par:
             continue
-        xp = o.getplus(x, mu)
-        S = np.zeros((3, 3), complex)
-        C = np.zeros((3, 3), complex)
-        f = []
-        for nu in range(D):
-            if nu == mu:
-                continue
-            xpn, xmn = o.getplus(x, nu), o.getminus(x, nu)
-            xpmn = o.getminus(xp, nu)
-            S += U[xp, nu] @ U[xpn, mu].conj().T @ U[x, nu].conj().T
-            S += U[xpmn, nu].conj().T @ U[xmn, mu].conj().T @ U[xmn, nu]
-            for la in range(nu + 1, D):
-                if la == mu:
-                    continue
-                xla = o.getplus(x, la)
-                P = U[x, nu] @ U[xpn, la] @ U[xla, nu].conj().T @ U[x, la].conj().T
-                f.append(np.trace(P).real / 3)
-                C += P
+        S, C, f = link_inputs(o, U, x, mu)
         Wl = U[x, mu] @ S
         Lam = tah(C)
         n = np.sqrt(2 * np.trace(Lam @ Lam).real + delta * delta)
         Lh = Lam / n
-        z = b2 + w2 @ np.tanh(W1 @ np.array(f) + b1)
+        z = b2 + w2 @ np.tanh(W1 @ f + b1)
         rho = rho_max * np.tanh(z)
         theta = rho * np.trace(Lh @ Wl).imag
         K = np.trace(Lh @ Lh @ Wl).real

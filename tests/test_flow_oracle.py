@@ -19,7 +19,7 @@ def _setup(oracle, ext, seed, nblocks=1):
     o = oracle.Oracle(3, ext)
     rng = np.random.default_rng(seed)
     V = random_su(oracle, rng, (o.V, o.dim), 3)
-    nf = (o.dim - 1) * (o.dim - 2) // 2
+    nf = 3 * (o.dim - 1) * (o.dim - 2) // 2  # plaquette and two rectangles per plane
     bsize = 32 * nf + 65
     w = (rng.standard_normal(nblocks * bsize) * 0.5).astype(np.float32)
     return o, rng, V, nf, bsize, w
@@ -71,3 +71,40 @@ def test_zero_coefficients_are_identity(oracle, fo):
     w[:] = 0
     U, lj = fo.forward(o, V, [(0, 0, 0), (1, 1, 0)], w, bsize, nf, 0.3, 1e-3)
     assert np.array_equal(U, V) and lj == 0.0
+
+
+def test_features_closed_form_and_invariance(oracle, fo):
+    """Plaquette and rectangle features: for diagonal (abelian) links the
+    rectangle is the product of its two plaquettes, so ReTr R / 3 is the mean
+    cosine of the summed plaquette angles; every feature is gauge invariant."""
+    o = oracle.Oracle(3, [4, 4, 2, 2])
+    rng = np.random.default_rng(5)
+    th = rng.uniform(-np.pi, np.pi, (o.V, o.dim, 2))
+    ph = np.stack([th[..., 0], th[..., 1], -th[..., 0] - th[..., 1]], -1)
+    U = np.zeros((o.V, o.dim, 3, 3), complex)
+    for c in range(3):
+        U[:, :, c, c] = np.exp(1j * ph[..., c])
+    x, mu = 5, 1
+
+    def pang(y, a, b):  # plaquette angles at y in plane (a, b)
+        return ph[y, a] + ph[o.getplus(y, a), b] - ph[o.getplus(y, b), a] - ph[y, b]
+
+    _, _, f = fo.link_inputs(o, U, x, mu)
+    planes = [(a, b) for a in range(4) for b in range(a + 1, 4) if mu not in (a, b)]
+    npl = len(planes)
+    assert len(f) == 3 * npl
+    for j, (a, b) in enumerate(planes):
+        p0 = pang(x, a, b)
+        assert abs(f[j] - np.cos(p0).mean()) < 1e-13
+        r1 = p0 + pang(o.getplus(x, a), a, b)
+        r2 = p0 + pang(o.getplus(x, b), a, b)
+        assert abs(f[npl + 2 * j] - np.cos(r1).mean()) < 1e-13
+        assert abs(f[npl + 2 * j + 1] - np.cos(r2).mean()) < 1e-13
+    # gauge invariance on a random SU(3) field
+    V = random_su(oracle, rng, (o.V, o.dim), 3)
+    g = random_su(oracle, rng, (o.V,), 3)
+    Vg = np.empty_like(V)
+    for y in range(o.V):
+        for d in range(o.dim):
+            Vg[y, d] = g[y] @ V[y, d] @ g[o.getplus(y, d)].conj().T
+    assert np.allclose(fo.link_inputs(o, V, x, mu)[2], fo.link_inputs(o, Vg, x, mu)[2], atol=1e-13)

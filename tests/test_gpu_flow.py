@@ -33,7 +33,7 @@ def _setup(oracle, seed, scale=0.5, prec=A.FP64):
     bsize = lat.flow_define([s[0] for s in SUBS], [s[1] for s in SUBS], [s[2] for s in SUBS], 2, RHO, DELTA)
     w = (rng.standard_normal(2 * bsize) * scale).astype(np.float32)
     lat.flow_set_weights(w)
-    nf = (o.dim - 1) * (o.dim - 2) // 2
+    nf = 3 * (o.dim - 1) * (o.dim - 2) // 2  # plaquette and two rectangles per plane
     return o, rng, V, lat, w, bsize, nf
 
 
@@ -130,8 +130,8 @@ lat = A.Lattice(3, [4, 4, 4, 4], A.FP64)
 lat.haar(3, 0)
 mu, par, blk = L.sweep_masks(4, 1)
 bsize = lat.flow_define(mu, par, blk, 1, 0.3, 1e-3)
-w0 = L.init_weights(1, bsize, 3, seed=1, scale=0.3)
-w0[32 * 3 + 32: 32 * 3 + 64] = 0.05
+w0 = L.init_weights(1, bsize, 9, seed=1, scale=0.3)
+w0[32 * 9 + 32: 32 * 9 + 64] = 0.05
 fa = L.flow_action_function()
 w = torch.tensor(w0, device="cuda", requires_grad=True)
 loss = fa.apply(w, lat, 6.0)
