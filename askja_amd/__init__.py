@@ -83,6 +83,8 @@ SIGNATURES = {
     "askja_uviol": (_I, [_P, _DP]),
     "askja_wilson_lines": (_I, [_P, _I, _DP]),
     "askja_line_matrix": (_I, [_P, _I, C.POINTER(C.c_int), _DP]),
+    "askja_slice_traces": (_I, [_P, _I, _DP]),
+    "askja_line_phases": (_I, [_P, _I, _I, C.POINTER(C.c_int64), _DP]),
     "askja_hmc_trajectory": (_I, [_P, C.POINTER(HmcParams), _U64, C.POINTER(HmcResult)]),
     "askja_uniform": (_I, [_P, _U64, _U64, _DP]),
     "askja_flow_define": (_I, [_P, _I, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), _I, _D, _D]),
@@ -267,6 +269,23 @@ class Lattice:
         self._ck(lib().askja_line_matrix(self._h, d, xx, _dptr(out)), "line_matrix")
         return out.view(np.complex128).reshape(self.nc, self.nc)
 
+    def slice_traces(self, d):
+        """scalarLoopcorr (measurements4d.cpp:357-408) for links of direction d:
+        mean over each time slice x0 of ReTr U_d / N."""
+        out = np.zeros(self.ext[0])
+        self._ck(lib().askja_slice_traces(self._h, d, _dptr(out)), "slice_traces")
+        return out
+
+    def line_phases(self, d, starts):
+        """Sorted, winding-recentred eigen-phases (getevs,
+        measurements4d.cpp:185-309) of the lines in direction d through the
+        lexicographic sites `starts`; shape (len(starts), N)."""
+        st = np.ascontiguousarray(starts, dtype=np.int64)
+        out = np.zeros(len(st) * self.nc)
+        self._ck(lib().askja_line_phases(self._h, d, len(st), st.ctypes.data_as(C.POINTER(C.c_int64)), _dptr(out)),
+                 "line_phases")
+        return out.reshape(len(st), self.nc)
+
     def WilsonLoop(self):
         """|Tr prod_t U_0(t, 0)|/N at the spatial origin (sun.cpp:307-345)."""
         return abs(self.wilson_lines(0)[0])
@@ -274,6 +293,22 @@ class Lattice:
     def polyakov(self):
         """Volume average of Tr P/N over all spatial sites (complex)."""
         return complex(self.wilson_lines(0).mean())
+
+    def polyakov_measure(self):
+        """measure4d's PolyakovLoop (measurements4d.cpp:410-466): mean over x1
+        (x2.. = 0) of |Tr prod_t U_0|/N."""
+        lines = self.wilson_lines(0)
+        stride = self.V // (self.ext[0] * self.ext[1])
+        return float(np.mean(np.abs(lines[:: stride][: self.ext[1]])))
+
+    def wilson_measure(self, d):
+        """measure4d's WilsonLoop(evs, d) (measurements4d.cpp:311-355): mean over
+        x0 of |Tr prod U_d|/N along the line through (x0, 0, ..., 0); the trace
+        of a closed line does not depend on where it starts, so the
+        reference's stale loop variable (the start x_d) does not matter."""
+        lines = self.wilson_lines(d)
+        stride = self.V // (self.ext[d] * self.ext[0])
+        return float(np.mean(np.abs(lines[:: stride][: self.ext[0]])))
 
     # -- HMC --------------------------------------------------------------------------
     def hmc_trajectory(self, beta, dt, ndrift, level, seed, counter, reunit=True):

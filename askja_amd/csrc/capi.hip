@@ -806,6 +806,56 @@ int askja_wilson_lines(askja_ctx *c, int dir, double *out)
     return ASKJA_OK;
 }
 
+int askja_slice_traces(askja_ctx *c, int dir, double *out)
+{
+    ENTER(c);
+    if (!out) return fail(c, ASKJA_EINVAL, "slice_traces: null output");
+    if (dir < 0 || dir >= c->dim) return fail(c, ASKJA_EINVAL, "slice_traces: dir %d out of range", dir);
+    const long nb = slice_trace_blocks(c->geom), n0 = c->ext[0];
+    int rc = ensure_part(c, (size_t)nb * n0 * 2);
+    if (rc) return rc;
+    double *d = nullptr;
+    HIPCHK(c, hipMallocAsync((void **)&d, (size_t)2 * n0 * sizeof(double), c->st));
+    hipError_t e = launch_slice_trace(c->geom, c->links[c->cur], dir, c->mode, c->part, c->st);
+    for (long x0 = 0; x0 < n0 && e == hipSuccess; x0++)
+        e = launch_finalize(c->part + 2 * x0 * nb, nb, 1, d + 2 * x0, c->st);
+    std::vector<double> h((size_t)2 * n0);
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), d, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipFreeAsync(d, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    if (e != hipSuccess) return fail(c, ASKJA_EHIP, "slice_traces: %s", hipGetErrorString(e));
+    const double slice = (double)(c->V / n0);
+    for (long x0 = 0; x0 < n0; x0++) out[x0] = (h[2 * x0] + h[2 * x0 + 1]) / slice;
+    return ASKJA_OK;
+}
+
+int askja_line_phases(askja_ctx *c, int dir, int nlines, const int64_t *starts, double *out)
+{
+    ENTER(c);
+    if (!starts || !out || nlines < 1) return fail(c, ASKJA_EINVAL, "line_phases: bad arguments");
+    if (dir < 0 || dir >= c->dim) return fail(c, ASKJA_EINVAL, "line_phases: dir %d out of range", dir);
+    for (int k = 0; k < nlines; k++)
+        if (starts[k] < 0 || starts[k] >= c->V) return fail(c, ASKJA_EINVAL, "line_phases: start %d out of range", k);
+    const size_t nn = (size_t)c->nc * c->nc;
+    long *ds = nullptr;
+    double *wb = nullptr, *dp = nullptr;
+    HIPCHK(c, hipMallocAsync((void **)&ds, nlines * sizeof(long), c->st));
+    HIPCHK(c, hipMallocAsync((void **)&wb, (size_t)nlines * 2 * nn * sizeof(double), c->st));
+    HIPCHK(c, hipMallocAsync((void **)&dp, (size_t)nlines * c->nc * sizeof(double), c->st));
+    std::vector<long> hs(starts, starts + nlines);
+    hipError_t e = hipMemcpyAsync(ds, hs.data(), nlines * sizeof(long), hipMemcpyHostToDevice, c->st);
+    if (e == hipSuccess) e = launch_line_phases(c->geom, c->links[c->cur], dir, c->mode, nlines, ds, wb, dp, c->st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out, dp, (size_t)nlines * c->nc * sizeof(double), hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    (void)hipFreeAsync(ds, c->st);
+    (void)hipFreeAsync(wb, c->st);
+    (void)hipFreeAsync(dp, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    if (e != hipSuccess) return fail(c, ASKJA_EHIP, "line_phases: %s", hipGetErrorString(e));
+    return ASKJA_OK;
+}
+
 int askja_line_matrix(askja_ctx *c, int dir, const int *x, double *out)
 {
     ENTER(c);

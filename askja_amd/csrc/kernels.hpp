@@ -135,6 +135,16 @@ hipError_t launch_mom_scale(const Geom &g, void *P, double f, int mode, hipStrea
 hipError_t launch_project(const Geom &g, void *U, int mode, int haar, uint64_t seed, uint64_t stream,
                           double *scratch, long scratch_doubles, hipStream_t st);
 
+// ---- measurement (measure.hip) ---------------------------------------------
+// blocks per time slice of the slice-trace partials
+long slice_trace_blocks(const Geom &g);
+// part[x0 * nb + b] (s, c): sums of ReTr U_dir / N over the sites of slice x0
+hipError_t launch_slice_trace(const Geom &g, const void *U, int dir, int mode, double *part, hipStream_t st);
+// sorted, winding-recentred eigen-phases of the lines in direction dir that
+// start at the lexicographic sites starts[0 .. nlines); wbuf: 2 N^2 per line
+hipError_t launch_line_phases(const Geom &g, const void *U, int dir, int mode, int nlines, const long *starts,
+                              double *wbuf, double *out, hipStream_t st);
+
 // ---- learned field transformation (flow.hip, N = 3) -------------------------
 struct FlowSub {
     int mu, par;     // updated links: direction mu on sites of parity par

@@ -165,6 +165,18 @@ int askja_wilson_lines(askja_ctx *ctx, int dir, double *out);
  * the Wilson-*.dat columns. */
 int askja_line_matrix(askja_ctx *ctx, int dir, const int *x, double *out);
 
+/* ---- measurement (the device side of measurements4d.cpp) ------------------------- */
+/* scalarLoopcorr (measurements4d.cpp:357-408): out[x0] = mean over the sites
+ * of time slice x0 of ReTr U_dir(x) / N (the reference uses dir = DIM-1).
+ * out holds extent[0] doubles. */
+int askja_slice_traces(askja_ctx *ctx, int dir, double *out);
+/* getevs (measurements4d.cpp:185-309): for each of nlines straight lines in
+ * direction dir starting at the lexicographic sites starts[k], the N
+ * eigen-phases arg(lambda) of the line product, shifted by 2 pi w / N when the
+ * winding w = sum(phases) / 2 pi has |w| > 0.1, sorted ascending.  out holds
+ * nlines * N doubles.  (Batched on the device: one workgroup per line.) */
+int askja_line_phases(askja_ctx *ctx, int dir, int nlines, const int64_t *starts, double *out);
+
 /* ---- HMC -------------------------------------------------------------------------- */
 typedef struct {
     double beta;

@@ -1015,8 +1015,9 @@ void oracle_md_trajectory(oracle_t *o, double dt, int ndrift)
 /* Plain HMC chain with counter-based randomness (momenta: Philox purpose 1,
  * counter = trajectory number; Metropolis uniform: Philox purpose 3, link
  * 0, counter = trajectory number).  Per trajectory it records
- * rec[0..7] = {dH, accepted, S/beta-normalised plaquette <ReTr P/N>,
- * Re P, Im P (volume-averaged Polyakov loop), |P|, H_old, H_new}.
+ * rec[0..9] = {dH, accepted, S/beta-normalised plaquette <ReTr P/N>,
+ * Re P, Im P (volume-averaged Polyakov loop), |P|, H_old, H_new,
+ * measure4d's spatial Wilson line W_1 and Polyakov loop P-bar}.
  * Reunitarises the chain state before each trajectory when reunit != 0. */
 void oracle_hmc(oracle_t *o, double dt, int ndrift, int ntraj, uint64_t seed,
                 uint64_t first_counter, int reunit, double *rec)
@@ -1040,7 +1041,7 @@ void oracle_hmc(oracle_t *o, double dt, int ndrift, int ntraj, uint64_t seed,
         double uni = u01(r[0]);
         int acc = (h1 < h0) || (uni < exp(h0 - h1));
         if (!acc) oracle_loadconfiguration(o);
-        double *R = rec + 8 * (size_t)t;
+        double *R = rec + 10 * (size_t)t;
         R[0] = dh;
         R[1] = acc;
         R[2] = 1.0 - oracle_energy_mag(o) / nplaq;
@@ -1057,6 +1058,8 @@ void oracle_hmc(oracle_t *o, double dt, int ndrift, int ntraj, uint64_t seed,
         R[5] = hypot(pr, pim);
         R[6] = h0;
         R[7] = h1;
+        R[8] = o->dim > 1 ? oracle_wilson_measure(o, 1) : 0.0; /* measurements4d.cpp:311-355, dir 1 */
+        R[9] = o->dim > 1 ? oracle_polyakov_measure(o) : 0.0;  /* measurements4d.cpp:410-466 */
     }
     free(pf);
 }

@@ -213,9 +213,9 @@ class Oracle:
         lib().oracle_md_trajectory(self._h, dt, ndrift)
 
     def hmc(self, dt, ndrift, ntraj, seed, first_counter=0, reunit=True):
-        rec = np.zeros(8 * ntraj, np.float64)
+        rec = np.zeros(10 * ntraj, np.float64)
         lib().oracle_hmc(self._h, dt, ndrift, ntraj, seed, first_counter, 1 if reunit else 0, rec)
-        return rec.reshape(ntraj, 8)
+        return rec.reshape(ntraj, 10)
 
 
 # -- stand-alone helpers -------------------------------------------------------

@@ -1,5 +1,6 @@
-"""Writes tests/golden/ensemble_su3_4x4.json: ensemble averages of plain HMC
-from the CPU oracle (SU(3), D=4, 4^4, beta 5.7, dt 0.05, 10 MD steps, LEVEL 8,
+"""Writes tests/golden/ensemble_su3_4x4.json: ensemble averages (plaquette,
+volume-averaged |Polyakov loop|, and measurements4d.cpp's spatial Wilson line
+W_1 and Polyakov loop P-bar) of plain HMC from the CPU oracle (SU(3), D=4, 4^4, beta 5.7, dt 0.05, 10 MD steps, LEVEL 8,
 Haar start, counter-based momenta, 4000 trajectories after 200 for
 thermalisation, reunitarised chain) with their autocorrelation-corrected
 errors.  The GPU chain (independent seed) must agree within these errors.
@@ -44,7 +45,7 @@ def main():
     rec = o.hmc(DT, NDRIFT, NTRAJ, SEED, NTHERM, True)
     out = {"workload": f"SU(3) D=4 4^4 beta {BETA}, dt {DT}, {NDRIFT} MD steps, LEVEL {LEVEL}",
            "ntraj": NTRAJ, "acceptance": float(rec[:, 1].mean())}
-    for name, col in (("plaquette", 2), ("abs_polyakov", 5)):
+    for name, col in (("plaquette", 2), ("abs_polyakov", 5), ("wilson_measure_x1", 8), ("polyakov_measure", 9)):
         x = rec[:, col]
         tau = tau_int(x)
         out[name] = {"mean": float(x.mean()), "err": float(x.std() * math.sqrt(tau / len(x))), "tau": tau}

@@ -161,3 +161,74 @@ def test_resume_continues_numbering(tmp_path):
     assert times == [0.0, 0.5, 1.0, 1.5, 2.0, 2.5, 3.0, 3.5, 4.0]
     # the seed travels after NTOT, where neither reference reader looks
     assert open(master).read().split()[-2] == "SEED"
+
+
+MPARAMS = """#Parameters_file
+NC              3
+DIM             4
+N0              4
+N1              6
+N2              2
+N3              2
+BETA            5.9
+DT              0.05
+STEPS           100
+UPDATE          20
+UPDATE2         20
+CONFTAKE        1
+LEVEL           6
+VERBOSE         0
+RUNCONTINUE     0
+RANDOM          0
+"""
+MTAG = "N3-Nt4-Ns6-BETA5.90-DT0.0500-U1.0000-CT1"
+
+
+def _rows(path):
+    return [list(map(float, l.rstrip("\t").split("\t"))) for l in open(path).read().splitlines() if not l.startswith("#")]
+
+
+@pytest.mark.gpu
+def test_measure_program_vs_oracle(tmp_path):
+    """askja-measure (measure4d's command line and files) on configurations
+    written by askja, against the oracle's restatement of measure4d on the
+    same files: Nrav running and final rows, W24, P5corr, Hist (numbers as
+    printed, 6 significant digits)."""
+    from oracle import measure_oracle as mo
+
+    (tmp_path / "p.txt").write_text(MPARAMS)
+    r = _run(["p.txt", "--reunit"], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    master = tmp_path / "output-DIM4" / f"master-{MTAG}.params"
+    taken = read_master(master, 4)["TAKEN"]
+    assert taken == 5
+    meas = os.path.join(ROOT, "askja_amd", "driver", "askja-measure")
+    r = subprocess.run([meas, str(master), "1", "0", "1"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and "Finished successfully" in r.stdout, r.stdout + r.stderr
+    ext = [4, 6, 2, 2]
+    paths = [str(tmp_path / "output-DIM4" / f"U-{MTAG}-{k}.conf") for k in range(1, taken)]
+    want = mo.measure(paths, 3, ext, 5.9)
+    rav = tmp_path / "rav-DIM4"
+
+    def close(a, b):
+        return abs(a - b) <= 1e-5 * abs(b) + 1e-9
+
+    nrav = _rows(rav / f"Nrav-{MTAG}.dat")
+    assert len(nrav) == len(want["running"]) + 1
+    for row, run in zip(nrav[:-1], want["running"]):
+        assert all(close(a, b) for a, b in zip([row[1], row[3], row[5], row[7]], run)), (row, run)
+        assert row[2] == row[4] == row[6] == row[8] == 0.0
+    fin = nrav[-1]
+    m, e = want["mean"], want["err"]
+    exp = [taken, m["P"], e["P"], m["Wx"], e["Wx"], m["Ut"], e["Ut"], m["Us"], e["Us"], 4, 6, 5.9]
+    assert all(close(a, b) for a, b in zip(fin, exp)), (fin, exp)
+    w24 = _rows(rav / f"W24-{MTAG}.dat")
+    assert all(close(a, b) for a, b in zip(w24[0], [3, 5.9, 6, m["W2"], e["W2"], taken - 1, 1])), w24
+    p5 = _rows(rav / f"P5corr-{MTAG}.dat")
+    assert [row[0] for row in p5] == [0, 1]
+    assert all(close(row[1], c) for row, c in zip(p5, want["corr"])), (p5, want["corr"])
+    hist = _rows(rav / f"Hist-{MTAG}.dat")
+    assert len(hist) == len(want["hist"]) == 12
+    for row, h in zip(hist, want["hist"]):
+        assert all(close(a, b) for a, b in zip(row, h)), (row, h)
+    assert abs(sum(row[1] for row in hist) - 3) < 1e-9 and abs(sum(row[2] for row in hist) - 3) < 1e-9

@@ -357,3 +357,14 @@ def test_example_params_regime_dim10(oracle):
     for _ in range(3):
         fast.kick_drift(0.01, 0.01, 8)
     assert np.max(np.abs(fast.links() - o.field("u"))) < 1e-12
+
+
+@pytest.mark.parametrize("n,ext", [(3, [4, 3, 2, 2]), (2, [3, 4, 2]), (4, [2, 3, 3, 2])])
+def test_measure_definitions_vs_oracle(oracle, n, ext):
+    """measure4d's PolyakovLoop and WilsonLoop(evs, d) from the GPU lines
+    against the oracle's restatement (incl. its loop-variable quirk)."""
+    o, u, p = _state(oracle, n, ext, 31)
+    lat = _gpu(n, ext, A.FP64, u, p)
+    assert abs(lat.polyakov_measure() - o.polyakov_measure()) < 1e-13
+    for d in range(1, len(ext)):
+        assert abs(lat.wilson_measure(d) - o.wilson_measure(d)) < 1e-13

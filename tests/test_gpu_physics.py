@@ -1,6 +1,7 @@
-"""Statistical parity of the GPU HMC chain: ensemble averages against the
-oracle's chain (tests/golden/ensemble_su3_4x4.json, written by
-make_ensemble.py) and the plaquette of 2D SU(3) against the exact
+"""Statistical parity of the GPU HMC chain: ensemble averages (plaquette,
+|Polyakov loop|, and measurements4d.cpp's Wilson line W_1 and Polyakov loop
+P-bar) against the oracle's chain (tests/golden/ensemble_su3_4x4.json, written
+by make_ensemble.py) and the plaquette of 2D SU(3) against the exact
 single-plaquette result (2D plaquettes are independent up to an exponentially
 small finite-volume correction)."""
 import json
@@ -44,13 +45,16 @@ def test_ensemble_matches_oracle_chain():
     lat.haar(4242, 0)
     for t in range(200):
         lat.hmc_trajectory(5.7, 0.05, 10, 8, 99, t)
-    plaq, pol, acc = [], [], 0
+    plaq, pol, wm, pm, acc = [], [], [], [], 0
     for t in range(4000):
         r = lat.hmc_trajectory(5.7, 0.05, 10, 8, 99, 1000 + t)
         acc += r.accepted
         plaq.append(1.0 - r.action / lat.n_plaquettes())
         pol.append(abs(lat.polyakov()))
-    for name, series in (("plaquette", plaq), ("abs_polyakov", pol)):
+        wm.append(lat.wilson_measure(1))
+        pm.append(lat.polyakov_measure())
+    for name, series in (("plaquette", plaq), ("abs_polyakov", pol), ("wilson_measure_x1", wm),
+                         ("polyakov_measure", pm)):
         m, e = mean_err(series)
         g = gold[name]
         z = abs(m - g["mean"]) / math.hypot(e, g["err"])
