@@ -62,6 +62,10 @@ SIGNATURES = {
     "askja_field_doubles": (C.c_size_t, [_P]),
     "askja_links_upload": (_I, [_P, _DP]),
     "askja_links_download": (_I, [_P, _DP]),
+    "askja_links_download_async": (_I, [_P, _DP]),
+    "askja_download_wait": (_I, [_P]),
+    "askja_host_alloc": (_I, [C.c_size_t, C.POINTER(_DP)]),
+    "askja_host_free": (_I, [_DP]),
     "askja_links_cold": (_I, [_P]),
     "askja_links_haar": (_I, [_P, _U64, _U64]),
     "askja_reunitarize": (_I, [_P]),

@@ -39,6 +39,11 @@ struct askja_ctx {
     size_t staging_doubles = 0;
     hipStream_t st = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // asynchronous configuration download: own staging buffer and copy stream
+    double *astaging = nullptr;
+    hipStream_t cst = nullptr;
+    hipEvent_t dl_ready = nullptr, dl_done = nullptr;
+    bool dl_pending = false;
     int sticky = 0;
     std::string err;
     Geom geom{};
@@ -298,38 +303,6 @@ int askja_create(int nc, int dim, const int *extents, int precision, askja_ctx *
         key[i] = (long)(xs / chunk) * V + i;
         if (par[i] == 0) n0++;
     }
-    if (const char *ord = getenv("ASKJA_SITE_ORDER")) {
-        // experimental: mixed-radix key over items "d/b" (block index of
-        // x_d), "d%b" (x_d within its block) or "d" (x_d)
-        struct Item { int d, b, kind; };
-        std::vector<Item> items;
-        for (const char *q = ord; *q;) {
-            Item it{(int)strtol(q, (char **)&q, 10), 0, 0};
-            if (*q == '/' || *q == '%') {
-                it.kind = *q == '/' ? 1 : 2;
-                it.b = (int)strtol(q + 1, (char **)&q, 10);
-            }
-            items.push_back(it);
-            if (*q == ',') q++;
-        }
-        for (long i = 0; i < V; i++) {
-            long k = 0;
-            for (const Item &it : items) {
-                const int xd = (int)((i / c->helparr[it.d]) % extents[it.d]);
-                const int e = extents[it.d];
-                if (it.kind == 1) k = k * ((e + it.b - 1) / it.b) + xd / it.b;
-                else if (it.kind == 2) k = k * it.b + xd % it.b;
-                else k = k * e + xd;
-            }
-            key[i] = k;
-        }
-        std::vector<long> chk(key);
-        std::sort(chk.begin(), chk.end());
-        if (std::adjacent_find(chk.begin(), chk.end()) != chk.end()) {
-            delete c;
-            return ASKJA_EINVAL;
-        }
-    }
     {
         std::vector<long> order(V);
         for (long i = 0; i < V; i++) order[i] = i;
@@ -414,6 +387,11 @@ int askja_destroy(askja_ctx *c)
                     c->flow_out, c->flow_work, c->flow_log, c->flow_contrib};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
+    if (c->cst) (void)hipStreamSynchronize(c->cst);
+    if (c->astaging) (void)hipFree(c->astaging);
+    if (c->dl_ready) (void)hipEventDestroy(c->dl_ready);
+    if (c->dl_done) (void)hipEventDestroy(c->dl_done);
+    if (c->cst) (void)hipStreamDestroy(c->cst);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->st) (void)hipStreamDestroy(c->st);
@@ -464,6 +442,54 @@ int askja_links_download(askja_ctx *c, double *aos)
     HIPCHK(c, launch_links_to_aos(c->geom, c->links[c->cur], c->staging, c->mode, c->st));
     HIPCHK(c, hipMemcpyAsync(aos, c->staging, c->staging_doubles * sizeof(double), hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
+    return ASKJA_OK;
+}
+
+int askja_links_download_async(askja_ctx *c, double *pinned)
+{
+    ENTER(c);
+    if (!pinned) return fail(c, ASKJA_EINVAL, "links_download_async: null buffer");
+    int rc = ensure_staging(c);  // sizes staging_doubles
+    if (rc) return rc;
+    if (!c->cst) {
+        HIPCHK(c, hipStreamCreateWithFlags(&c->cst, hipStreamNonBlocking));
+        HIPCHK(c, hipEventCreateWithFlags(&c->dl_ready, hipEventDisableTiming));
+        HIPCHK(c, hipEventCreateWithFlags(&c->dl_done, hipEventDisableTiming));
+        HIPCHK(c, hipMalloc((void **)&c->astaging, c->staging_doubles * sizeof(double)));
+    }
+    // the previous copy must have left the staging buffer before it is refilled
+    if (c->dl_pending) HIPCHK(c, hipStreamWaitEvent(c->st, c->dl_done, 0));
+    HIPCHK(c, launch_links_to_aos(c->geom, c->links[c->cur], c->astaging, c->mode, c->st));
+    HIPCHK(c, hipEventRecord(c->dl_ready, c->st));
+    HIPCHK(c, hipStreamWaitEvent(c->cst, c->dl_ready, 0));
+    HIPCHK(c, hipMemcpyAsync(pinned, c->astaging, c->staging_doubles * sizeof(double), hipMemcpyDeviceToHost, c->cst));
+    HIPCHK(c, hipEventRecord(c->dl_done, c->cst));
+    c->dl_pending = true;
+    return ASKJA_OK;
+}
+
+int askja_download_wait(askja_ctx *c)
+{
+    ENTER(c);
+    if (c->dl_pending) HIPCHK(c, hipEventSynchronize(c->dl_done));
+    c->dl_pending = false;
+    return ASKJA_OK;
+}
+
+int askja_host_alloc(size_t ndoubles, double **out)
+{
+    if (!out || ndoubles == 0) return ASKJA_EINVAL;
+    *out = nullptr;
+    if (hipHostMalloc((void **)out, ndoubles * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+        *out = nullptr;
+        return ASKJA_ENOMEM;
+    }
+    return ASKJA_OK;
+}
+
+int askja_host_free(double *p)
+{
+    if (p && hipHostFree(p) != hipSuccess) return ASKJA_EHIP;
     return ASKJA_OK;
 }
 

@@ -37,6 +37,8 @@
 #include <sstream>
 #include <string>
 #include <vector>
+#include <atomic>
+#include <thread>
 
 namespace {
 
@@ -104,6 +106,25 @@ bool store_parameters(const char *infile, const char *outfile, const Params &p, 
     myout << "SEED\t\t" << p.seed << "\n";
     return (bool)myout;
 }
+
+// Configuration output on a host thread: two page-locked buffers, one
+// writer at a time (joined before the next starts and before exit).
+struct ConfWriter {
+    double *pinned[2] = {nullptr, nullptr};
+    int next = 0;
+    std::thread t;
+    std::atomic<bool> ok{true};
+    bool join()
+    {
+        if (t.joinable()) t.join();
+        return ok.load();
+    }
+    ~ConfWriter()
+    {
+        join();
+        for (double *p : pinned) askja_host_free(p);
+    }
+};
 
 int die(askja_ctx *c, int rc, const char *what)
 {
@@ -304,6 +325,7 @@ int main(int argc, char *argv[])
 
     const clock_t c0 = clock();
     const time_t w0 = time(NULL);
+    ConfWriter out;
     int timekeep;
     for (timekeep = P.START + 1; timekeep < P.STEPS + P.START + 1; timekeep++) {
         CK(drift());
@@ -378,22 +400,36 @@ int main(int argc, char *argv[])
                 CK(flush());
                 snprintf(buffer, sizeof buffer, "%s/U-%s-%i.conf", outdir, tag, P.TAKEN);
                 printf("==>Info: storing configuration %s\n", buffer);
-                std::vector<double> u(askja_field_doubles(ctx));
-                CK(askja_links_download(ctx, u.data()));
-                FILE *f = fopen(buffer, "wb");
-                if (!f || fwrite(u.data(), sizeof(double), u.size(), f) != u.size()) {
-                    fprintf(stderr, "askja: writing %s failed\n", buffer);
-                    return 1;
-                }
-                fclose(f);
+                // device -> pinned buffer, then the file and the master file
+                // are written on a host thread while the chain continues; the
+                // writes stay in order (one writer at a time)
+                const size_t nd = askja_field_doubles(ctx);
+                if (!out.pinned[out.next]) CK(askja_host_alloc(nd, &out.pinned[out.next]));
+                CK(askja_links_download_async(ctx, out.pinned[out.next]));
+                CK(askja_download_wait(ctx));
+                if (!out.join()) return 1;
                 P.TAKEN++;
+                const std::string conf = buffer;
                 snprintf(buffer, sizeof buffer, "%s/master-%s.params", outdir, tag);
-                if (!store_parameters(paramsfile, buffer, P, timekeep, P.TAKEN)) return 1;
+                const std::string master = buffer;
+                const Params snap = P;
+                const int tk = timekeep;
+                double *data = out.pinned[out.next];
+                out.t = std::thread([&out = out, conf, master, snap, tk, data, nd, paramsfile]() {
+                    FILE *f = fopen(conf.c_str(), "wb");
+                    bool ok = f && fwrite(data, sizeof(double), nd, f) == nd;
+                    if (f) fclose(f);
+                    if (!ok) fprintf(stderr, "askja: writing %s failed\n", conf.c_str());
+                    else ok = store_parameters(paramsfile, master.c_str(), snap, tk, snap.TAKEN);
+                    if (!ok) out.ok = false;
+                });
+                out.next ^= 1;
             }
         }
     }
     CK(flush());
     CK(askja_sync(ctx));
+    if (!out.join()) return 1;
     fclose(constraints);
     fclose(wilson);
     const double wall = difftime(time(NULL), w0);

@@ -82,6 +82,17 @@ size_t askja_field_doubles(const askja_ctx *ctx);
 int askja_links_upload(askja_ctx *ctx, const double *aos);
 /* storeconf (sun.cpp:289-302), its in-memory half: host buffer <- links */
 int askja_links_download(askja_ctx *ctx, double *aos);
+/* Asynchronous askja_links_download for writing configurations while the
+ * chain continues: the AoS conversion runs on the context's stream (ordered
+ * after everything queued so far), the device-to-host copy into `pinned`
+ * (from askja_host_alloc, askja_field_doubles long) on a second stream.
+ * Complete when askja_download_wait returns; `pinned` must not be read
+ * before.  A new download first waits for the previous copy. */
+int askja_links_download_async(askja_ctx *ctx, double *pinned);
+int askja_download_wait(askja_ctx *ctx);
+/* Page-locked host buffers for askja_links_download_async. */
+int askja_host_alloc(size_t ndoubles, double **out);
+int askja_host_free(double *p);
 /* initializeU (sun.cpp:194-207): cold start, all links = identity */
 int askja_links_cold(askja_ctx *ctx);
 /* Haar-random SU(N) start (no reference counterpart): Ginibre matrix from

@@ -368,3 +368,35 @@ def test_measure_definitions_vs_oracle(oracle, n, ext):
     assert abs(lat.polyakov_measure() - o.polyakov_measure()) < 1e-13
     for d in range(1, len(ext)):
         assert abs(lat.wilson_measure(d) - o.wilson_measure(d)) < 1e-13
+
+
+def test_async_download_into_pinned_buffers(oracle):
+    """askja_links_download_async (conversion on the context's stream, copy on
+    a second stream into page-locked memory) gives the synchronous download's
+    field, also when a second download is queued behind MD work."""
+    import ctypes as C
+
+    n, ext = 3, [4, 4, 2, 2]
+    lat = _gpu(n, ext, A.FP64)
+    lat.haar(5, 0)
+    nd = A.lib().askja_field_doubles(lat._h)
+    bufs = []
+    for _ in range(2):
+        p = C.POINTER(C.c_double)()
+        assert A.lib().askja_host_alloc(nd, C.byref(p)) == 0
+        bufs.append(p)
+    try:
+        want0 = lat.links()
+        assert A.lib().askja_links_download_async(lat._h, bufs[0]) == 0
+        lat.mom_refresh(1, 0, 5.0)
+        lat.kick_drift(0.02, 0.02, 8)  # queued after the first conversion
+        assert A.lib().askja_links_download_async(lat._h, bufs[1]) == 0
+        assert A.lib().askja_download_wait(lat._h) == 0
+        got0 = np.ctypeslib.as_array(bufs[0], (nd,)).view(np.complex128).reshape(want0.shape)
+        got1 = np.ctypeslib.as_array(bufs[1], (nd,)).view(np.complex128).reshape(want0.shape)
+        assert np.array_equal(got0, want0)
+        assert np.array_equal(got1, lat.links())
+        assert not np.array_equal(got0, got1)
+    finally:
+        for p in bufs:
+            A.lib().askja_host_free(p)
