@@ -843,6 +843,7 @@ int askja_slice_traces(askja_ctx *c, int dir, double *out)
     double *d = nullptr;
     HIPCHK(c, hipMallocAsync((void **)&d, (size_t)2 * n0 * sizeof(double), c->st));
     hipError_t e = launch_slice_trace(c->geom, c->links[c->cur], dir, c->mode, c->part, c->st);
+    if (e != hipSuccess) (void)hipFreeAsync(d, c->st);
     for (long x0 = 0; x0 < n0 && e == hipSuccess; x0++)
         e = launch_finalize(c->part + 2 * x0 * nb, nb, 1, d + 2 * x0, c->st);
     std::vector<double> h((size_t)2 * n0);
@@ -865,19 +866,19 @@ int askja_line_phases(askja_ctx *c, int dir, int nlines, const int64_t *starts, 
     const size_t nn = (size_t)c->nc * c->nc;
     long *ds = nullptr;
     double *wb = nullptr, *dp = nullptr;
-    HIPCHK(c, hipMallocAsync((void **)&ds, nlines * sizeof(long), c->st));
-    HIPCHK(c, hipMallocAsync((void **)&wb, (size_t)nlines * 2 * nn * sizeof(double), c->st));
-    HIPCHK(c, hipMallocAsync((void **)&dp, (size_t)nlines * c->nc * sizeof(double), c->st));
     std::vector<long> hs(starts, starts + nlines);
-    hipError_t e = hipMemcpyAsync(ds, hs.data(), nlines * sizeof(long), hipMemcpyHostToDevice, c->st);
+    hipError_t e = hipMallocAsync((void **)&ds, nlines * sizeof(long), c->st);
+    if (e == hipSuccess) e = hipMallocAsync((void **)&wb, (size_t)nlines * 2 * nn * sizeof(double), c->st);
+    if (e == hipSuccess) e = hipMallocAsync((void **)&dp, (size_t)nlines * c->nc * sizeof(double), c->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(ds, hs.data(), nlines * sizeof(long), hipMemcpyHostToDevice, c->st);
     if (e == hipSuccess) e = launch_line_phases(c->geom, c->links[c->cur], dir, c->mode, nlines, ds, wb, dp, c->st);
     if (e == hipSuccess)
         e = hipMemcpyAsync(out, dp, (size_t)nlines * c->nc * sizeof(double), hipMemcpyDeviceToHost, c->st);
     if (e == hipSuccess) e = hipStreamSynchronize(c->st);
-    (void)hipFreeAsync(ds, c->st);
-    (void)hipFreeAsync(wb, c->st);
-    (void)hipFreeAsync(dp, c->st);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    for (void *q : {(void *)ds, (void *)wb, (void *)dp})
+        if (q) (void)hipFreeAsync(q, c->st);
+    const hipError_t e2 = hipStreamSynchronize(c->st);
+    if (e == hipSuccess) e = e2;
     if (e != hipSuccess) return fail(c, ASKJA_EHIP, "line_phases: %s", hipGetErrorString(e));
     return ASKJA_OK;
 }

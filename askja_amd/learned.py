@@ -76,8 +76,9 @@ def flow_action_function():
     return FlowAction
 
 
-def plain_chain(lat, beta, dt, ndrift, level, seed, first, n, measure=True):
-    """n plain HMC trajectories; returns (|P| series, acceptance, seconds)."""
+def plain_chain(lat, beta, dt, ndrift, level, seed, first, n, measure=True, plaq=None):
+    """n plain HMC trajectories; returns (|P| series, acceptance, seconds);
+    the plaquette series is appended to `plaq` when given."""
     series, acc = [], 0
     lat.sync()
     t0 = time.perf_counter()
@@ -86,12 +87,15 @@ def plain_chain(lat, beta, dt, ndrift, level, seed, first, n, measure=True):
         acc += r.accepted
         if measure:
             series.append(abs(lat.polyakov()))
+            if plaq is not None:
+                plaq.append(1.0 - r.action / lat.n_plaquettes())
     lat.sync()
     return np.array(series), acc / max(n, 1), time.perf_counter() - t0
 
 
-def flow_chain(lat, beta, dt, ndrift, level, seed, first, n, log=None):
-    """n FTHMC trajectories; |P| measured on U = F(V)."""
+def flow_chain(lat, beta, dt, ndrift, level, seed, first, n, log=None, plaq=None):
+    """n FTHMC trajectories; |P| (and the plaquette, into `plaq`) measured on
+    the physical configuration U = F(V)."""
     series, acc = [], 0
     lat.sync()
     t0 = time.perf_counter()
@@ -99,10 +103,20 @@ def flow_chain(lat, beta, dt, ndrift, level, seed, first, n, log=None):
         r = lat.flow_hmc_trajectory(beta, dt, ndrift, level, seed, first + k, reunit=True)
         acc += r.accepted
         series.append(abs(lat.flow_polyakov()))
+        if plaq is not None:
+            plaq.append(1.0 - r.action / lat.n_plaquettes())
         if log and (k + 1) % 50 == 0:
             log(f"[learned] flow trajectory {k + 1}/{n} ({time.perf_counter() - t0:.1f} s)")
     lat.sync()
     return np.array(series), acc / max(n, 1), time.perf_counter() - t0
+
+
+def mean_err(x):
+    """Mean and its error sqrt(var tau / n) with tau from tau_int."""
+    x = np.asarray(x, np.float64)
+    t = tau_int(x)
+    t = 1.0 if not np.isfinite(t) or t < 1.0 else t
+    return float(x.mean()), float(x.std() * math.sqrt(t / len(x)))
 
 
 def tau_int(x, S=1.5):
@@ -192,19 +206,26 @@ def main():
     plain_chain(lat, args.beta, args.dt, args.ndrift, args.level, args.seed, 2 * 10**6, args.therm, measure=False)
     start = lat.links()
     res = {}
-    pl, acc_p, t_p = plain_chain(lat, args.beta, args.dt, args.ndrift, args.level, args.seed, 3 * 10**6, args.ntraj)
+    pq = []
+    pl, acc_p, t_p = plain_chain(lat, args.beta, args.dt, args.ndrift, args.level, args.seed, 3 * 10**6, args.ntraj,
+                                 plaq=pq)
     res["plain"] = {"traj_per_s": args.ntraj / t_p, "acceptance": acc_p, "tau": tau_int(pl),
-                    "mean_abs_polyakov": float(pl.mean())}
+                    "abs_polyakov": mean_err(pl), "plaquette": mean_err(pq)}
     for name, weights in (("flow_untrained", w0), ("flow_trained", w)):
         lat.set_links(start)
         lat.flow_set_weights(weights)
         log(f"[learned] {name} chain")
+        fq = []
         fl, acc_f, t_f = flow_chain(lat, args.beta, args.dt, args.ndrift, args.level, args.seed, 4 * 10**6, args.ntraj,
-                                    log=log)
+                                    log=log, plaq=fq)
         res[name] = {"traj_per_s": args.ntraj / t_f, "acceptance": acc_f, "tau": tau_int(fl),
-                     "mean_abs_polyakov": float(fl.mean())}
+                     "abs_polyakov": mean_err(fl), "plaquette": mean_err(fq)}
     for k, v in res.items():
         v["indep_configs_per_s"] = v["traj_per_s"] / v["tau"]
+        if k != "plain":  # agreement with the plain chain, in combined standard errors
+            for obs in ("abs_polyakov", "plaquette"):
+                (m, e), (m0, e0) = v[obs], res["plain"][obs]
+                v[obs + "_z_vs_plain"] = (m - m0) / math.hypot(e, e0)
     out = {"workload": f"SU(3) D=4 {args.extent}^4 beta={args.beta}, FTHMC {args.layers} sweep layer(s) "
                        f"({len(mu)} masked sub-layers), trained on {args.ntrain} plain-HMC configs at "
                        f"beta={args.beta_train}, dt={args.dt}, {args.ndrift} MD steps",
