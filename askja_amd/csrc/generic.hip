@@ -33,6 +33,8 @@ ASKJA_TEAM_DECL(64)
 hipError_t mfma_kick_16(const KickArgs &a, int mode, bool drift, hipStream_t st);
 hipError_t mfma_kick_32(const KickArgs &a, int mode, bool drift, hipStream_t st);
 hipError_t mfma_kick_64(const KickArgs &a, int mode, bool drift, hipStream_t st);
+hipError_t rows_kick(const KickArgs &a, int mode, bool drift, hipStream_t st);
+long rows_kick_blocks(const Geom &g);
 hipError_t mfma_action_16(const KickArgs &a, hipStream_t st);
 hipError_t mfma_action_32(const KickArgs &a, hipStream_t st);
 hipError_t mfma_action_64(const KickArgs &a, hipStream_t st);
@@ -567,6 +569,7 @@ long generic_scratch_per_thread(int n) { return 12L * n * n; }
 long kick_partials(const Geom &g)
 {
     if (g.N <= 3) return kick_site_blocks(g) * g.D;
+    if (g.N <= 8) return rows_kick_blocks(g);
     return g.V * g.D;
 }
 
@@ -582,7 +585,7 @@ hipError_t launch_kick(const KickArgs &a, int mode, bool drift, hipStream_t st)
     }
     if (mode == kModeBF16x3) mode = kModeF32;
     if (n <= 3) return small_kick(a, mode, drift, st);
-    if (n <= 8) return team_kick_8(a, mode, drift, st);
+    if (n <= 8) return rows_kick(a, mode, drift, st);
     if (n <= 16) return team_kick_16(a, mode, drift, st);
     if (n <= 32) return team_kick_32(a, mode, drift, st);
     return team_kick_64(a, mode, drift, st);

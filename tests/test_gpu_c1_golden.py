@@ -1,7 +1,10 @@
 """One fixed-momentum trajectory of the reference's example workload at
 DIM 4 (NC 8, 10^4, BETA 20, DT 0.01, LEVEL 8, cold start, mt19937 seed
 120351616 momenta, 999 drifts) against the oracle's result committed in
-tests/golden/c1_trajectory1.json (written by make_c1_trajectory.py).
+tests/golden/c1_trajectory1.json (written by make_c1_trajectory.py), and the
+same for the reference's own example params file as it stands (DIM 10,
+10 x 10 x 1^8, BETA 20000; tests/golden/example_dim10_trajectory1.json,
+written by make_example_dim10.py).
 
 FP64_EXACT must reproduce the oracle's links and momenta bit for bit (equal
 SHA-256); FP64 (FMA, factored staples) must agree link by link to 1e-10.
@@ -50,3 +53,25 @@ def test_c1_trajectory_fast_link_by_link():
     assert d <= 1e-10, d
     beta, n2 = GOLD["beta"], 2.0 * GOLD["nc"]
     assert abs(beta * (e[2] + e[3] / n2) - GOLD["h_new"]) < 1e-6
+
+
+EXAMPLE = json.load(open(os.path.join(ROOT, "tests", "golden", "example_dim10_trajectory1.json")))
+
+
+def test_example_params_trajectory_exact_bitwise():
+    lat = A.Lattice(EXAMPLE["nc"], EXAMPLE["extents"], A.FP64_EXACT)
+    lat.cold()
+    lat.mom_gsl(EXAMPLE["seed"], EXAMPLE["beta"])
+    e = lat.md_trajectory(EXAMPLE["dt"], EXAMPLE["ndrift"], EXAMPLE["level"])
+    u = lat.links()
+    assert hashlib.sha256(np.ascontiguousarray(u).tobytes()).hexdigest() == EXAMPLE["links_sha256"]
+    assert hashlib.sha256(np.ascontiguousarray(lat.momenta()).tobytes()).hexdigest() == EXAMPLE["momenta_sha256"]
+    for s in EXAMPLE["samples"]:
+        assert np.array_equal(u[s["site"], s["dir"]].ravel(), np.array(s["re"]) + 1j * np.array(s["im"]))
+    beta, n2 = EXAMPLE["beta"], 2.0 * EXAMPLE["nc"]
+    assert abs(beta * (e[2] + e[3] / n2) - EXAMPLE["h_new"]) < 1e-9 * abs(EXAMPLE["h_old"])
+    fast = A.Lattice(EXAMPLE["nc"], EXAMPLE["extents"], A.FP64)
+    fast.cold()
+    fast.mom_gsl(EXAMPLE["seed"], EXAMPLE["beta"])
+    fast.md_trajectory(EXAMPLE["dt"], EXAMPLE["ndrift"], EXAMPLE["level"])
+    assert np.max(np.abs(fast.links() - u)) <= 1e-10
