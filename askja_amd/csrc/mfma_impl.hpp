@@ -16,7 +16,7 @@
 // checked on gfx950 with one-hot operands.  The A operand of lane l is
 // A[l%16][k0 + l/16] and the B operand B[k0 + l/16][l%16] for both 16x16x4
 // forms.
-// Operands are staged in two LDS slots (planar re/im, row stride NMAX).
+// Operands are staged in two LDS slots (planar re/im, row stride NMAX + 4).
 #pragma once
 #include "kernels.hpp"
 #include "common.hpp"
@@ -53,7 +53,15 @@ struct Cfg {
     static constexpr int WPR = (P == kF64 && NMAX == 64) ? 2 : 1;  // wavefronts per 16-row block
     static constexpr int NT = 64 * (NMAX / 16) * WPR;
     static constexpr int NCT = NMAX / 16 / WPR;  // column tiles per wavefront
-    static constexpr int SLOT = NMAX * NMAX;
+    // operand prefetch through registers, except at NMAX = 64 in FP32 and
+    // BF16x3, where the extra registers would halve the occupancy
+    static constexpr bool PF = !(P != kF64 && NMAX == 64);
+    // LDS row stride: NMAX + 4 elements puts the 16 rows a wavefront reads
+    // for one operand on distinct banks (an unpadded stride maps them all
+    // to one bank group).  The BF16x3 form without prefetch measured faster
+    // unpadded.
+    static constexpr int LD = NMAX + ((P == kBF16X3 && !PF) ? 0 : 4);
+    static constexpr int SLOT = NMAX * LD;
 };
 
 template <int P, int NMAX>
@@ -92,15 +100,47 @@ __device__ inline void split8(const float (&x)[8], bf16x8 &hi, bf16x8 &lo)
     }
 }
 
+// One half of the stacked product on the 16x16x4 forms: IM = false takes
+// k < NMAX ([Ar] against [Br ; Bi] rows), IM = true the rest ([Ai] against
+// [-Bi ; Br]).  Lane l holds A[row][k] and B[k][col], k = 4 ks + l/16.
+template <int P, int NMAX, bool DA, bool DB, bool IM, typename R>
+__device__ inline void cmul_half(const R *A, const R *Br, const R *Bi, Acc<P, NMAX> &acc)
+{
+    constexpr int NCT = Cfg<P, NMAX>::NCT, LD = Cfg<P, NMAX>::LD;
+    const int lane = threadIdx.x & 63;
+    const int row = 16 * wave_rowblock<P, NMAX>() + (lane & 15);
+    const int t0 = wave_tile0<P, NMAX>();
+#pragma unroll 4
+    for (int ks = 0; ks < NMAX / 4; ks++) {
+        const int k = 4 * ks + (lane >> 4);
+        const R a = DA ? A[k * LD + row] : A[row * LD + k];
+        const R av = (IM && DA) ? -a : a;
+#pragma unroll
+        for (int t = 0; t < NCT; t++) {
+            const int col = 16 * (t0 + t) + (lane & 15);
+            const R bre = DB ? Br[col * LD + k] : Br[k * LD + col];
+            const R bim = DB ? -Bi[col * LD + k] : Bi[k * LD + col];
+            const R br = IM ? -bim : bre, bi = IM ? bre : bim;
+            if constexpr (P == kF64) {
+                acc.r[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, br, acc.r[t], 0, 0, 0);
+                acc.i[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bi, acc.i[t], 0, 0, 0);
+            } else {
+                acc.r[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, br, acc.r[t], 0, 0, 0);
+                acc.i[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bi, acc.i[t], 0, 0, 0);
+            }
+        }
+    }
+}
+
 // acc (+)= op(A) op(B) on the wavefront's rows and column tiles; A, B in LDS.
 template <int P, int NMAX, bool DA, bool DB, typename R>
 __device__ inline void cmul(const R *Ar, const R *Ai, const R *Br, const R *Bi, Acc<P, NMAX> &acc)
 {
-    constexpr int NCT = Cfg<P, NMAX>::NCT;
-    const int lane = threadIdx.x & 63;
-    const int row = 16 * wave_rowblock<P, NMAX>() + (lane & 15);
-    const int t0 = wave_tile0<P, NMAX>();
     if constexpr (P == kBF16X3) {
+        constexpr int NCT = Cfg<P, NMAX>::NCT, LD = Cfg<P, NMAX>::LD;
+        const int lane = threadIdx.x & 63;
+        const int row = 16 * wave_rowblock<P, NMAX>() + (lane & 15);
+        const int t0 = wave_tile0<P, NMAX>();
         const int kq = 8 * (lane >> 4);
 #pragma unroll
         for (int ks = 0; ks < (2 * NMAX) / 32; ks++) {
@@ -111,8 +151,8 @@ __device__ inline void cmul(const R *Ar, const R *Ai, const R *Br, const R *Bi, 
 #pragma unroll
             for (int j = 0; j < 8; j++) {
                 const int k = kb + j;
-                const float re = DA ? Ar[k * NMAX + row] : Ar[row * NMAX + k];
-                const float im = DA ? -Ai[k * NMAX + row] : Ai[row * NMAX + k];
+                const float re = DA ? Ar[k * LD + row] : Ar[row * LD + k];
+                const float im = DA ? -Ai[k * LD + row] : Ai[row * LD + k];
                 av[j] = imag ? im : re;
             }
             bf16x8 ah, al;
@@ -124,8 +164,8 @@ __device__ inline void cmul(const R *Ar, const R *Ai, const R *Br, const R *Bi, 
 #pragma unroll
                 for (int j = 0; j < 8; j++) {
                     const int k = kb + j;
-                    const float re = DB ? Br[col * NMAX + k] : Br[k * NMAX + col];
-                    const float im = DB ? -Bi[col * NMAX + k] : Bi[k * NMAX + col];
+                    const float re = DB ? Br[col * LD + k] : Br[k * LD + col];
+                    const float im = DB ? -Bi[col * LD + k] : Bi[k * LD + col];
                     brv[j] = imag ? -im : re;  // [Br ; -Bi]
                     biv[j] = imag ? re : im;   // [Bi ; Br]
                 }
@@ -141,30 +181,8 @@ __device__ inline void cmul(const R *Ar, const R *Ai, const R *Br, const R *Bi, 
             }
         }
     } else {
-        // 16x16x4: lane l holds A[row][k0] and B[k0][col], k0 = 4 ks + l/16
-#pragma unroll 4
-        for (int ks = 0; ks < (2 * NMAX) / 4; ks++) {
-            const int k0 = 4 * ks + (lane >> 4);
-            const bool imag = k0 >= NMAX;
-            const int k = imag ? k0 - NMAX : k0;
-            const R are = DA ? Ar[k * NMAX + row] : Ar[row * NMAX + k];
-            const R aim = DA ? -Ai[k * NMAX + row] : Ai[row * NMAX + k];
-            const R av = imag ? aim : are;
-#pragma unroll
-            for (int t = 0; t < NCT; t++) {
-                const int col = 16 * (t0 + t) + (lane & 15);
-                const R bre = DB ? Br[col * NMAX + k] : Br[k * NMAX + col];
-                const R bim = DB ? -Bi[col * NMAX + k] : Bi[k * NMAX + col];
-                const R br = imag ? -bim : bre, bi = imag ? bre : bim;
-                if constexpr (P == kF64) {
-                    acc.r[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, br, acc.r[t], 0, 0, 0);
-                    acc.i[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bi, acc.i[t], 0, 0, 0);
-                } else {
-                    acc.r[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, br, acc.r[t], 0, 0, 0);
-                    acc.i[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bi, acc.i[t], 0, 0, 0);
-                }
-            }
-        }
+        cmul_half<P, NMAX, DA, DB, false>(Ar, Br, Bi, acc);
+        cmul_half<P, NMAX, DA, DB, true>(Ai, Br, Bi, acc);
     }
 }
 
@@ -188,8 +206,8 @@ __device__ inline void acc_store(const Acc<P, NMAX> &a, R *Sr, R *Si)
     for (int t = 0; t < Cfg<P, NMAX>::NCT; t++)
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            Sr[acc_row<P, NMAX>(q) * NMAX + acc_col<P, NMAX>(t)] = a.r[t][q];
-            Si[acc_row<P, NMAX>(q) * NMAX + acc_col<P, NMAX>(t)] = a.i[t][q];
+            Sr[acc_row<P, NMAX>(q) * Cfg<P, NMAX>::LD + acc_col<P, NMAX>(t)] = a.r[t][q];
+            Si[acc_row<P, NMAX>(q) * Cfg<P, NMAX>::LD + acc_col<P, NMAX>(t)] = a.i[t][q];
         }
 }
 
@@ -202,10 +220,55 @@ __device__ inline void ld_link(R *Sr, R *Si, const S *__restrict__ f, long l, in
     for (int e = threadIdx.x; e < NMAX * NMAX; e += NT) {
         const int r = e / NMAX, c = e % NMAX;
         const bool in = r < n && c < n;
-        Sr[e] = in ? (R)b[2 * (r * n + c)] : R(0);
-        Si[e] = in ? (R)b[2 * (r * n + c) + 1] : R(0);
+        Sr[r * Cfg<P, NMAX>::LD + c] = in ? (R)b[2 * (r * n + c)] : R(0);
+        Si[r * Cfg<P, NMAX>::LD + c] = in ? (R)b[2 * (r * n + c) + 1] : R(0);
     }
 }
+
+// Register staging of one link operand: fetch() issues its global loads,
+// stash() writes it to an LDS slot.  With PF the fetch of the next operands
+// is issued before the current product and the stash follows it, so the
+// loads run under the MFMA work; without PF the operand is loaded straight
+// into its slot at the stash point (fewer live registers).
+template <int P, int NMAX, typename S>
+struct Stage {
+    static constexpr int NT = Cfg<P, NMAX>::NT, E = NMAX * NMAX / NT;
+    static_assert(E * NT == NMAX * NMAX, "slot not a multiple of the workgroup");
+    typedef S S2 __attribute__((ext_vector_type(2)));
+    S2 v[E];
+    __device__ inline void fetch(const S *__restrict__ f, long l, int n)
+    {
+        const S2 *b = (const S2 *)(f + (size_t)l * 2 * n * n);
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+            const int e = threadIdx.x + j * NT, r = e / NMAX, c = e % NMAX;
+            v[j] = (r < n && c < n) ? b[r * n + c] : S2{S(0), S(0)};
+        }
+    }
+    template <typename R>
+    __device__ inline void stash(R *Sr, R *Si) const
+    {
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+            const int e = threadIdx.x + j * NT, r = e / NMAX, c = e % NMAX;
+            Sr[r * Cfg<P, NMAX>::LD + c] = (R)v[j][0];
+            Si[r * Cfg<P, NMAX>::LD + c] = (R)v[j][1];
+        }
+    }
+    template <bool PF>
+    __device__ inline void early(const S *__restrict__ f, long l, int n)
+    {
+        if constexpr (PF) fetch(f, l, n);
+    }
+    template <bool PF, typename R>
+    __device__ inline void late(const S *__restrict__ f, long l, int n, R *Sr, R *Si)
+    {
+        if constexpr (PF)
+            stash(Sr, Si);
+        else
+            ld_link<P, NMAX>(Sr, Si, f, l, n);
+    }
+};
 
 template <typename R>
 __device__ inline R mom_el(const R *__restrict__ P, long l, int n, int i, int j, R &im)
@@ -241,42 +304,61 @@ __global__ __launch_bounds__((Cfg<P, NMAX>::NT)) void k_kick(KickArgs a)
     const LS *U = (const LS *)a.U;
     const long xp = g.nbr[(2 * mu) * V + s];
 
+    // operands in the order the products consume them, per direction nu:
+    // U_nu(x+mu), U_mu(x+nu) | U_nu(x) | U_nu(x+mu-nu), U_mu(x-nu) | U_nu(x-nu)
+    constexpr bool PF = Cfg<P, NMAX>::PF;
+    Stage<P, NMAX, LS> s0, s1;
+    int nu = mu == 0 ? 1 : 0;
+    s0.template early<PF>(U, nu * V + xp, n);
+    s1.template early<PF>(U, mu * V + g.nbr[(2 * nu) * V + s], n);
     Acc<P, NMAX> S, T;
     acc_zero(S);
-    for (int nu = 0; nu < g.D; nu++) {
-        if (nu == mu) continue;
+    while (nu < g.D) {
+        int nx = nu + 1;
+        if (nx == mu) nx++;
         const long xpn = g.nbr[(2 * nu) * V + s];
         const long xmn = g.nbr[(2 * nu + 1) * V + s];
         const long xpmn = g.nbr[(2 * nu + 1) * V + xp];
         // S += U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+
         __syncthreads();
-        ld_link<P, NMAX>(Are, Aim, U, nu * V + xp, n);
-        ld_link<P, NMAX>(Bre, Bim, U, mu * V + xpn, n);
+        s0.template late<PF>(U, nu * V + xp, n, Are, Aim);
+        s1.template late<PF>(U, mu * V + xpn, n, Bre, Bim);
         __syncthreads();
+        s0.template early<PF>(U, nu * V + s, n);
         acc_zero(T);
         cmul<P, NMAX, false, true>(Are, Aim, Bre, Bim, T);
         __syncthreads();
         acc_store(T, Are, Aim);
-        ld_link<P, NMAX>(Bre, Bim, U, nu * V + s, n);
+        s0.template late<PF>(U, nu * V + s, n, Bre, Bim);
         __syncthreads();
+        s0.template early<PF>(U, nu * V + xpmn, n);
+        s1.template early<PF>(U, mu * V + xmn, n);
         cmul<P, NMAX, false, true>(Are, Aim, Bre, Bim, S);
         // S += U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu)
         __syncthreads();
-        ld_link<P, NMAX>(Are, Aim, U, nu * V + xpmn, n);
-        ld_link<P, NMAX>(Bre, Bim, U, mu * V + xmn, n);
+        s0.template late<PF>(U, nu * V + xpmn, n, Are, Aim);
+        s1.template late<PF>(U, mu * V + xmn, n, Bre, Bim);
         __syncthreads();
+        s0.template early<PF>(U, nu * V + xmn, n);
         acc_zero(T);
         cmul<P, NMAX, true, true>(Are, Aim, Bre, Bim, T);
         __syncthreads();
         acc_store(T, Are, Aim);
-        ld_link<P, NMAX>(Bre, Bim, U, nu * V + xmn, n);
+        s0.template late<PF>(U, nu * V + xmn, n, Bre, Bim);
         __syncthreads();
+        if (nx < g.D) {
+            s0.template early<PF>(U, nx * V + xp, n);
+            s1.template early<PF>(U, mu * V + g.nbr[(2 * nx) * V + s], n);
+        } else {
+            s0.template early<PF>(U, mu * V + s, n);
+        }
         cmul<P, NMAX, false, false>(Are, Aim, Bre, Bim, S);
+        nu = nx;
     }
     // W = U_mu(x) S
     __syncthreads();
     acc_store(S, Are, Aim);
-    ld_link<P, NMAX>(Bre, Bim, U, mu * V + s, n);
+    s0.template late<PF>(U, mu * V + s, n, Bre, Bim);
     __syncthreads();
     acc_zero(T);
     cmul<P, NMAX, false, false>(Bre, Bim, Are, Aim, T);
@@ -290,7 +372,7 @@ __global__ __launch_bounds__((Cfg<P, NMAX>::NT)) void k_kick(KickArgs a)
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const int i = acc_row<P, NMAX>(q), j = acc_col<P, NMAX>(t);
-                if (i == j && i < n) sum[0] += (double)Are[i * NMAX + i];
+                if (i == j && i < n) sum[0] += (double)Are[i * Cfg<P, NMAX>::LD + i];
             }
         block_reduce_store<3>(sum, cmp, a.part + 6 * (size_t)lb);
         return;
@@ -299,7 +381,7 @@ __global__ __launch_bounds__((Cfg<P, NMAX>::NT)) void k_kick(KickArgs a)
     const R eps = (R)a.eps_kick, inv_n = (R)a.inv_n;
     R *Pm = (R *)a.P;
     R tri = R(0);
-    for (int i = 0; i < n; i++) tri += Aim[i * NMAX + i] + Aim[i * NMAX + i];
+    for (int i = 0; i < n; i++) tri += Aim[i * Cfg<P, NMAX>::LD + i] + Aim[i * Cfg<P, NMAX>::LD + i];
     const R corr = tri * inv_n;
     double sum[3] = {0.0, 0.0, 0.0}, cmp[3] = {0.0, 0.0, 0.0};
     R pr[NCT][4], pim[NCT][4], opr[NCT][4], opi[NCT][4];
@@ -315,13 +397,13 @@ __global__ __launch_bounds__((Cfg<P, NMAX>::NT)) void k_kick(KickArgs a)
             opr[t][q] = ir;
             opi[t][q] = ii;
             if (i == j) {
-                const R mi = (Aim[i * NMAX + i] + Aim[i * NMAX + i]) - corr;
+                const R mi = (Aim[i * Cfg<P, NMAX>::LD + i] + Aim[i * Cfg<P, NMAX>::LD + i]) - corr;
                 pr[t][q] = ir - (R(0.5) * mi) * eps;
-                sum[0] += (double)Are[i * NMAX + i];
+                sum[0] += (double)Are[i * Cfg<P, NMAX>::LD + i];
             } else {
                 const int lo = i < j ? i : j, hi = i < j ? j : i;
-                const R dr = Are[lo * NMAX + hi] - Are[hi * NMAX + lo];
-                const R di = Aim[lo * NMAX + hi] + Aim[hi * NMAX + lo];
+                const R dr = Are[lo * Cfg<P, NMAX>::LD + hi] - Are[hi * Cfg<P, NMAX>::LD + lo];
+                const R di = Aim[lo * Cfg<P, NMAX>::LD + hi] + Aim[hi * Cfg<P, NMAX>::LD + lo];
                 R ui;
                 const R ur = mom_el(Pm, l, n, lo, hi, ui);
                 const R vr = ur - (R(0.5) * di) * eps, vi = ui + (R(0.5) * dr) * eps;
@@ -367,9 +449,10 @@ __global__ __launch_bounds__((Cfg<P, NMAX>::NT)) void k_kick(KickArgs a)
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const int i = acc_row<P, NMAX>(q), j = acc_col<P, NMAX>(t);
-                Are[i * NMAX + j] = -(pim[t][q] * ed);
-                Aim[i * NMAX + j] = pr[t][q] * ed;
+                Are[i * Cfg<P, NMAX>::LD + j] = -(pim[t][q] * ed);
+                Aim[i * Cfg<P, NMAX>::LD + j] = pr[t][q] * ed;
             }
+        s0.template early<PF>(U, l, n);  // U_mu(x) for the final product
         Acc<P, NMAX> X;
         const R cl = level <= 1 ? R(1) : (R)a.cinv[level];
 #pragma unroll
@@ -400,7 +483,7 @@ __global__ __launch_bounds__((Cfg<P, NMAX>::NT)) void k_kick(KickArgs a)
         }
         __syncthreads();
         acc_store(X, Bre, Bim);
-        ld_link<P, NMAX>(Are, Aim, U, l, n);
+        s0.template late<PF>(U, l, n, Are, Aim);
         __syncthreads();
         Acc<P, NMAX> Un;
         acc_zero(Un);
