@@ -53,14 +53,14 @@ struct Cfg {
     static constexpr int WPR = (P == kF64 && NMAX == 64) ? 2 : 1;  // wavefronts per 16-row block
     static constexpr int NT = 64 * (NMAX / 16) * WPR;
     static constexpr int NCT = NMAX / 16 / WPR;  // column tiles per wavefront
-    // operand prefetch through registers, except at NMAX = 64 in FP32 and
-    // BF16x3, where the extra registers would halve the occupancy
-    static constexpr bool PF = !(P != kF64 && NMAX == 64);
-    // LDS row stride: NMAX + 4 elements puts the 16 rows a wavefront reads
-    // for one operand on distinct banks (an unpadded stride maps them all
-    // to one bank group).  The BF16x3 form without prefetch measured faster
-    // unpadded.
-    static constexpr int LD = NMAX + ((P == kBF16X3 && !PF) ? 0 : 4);
+    // operand prefetch through registers, except at NMAX = 64 in FP32, where
+    // the extra registers would halve the occupancy
+    static constexpr bool PF = !(P == kF32 && NMAX == 64);
+    // LDS row stride in elements.  FP64 / FP32: NMAX + 4 puts the 16 rows a
+    // wavefront reads for one operand on distinct banks (an unpadded stride
+    // maps them all to one bank group).  BF16x3: NMAX + 8 bf16 (16 bytes)
+    // keeps the 16-byte reads of eight consecutive rows on distinct banks.
+    static constexpr int LD = NMAX + (P == kBF16X3 ? 8 : 4);
     static constexpr int SLOT = NMAX * LD;
 };
 
@@ -90,13 +90,45 @@ __device__ inline int wave_tile0()
     return ((threadIdx.x >> 6) % Cfg<P, NMAX>::WPR) * Cfg<P, NMAX>::NCT;
 }
 
-__device__ inline void split8(const float (&x)[8], bf16x8 &hi, bf16x8 &lo)
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+// bf16x8 with the sign of every element flipped where m = 0x80008000
+__device__ inline bf16x8 xsign(bf16x8 v, unsigned m)
 {
-#pragma unroll
-    for (int j = 0; j < 8; j++) {
-        const __bf16 h = (__bf16)x[j];
-        hi[j] = h;
-        lo[j] = (__bf16)(x[j] - (float)h);
+    u32x4 x = __builtin_bit_cast(u32x4, v);
+    x ^= m;
+    return __builtin_bit_cast(bf16x8, x);
+}
+
+__device__ inline bf16x8 ld8(const __bf16 *p)
+{
+    return *(const bf16x8 *)p;
+}
+
+// Operand slots.  FP64 / FP32: planar re / im matrices, row-major, row
+// stride LD.  BF16x3: every element is split once, when it is written, as
+// x = hi + lo (hi = bf16(x), lo = bf16(x - hi)); each of the re and im
+// arrays then holds a hi plane followed by a lo plane of bf16, row stride LD,
+// stored so that the MFMA's k index runs along the row: as A for A and for a
+// daggered B, transposed (TR) for a daggered A and for a B.  The products
+// read eight consecutive k of one plane with one 16-byte load.
+template <int P, int NMAX, bool TR, typename R>
+__device__ inline void put(R *Sr, R *Si, int r, int c, R re, R im)
+{
+    constexpr int LD = Cfg<P, NMAX>::LD;
+    if constexpr (P == kBF16X3) {
+        // no contraction: x - hi must not fuse with a product that formed x
+#pragma clang fp contract(off)
+        const int o = TR ? c * LD + r : r * LD + c;
+        __bf16 *rh = (__bf16 *)Sr, *ih = (__bf16 *)Si;
+        const __bf16 hr = (__bf16)re, hi = (__bf16)im;
+        rh[o] = hr;
+        rh[NMAX * LD + o] = (__bf16)(re - (float)hr);
+        ih[o] = hi;
+        ih[NMAX * LD + o] = (__bf16)(im - (float)hi);
+    } else {
+        Sr[r * LD + c] = re;
+        Si[r * LD + c] = im;
     }
 }
 
@@ -137,41 +169,35 @@ template <int P, int NMAX, bool DA, bool DB, typename R>
 __device__ inline void cmul(const R *Ar, const R *Ai, const R *Br, const R *Bi, Acc<P, NMAX> &acc)
 {
     if constexpr (P == kBF16X3) {
-        constexpr int NCT = Cfg<P, NMAX>::NCT, LD = Cfg<P, NMAX>::LD;
+        // lane l: A[row][k0..k0+7] and B[k0..k0+7][col], k0 = 32 ks + 8 (l/16)
+        // in the stacked index; the slots hold A and B in the orientation
+        // that makes k contiguous (conjugation = sign flip of the im planes)
+        constexpr int NCT = Cfg<P, NMAX>::NCT, LD = Cfg<P, NMAX>::LD, PL = NMAX * LD;
+        const __bf16 *arh = (const __bf16 *)Ar, *aih = (const __bf16 *)Ai;
+        const __bf16 *brh = (const __bf16 *)Br, *bih = (const __bf16 *)Bi;
         const int lane = threadIdx.x & 63;
         const int row = 16 * wave_rowblock<P, NMAX>() + (lane & 15);
         const int t0 = wave_tile0<P, NMAX>();
         const int kq = 8 * (lane >> 4);
-#pragma unroll
+        constexpr unsigned SG = 0x80008000u;
+#pragma unroll 1  // unrolled, the 16-byte operand loads of every step stay live
         for (int ks = 0; ks < (2 * NMAX) / 32; ks++) {
             const int k0 = 32 * ks + kq;           // first of this lane's 8 stacked k
             const bool imag = k0 >= NMAX;          // all 8 in the same half
             const int kb = imag ? k0 - NMAX : k0;  // column of A / row of B
-            float av[8];
-#pragma unroll
-            for (int j = 0; j < 8; j++) {
-                const int k = kb + j;
-                const float re = DA ? Ar[k * LD + row] : Ar[row * LD + k];
-                const float im = DA ? -Ai[k * LD + row] : Ai[row * LD + k];
-                av[j] = imag ? im : re;
-            }
-            bf16x8 ah, al;
-            split8(av, ah, al);
+            // [Ar | Ai] (Ai -> -Ai for A^+)
+            const __bf16 *ap = (imag ? aih : arh) + row * LD + kb;
+            const unsigned ma = (DA && imag) ? SG : 0u;
+            const bf16x8 ah = xsign(ld8(ap), ma), al = xsign(ld8(ap + PL), ma);
+            // real output: [Br ; -Bi], imaginary output: [Bi ; Br]
+            // (Bi -> -Bi for B^+)
+            const __bf16 *bp = imag ? bih : brh, *cp = imag ? brh : bih;
+            const unsigned mb = (imag && !DB) ? SG : 0u, mc = (!imag && DB) ? SG : 0u;
 #pragma unroll
             for (int t = 0; t < NCT; t++) {
-                const int col = 16 * (t0 + t) + (lane & 15);
-                float brv[8], biv[8];
-#pragma unroll
-                for (int j = 0; j < 8; j++) {
-                    const int k = kb + j;
-                    const float re = DB ? Br[col * LD + k] : Br[k * LD + col];
-                    const float im = DB ? -Bi[col * LD + k] : Bi[k * LD + col];
-                    brv[j] = imag ? -im : re;  // [Br ; -Bi]
-                    biv[j] = imag ? re : im;   // [Bi ; Br]
-                }
-                bf16x8 bh, bl, ch, cl;
-                split8(brv, bh, bl);
-                split8(biv, ch, cl);
+                const int o = (16 * (t0 + t) + (lane & 15)) * LD + kb;
+                const bf16x8 bh = xsign(ld8(bp + o), mb), bl = xsign(ld8(bp + PL + o), mb);
+                const bf16x8 ch = xsign(ld8(cp + o), mc), cl = xsign(ld8(cp + PL + o), mc);
                 acc.r[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc.r[t], 0, 0, 0);
                 acc.r[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc.r[t], 0, 0, 0);
                 acc.r[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc.r[t], 0, 0, 0);
@@ -199,8 +225,21 @@ __device__ inline int acc_col(int t)
     return 16 * (wave_tile0<P, NMAX>() + t) + (threadIdx.x & 15);
 }
 
-template <int P, int NMAX, typename R>
+// accumulators -> operand slot (orientation TR, see put)
+template <bool TR, int P, int NMAX, typename R>
 __device__ inline void acc_store(const Acc<P, NMAX> &a, R *Sr, R *Si)
+{
+#pragma unroll
+    for (int t = 0; t < Cfg<P, NMAX>::NCT; t++)
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            put<P, NMAX, TR, R>(Sr, Si, acc_row<P, NMAX>(q), acc_col<P, NMAX>(t), a.r[t][q], a.i[t][q]);
+}
+
+// accumulators -> planar row-major matrix at full precision (read element
+// by element, not as a product operand)
+template <int P, int NMAX, typename R>
+__device__ inline void acc_store_full(const Acc<P, NMAX> &a, R *Sr, R *Si)
 {
 #pragma unroll
     for (int t = 0; t < Cfg<P, NMAX>::NCT; t++)
@@ -212,7 +251,7 @@ __device__ inline void acc_store(const Acc<P, NMAX> &a, R *Sr, R *Si)
 }
 
 // global link (matrix contiguous, n x n complex) -> zero-padded slot
-template <int P, int NMAX, typename R, typename S>
+template <int P, int NMAX, bool TR, typename R, typename S>
 __device__ inline void ld_link(R *Sr, R *Si, const S *__restrict__ f, long l, int n)
 {
     constexpr int NT = Cfg<P, NMAX>::NT;
@@ -220,8 +259,7 @@ __device__ inline void ld_link(R *Sr, R *Si, const S *__restrict__ f, long l, in
     for (int e = threadIdx.x; e < NMAX * NMAX; e += NT) {
         const int r = e / NMAX, c = e % NMAX;
         const bool in = r < n && c < n;
-        Sr[r * Cfg<P, NMAX>::LD + c] = in ? (R)b[2 * (r * n + c)] : R(0);
-        Si[r * Cfg<P, NMAX>::LD + c] = in ? (R)b[2 * (r * n + c) + 1] : R(0);
+        put<P, NMAX, TR, R>(Sr, Si, r, c, in ? (R)b[2 * (r * n + c)] : R(0), in ? (R)b[2 * (r * n + c) + 1] : R(0));
     }
 }
 
@@ -245,14 +283,13 @@ struct Stage {
             v[j] = (r < n && c < n) ? b[r * n + c] : S2{S(0), S(0)};
         }
     }
-    template <typename R>
+    template <bool TR, typename R>
     __device__ inline void stash(R *Sr, R *Si) const
     {
 #pragma unroll
         for (int j = 0; j < E; j++) {
             const int e = threadIdx.x + j * NT, r = e / NMAX, c = e % NMAX;
-            Sr[r * Cfg<P, NMAX>::LD + c] = (R)v[j][0];
-            Si[r * Cfg<P, NMAX>::LD + c] = (R)v[j][1];
+            put<P, NMAX, TR, R>(Sr, Si, r, c, (R)v[j][0], (R)v[j][1]);
         }
     }
     template <bool PF>
@@ -260,13 +297,13 @@ struct Stage {
     {
         if constexpr (PF) fetch(f, l, n);
     }
-    template <bool PF, typename R>
+    template <bool PF, bool TR, typename R>
     __device__ inline void late(const S *__restrict__ f, long l, int n, R *Sr, R *Si)
     {
         if constexpr (PF)
-            stash(Sr, Si);
+            stash<TR>(Sr, Si);
         else
-            ld_link<P, NMAX>(Sr, Si, f, l, n);
+            ld_link<P, NMAX, TR>(Sr, Si, f, l, n);
     }
 };
 
@@ -292,7 +329,11 @@ __global__ __launch_bounds__((Cfg<P, NMAX>::NT)) void k_kick(KickArgs a)
 {
     typedef typename Pol<P>::R R;
     constexpr int NCT = Cfg<P, NMAX>::NCT;
-    __shared__ R Are[Cfg<P, NMAX>::SLOT], Aim[Cfg<P, NMAX>::SLOT], Bre[Cfg<P, NMAX>::SLOT], Bim[Cfg<P, NMAX>::SLOT];
+    // 16-byte aligned: the BF16x3 products read the slots 16 bytes at a time
+    __shared__ __align__(16) R Are[Cfg<P, NMAX>::SLOT];
+    __shared__ __align__(16) R Aim[Cfg<P, NMAX>::SLOT];
+    __shared__ __align__(16) R Bre[Cfg<P, NMAX>::SLOT];
+    __shared__ __align__(16) R Bim[Cfg<P, NMAX>::SLOT];
     const Geom &g = a.g;
     const int n = g.N;
     const long V = g.V;
@@ -321,30 +362,30 @@ __global__ __launch_bounds__((Cfg<P, NMAX>::NT)) void k_kick(KickArgs a)
         const long xpmn = g.nbr[(2 * nu + 1) * V + xp];
         // S += U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+
         __syncthreads();
-        s0.template late<PF>(U, nu * V + xp, n, Are, Aim);
-        s1.template late<PF>(U, mu * V + xpn, n, Bre, Bim);
+        s0.template late<PF, false>(U, nu * V + xp, n, Are, Aim);
+        s1.template late<PF, false>(U, mu * V + xpn, n, Bre, Bim);
         __syncthreads();
         s0.template early<PF>(U, nu * V + s, n);
         acc_zero(T);
         cmul<P, NMAX, false, true>(Are, Aim, Bre, Bim, T);
         __syncthreads();
-        acc_store(T, Are, Aim);
-        s0.template late<PF>(U, nu * V + s, n, Bre, Bim);
+        acc_store<false>(T, Are, Aim);
+        s0.template late<PF, false>(U, nu * V + s, n, Bre, Bim);
         __syncthreads();
         s0.template early<PF>(U, nu * V + xpmn, n);
         s1.template early<PF>(U, mu * V + xmn, n);
         cmul<P, NMAX, false, true>(Are, Aim, Bre, Bim, S);
         // S += U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu)
         __syncthreads();
-        s0.template late<PF>(U, nu * V + xpmn, n, Are, Aim);
-        s1.template late<PF>(U, mu * V + xmn, n, Bre, Bim);
+        s0.template late<PF, true>(U, nu * V + xpmn, n, Are, Aim);
+        s1.template late<PF, false>(U, mu * V + xmn, n, Bre, Bim);
         __syncthreads();
         s0.template early<PF>(U, nu * V + xmn, n);
         acc_zero(T);
         cmul<P, NMAX, true, true>(Are, Aim, Bre, Bim, T);
         __syncthreads();
-        acc_store(T, Are, Aim);
-        s0.template late<PF>(U, nu * V + xmn, n, Bre, Bim);
+        acc_store<false>(T, Are, Aim);
+        s0.template late<PF, true>(U, nu * V + xmn, n, Bre, Bim);
         __syncthreads();
         if (nx < g.D) {
             s0.template early<PF>(U, nx * V + xp, n);
@@ -357,13 +398,13 @@ __global__ __launch_bounds__((Cfg<P, NMAX>::NT)) void k_kick(KickArgs a)
     }
     // W = U_mu(x) S
     __syncthreads();
-    acc_store(S, Are, Aim);
-    s0.template late<PF>(U, mu * V + s, n, Bre, Bim);
+    acc_store<true>(S, Are, Aim);
+    s0.template late<PF, false>(U, mu * V + s, n, Bre, Bim);
     __syncthreads();
     acc_zero(T);
     cmul<P, NMAX, false, false>(Bre, Bim, Are, Aim, T);
     __syncthreads();
-    acc_store(T, Are, Aim);
+    acc_store_full(T, Are, Aim);
     __syncthreads();
     if constexpr (ActionOnly) {
         double sum[3] = {0.0, 0.0, 0.0}, cmp[3] = {0.0, 0.0, 0.0};
@@ -449,8 +490,7 @@ __global__ __launch_bounds__((Cfg<P, NMAX>::NT)) void k_kick(KickArgs a)
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const int i = acc_row<P, NMAX>(q), j = acc_col<P, NMAX>(t);
-                Are[i * Cfg<P, NMAX>::LD + j] = -(pim[t][q] * ed);
-                Aim[i * Cfg<P, NMAX>::LD + j] = pr[t][q] * ed;
+                put<P, NMAX, false, R>(Are, Aim, i, j, -(pim[t][q] * ed), pr[t][q] * ed);  // rhs as A
             }
         s0.template early<PF>(U, l, n);  // U_mu(x) for the final product
         Acc<P, NMAX> X;
@@ -466,7 +506,7 @@ __global__ __launch_bounds__((Cfg<P, NMAX>::NT)) void k_kick(KickArgs a)
             }
         for (int k = level - 1; k >= 1; k--) {
             __syncthreads();
-            acc_store(X, Bre, Bim);
+            acc_store<true>(X, Bre, Bim);
             __syncthreads();
             Acc<P, NMAX> Q;
             acc_zero(Q);
@@ -482,8 +522,8 @@ __global__ __launch_bounds__((Cfg<P, NMAX>::NT)) void k_kick(KickArgs a)
                 }
         }
         __syncthreads();
-        acc_store(X, Bre, Bim);
-        s0.template late<PF>(U, l, n, Are, Aim);
+        acc_store<false>(X, Bre, Bim);
+        s0.template late<PF, true>(U, l, n, Are, Aim);
         __syncthreads();
         Acc<P, NMAX> Un;
         acc_zero(Un);
