@@ -53,14 +53,15 @@ struct Cfg {
     static constexpr int WPR = (P == kF64 && NMAX == 64) ? 2 : 1;  // wavefronts per 16-row block
     static constexpr int NT = 64 * (NMAX / 16) * WPR;
     static constexpr int NCT = NMAX / 16 / WPR;  // column tiles per wavefront
-    // operand prefetch through registers, except at NMAX = 64 in FP32, where
-    // the extra registers would halve the occupancy
-    static constexpr bool PF = !(P == kF32 && NMAX == 64);
-    // LDS row stride in elements.  FP64 / FP32: NMAX + 4 puts the 16 rows a
-    // wavefront reads for one operand on distinct banks (an unpadded stride
-    // maps them all to one bank group).  BF16x3: NMAX + 8 bf16 (16 bytes)
-    // keeps the 16-byte reads of eight consecutive rows on distinct banks.
-    static constexpr int LD = NMAX + (P == kBF16X3 ? 8 : 4);
+    // operand prefetch through registers, except at NMAX = 64 in FP32 and
+    // BF16x3, where the extra registers would halve the occupancy
+    static constexpr bool PF = !(P != kF64 && NMAX == 64);
+    // LDS row stride in elements.  FP64 / FP32: NMAX + 2 spreads the 16 rows
+    // a wavefront reads for one operand over the banks (an unpadded stride
+    // maps them all to one bank group; + 2 measured a little faster than
+    // + 4).  BF16x3: NMAX + 8 bf16 (16 bytes) keeps the 16-byte reads of
+    // eight consecutive rows on distinct banks.
+    static constexpr int LD = NMAX + (P == kBF16X3 ? 8 : 2);
     static constexpr int SLOT = NMAX * LD;
 };
 
@@ -164,47 +165,65 @@ __device__ inline void cmul_half(const R *A, const R *Br, const R *Bi, Acc<P, NM
     }
 }
 
+// BF16x3 steps ks in [KS0, KS1) of the stacked product.  Lane l holds
+// A[row][k0..k0+7] and B[k0..k0+7][col], k0 = 32 ks + 8 (l/16) in the
+// stacked index; the slots hold A and B in the orientation that makes k
+// contiguous.  Every sign (conjugation, the -Bi block of [Br ; -Bi]) is
+// folded into the A operand, once per step: the real output takes
+// (+Ar | -+Ai) against [Br ; Bi] rows, the imaginary output (+-Ar | +-Ai)
+// against [Bi ; Br].  H = 0 / 1: all lanes in the real / imaginary half
+// (NMAX >= 32); H = -1: the half is decided per lane and step.
+template <int NMAX, bool DA, bool DB, int H>
+__device__ inline void bf_steps(const __bf16 *arh, const __bf16 *aih, const __bf16 *brh, const __bf16 *bih,
+                                Acc<kBF16X3, NMAX> &acc)
+{
+    constexpr int P = kBF16X3, NCT = Cfg<P, NMAX>::NCT, LD = Cfg<P, NMAX>::LD, PL = NMAX * LD;
+    constexpr int KS0 = H == 1 ? NMAX / 32 : 0, KS1 = H == 0 ? NMAX / 32 : (2 * NMAX) / 32;
+    constexpr unsigned SG = 0x80008000u;
+    const int lane = threadIdx.x & 63;
+    const int row = 16 * wave_rowblock<P, NMAX>() + (lane & 15);
+    const int t0 = wave_tile0<P, NMAX>();
+    const int kq = 8 * (lane >> 4);
+#pragma unroll 1  // unrolled, the 16-byte operand loads of every step stay live
+    for (int ks = KS0; ks < KS1; ks++) {
+        const int k0 = 32 * ks + kq;
+        const bool imag = H < 0 ? k0 >= NMAX : H == 1;
+        const int kb = imag ? k0 - NMAX : k0;  // column of A / row of B
+        // real output: Ar Br - Ai_eff Bi_eff; imaginary: Ar Bi_eff + Ai_eff Br,
+        // with Ai_eff = -Ai for A^+ and Bi_eff = -Bi for B^+
+        const unsigned mr = imag ? ((DA == DB) ? SG : 0u) : 0u;
+        const unsigned mi = imag ? (DA ? SG : 0u) : (DB ? SG : 0u);
+        const __bf16 *ap = (imag ? aih : arh) + row * LD + kb;
+        const bf16x8 ah = ld8(ap), al = ld8(ap + PL);
+        const bf16x8 rh = xsign(ah, mr), rl = xsign(al, mr), ih = xsign(ah, mi), il = xsign(al, mi);
+        const __bf16 *bp = imag ? bih : brh, *cp = imag ? brh : bih;
+#pragma unroll
+        for (int t = 0; t < NCT; t++) {
+            const int o = (16 * (t0 + t) + (lane & 15)) * LD + kb;
+            const bf16x8 bh = ld8(bp + o), bl = ld8(bp + PL + o);
+            const bf16x8 ch = ld8(cp + o), cl = ld8(cp + PL + o);
+            acc.r[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(rh, bh, acc.r[t], 0, 0, 0);
+            acc.r[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(rh, bl, acc.r[t], 0, 0, 0);
+            acc.r[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(rl, bh, acc.r[t], 0, 0, 0);
+            acc.i[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ih, ch, acc.i[t], 0, 0, 0);
+            acc.i[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ih, cl, acc.i[t], 0, 0, 0);
+            acc.i[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(il, ch, acc.i[t], 0, 0, 0);
+        }
+    }
+}
+
 // acc (+)= op(A) op(B) on the wavefront's rows and column tiles; A, B in LDS.
 template <int P, int NMAX, bool DA, bool DB, typename R>
 __device__ inline void cmul(const R *Ar, const R *Ai, const R *Br, const R *Bi, Acc<P, NMAX> &acc)
 {
     if constexpr (P == kBF16X3) {
-        // lane l: A[row][k0..k0+7] and B[k0..k0+7][col], k0 = 32 ks + 8 (l/16)
-        // in the stacked index; the slots hold A and B in the orientation
-        // that makes k contiguous (conjugation = sign flip of the im planes)
-        constexpr int NCT = Cfg<P, NMAX>::NCT, LD = Cfg<P, NMAX>::LD, PL = NMAX * LD;
         const __bf16 *arh = (const __bf16 *)Ar, *aih = (const __bf16 *)Ai;
         const __bf16 *brh = (const __bf16 *)Br, *bih = (const __bf16 *)Bi;
-        const int lane = threadIdx.x & 63;
-        const int row = 16 * wave_rowblock<P, NMAX>() + (lane & 15);
-        const int t0 = wave_tile0<P, NMAX>();
-        const int kq = 8 * (lane >> 4);
-        constexpr unsigned SG = 0x80008000u;
-#pragma unroll 1  // unrolled, the 16-byte operand loads of every step stay live
-        for (int ks = 0; ks < (2 * NMAX) / 32; ks++) {
-            const int k0 = 32 * ks + kq;           // first of this lane's 8 stacked k
-            const bool imag = k0 >= NMAX;          // all 8 in the same half
-            const int kb = imag ? k0 - NMAX : k0;  // column of A / row of B
-            // [Ar | Ai] (Ai -> -Ai for A^+)
-            const __bf16 *ap = (imag ? aih : arh) + row * LD + kb;
-            const unsigned ma = (DA && imag) ? SG : 0u;
-            const bf16x8 ah = xsign(ld8(ap), ma), al = xsign(ld8(ap + PL), ma);
-            // real output: [Br ; -Bi], imaginary output: [Bi ; Br]
-            // (Bi -> -Bi for B^+)
-            const __bf16 *bp = imag ? bih : brh, *cp = imag ? brh : bih;
-            const unsigned mb = (imag && !DB) ? SG : 0u, mc = (!imag && DB) ? SG : 0u;
-#pragma unroll
-            for (int t = 0; t < NCT; t++) {
-                const int o = (16 * (t0 + t) + (lane & 15)) * LD + kb;
-                const bf16x8 bh = xsign(ld8(bp + o), mb), bl = xsign(ld8(bp + PL + o), mb);
-                const bf16x8 ch = xsign(ld8(cp + o), mc), cl = xsign(ld8(cp + PL + o), mc);
-                acc.r[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc.r[t], 0, 0, 0);
-                acc.r[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc.r[t], 0, 0, 0);
-                acc.r[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc.r[t], 0, 0, 0);
-                acc.i[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, ch, acc.i[t], 0, 0, 0);
-                acc.i[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, cl, acc.i[t], 0, 0, 0);
-                acc.i[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, ch, acc.i[t], 0, 0, 0);
-            }
+        if constexpr (NMAX >= 64) {  // NMAX = 32 measured faster with the per-lane form
+            bf_steps<NMAX, DA, DB, 0>(arh, aih, brh, bih, acc);
+            bf_steps<NMAX, DA, DB, 1>(arh, aih, brh, bih, acc);
+        } else {
+            bf_steps<NMAX, DA, DB, -1>(arh, aih, brh, bih, acc);
         }
     } else {
         cmul_half<P, NMAX, DA, DB, false>(Ar, Br, Bi, acc);
@@ -225,15 +244,47 @@ __device__ inline int acc_col(int t)
     return 16 * (wave_tile0<P, NMAX>() + t) + (threadIdx.x & 15);
 }
 
-// accumulators -> operand slot (orientation TR, see put)
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+
+// four consecutive elements x -> hi / lo bf16 (see put)
+__device__ inline void split4(const f32x4 &x, bf16x4 &hi, bf16x4 &lo)
+{
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const __bf16 h = (__bf16)x[q];
+        hi[q] = h;
+        lo[q] = (__bf16)(x[q] - (float)h);
+    }
+}
+
+// accumulators -> operand slot (orientation TR, see put).  A BF16x3 slot
+// stored transposed takes the lane's four rows q = 0..3 (consecutive in the
+// f32 accumulator layout) with one 8-byte write per plane.
 template <bool TR, int P, int NMAX, typename R>
 __device__ inline void acc_store(const Acc<P, NMAX> &a, R *Sr, R *Si)
 {
+    if constexpr (P == kBF16X3 && TR) {
+        constexpr int LD = Cfg<P, NMAX>::LD, PL = NMAX * LD;
+        __bf16 *rp = (__bf16 *)Sr, *ip = (__bf16 *)Si;
 #pragma unroll
-    for (int t = 0; t < Cfg<P, NMAX>::NCT; t++)
+        for (int t = 0; t < Cfg<P, NMAX>::NCT; t++) {
+            const int o = acc_col<P, NMAX>(t) * LD + acc_row<P, NMAX>(0);
+            bf16x4 h, l;
+            split4(a.r[t], h, l);
+            *(bf16x4 *)(rp + o) = h;
+            *(bf16x4 *)(rp + PL + o) = l;
+            split4(a.i[t], h, l);
+            *(bf16x4 *)(ip + o) = h;
+            *(bf16x4 *)(ip + PL + o) = l;
+        }
+    } else {
 #pragma unroll
-        for (int q = 0; q < 4; q++)
-            put<P, NMAX, TR, R>(Sr, Si, acc_row<P, NMAX>(q), acc_col<P, NMAX>(t), a.r[t][q], a.i[t][q]);
+        for (int t = 0; t < Cfg<P, NMAX>::NCT; t++)
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                put<P, NMAX, TR, R>(Sr, Si, acc_row<P, NMAX>(q), acc_col<P, NMAX>(t), a.r[t][q], a.i[t][q]);
+    }
 }
 
 // accumulators -> planar row-major matrix at full precision (read element
@@ -250,16 +301,80 @@ __device__ inline void acc_store_full(const Acc<P, NMAX> &a, R *Sr, R *Si)
         }
 }
 
+// Operand items: each thread moves E elements of a slot, in items of W
+// consecutive elements along the slot's row (W = 2 for BF16x3, whose split
+// planes then take one 4-byte write per item and plane; W = 1 otherwise).
+// Item j of this thread covers (r, c) .. (r, c + W - 1) of the stored
+// matrix, i.e. rows r .. r + W - 1 of column c of the link when TR.
+template <int P, int NMAX>
+struct Items {
+    static constexpr int NT = Cfg<P, NMAX>::NT, E = NMAX * NMAX / NT;
+    static constexpr int W = P == kBF16X3 ? 2 : 1, EI = E / W;
+    static_assert(E * NT == NMAX * NMAX && EI * W == E, "slot not a multiple of the workgroup");
+    // link element (r, c) of item p, element w
+    template <bool TR>
+    static __device__ inline void rc(int p, int w, int &r, int &c)
+    {
+        if constexpr (W == 1) {
+            r = p / NMAX;
+            c = p % NMAX;
+        } else if constexpr (!TR) {
+            r = p / (NMAX / W);
+            c = W * (p % (NMAX / W)) + w;
+        } else {
+            c = p % NMAX;
+            r = W * (p / NMAX) + w;
+        }
+    }
+    // write the item's W complex values (link elements of rc) to the slot
+    template <bool TR, typename R>
+    static __device__ inline void write(R *Sr, R *Si, int p, const R (&re)[W], const R (&im)[W])
+    {
+        if constexpr (W == 1) {
+            int r, c;
+            rc<TR>(p, 0, r, c);
+            put<P, NMAX, TR, R>(Sr, Si, r, c, re[0], im[0]);
+        } else {
+#pragma clang fp contract(off)
+            constexpr int LD = Cfg<P, NMAX>::LD, PL = NMAX * LD;
+            typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+            int r, c;
+            rc<TR>(p, 0, r, c);
+            const int o = TR ? c * LD + r : r * LD + c;  // the W elements sit at o, o + 1
+            bf16x2 rh, rl, ih, il;
+#pragma unroll
+            for (int w = 0; w < W; w++) {
+                rh[w] = (__bf16)re[w];
+                rl[w] = (__bf16)(re[w] - (float)rh[w]);
+                ih[w] = (__bf16)im[w];
+                il[w] = (__bf16)(im[w] - (float)ih[w]);
+            }
+            __bf16 *rp = (__bf16 *)Sr, *ip = (__bf16 *)Si;
+            *(bf16x2 *)(rp + o) = rh;
+            *(bf16x2 *)(rp + PL + o) = rl;
+            *(bf16x2 *)(ip + o) = ih;
+            *(bf16x2 *)(ip + PL + o) = il;
+        }
+    }
+};
+
 // global link (matrix contiguous, n x n complex) -> zero-padded slot
 template <int P, int NMAX, bool TR, typename R, typename S>
 __device__ inline void ld_link(R *Sr, R *Si, const S *__restrict__ f, long l, int n)
 {
-    constexpr int NT = Cfg<P, NMAX>::NT;
+    typedef Items<P, NMAX> I;
     const S *b = f + (size_t)l * 2 * n * n;
-    for (int e = threadIdx.x; e < NMAX * NMAX; e += NT) {
-        const int r = e / NMAX, c = e % NMAX;
-        const bool in = r < n && c < n;
-        put<P, NMAX, TR, R>(Sr, Si, r, c, in ? (R)b[2 * (r * n + c)] : R(0), in ? (R)b[2 * (r * n + c) + 1] : R(0));
+    for (int p = threadIdx.x; p < I::EI * I::NT; p += I::NT) {
+        R re[I::W], im[I::W];
+#pragma unroll
+        for (int w = 0; w < I::W; w++) {
+            int r, c;
+            I::template rc<TR>(p, w, r, c);
+            const bool in = r < n && c < n;
+            re[w] = in ? (R)b[2 * (r * n + c)] : R(0);
+            im[w] = in ? (R)b[2 * (r * n + c) + 1] : R(0);
+        }
+        I::template write<TR, R>(Sr, Si, p, re, im);
     }
 }
 
@@ -267,35 +382,44 @@ __device__ inline void ld_link(R *Sr, R *Si, const S *__restrict__ f, long l, in
 // stash() writes it to an LDS slot.  With PF the fetch of the next operands
 // is issued before the current product and the stash follows it, so the
 // loads run under the MFMA work; without PF the operand is loaded straight
-// into its slot at the stash point (fewer live registers).
+// into its slot at the stash point (fewer live registers).  TR is the
+// orientation of the slot the operand goes to (see put).
 template <int P, int NMAX, typename S>
 struct Stage {
-    static constexpr int NT = Cfg<P, NMAX>::NT, E = NMAX * NMAX / NT;
-    static_assert(E * NT == NMAX * NMAX, "slot not a multiple of the workgroup");
+    typedef Items<P, NMAX> I;
     typedef S S2 __attribute__((ext_vector_type(2)));
-    S2 v[E];
+    S2 v[I::E];
+    template <bool TR>
     __device__ inline void fetch(const S *__restrict__ f, long l, int n)
     {
         const S2 *b = (const S2 *)(f + (size_t)l * 2 * n * n);
 #pragma unroll
-        for (int j = 0; j < E; j++) {
-            const int e = threadIdx.x + j * NT, r = e / NMAX, c = e % NMAX;
-            v[j] = (r < n && c < n) ? b[r * n + c] : S2{S(0), S(0)};
-        }
+        for (int j = 0; j < I::EI; j++)
+#pragma unroll
+            for (int w = 0; w < I::W; w++) {
+                int r, c;
+                I::template rc<TR>(threadIdx.x + j * I::NT, w, r, c);
+                v[j * I::W + w] = (r < n && c < n) ? b[r * n + c] : S2{S(0), S(0)};
+            }
     }
     template <bool TR, typename R>
     __device__ inline void stash(R *Sr, R *Si) const
     {
 #pragma unroll
-        for (int j = 0; j < E; j++) {
-            const int e = threadIdx.x + j * NT, r = e / NMAX, c = e % NMAX;
-            put<P, NMAX, TR, R>(Sr, Si, r, c, (R)v[j][0], (R)v[j][1]);
+        for (int j = 0; j < I::EI; j++) {
+            R re[I::W], im[I::W];
+#pragma unroll
+            for (int w = 0; w < I::W; w++) {
+                re[w] = (R)v[j * I::W + w][0];
+                im[w] = (R)v[j * I::W + w][1];
+            }
+            I::template write<TR, R>(Sr, Si, threadIdx.x + j * I::NT, re, im);
         }
     }
-    template <bool PF>
+    template <bool PF, bool TR>
     __device__ inline void early(const S *__restrict__ f, long l, int n)
     {
-        if constexpr (PF) fetch(f, l, n);
+        if constexpr (PF) fetch<TR>(f, l, n);
     }
     template <bool PF, bool TR, typename R>
     __device__ inline void late(const S *__restrict__ f, long l, int n, R *Sr, R *Si)
@@ -350,8 +474,8 @@ __global__ __launch_bounds__((Cfg<P, NMAX>::NT)) void k_kick(KickArgs a)
     constexpr bool PF = Cfg<P, NMAX>::PF;
     Stage<P, NMAX, LS> s0, s1;
     int nu = mu == 0 ? 1 : 0;
-    s0.template early<PF>(U, nu * V + xp, n);
-    s1.template early<PF>(U, mu * V + g.nbr[(2 * nu) * V + s], n);
+    s0.template early<PF, false>(U, nu * V + xp, n);
+    s1.template early<PF, false>(U, mu * V + g.nbr[(2 * nu) * V + s], n);
     Acc<P, NMAX> S, T;
     acc_zero(S);
     while (nu < g.D) {
@@ -365,22 +489,22 @@ __global__ __launch_bounds__((Cfg<P, NMAX>::NT)) void k_kick(KickArgs a)
         s0.template late<PF, false>(U, nu * V + xp, n, Are, Aim);
         s1.template late<PF, false>(U, mu * V + xpn, n, Bre, Bim);
         __syncthreads();
-        s0.template early<PF>(U, nu * V + s, n);
+        s0.template early<PF, false>(U, nu * V + s, n);
         acc_zero(T);
         cmul<P, NMAX, false, true>(Are, Aim, Bre, Bim, T);
         __syncthreads();
         acc_store<false>(T, Are, Aim);
         s0.template late<PF, false>(U, nu * V + s, n, Bre, Bim);
         __syncthreads();
-        s0.template early<PF>(U, nu * V + xpmn, n);
-        s1.template early<PF>(U, mu * V + xmn, n);
+        s0.template early<PF, true>(U, nu * V + xpmn, n);
+        s1.template early<PF, false>(U, mu * V + xmn, n);
         cmul<P, NMAX, false, true>(Are, Aim, Bre, Bim, S);
         // S += U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu)
         __syncthreads();
         s0.template late<PF, true>(U, nu * V + xpmn, n, Are, Aim);
         s1.template late<PF, false>(U, mu * V + xmn, n, Bre, Bim);
         __syncthreads();
-        s0.template early<PF>(U, nu * V + xmn, n);
+        s0.template early<PF, true>(U, nu * V + xmn, n);
         acc_zero(T);
         cmul<P, NMAX, true, true>(Are, Aim, Bre, Bim, T);
         __syncthreads();
@@ -388,10 +512,10 @@ __global__ __launch_bounds__((Cfg<P, NMAX>::NT)) void k_kick(KickArgs a)
         s0.template late<PF, true>(U, nu * V + xmn, n, Bre, Bim);
         __syncthreads();
         if (nx < g.D) {
-            s0.template early<PF>(U, nx * V + xp, n);
-            s1.template early<PF>(U, mu * V + g.nbr[(2 * nx) * V + s], n);
+            s0.template early<PF, false>(U, nx * V + xp, n);
+            s1.template early<PF, false>(U, mu * V + g.nbr[(2 * nx) * V + s], n);
         } else {
-            s0.template early<PF>(U, mu * V + s, n);
+            s0.template early<PF, false>(U, mu * V + s, n);
         }
         cmul<P, NMAX, false, false>(Are, Aim, Bre, Bim, S);
         nu = nx;
@@ -492,7 +616,7 @@ __global__ __launch_bounds__((Cfg<P, NMAX>::NT)) void k_kick(KickArgs a)
                 const int i = acc_row<P, NMAX>(q), j = acc_col<P, NMAX>(t);
                 put<P, NMAX, false, R>(Are, Aim, i, j, -(pim[t][q] * ed), pr[t][q] * ed);  // rhs as A
             }
-        s0.template early<PF>(U, l, n);  // U_mu(x) for the final product
+        s0.template early<PF, true>(U, l, n);  // U_mu(x) for the final product
         Acc<P, NMAX> X;
         const R cl = level <= 1 ? R(1) : (R)a.cinv[level];
 #pragma unroll
