@@ -408,19 +408,29 @@ __global__ __launch_bounds__(NT) void k_drift(Geom g, R *U, const R *P, double e
 }
 
 // SU(N) projection of one link per workgroup (N >= 4), FP64 in LDS:
-// classical Gram-Schmidt with one re-orthogonalisation pass on the rows
-// (all projections of a row computed in parallel), then the determinant by LU
-// with partial pivoting on a copy and multiplication by exp(-i arg(det)/N).
+// classical Gram-Schmidt with one re-orthogonalisation pass on the rows,
+// then the determinant of the orthonormal rows by LU with partial pivoting
+// and multiplication by exp(-i arg(det)/N).  Each projection coefficient
+// and each updated element is summed by a group of G = NT / NMAX adjacent
+// lanes and reduced with shuffles; the pivot search is one wavefront's
+// argmax.  The orthonormal rows are written to the link, the LU then
+// overwrites them in LDS, and the phase is applied to the link in place
+// (one LDS matrix, odd row stride).
 // haar: the link is first filled with a Ginibre matrix from Philox (purpose
 // 2, counter = stream, link index = dir * V + lexicographic site; normal 2e
 // and 2e+1 are the real and imaginary parts of element e, scaled 1/sqrt 2).
 template <int NMAX, int NT, typename R>
 __global__ __launch_bounds__(NT) void k_project(Geom g, R *U, int haar, uint64_t seed, uint64_t stream)
 {
-    __shared__ double ar[NMAX * NMAX], ai[NMAX * NMAX], mr[NMAX * NMAX], mi[NMAX * NMAX];
-    __shared__ double dr[NMAX], di[NMAX], red[NT];
+    constexpr int LDA = NMAX + 1;  // odd row stride: rows on different banks
+    constexpr int G = NT / NMAX;   // lanes per coefficient
+    constexpr int NW = NT / 64;
+    static_assert(G * NMAX == NT && (G & (G - 1)) == 0 && G <= 64 && NW * 64 == NT, "workgroup shape");
+    __shared__ double ar[NMAX * LDA], ai[NMAX * LDA];
+    __shared__ double dr[NMAX], di[NMAX], red[NW];
     __shared__ int piv;
     const int n = g.N, tid = threadIdx.x;
+    const int grp = tid / G, gl = tid % G;  // row q / column k of this group, lane in the group
     const long l = blockIdx.x;
     R *b = U + (size_t)l * 2 * n * n;
     if (haar) {
@@ -430,96 +440,121 @@ __global__ __launch_bounds__(NT) void k_project(Geom g, R *U, int haar, uint64_t
             double z[4];
             normals4(philox_block(seed, stream, kPurposeHaar, link, (uint32_t)blk), z);
             for (int q = 0; q < 4 && 4 * blk + q < 2 * n * n; q++) {
-                const int a = 4 * blk + q;
-                if (a & 1) ai[a >> 1] = z[q] * 0.7071067811865476;
-                else ar[a >> 1] = z[q] * 0.7071067811865476;
+                const int a = 4 * blk + q, e = a >> 1;
+                double *dst = (a & 1) ? ai : ar;
+                dst[(e / n) * LDA + e % n] = z[q] * 0.7071067811865476;
             }
         }
     } else {
         for (int e = tid; e < n * n; e += NT) {
-            ar[e] = (double)b[2 * e];
-            ai[e] = (double)b[2 * e + 1];
+            ar[(e / n) * LDA + e % n] = (double)b[2 * e];
+            ai[(e / n) * LDA + e % n] = (double)b[2 * e + 1];
         }
     }
     __syncthreads();
     for (int r = 0; r < n; r++) {
         for (int pass = 0; pass < 2; pass++) {
             // d_q = <row q, row r> = sum_k conj(a_qk) a_rk for q < r
-            for (int q = tid; q < r; q += NT) {
+            {
+                const int q = grp;
                 double sr = 0.0, si = 0.0;
-                for (int k = 0; k < n; k++) {
-                    const double qr = ar[q * n + k], qi = ai[q * n + k], xr = ar[r * n + k], xi = ai[r * n + k];
-                    sr += qr * xr + qi * xi;
-                    si += qr * xi - qi * xr;
+                if (q < r)
+                    for (int k = gl; k < n; k += G) {
+                        const double qr = ar[q * LDA + k], qi = ai[q * LDA + k];
+                        const double xr = ar[r * LDA + k], xi = ai[r * LDA + k];
+                        sr += qr * xr + qi * xi;
+                        si += qr * xi - qi * xr;
+                    }
+#pragma unroll
+                for (int off = G / 2; off > 0; off >>= 1) {
+                    sr += __shfl_xor(sr, off, 64);
+                    si += __shfl_xor(si, off, 64);
                 }
-                dr[q] = sr;
-                di[q] = si;
+                if (q < r && gl == 0) {
+                    dr[q] = sr;
+                    di[q] = si;
+                }
             }
             __syncthreads();
-            for (int k = tid; k < n; k += NT) {
-                double xr = ar[r * n + k], xi = ai[r * n + k];
-                for (int q = 0; q < r; q++) {
-                    const double qr = ar[q * n + k], qi = ai[q * n + k];
-                    xr -= dr[q] * qr - di[q] * qi;
-                    xi -= dr[q] * qi + di[q] * qr;
+            // a_rk -= sum_q d_q a_qk
+            {
+                const int k = grp;
+                double xr = 0.0, xi = 0.0;
+                if (k < n)
+                    for (int q = gl; q < r; q += G) {
+                        const double qr = ar[q * LDA + k], qi = ai[q * LDA + k];
+                        xr += dr[q] * qr - di[q] * qi;
+                        xi += dr[q] * qi + di[q] * qr;
+                    }
+#pragma unroll
+                for (int off = G / 2; off > 0; off >>= 1) {
+                    xr += __shfl_xor(xr, off, 64);
+                    xi += __shfl_xor(xi, off, 64);
                 }
-                ar[r * n + k] = xr;
-                ai[r * n + k] = xi;
+                if (k < n && gl == 0) {
+                    ar[r * LDA + k] -= xr;
+                    ai[r * LDA + k] -= xi;
+                }
             }
             __syncthreads();
         }
+        // normalise row r
         double part = 0.0;
-        for (int k = tid; k < n; k += NT) part += ar[r * n + k] * ar[r * n + k] + ai[r * n + k] * ai[r * n + k];
-        red[tid] = part;
+        for (int k = tid; k < n; k += NT) part += ar[r * LDA + k] * ar[r * LDA + k] + ai[r * LDA + k] * ai[r * LDA + k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+        if ((tid & 63) == 0) red[tid >> 6] = part;
         __syncthreads();
-        for (int off = NT / 2; off > 0; off >>= 1) {
-            if (tid < off) red[tid] += red[tid + off];
-            __syncthreads();
-        }
-        const double inv = 1.0 / sqrt(red[0]);
-        __syncthreads();
+        double nrm = 0.0;
+#pragma unroll
+        for (int w = 0; w < NW; w++) nrm += red[w];
+        const double inv = 1.0 / sqrt(nrm);
         for (int k = tid; k < n; k += NT) {
-            ar[r * n + k] *= inv;
-            ai[r * n + k] *= inv;
+            ar[r * LDA + k] *= inv;
+            ai[r * LDA + k] *= inv;
         }
         __syncthreads();
     }
-    // determinant of the orthonormalised matrix by LU on a copy
+    // orthonormal rows -> link; determinant by LU in place
     for (int e = tid; e < n * n; e += NT) {
-        mr[e] = ar[e];
-        mi[e] = ai[e];
+        b[2 * e] = (R)ar[(e / n) * LDA + e % n];
+        b[2 * e + 1] = (R)ai[(e / n) * LDA + e % n];
     }
-    __syncthreads();
     double detr = 1.0, deti = 0.0;  // kept identical in every thread
     for (int c = 0; c < n; c++) {
-        if (tid == 0) {
-            int p = c;
-            double best = -1.0;
-            for (int r = c; r < n; r++) {
-                const double v = mr[r * n + c] * mr[r * n + c] + mi[r * n + c] * mi[r * n + c];
-                if (v > best) {
-                    best = v;
-                    p = r;
+        if (tid < 64) {
+            // first row of largest |m_rc|^2, r >= c (n <= 64 rows, one per lane)
+            const int rr = tid;
+            double v = -1.0;
+            if (rr >= c && rr < n) v = ar[rr * LDA + c] * ar[rr * LDA + c] + ai[rr * LDA + c] * ai[rr * LDA + c];
+            int idx = rr;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const double v2 = __shfl_xor(v, off, 64);
+                const int i2 = __shfl_xor(idx, off, 64);
+                if (v2 > v || (v2 == v && i2 < idx)) {
+                    v = v2;
+                    idx = i2;
                 }
             }
-            piv = p;
+            if (tid == 0) piv = idx;
         }
         __syncthreads();
         const int p = piv;
         if (p != c) {
             for (int k = tid; k < n; k += NT) {
-                double t = mr[c * n + k];
-                mr[c * n + k] = mr[p * n + k];
-                mr[p * n + k] = t;
-                t = mi[c * n + k];
-                mi[c * n + k] = mi[p * n + k];
-                mi[p * n + k] = t;
+                double t = ar[c * LDA + k];
+                ar[c * LDA + k] = ar[p * LDA + k];
+                ar[p * LDA + k] = t;
+                t = ai[c * LDA + k];
+                ai[c * LDA + k] = ai[p * LDA + k];
+                ai[p * LDA + k] = t;
             }
             detr = -detr;
             deti = -deti;
         }
         __syncthreads();
-        const double pr = mr[c * n + c], pim = mi[c * n + c];
+        const double pr = ar[c * LDA + c], pim = ai[c * LDA + c];
         const double nr = detr * pr - deti * pim, ni = detr * pim + deti * pr;
         detr = nr;
         deti = ni;
@@ -529,19 +564,20 @@ __global__ __launch_bounds__(NT) void k_project(Geom g, R *U, int haar, uint64_t
         const int m = n - c - 1;
         for (int e = tid; e < m * m; e += NT) {
             const int r = c + 1 + e / m, k = c + 1 + e % m;
-            const double xr = mr[r * n + c], xi = mi[r * n + c];
+            const double xr = ar[r * LDA + c], xi = ai[r * LDA + c];
             const double fr = (xr * pr + xi * pim) / den, fi = (xi * pr - xr * pim) / den;
-            const double br = mr[c * n + k], bi = mi[c * n + k];
-            mr[r * n + k] -= fr * br - fi * bi;
-            mi[r * n + k] -= fr * bi + fi * br;
+            const double br = ar[c * LDA + k], bi = ai[c * LDA + k];
+            ar[r * LDA + k] -= fr * br - fi * bi;
+            ai[r * LDA + k] -= fr * bi + fi * br;
         }
         __syncthreads();
     }
     const double ph = -atan2(deti, detr) / n;
     const double cr = cos(ph), ci = sin(ph);
     for (int e = tid; e < n * n; e += NT) {
-        b[2 * e] = (R)(ar[e] * cr - ai[e] * ci);
-        b[2 * e + 1] = (R)(ar[e] * ci + ai[e] * cr);
+        const double qr = (double)b[2 * e], qi = (double)b[2 * e + 1];
+        b[2 * e] = (R)(qr * cr - qi * ci);
+        b[2 * e + 1] = (R)(qr * ci + qi * cr);
     }
 }
 
