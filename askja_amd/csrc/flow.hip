@@ -32,6 +32,13 @@ namespace askja {
 namespace flow {
 
 constexpr int kB = 256;
+// Block of the per-updated-link kernels (forward, backward).  They hold 256
+// VGPRs (one wavefront per SIMD) and a sub-layer has few links (32768 at
+// 16^4), so small blocks spread the wavefronts over every CU.
+#ifndef ASKJA_FLOW_BLOCK
+#define ASKJA_FLOW_BLOCK 64
+#endif
+constexpr int kBL = ASKJA_FLOW_BLOCK;
 constexpr int kHid = 32;
 
 struct C3 {
@@ -422,9 +429,9 @@ __device__ inline bool site_of(const Geom &g, int par, long t, long &s)
 // thread's updated link).  With log, the sub-layer's input links are kept,
 // link of parity-site t at index t (64-link tiles), for the backward.
 template <typename R, int D, int MU>
-__global__ __launch_bounds__(kB) void k_forward(Geom g, const R *Uin, R *Uout, Sub sb, double *part, R *log)
+__global__ __launch_bounds__(kBL) void k_forward(Geom g, const R *Uin, R *Uout, Sub sb, double *part, R *log)
 {
-    const long t = (long)blockIdx.x * kB + threadIdx.x;
+    const long t = (long)blockIdx.x * kBL + threadIdx.x;
     double sum[1] = {0.0}, cmp[1] = {0.0};
     long s;
     if (site_of(g, sb.par, t, s)) {
@@ -484,11 +491,11 @@ __device__ inline void rect_grad(const R *U, const Rect &r, double c, double *co
 }
 
 template <typename R, int D, int MU>
-__global__ __launch_bounds__(kB) void k_backward(Geom g, R *U, Sub sb, double *gr, double c_logj, double *gw,
+__global__ __launch_bounds__(kBL) void k_backward(Geom g, R *U, Sub sb, double *gr, double c_logj, double *gw,
                                                  const R *log, double *contrib, long T)
 {
     constexpr int nf = nfeat(D);
-    const long t = (long)blockIdx.x * kB + threadIdx.x;
+    const long t = (long)blockIdx.x * kBL + threadIdx.x;
     long s;
     const bool ok = site_of(g, sb.par, t, s);
     const long V = g.V;
@@ -782,7 +789,7 @@ static flow::Sub to_sub(const FlowSub &f) { return flow::Sub{f.mu, f.par, f.w, f
 static unsigned par_blocks(const Geom &g, int par)
 {
     const long len = par ? g.V - g.V0 : g.V0;
-    return (unsigned)((len + flow::kB - 1) / flow::kB);
+    return (unsigned)((len + flow::kBL - 1) / flow::kBL);
 }
 
 long flow_forward_partials(const Geom &g, int par) { return par_blocks(g, par); }
@@ -800,7 +807,7 @@ static hipError_t flow_dispatch(int D, int mu, unsigned nb, hipStream_t st, Args
 {
 #define ASKJA_FLOW_CASE(DD, MM)                                                              \
     if (D == DD && mu == MM) {                                                               \
-        hipLaunchKernelGGL((K<R, DD, MM>::fn), dim3(nb), dim3(flow::kB), 0, st, args...);   \
+        hipLaunchKernelGGL((K<R, DD, MM>::fn), dim3(nb), dim3(K<R, DD, MM>::block), 0, st, args...); \
         return hipGetLastError();                                                            \
     }
     ASKJA_FLOW_CASE(3, 0) ASKJA_FLOW_CASE(3, 1) ASKJA_FLOW_CASE(3, 2)
@@ -812,14 +819,17 @@ static hipError_t flow_dispatch(int D, int mu, unsigned nb, hipStream_t st, Args
 template <typename R, int D, int MU>
 struct FwdK {
     static constexpr auto fn = flow::k_forward<R, D, MU>;
+    static constexpr int block = flow::kBL;
 };
 template <typename R, int D, int MU>
 struct BwdK {
     static constexpr auto fn = flow::k_backward<R, D, MU>;
+    static constexpr int block = flow::kBL;
 };
 template <typename R, int D, int MU>
 struct GatK {
     static constexpr auto fn = flow::k_gather<R, D, MU>;
+    static constexpr int block = flow::kB;
 };
 
 hipError_t launch_flow_forward(const Geom &g, const void *Uin, void *Uout, const FlowSub &f, int mode, double *part,

@@ -566,10 +566,25 @@ inline long threads_for(long work, long per_thread, long scratch_doubles)
 long generic_scratch_per_thread(int n) { return 12L * n * n; }
 
 // ---------------------------------------------------------------- dispatch
+// 4 <= N <= 8: the row kernel puts 64/N links in one single-wavefront
+// workgroup and walks each link's staples and Horner products serially, so a
+// small lattice leaves most SIMDs idle and the launch is latency bound (the
+// reference's example parameters, N 8, D 10, 100 sites: 1000 links in 125
+// workgroups).  Below 2560 links (2.5 per SIMD) the workgroup-per-link
+// kernel, one matrix element per lane, runs instead.  Measured crossovers
+// (kick+drift, MI355X): N 8 about 3600 links in FP64 and 6000 in the exact
+// mode, N 4 about 2200 and 4300.
+constexpr long kRowsMinLinks = 2560;
+
+static bool small_lattice_team(const Geom &g)
+{
+    return g.N >= 4 && g.N <= 8 && g.V * g.D < kRowsMinLinks;
+}
+
 long kick_partials(const Geom &g)
 {
     if (g.N <= 3) return kick_site_blocks(g) * g.D;
-    if (g.N <= 8) return rows_kick_blocks(g);
+    if (g.N <= 8 && !small_lattice_team(g)) return rows_kick_blocks(g);
     return g.V * g.D;
 }
 
@@ -585,7 +600,7 @@ hipError_t launch_kick(const KickArgs &a, int mode, bool drift, hipStream_t st)
     }
     if (mode == kModeBF16x3) mode = kModeF32;
     if (n <= 3) return small_kick(a, mode, drift, st);
-    if (n <= 8) return rows_kick(a, mode, drift, st);
+    if (n <= 8) return small_lattice_team(a.g) ? team_kick_8(a, mode, drift, st) : rows_kick(a, mode, drift, st);
     if (n <= 16) return team_kick_16(a, mode, drift, st);
     if (n <= 32) return team_kick_32(a, mode, drift, st);
     return team_kick_64(a, mode, drift, st);
