@@ -32,13 +32,6 @@ namespace askja {
 namespace flow {
 
 constexpr int kB = 256;
-// Block of the per-updated-link kernels (forward, backward).  They hold 256
-// VGPRs (one wavefront per SIMD) and a sub-layer has few links (32768 at
-// 16^4), so small blocks spread the wavefronts over every CU.
-#ifndef ASKJA_FLOW_BLOCK
-#define ASKJA_FLOW_BLOCK 64
-#endif
-constexpr int kBL = ASKJA_FLOW_BLOCK;
 constexpr int kHid = 32;
 
 struct C3 {
@@ -341,26 +334,61 @@ struct Fwd {
     double f[nfeat(D)];
 };
 
-// f[j] = v for a run-time j, keeping f in registers
-template <int NF>
-__device__ inline void set_feature(double (&f)[NF], int j, double v)
+// Two wavefronts share the work of 64 updated links: the forward and
+// backward kernels run workgroups of 2 x kLinks threads, and wavefront `role`
+// (0 or 1) holds lane t of the same kLinks links.  Role 0 computes the
+// plaquettes at x (their sum C and their features) and the 2x1 rectangles,
+// role 1 the 1x2 rectangles and the staple sum S; the MLP is split by hidden
+// unit.  The parts meet in LDS (one column per link); both wavefronts then
+// hold C, S and every feature and compute the rest alike.  A sub-layer
+// updates few links (32768 at 16^4) with 256-VGPR threads, so the split is
+// what gives every SIMD a wavefront.
+constexpr int kLinks = 64;
+
+template <int D>
+struct Xch {
+    double c[18][kLinks], sm[18][kLinks];  // C (role 0), S (role 1): re then im, row-major
+    double f[nfeat(D)][kLinks];            // features, each written by the role that computes it
+    double z[2][kLinks];                   // MLP output partials of the two roles
+    double df[2][nfeat(D)][kLinks];        // backward: dPhi/df partials of the two roles
+};
+
+__device__ inline void xput(double (*d)[kLinks], int lane, const C3 &m)
 {
 #pragma unroll
-    for (int q = 0; q < NF; q++)
-        if (q == j) f[q] = v;
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            d[p * 3 + q][lane] = m.r[p][q];
+            d[9 + p * 3 + q][lane] = m.i[p][q];
+        }
 }
 
-// Plaquette features, MLP, staples and rotation angle of link (x = s, MU).
-// F.U must hold the own link on entry.  The plaquettes come first (C,
-// features, Lhat), then the staples (S, W = U S), so that only one
-// accumulator is live at a time; the loops stay rolled (a fully unrolled
-// stencil hoists every neighbour load and spills).
-template <typename R, int D, int MU>
-__device__ inline void forward_link(const Geom &g, const R *U, const Sub &sb, long s, Fwd<D> &F)
+__device__ inline C3 xget(const double (*d)[kLinks], int lane)
 {
-    constexpr int nf = nfeat(D);
+    C3 m;
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            m.r[p][q] = d[p * 3 + q][lane];
+            m.i[p][q] = d[9 + p * 3 + q][lane];
+        }
+    return m;
+}
+
+// Plaquette and rectangle features, MLP, staples and rotation angle of link
+// (x = s, MU), split between the two roles as above.  F.U must hold the own
+// link on entry (ok lanes).  Every thread of the workgroup calls it (it has
+// two barriers).  The loops stay rolled (a fully unrolled stencil hoists
+// every neighbour load and spills).
+template <typename R, int D, int MU>
+__device__ inline void forward_link(const Geom &g, const R *U, const Sub &sb, long s, bool ok, int role, int lane,
+                                    Xch<D> &x, Fwd<D> &F)
+{
+    constexpr int nf = nfeat(D), np = nplanes(D);
     const long V = g.V;
-    {
+    if (ok && role == 0) {
         C3 C;
         zero(C);
         int j = 0;
@@ -375,30 +403,29 @@ __device__ inline void forward_link(const Geom &g, const R *U, const Sub &sb, lo
                 const C3 P = mm<false, true>(
                     mm<false, true>(mm<false, false>(ld(U, nu * V + s), ld(U, la * V + xpn)), ld(U, nu * V + xla)),
                     ld(U, la * V + s));
-                set_feature(F.f, j, retr(P) / 3.0);
+                x.f[j][lane] = retr(P) / 3.0;
                 addto(C, P);
                 Rect r1, r2;
                 rectangles(g, s, nu, la, r1, r2);
-                set_feature(F.f, nplanes(D) + 2 * j, retr(rect_loop(U, r1)) / 3.0);
-                set_feature(F.f, nplanes(D) + 2 * j + 1, retr(rect_loop(U, r2)) / 3.0);
+                x.f[np + 2 * j][lane] = retr(rect_loop(U, r1)) / 3.0;
                 j++;
             }
         }
-        const C3 Lam = tah(C);
-        F.n = sqrt(2.0 * retr(mm<false, false>(Lam, Lam)) + sb.delta * sb.delta);
-        F.Lh = cscale(Lam, 1.0 / F.n, 0.0);
-    }
-    const float *w = sb.w;
-    double z = (double)w[kHid * nf + 2 * kHid];
+        xput(x.c, lane, C);
+    } else if (ok) {
+        int j = 0;
 #pragma unroll 1
-    for (int k = 0; k < kHid; k++) {
-        double a = (double)w[kHid * nf + k];
-#pragma unroll
-        for (int q = 0; q < nf; q++) a += (double)w[k * nf + q] * F.f[q];
-        z += (double)w[kHid * nf + kHid + k] * tanh(a);
-    }
-    F.rho = sb.rho_max * tanh(z);
-    {
+        for (int nu = 0; nu < D; nu++) {
+            if (nu == MU) continue;
+#pragma unroll 1
+            for (int la = nu + 1; la < D; la++) {
+                if (la == MU) continue;
+                Rect r1, r2;
+                rectangles(g, s, nu, la, r1, r2);
+                x.f[np + 2 * j + 1][lane] = retr(rect_loop(U, r2)) / 3.0;
+                j++;
+            }
+        }
         const long xp = g.nbr[(2 * MU) * V + s];
         C3 S;
         zero(S);
@@ -411,10 +438,34 @@ __device__ inline void forward_link(const Geom &g, const R *U, const Sub &sb, lo
             addto(S, mm<false, true>(mm<false, true>(ld(U, nu * V + xp), ld(U, MU * V + xpn)), ld(U, nu * V + s)));
             addto(S, mm<false, false>(mm<true, true>(ld(U, nu * V + xpmn), ld(U, MU * V + xmn)), ld(U, nu * V + xmn)));
         }
-        F.W = mm<false, false>(F.U, S);
+        xput(x.sm, lane, S);
     }
-    F.theta = F.rho * imtr(mm<false, false>(F.Lh, F.W));
-    F.K = retr(mm<false, false>(mm<false, false>(F.Lh, F.Lh), F.W));
+    __syncthreads();
+    const float *w = sb.w;
+    if (ok) {
+#pragma unroll
+        for (int q = 0; q < nf; q++) F.f[q] = x.f[q][lane];
+        const C3 Lam = tah(xget(x.c, lane));
+        F.n = sqrt(2.0 * retr(mm<false, false>(Lam, Lam)) + sb.delta * sb.delta);
+        F.Lh = cscale(Lam, 1.0 / F.n, 0.0);
+        // hidden units [role kHid/2, (role + 1) kHid/2); role 0 adds b2
+        double zp = role == 0 ? (double)w[kHid * nf + 2 * kHid] : 0.0;
+#pragma unroll 1
+        for (int k = role * (kHid / 2); k < (role + 1) * (kHid / 2); k++) {
+            double a = (double)w[kHid * nf + k];
+#pragma unroll
+            for (int q = 0; q < nf; q++) a += (double)w[k * nf + q] * F.f[q];
+            zp += (double)w[kHid * nf + kHid + k] * tanh(a);
+        }
+        x.z[role][lane] = zp;
+    }
+    __syncthreads();
+    if (ok) {
+        F.rho = sb.rho_max * tanh(x.z[0][lane] + x.z[1][lane]);
+        F.W = mm<false, false>(F.U, xget(x.sm, lane));
+        F.theta = F.rho * imtr(mm<false, false>(F.Lh, F.W));
+        F.K = retr(mm<false, false>(mm<false, false>(F.Lh, F.Lh), F.W));
+    }
 }
 
 // sites of parity p: device sites [0, V0) or [V0, V)
@@ -429,16 +480,21 @@ __device__ inline bool site_of(const Geom &g, int par, long t, long &s)
 // thread's updated link).  With log, the sub-layer's input links are kept,
 // link of parity-site t at index t (64-link tiles), for the backward.
 template <typename R, int D, int MU>
-__global__ __launch_bounds__(kBL) void k_forward(Geom g, const R *Uin, R *Uout, Sub sb, double *part, R *log)
+__global__ __launch_bounds__(2 * kLinks) void k_forward(Geom g, const R *Uin, R *Uout, Sub sb, double *part, R *log)
 {
-    const long t = (long)blockIdx.x * kBL + threadIdx.x;
+    __shared__ Xch<D> x;
+    const int lane = (int)(threadIdx.x % kLinks), role = (int)(threadIdx.x / kLinks);
+    const long t = (long)blockIdx.x * kLinks + lane;
     double sum[1] = {0.0}, cmp[1] = {0.0};
     long s;
-    if (site_of(g, sb.par, t, s)) {
-        Fwd<D> F;
+    const bool ok = site_of(g, sb.par, t, s);
+    Fwd<D> F;
+    if (ok) {
         F.U = ld(Uin, MU * g.V + s);
-        if (log) st(log, t, F.U);
-        forward_link<R, D, MU>(g, Uin, sb, s, F);
+        if (log && role == 0) st(log, t, F.U);
+    }
+    forward_link<R, D, MU>(g, Uin, sb, s, ok, role, lane, x, F);
+    if (ok && role == 0) {
         const C3 E = expi(cscale(F.Lh, F.theta, 0.0));
         st(Uout, MU * g.V + s, mm<false, false>(E, F.U));
         sum[0] = log1p(F.rho * F.K);
@@ -491,79 +547,172 @@ __device__ inline void rect_grad(const R *U, const Rect &r, double c, double *co
 }
 
 template <typename R, int D, int MU>
-__global__ __launch_bounds__(kBL) void k_backward(Geom g, R *U, Sub sb, double *gr, double c_logj, double *gw,
-                                                 const R *log, double *contrib, long T)
+__global__ __launch_bounds__(2 * kLinks) void k_backward(Geom g, R *U, Sub sb, double *gr, double c_logj, double *gw,
+                                                         const R *log, double *contrib, long T)
 {
-    constexpr int nf = nfeat(D);
-    const long t = (long)blockIdx.x * kBL + threadIdx.x;
+    constexpr int nf = nfeat(D), np = nplanes(D);
+    __shared__ Xch<D> x;
+    const int lane = (int)(threadIdx.x % kLinks), role = (int)(threadIdx.x / kLinks);
+    const long t = (long)blockIdx.x * kLinks + lane;
     long s;
     const bool ok = site_of(g, sb.par, t, s);
     const long V = g.V;
+    const long l = MU * V + s;
     Fwd<D> F;
 #pragma unroll
     for (int q = 0; q < nf; q++) F.f[q] = 0.0;
-    double dz = 0.0;
-    C3 GC;
     if (ok) {
-        const long l = MU * V + s;
         F.U = ld(log, t);
-        st(U, l, F.U);
-        forward_link<R, D, MU>(g, U, sb, s, F);
+        if (role == 0) st(U, l, F.U);
+    }
+    forward_link<R, D, MU>(g, U, sb, s, ok, role, lane, x, F);
+    // Both roles need dPhi/dz; role 0 then needs dPhi/dC (through Lhat, the
+    // dexp adjoint series), role 1 the own link's gradient (through E) and
+    // dPhi/dS.
+    double dz = 0.0;
+    C3 gU, GC, GS;
+    if (ok) {
         const C3 gp = gget(gr, l);
-        // U' = E U,  E = exp(i theta Lhat)
-        C3 gU;
-        {
-            const C3 E = expi(cscale(F.Lh, F.theta, 0.0));
-            gU = proj(mm<true, false>(E, mm<false, false>(gp, E)));
-        }
         const double dtheta = 2.0 * retr(mm<false, false>(gp, F.Lh));
-        C3 gLh = cscale(avg_ad(cscale(F.Lh, F.theta, 0.0), gp), F.theta, 0.0);
         // theta = rho Im Tr(Lhat W)
         double drho = dtheta * imtr(mm<false, false>(F.Lh, F.W));
-        {
-            C3 WmWd;
-#pragma unroll
-            for (int p = 0; p < 3; p++)
-#pragma unroll
-                for (int q = 0; q < 3; q++) {
-                    WmWd.r[p][q] = F.W.r[p][q] - F.W.r[q][p];
-                    WmWd.i[p][q] = F.W.i[p][q] + F.W.i[q][p];
-                }
-            addto(gLh, cscale(WmWd, 0.0, -0.25 * dtheta * F.rho));  // dtheta rho (W - W^+)/(4i)
-        }
-        C3 GW = cscale(F.Lh, 0.0, -0.5 * dtheta * F.rho);  // dtheta rho (-i Lhat)/2
         // log J = log(1 + rho K), K = ReTr(Lhat^2 W)
         const double a = c_logj / (1.0 + F.rho * F.K);
         drho += a * F.K;
-        {
-            C3 M = mm<false, false>(F.Lh, F.W);
-            addto(M, mm<false, false>(F.W, F.Lh));
-            C3 Md;
-#pragma unroll
-            for (int p = 0; p < 3; p++)
-#pragma unroll
-                for (int q = 0; q < 3; q++) {
-                    Md.r[p][q] = M.r[p][q] + M.r[q][p];
-                    Md.i[p][q] = M.i[p][q] - M.i[q][p];
-                }
-            addto(gLh, Md, 0.25 * a * F.rho);
-        }
-        addto(GW, mm<false, false>(F.Lh, F.Lh), 0.5 * a * F.rho);
         // rho = rho_max tanh(z)
         const double th = F.rho / sb.rho_max;
         dz = drho * sb.rho_max * (1.0 - th * th);
-        // Lhat = Lam / n, Lam = n Lhat
-        {
+        if (role == 0) {
+            // U' = E U with E = exp(i theta Lhat): through Lhat
+            C3 gLh = cscale(avg_ad(cscale(F.Lh, F.theta, 0.0), gp), F.theta, 0.0);
+            {
+                C3 WmWd;
+#pragma unroll
+                for (int p = 0; p < 3; p++)
+#pragma unroll
+                    for (int q = 0; q < 3; q++) {
+                        WmWd.r[p][q] = F.W.r[p][q] - F.W.r[q][p];
+                        WmWd.i[p][q] = F.W.i[p][q] + F.W.i[q][p];
+                    }
+                addto(gLh, cscale(WmWd, 0.0, -0.25 * dtheta * F.rho));  // dtheta rho (W - W^+)/(4i)
+            }
+            {
+                C3 M = mm<false, false>(F.Lh, F.W);
+                addto(M, mm<false, false>(F.W, F.Lh));
+                C3 Md;
+#pragma unroll
+                for (int p = 0; p < 3; p++)
+#pragma unroll
+                    for (int q = 0; q < 3; q++) {
+                        Md.r[p][q] = M.r[p][q] + M.r[q][p];
+                        Md.i[p][q] = M.i[p][q] - M.i[q][p];
+                    }
+                addto(gLh, Md, 0.25 * a * F.rho);
+            }
+            // Lhat = Lam / n, Lam = n Lhat
             const double tgl = 2.0 * F.n * retr(mm<false, false>(gLh, F.Lh));
             C3 gLam = cscale(gLh, 1.0 / F.n, 0.0);
             addto(gLam, F.Lh, -tgl / (F.n * F.n));
             GC = cscale(proj(gLam), 0.0, -1.0);  // Lam = TAH(C)
+        } else {
+            C3 GW = cscale(F.Lh, 0.0, -0.5 * dtheta * F.rho);  // dtheta rho (-i Lhat)/2
+            addto(GW, mm<false, false>(F.Lh, F.Lh), 0.5 * a * F.rho);
+            // own link: through E, then through W = U S
+            {
+                const C3 E = expi(cscale(F.Lh, F.theta, 0.0));
+                gU = proj(mm<true, false>(E, mm<false, false>(gp, E)));
+            }
+            addto(gU, proj(cscale(mm<false, false>(F.W, GW), 0.0, 1.0)));
+            // staple links: G_S = G_W U
+            GS = mm<false, false>(GW, F.U);
         }
-        // own link through W = U S
-        addto(gU, proj(cscale(mm<false, false>(F.W, GW), 0.0, 1.0)));
+    }
+    // MLP backward over this role's hidden units, every lane (inactive lanes
+    // carry dz = 0, f = 0) so that the weight gradient can be summed over
+    // the wavefront first; dPhi/df partials meet in LDS
+    const float *w = sb.w;
+    double df[nf > 0 ? nf : 1];
+#pragma unroll
+    for (int q = 0; q < nf; q++) df[q] = 0.0;
+    const bool lead = lane == 0;
+#pragma unroll 1
+    for (int k = role * (kHid / 2); k < (role + 1) * (kHid / 2); k++) {
+        double a = (double)w[kHid * nf + k];
+#pragma unroll
+        for (int q = 0; q < nf; q++) a += (double)w[k * nf + q] * F.f[q];
+        const double h = tanh(a);
+        const double dh = dz * (double)w[kHid * nf + kHid + k];
+        const double da = dh * (1.0 - h * h);
+#pragma unroll
+        for (int q = 0; q < nf; q++) df[q] += da * (double)w[k * nf + q];
+        if (gw) {
+            const double s_w2 = wave_sum(dz * h), s_b1 = wave_sum(da);
+            if (lead) {
+                atomicAdd(gw + kHid * nf + kHid + k, s_w2);
+                atomicAdd(gw + kHid * nf + k, s_b1);
+            }
+#pragma unroll
+            for (int q = 0; q < nf; q++) {
+                const double s_w1 = wave_sum(da * F.f[q]);
+                if (lead) atomicAdd(gw + k * nf + q, s_w1);
+            }
+        }
+    }
+    if (gw && role == 0) {
+        const double s_b2 = wave_sum(dz);
+        if (lead) atomicAdd(gw + kHid * nf + 2 * kHid, s_b2);
+    }
+#pragma unroll
+    for (int q = 0; q < nf; q++) x.df[role][q][lane] = df[q];
+    // also orders every read of gr[l] above before its write below
+    __syncthreads();
+    if (!ok) return;
+#pragma unroll
+    for (int q = 0; q < nf; q++) df[q] = x.df[0][q][lane] + x.df[1][q][lane];
+    if (role == 0) {
+        // plaquettes P = p1 p2 p3^+ p4^+ in the planes (nu, la) without MU:
+        // dPhi/dLam through C = sum P, and dPhi/df through ReTr P / 3; then
+        // the 2x1 rectangles: dPhi/df (f = ReTr R / 3) only
+        int j = 0;
+#pragma unroll 1
+        for (int nu = 0; nu < D; nu++) {
+            if (nu == MU) continue;
+            const long xpn = g.nbr[(2 * nu) * V + s];
+#pragma unroll 1
+            for (int la = nu + 1; la < D; la++) {
+                if (la == MU) continue;
+                const long xla = g.nbr[(2 * la) * V + s];
+                const C3 p1 = ld(U, nu * V + s), p2 = ld(U, la * V + xpn), p3 = ld(U, nu * V + xla),
+                         p4 = ld(U, la * V + s);
+                C3 GP = GC;
+                double dfj = 0.0, df1 = 0.0;
+#pragma unroll
+                for (int q = 0; q < nf; q++) {
+                    if (q == j) dfj = df[q];
+                    if (q == np + 2 * j) df1 = df[q];
+                }
+                const double cf_ = dfj / 6.0;
+#pragma unroll
+                for (int p = 0; p < 3; p++) GP.r[p][p] += cf_;
+                const C3 p12 = mm<false, false>(p1, p2);
+                const C3 p123 = mm<false, true>(p12, p3);
+                const C3 P = mm<false, true>(p123, p4);
+                const int sl = 6 * (D - 1) + 16 * j;
+                cput(contrib, sl + 0, t, T, proj(cscale(mm<false, false>(P, GP), 0.0, 1.0)));
+                cput(contrib, sl + 1, t, T,
+                     proj(cscale(mm<false, false>(mm<false, true>(mm<false, true>(p2, p3), p4), mm<false, false>(GP, p1)),
+                                 0.0, 1.0)));
+                cput(contrib, sl + 2, t, T, proj(cscale(mm<true, false>(p4, mm<false, false>(GP, p123)), 0.0, -1.0)));
+                cput(contrib, sl + 3, t, T, proj(cscale(mm<false, false>(GP, P), 0.0, -1.0)));
+                Rect r1, r2;
+                rectangles(g, s, nu, la, r1, r2);
+                rect_grad(U, r1, df1 / 6.0, contrib, sl + 4, t, T);
+                j++;
+            }
+        }
+    } else {
         gput(gr, l, gU);
-        // staple links: G_S = G_W U
-        const C3 GS = mm<false, false>(GW, F.U);
+        // staple links through S
         const long xp = g.nbr[(2 * MU) * V + s];
 #pragma unroll 1
         for (int nu = 0; nu < D; nu++) {
@@ -592,84 +741,23 @@ __global__ __launch_bounds__(kBL) void k_backward(Geom g, R *U, Sub sb, double *
                 cput(contrib, sl + 5, t, T, proj(cscale(fd, 0.0, 1.0)));
             }
         }
-    }
-    // MLP backward, every lane (inactive lanes carry dz = 0, f = 0) so that
-    // the weight gradient can be summed over the wavefront first
-    const float *w = sb.w;
-    double df[nf > 0 ? nf : 1];
-#pragma unroll
-    for (int q = 0; q < nf; q++) df[q] = 0.0;
-    const bool lead = (threadIdx.x & 63) == 0;
+        // the 1x2 rectangles: dPhi/df only
+        int j = 0;
 #pragma unroll 1
-    for (int k = 0; k < kHid; k++) {
-        double a = (double)w[kHid * nf + k];
-#pragma unroll
-        for (int q = 0; q < nf; q++) a += (double)w[k * nf + q] * F.f[q];
-        const double h = tanh(a);
-        const double dh = dz * (double)w[kHid * nf + kHid + k];
-        const double da = dh * (1.0 - h * h);
-#pragma unroll
-        for (int q = 0; q < nf; q++) df[q] += da * (double)w[k * nf + q];
-        if (gw) {
-            const double s_w2 = wave_sum(dz * h), s_b1 = wave_sum(da);
-            if (lead) {
-                atomicAdd(gw + kHid * nf + kHid + k, s_w2);
-                atomicAdd(gw + kHid * nf + k, s_b1);
-            }
-#pragma unroll
-            for (int q = 0; q < nf; q++) {
-                const double s_w1 = wave_sum(da * F.f[q]);
-                if (lead) atomicAdd(gw + k * nf + q, s_w1);
-            }
-        }
-    }
-    if (gw) {
-        const double s_b2 = wave_sum(dz);
-        if (lead) atomicAdd(gw + kHid * nf + 2 * kHid, s_b2);
-    }
-    if (!ok) return;
-    // plaquettes P = p1 p2 p3^+ p4^+ in the planes (nu, la) without MU:
-    // dPhi/dLam through C = sum P, and dPhi/df through ReTr P / 3
-    int j = 0;
+        for (int nu = 0; nu < D; nu++) {
+            if (nu == MU) continue;
 #pragma unroll 1
-    for (int nu = 0; nu < D; nu++) {
-        if (nu == MU) continue;
-        const long xpn = g.nbr[(2 * nu) * V + s];
-#pragma unroll 1
-        for (int la = nu + 1; la < D; la++) {
-            if (la == MU) continue;
-            const long xla = g.nbr[(2 * la) * V + s];
-            const C3 p1 = ld(U, nu * V + s), p2 = ld(U, la * V + xpn), p3 = ld(U, nu * V + xla), p4 = ld(U, la * V + s);
-            C3 GP = GC;
-            double dfj = 0.0;
+            for (int la = nu + 1; la < D; la++) {
+                if (la == MU) continue;
+                double df2 = 0.0;
 #pragma unroll
-            for (int q = 0; q < nf; q++)
-                if (q == j) dfj = df[q];
-            j++;
-            const double cf_ = dfj / 6.0;
-#pragma unroll
-            for (int p = 0; p < 3; p++) GP.r[p][p] += cf_;
-            const C3 p12 = mm<false, false>(p1, p2);
-            const C3 p123 = mm<false, true>(p12, p3);
-            const C3 P = mm<false, true>(p123, p4);
-            const int sl = 6 * (D - 1) + 16 * (j - 1);
-            cput(contrib, sl + 0, t, T, proj(cscale(mm<false, false>(P, GP), 0.0, 1.0)));
-            cput(contrib, sl + 1, t, T,
-                 proj(cscale(mm<false, false>(mm<false, true>(mm<false, true>(p2, p3), p4), mm<false, false>(GP, p1)),
-                             0.0, 1.0)));
-            cput(contrib, sl + 2, t, T, proj(cscale(mm<true, false>(p4, mm<false, false>(GP, p123)), 0.0, -1.0)));
-            cput(contrib, sl + 3, t, T, proj(cscale(mm<false, false>(GP, P), 0.0, -1.0)));
-            // rectangles: dPhi/df (f = ReTr R / 3) only
-            Rect r1, r2;
-            rectangles(g, s, nu, la, r1, r2);
-            double df1 = 0.0, df2 = 0.0;
-#pragma unroll
-            for (int e = 0; e < nf; e++) {
-                if (e == nplanes(D) + 2 * (j - 1)) df1 = df[e];
-                if (e == nplanes(D) + 2 * (j - 1) + 1) df2 = df[e];
+                for (int q = 0; q < nf; q++)
+                    if (q == np + 2 * j + 1) df2 = df[q];
+                Rect r1, r2;
+                rectangles(g, s, nu, la, r1, r2);
+                rect_grad(U, r2, df2 / 6.0, contrib, 6 * (D - 1) + 16 * j + 10, t, T);
+                j++;
             }
-            rect_grad(U, r1, df1 / 6.0, contrib, sl + 4, t, T);
-            rect_grad(U, r2, df2 / 6.0, contrib, sl + 10, t, T);
         }
     }
 }
@@ -789,7 +877,7 @@ static flow::Sub to_sub(const FlowSub &f) { return flow::Sub{f.mu, f.par, f.w, f
 static unsigned par_blocks(const Geom &g, int par)
 {
     const long len = par ? g.V - g.V0 : g.V0;
-    return (unsigned)((len + flow::kBL - 1) / flow::kBL);
+    return (unsigned)((len + flow::kLinks - 1) / flow::kLinks);
 }
 
 long flow_forward_partials(const Geom &g, int par) { return par_blocks(g, par); }
@@ -819,12 +907,12 @@ static hipError_t flow_dispatch(int D, int mu, unsigned nb, hipStream_t st, Args
 template <typename R, int D, int MU>
 struct FwdK {
     static constexpr auto fn = flow::k_forward<R, D, MU>;
-    static constexpr int block = flow::kBL;
+    static constexpr int block = 2 * flow::kLinks;
 };
 template <typename R, int D, int MU>
 struct BwdK {
     static constexpr auto fn = flow::k_backward<R, D, MU>;
-    static constexpr int block = flow::kBL;
+    static constexpr int block = 2 * flow::kLinks;
 };
 template <typename R, int D, int MU>
 struct GatK {
