@@ -91,6 +91,7 @@ SIGNATURES = {
     "askja_line_phases": (_I, [_P, _I, _I, C.POINTER(C.c_int64), _DP]),
     "askja_hmc_trajectory": (_I, [_P, C.POINTER(HmcParams), _U64, C.POINTER(HmcResult)]),
     "askja_uniform": (_I, [_P, _U64, _U64, _DP]),
+    "askja_tau_int": (_I, [_DP, _L, _D, _DP, C.POINTER(C.c_long)]),
     "askja_flow_define": (_I, [_P, _I, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), _I, _D, _D]),
     "askja_flow_block_size": (_I, [_P, C.POINTER(C.c_int)]),
     "askja_flow_set_weights": (_I, [_P, _P, _I]),
@@ -132,6 +133,17 @@ def lib():
 
 def _dptr(a):
     return a.ctypes.data_as(_DP)
+
+
+def tau_int(x, S=1.5):
+    """Integrated autocorrelation time tau = 1 + 2 sum rho(t) of a series and
+    Wolff's automatic window W (askja_tau_int; host code, no GPU)."""
+    x = np.ascontiguousarray(x, np.float64)
+    tau, w = C.c_double(), C.c_long()
+    rc = lib().askja_tau_int(_dptr(x), len(x), S, C.byref(tau), C.byref(w))
+    if rc:
+        raise AskjaError(f"tau_int: {_ERRORS.get(rc, rc)} (n = {len(x)}, S = {S})")
+    return tau.value, w.value
 
 
 class Lattice:

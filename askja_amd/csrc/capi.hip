@@ -946,6 +946,44 @@ int askja_uniform(askja_ctx *c, uint64_t seed, uint64_t counter, double *out)
     return ASKJA_OK;
 }
 
+// --------------------------------------------------------- autocorrelation
+int askja_tau_int(const double *x, long n, double S, double *tau, long *window)
+{
+    if (!x || !tau || n < 8 || !(S > 0)) return ASKJA_EINVAL;
+    double mean = 0.0;
+    for (long i = 0; i < n; i++) mean += x[i];
+    mean /= (double)n;
+    std::vector<double> d((size_t)n);
+    double var = 0.0;
+    for (long i = 0; i < n; i++) {
+        d[i] = x[i] - mean;
+        var += d[i] * d[i];
+    }
+    var /= (double)n;
+    if (!(var > 0.0)) {
+        *tau = 1.0;
+        if (window) *window = 0;
+        return ASKJA_OK;
+    }
+    // Wolff's window: the first W where exp(-W/tau_W) - tau_W/sqrt(W n) < 0,
+    // tau_W = S / log((2 tint + 1) / (2 tint - 1)) from the running tint
+    double tint = 0.5;
+    for (long W = 1; W < n / 2; W++) {
+        double c = 0.0;
+        for (long i = 0; i + W < n; i++) c += d[i] * d[i + W];
+        tint += c / (double)(n - W) / var;
+        const double tw = tint > 0.5 ? S / std::log((2.0 * tint + 1.0) / (2.0 * tint - 1.0)) : 1e-9;
+        if (std::exp(-(double)W / tw) - tw / std::sqrt((double)W * (double)n) < 0.0) {
+            *tau = 2.0 * tint;
+            if (window) *window = W;
+            return ASKJA_OK;
+        }
+    }
+    *tau = 2.0 * tint;
+    if (window) *window = n / 2;
+    return ASKJA_OK;
+}
+
 // ------------------------------------------------------------------ flow
 namespace {
 

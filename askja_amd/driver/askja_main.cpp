@@ -27,6 +27,7 @@
 #include <sys/stat.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -279,6 +280,23 @@ int main(int argc, char *argv[])
         fflush(wilson);
     };
 
+    // |volume average of Tr P / N| of the chain state after every trajectory,
+    // for the integrated autocorrelation time reported at the end
+    long nlines = 1;
+    for (int i = 1; i < DIM; i++) nlines *= P.x[i];
+    std::vector<double> lines((size_t)2 * nlines), poly;
+    auto polyakov_abs = [&](double &out) -> int {
+        int rc = askja_wilson_lines(ctx, 0, lines.data());
+        if (rc) return rc;
+        double re = 0.0, im = 0.0;
+        for (long k = 0; k < nlines; k++) {
+            re += lines[2 * k];
+            im += lines[2 * k + 1];
+        }
+        out = std::hypot(re, im) / (double)nlines;
+        return ASKJA_OK;
+    };
+
     uint64_t traj_counter = (uint64_t)P.NTOT;
     CK(refresh(traj_counter));
     double CRITENERGY;
@@ -325,6 +343,7 @@ int main(int argc, char *argv[])
 
     const clock_t c0 = clock();
     const time_t w0 = time(NULL);
+    const auto t0 = std::chrono::steady_clock::now();
     ConfWriter out;
     int timekeep;
     for (timekeep = P.START + 1; timekeep < P.STEPS + P.START + 1; timekeep++) {
@@ -384,6 +403,11 @@ int main(int argc, char *argv[])
             } else {
                 CK(askja_snapshot(ctx, 1));
             }
+            {
+                double pa;
+                CK(polyakov_abs(pa));
+                poly.push_back(pa);
+            }
             if (P.VERBOSE) {
                 double em;
                 CK(energy_mag(em));
@@ -433,11 +457,22 @@ int main(int argc, char *argv[])
     fclose(constraints);
     fclose(wilson);
     const double wall = difftime(time(NULL), w0);
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     (void)c0;
     askja_destroy(ctx);
     printf("===>Info: Statistics: acc=%i (%i) rec=%i, rate=%f\n", P.NACC, P.NAC1, P.NREC, (P.NACC * 1.0) / (P.NTOT));
     printf("===>Took %i configurations\n", P.TAKEN);
     printf("===>Info: %d MD steps in %.0f s\n", P.STEPS, wall);
+    {
+        // integrated autocorrelation time of |P| (Wolff's window); the
+        // reference's statistics assume independent samples
+        double tau;
+        long W;
+        if (askja_tau_int(poly.data(), (long)poly.size(), 1.5, &tau, &W) == ASKJA_OK)
+            printf("===>Info: tau_int(|P|) = %f (window %ld) over %zu trajectories, %f independent configurations/s\n",
+                   tau, W, poly.size(), secs > 0 ? poly.size() / secs / tau : 0.0);
+        else printf("===>Info: tau_int(|P|) needs at least 8 trajectories (%zu)\n", poly.size());
+    }
     printf("Finished successfully\n");
     return 0;
 }

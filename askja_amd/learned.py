@@ -120,22 +120,11 @@ def mean_err(x):
 
 
 def tau_int(x, S=1.5):
-    """tau = 1 + 2 sum rho(t) with Wolff's automatic window."""
-    x = np.asarray(x, np.float64)
-    n = len(x)
-    d = x - x.mean()
-    var = np.dot(d, d) / n
-    if n < 8 or var <= 0:
+    """tau = 1 + 2 sum rho(t) with Wolff's automatic window (the library's
+    askja_tau_int); NaN below 8 samples."""
+    if len(x) < 8:
         return float("nan")
-    f = np.fft.rfft(d, 2 * n)
-    acf = np.fft.irfft(f * np.conj(f))[:n] / (n - np.arange(n)) / var
-    tint = 0.5
-    for W in range(1, n // 2):
-        tint += acf[W]
-        tau = S / math.log((2 * tint + 1) / (2 * tint - 1)) if tint > 0.5 else 1e-9
-        if math.exp(-W / tau) - tau / math.sqrt(W * n) < 0:
-            return 2.0 * tint
-    return 2.0 * tint
+    return A.tau_int(x, S)[0]
 
 
 def train(lat, configs, beta, w0, epochs=2, lr=2e-3, log=None):

@@ -43,27 +43,15 @@ PMC_TRAFFIC_BYTES = {"fp64": 999.5e6, "fp32": 377.8e6}
 
 
 def tau_int_wolff(x, S=1.5):
-    """Integrated autocorrelation time (Wolff, Comput. Phys. Commun. 156
-    (2004) 143): automatic window W minimising the sum of statistical and
-    systematic errors.  Returns tau = 2 * tau_int(Wolff) = 1 + 2 sum rho."""
-    x = np.asarray(x, np.float64)
-    n = len(x)
-    if n < 8:
+    """Integrated autocorrelation time tau = 1 + 2 sum rho(t) (1 for
+    independent samples) with Wolff's automatic window (Comput. Phys.
+    Commun. 156 (2004) 143), from the library's host code askja_tau_int.
+    Returns (tau, window); NaN below 8 samples."""
+    import askja_amd as A
+
+    if len(x) < 8:
         return float("nan"), 0
-    d = x - x.mean()
-    var = np.dot(d, d) / n
-    if var <= 0:
-        return 1.0, 0
-    f = np.fft.rfft(d, 2 * n)
-    acf = np.fft.irfft(f * np.conj(f))[:n] / (n - np.arange(n)) / var
-    tint = 0.5
-    for W in range(1, n // 2):
-        tint += acf[W]
-        tau = S / math.log((2 * tint + 1) / (2 * tint - 1)) if tint > 0.5 else 1e-9
-        g = math.exp(-W / tau) - tau / math.sqrt(W * n)
-        if g < 0:
-            return 2.0 * tint, W
-    return 2.0 * tint, n // 2
+    return A.tau_int(x, S)
 
 
 def cpu_baseline(steps_sample):
@@ -262,6 +250,38 @@ def kernel_study(reps=None):
             lat.close()
 
 
+def example_study(reps=200, ntraj=5):
+    """The reference's example parameters (params_N8S10T10B20k.txt: NC 8,
+    DIM 10, N0 = N1 = 10, the other eight extents 1, BETA 20000, DT 0.01,
+    LEVEL 8, UPDATE2 1000): time of one kick+drift launch (HIP events on the
+    library's stream) and of one HMC trajectory of 1000 drifts, per
+    precision.  Haar start, counter-based momenta."""
+    import askja_amd as A
+
+    ext = [10, 10] + [1] * 8
+    for name, prec in (("fp64", A.FP64), ("fp64-exact", A.FP64_EXACT), ("fp32", A.FP32)):
+        lat = A.Lattice(8, ext, prec)
+        lat.haar(SEED, 1)
+        lat.mom_refresh(SEED, 0, 20000.0)
+        lat.kick_drift(0.01, 0.01, 8)
+        lat.sync()
+        lat.timer_start()
+        for _ in range(reps):
+            lat.kick_drift(0.01, 0.01, 8)
+        us = lat.timer_stop() / reps * 1e3
+        lat.sync()
+        t0 = time.perf_counter()
+        acc = 0
+        for k in range(ntraj):
+            acc += lat.hmc_trajectory(20000.0, 0.01, 1000, 8, SEED, k).accepted
+        lat.sync()
+        T = (time.perf_counter() - t0) / ntraj
+        print(json.dumps({"study": "example", "nc": 8, "extents": ext, "precision": name,
+                          "us_kick_drift": round(us, 2), "s_per_trajectory": round(T, 4), "acceptance": acc / ntraj}),
+              flush=True)
+        lat.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -272,8 +292,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--roofline-launches", type=int, default=100)
     ap.add_argument("--copy-probe", action="store_true", help="only run a 1 GiB device copy (counter calibration)")
-    ap.add_argument("--study", choices=["precision", "c3scan", "c5exact", "kernels"],
-                    help="print a precision, beta-scan, exact-2D or per-config kernel study instead")
+    ap.add_argument("--study", choices=["precision", "c3scan", "c5exact", "kernels", "example"],
+                    help="print a precision, beta-scan, exact-2D, per-config kernel or example-params study instead")
     ap.add_argument("--study-configs", default="C2,C3,C5")
     ap.add_argument("--study-therm", type=int, default=20, help="thermalising trajectories (c5exact)")
     ap.add_argument("--study-precisions", default="bf16x3,fp64", help="precisions (c5exact)")
@@ -289,6 +309,9 @@ def main():
         return
     if args.study == "kernels":
         kernel_study()
+        return
+    if args.study == "example":
+        example_study()
         return
     if args.study == "c5exact":
         c5_exact(max(args.steps, 2), args.study_therm, tuple(args.study_precisions.split(",")))

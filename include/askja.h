@@ -222,6 +222,17 @@ int askja_hmc_trajectory(askja_ctx *ctx, const askja_hmc_params *p, uint64_t tra
  * time(NULL), sun.cpp:1277-1287). */
 int askja_uniform(askja_ctx *ctx, uint64_t seed, uint64_t counter, double *out);
 
+/* ---- autocorrelation (no reference counterpart) ----------------------------------- */
+/* Integrated autocorrelation time of the series x[0..n-1] by automatic
+ * windowing (U. Wolff, Comput. Phys. Commun. 156 (2004) 143; S is the window
+ * factor, 1.5 customary): *tau = 1 + 2 sum_{t=1}^{W} rho(t), 1 for
+ * uncorrelated data, with rho(t) the autocorrelation normalised by the n - t
+ * pairs, and *window = W (may be NULL).  The reference's error bars assume
+ * independent samples (measurements4d.cpp:1213-1257).  Host code, no
+ * context: returns ASKJA_EINVAL for n < 8, S <= 0 or null pointers.  A
+ * constant series gives tau = 1, window 0. */
+int askja_tau_int(const double *x, long n, double S, double *tau, long *window);
+
 /* ---- learned field transformation (no reference counterpart; SU(3), D = 3 or 4) ----
  * U = F(V): nsub masked sub-layers; sub-layer k updates the links of
  * direction mu[k] on the sites of parity parity[k] as
