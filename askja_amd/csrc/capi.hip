@@ -199,10 +199,8 @@ int exact_action(askja_ctx *c, const void *U, double *s)
         *s = (double)n_plaquettes(c) - pair_val(h, 0) / (4.0 * c->nc);
         return ASKJA_OK;
     }
-    if ((rc = ensure_scratch(c))) return rc;
     long np = 0;
-    const long cap = (c->V + 255) / 256 + 1024;
-    if ((rc = ensure_part(c, (size_t)cap * 4))) return rc;
+    if ((rc = ensure_part(c, (size_t)plaq_partials(c->geom) * 4))) return rc;
     HIPCHK(c, launch_plaq(c->geom, U, c->mode, 1. / (double)c->nc, c->part, &np, c->scratch, (long)c->scratch_doubles,
                           c->st));
     HIPCHK(c, launch_finalize(c->part, np, 2, c->res, c->st));
@@ -713,11 +711,9 @@ int askja_md_trajectory(askja_ctx *c, double dt, int ndrift, int level, double *
 int askja_action(askja_ctx *c, double *s_t, double *s_s)
 {
     ENTER(c);
-    int rc = ensure_scratch(c);
-    if (rc) return rc;
     long np = 0;
-    const long cap = (c->V + 255) / 256 + 1024;
-    if ((rc = ensure_part(c, (size_t)cap * 4))) return rc;
+    int rc = ensure_part(c, (size_t)plaq_partials(c->geom) * 4);
+    if (rc) return rc;
     HIPCHK(c, launch_plaq(c->geom, c->links[c->cur], c->mode, 1. / (double)c->nc, c->part, &np, c->scratch,
                           (long)c->scratch_doubles, c->st));
     HIPCHK(c, launch_finalize(c->part, np, 2, c->res, c->st));

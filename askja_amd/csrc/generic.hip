@@ -24,7 +24,8 @@ hipError_t small_project(const Geom &g, void *U, int mode, int haar, uint64_t se
     hipError_t team_drift_##NM(const Geom &g, void *U, const void *P, double eps, int level, const double *c, \
                                int mode, hipStream_t st);                                                     \
     hipError_t team_project_##NM(const Geom &g, void *U, int mode, int haar, uint64_t seed, uint64_t stream,    \
-                                 hipStream_t st);
+                                 hipStream_t st);                                                             \
+    hipError_t team_plaq_##NM(const Geom &g, const void *U, int mode, double inv_n, double *part, hipStream_t st);
 ASKJA_TEAM_DECL(8)
 ASKJA_TEAM_DECL(16)
 ASKJA_TEAM_DECL(32)
@@ -416,37 +417,6 @@ __device__ inline void gload(const R *U, const Geom &g, long l, double *A)
     for (int e = 0; e < nr; e++) A[e] = (double)lkc(U, g, l, e);
 }
 
-// plaquette sums, thread per site (grid-stride over sites, 4 work matrices)
-template <typename R>
-__global__ __launch_bounds__(kB) void k_plaq_g(Geom g, const R *U, double inv_n, double *scratch, long nthr,
-                                                double *part)
-{
-    const long t = (long)blockIdx.x * kB + threadIdx.x;
-    double s[2] = {0.0, 0.0}, c[2] = {0.0, 0.0};
-    if (t < nthr) {
-        const int n = g.N, nr = 2 * n * n;
-        double *A = scratch + (size_t)t * 4 * nr, *B = A + nr, *T1 = B + nr, *T2 = T1 + nr;
-        for (long x = t; x < g.V; x += nthr)
-            for (int j = 0; j < g.D; j++) {
-                const long xj = g.nbr[(2 * j) * g.V + x];
-                for (int k = j + 1; k < g.D; k++) {
-                    const long xk = g.nbr[(2 * k) * g.V + x];
-                    gload(U, g, j * g.V + x, A);
-                    gload(U, g, k * g.V + xj, B);
-                    gmul(A, false, B, false, T1, n);
-                    gload(U, g, j * g.V + xk, B);
-                    gmul(T1, false, B, true, T2, n);
-                    gload(U, g, k * g.V + x, B);
-                    gmul(T2, false, B, true, T1, n);
-                    double tr = 0.0;
-                    for (int i = 0; i < n; i++) tr = tr + T1[2 * (i * n + i)];
-                    kahan_add(s[j == 0 ? 0 : 1], c[j == 0 ? 0 : 1], 1.0 - tr * inv_n);
-                }
-            }
-    }
-    block_reduce_store<2>(s, c, part + 4 * (size_t)blockIdx.x);
-}
-
 template <typename R>
 __global__ __launch_bounds__(kB) void k_uviol_g(Geom g, const R *U, double *scratch, long nthr, double *part)
 {
@@ -581,6 +551,12 @@ static bool small_lattice_team(const Geom &g)
     return g.N >= 4 && g.N <= 8 && g.V * g.D < kRowsMinLinks;
 }
 
+long plaq_partials(const Geom &g)
+{
+    if (g.N <= 3) return (g.V + 255) / 256;
+    return g.V;
+}
+
 long kick_partials(const Geom &g)
 {
     if (g.N <= 3) return kick_site_blocks(g) * g.D;
@@ -642,14 +618,12 @@ hipError_t launch_plaq(const Geom &g, const void *U, int mode, double inv_n, dou
                        double *scratch, long scratch_doubles, hipStream_t st)
 {
     if (g.N <= 3) return small_plaq(g, U, mode, inv_n, part, np, st);
-    const long nthr = threads_for(g.V, 4L * 2 * g.N * g.N, scratch_doubles);
-    *np = nblk(nthr);
-    if (f32_mode(mode))
-        hipLaunchKernelGGL(k_plaq_g<float>, dim3(nblk(nthr)), dim3(kB), 0, st, g, (const float *)U, inv_n, scratch, nthr, part);
-    else
-        hipLaunchKernelGGL(k_plaq_g<double>, dim3(nblk(nthr)), dim3(kB), 0, st, g, (const double *)U, inv_n, scratch, nthr,
-                           part);
-    return hipGetLastError();
+    // one workgroup per site, products through LDS
+    *np = g.V;
+    if (g.N <= 8) return team_plaq_8(g, U, mode, inv_n, part, st);
+    if (g.N <= 16) return team_plaq_16(g, U, mode, inv_n, part, st);
+    if (g.N <= 32) return team_plaq_32(g, U, mode, inv_n, part, st);
+    return team_plaq_64(g, U, mode, inv_n, part, st);
 }
 
 hipError_t launch_uviol(const Geom &g, const void *U, int mode, double *part, long *np, double *scratch,

@@ -93,7 +93,9 @@ hipError_t launch_drift(const Geom &g, void *U, const void *P, double eps, int l
 hipError_t launch_finalize(const double *part, long nparts, int K, double *out, hipStream_t st);
 
 // Plaquette sums: {sum over (0,k) planes, sum over (j,k), 1 <= j < k} of
-// (1 - ReTr P / N).  Returns the number of partial blocks written.
+// (1 - ReTr P / N).  Returns the number of partial blocks written, at most
+// plaq_partials(g) (2 (s, c) pairs each).
+long plaq_partials(const Geom &g);
 hipError_t launch_plaq(const Geom &g, const void *U, int mode, double inv_n, double *part,
                        long *nparts, double *scratch, long scratch_doubles, hipStream_t st);
 // sum over links of ||U U^+ - 1||_F^2 (tr2 of U U^+ - 1)

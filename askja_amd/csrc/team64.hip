@@ -20,4 +20,9 @@ hipError_t team_project_64(const Geom &g, void *U, int mode, int haar, uint64_t 
     return team::project<64, 256>(g, U, mode, haar, seed, stream, st);
 }
 
+hipError_t team_plaq_64(const Geom &g, const void *U, int mode, double inv_n, double *part, hipStream_t st)
+{
+    return team::plaq<64, 256>(g, U, mode, inv_n, part, st);
+}
+
 }  // namespace askja

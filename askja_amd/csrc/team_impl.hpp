@@ -581,6 +581,77 @@ __global__ __launch_bounds__(NT) void k_project(Geom g, R *U, int haar, uint64_t
     }
 }
 
+// global link (matrix contiguous, stored as LS) -> LDS slot in FP64
+template <int NT, typename LS>
+__device__ inline void ld_link_f64(double *sre, double *sim, const LS *__restrict__ f, long l, int n)
+{
+    const int nr = 2 * n * n;
+    const LS *b = f + (size_t)l * nr;
+    for (int r = threadIdx.x; r < nr; r += NT) {
+        const double v = (double)b[r];
+        if (r & 1) sim[r >> 1] = v;
+        else sre[r >> 1] = v;
+    }
+}
+
+// Plaquette sums of one site per workgroup, FP64 arithmetic whatever the
+// storage (EnergyMag sun.cpp:507-537, split as PlaqExp
+// measurements4d.cpp:885-927): 1 - ReTr P_jk(x) / N over the temporal planes
+// (0, k) and, separately, the spatial planes (j, k), j >= 1, with
+// P_jk(x) = U_j(x) U_k(x+j) U_j(x+k)^+ U_k(x)^+.  Each thread adds
+// 1/N - P_ii/N for the diagonal elements it owns; the workgroup sum of
+// those is 1 - ReTr P / N.
+template <int NMAX, int NT, typename LS>
+__global__ __launch_bounds__(NT) void k_plaq(Geom g, const LS *U, double inv_n, double *part)
+{
+    using C = Cfg<NMAX, NT, double>;
+    constexpr int EPL = C::EPL;
+    __shared__ double Are[C::SLOT], Aim[C::SLOT], Bre[C::SLOT], Bim[C::SLOT];
+    const int n = g.N;
+    const long V = g.V;
+    const long x = blockIdx.x;
+    Own<EPL, NT> o;
+    o.init(n);
+    double s[2] = {0.0, 0.0}, c[2] = {0.0, 0.0};
+    for (int j = 0; j < g.D; j++) {
+        const long xj = g.nbr[(2 * j) * V + x];
+        for (int k = j + 1; k < g.D; k++) {
+            const long xk = g.nbr[(2 * k) * V + x];
+            double tr[EPL], ti[EPL];
+            __syncthreads();
+            ld_link_f64<NT>(Are, Aim, U, j * V + x, n);
+            ld_link_f64<NT>(Bre, Bim, U, k * V + xj, n);
+            __syncthreads();
+            mul<false, false, false, false>(o, Are, Aim, Bre, Bim, n, 1.0, tr, ti);
+            __syncthreads();
+            st_regs(o, Are, Aim, n, tr, ti);
+            ld_link_f64<NT>(Bre, Bim, U, j * V + xk, n);
+            __syncthreads();
+            mul<false, true, false, false>(o, Are, Aim, Bre, Bim, n, 1.0, tr, ti);
+            __syncthreads();
+            st_regs(o, Are, Aim, n, tr, ti);
+            ld_link_f64<NT>(Bre, Bim, U, k * V + x, n);
+            __syncthreads();
+            mul<false, true, false, false>(o, Are, Aim, Bre, Bim, n, 1.0, tr, ti);
+#pragma unroll
+            for (int q = 0; q < EPL; q++)
+                if (o.ok[q] && o.i[q] == o.j[q]) kahan_add(s[j == 0 ? 0 : 1], c[j == 0 ? 0 : 1], inv_n - tr[q] * inv_n);
+        }
+    }
+    block_reduce_store<2>(s, c, part + 4 * (size_t)blockIdx.x);
+}
+
+template <int NMAX, int NT>
+hipError_t plaq(const Geom &g, const void *U, int mode, double inv_n, double *part, hipStream_t st)
+{
+    const unsigned nb = (unsigned)g.V;
+    if (f32_mode(mode))
+        hipLaunchKernelGGL((k_plaq<NMAX, NT, float>), dim3(nb), dim3(NT), 0, st, g, (const float *)U, inv_n, part);
+    else
+        hipLaunchKernelGGL((k_plaq<NMAX, NT, double>), dim3(nb), dim3(NT), 0, st, g, (const double *)U, inv_n, part);
+    return hipGetLastError();
+}
+
 template <int NMAX, int NT>
 hipError_t project(const Geom &g, void *U, int mode, int haar, uint64_t seed, uint64_t stream, hipStream_t st)
 {
