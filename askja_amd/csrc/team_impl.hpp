@@ -28,6 +28,50 @@ __device__ inline void ld_link(R *sre, R *sim, const R *__restrict__ f, long l, 
     }
 }
 
+// Register staging of one link operand: fetch() issues the global loads of
+// link l (K = 2 NMAX^2 / NT values per thread), put() writes them to an LDS
+// slot.  The kick fetches the next operand before the current product and
+// puts it after, so that the loads run under the product instead of in
+// series with it.  That pays where a link is one small matrix (NMAX = 8, two
+// values per thread: the small lattices of 4 <= N <= 8, latency bound); for
+// the larger size classes the staging registers would cost occupancy or
+// spill, and there fetch() only records the link and put() loads it straight
+// into the slot.
+template <int NT, int K, typename R, bool Stage = (K <= 2)>
+struct Pref {
+    R v[K];
+    __device__ inline void fetch(const R *__restrict__ f, long l, int n)
+    {
+        const int nr = 2 * n * n;
+        const R *b = f + (size_t)l * nr;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int r = (int)threadIdx.x + k * NT;
+            v[k] = r < nr ? b[r] : R(0);
+        }
+    }
+    __device__ inline void put(R *sre, R *sim, int n) const
+    {
+        const int nr = 2 * n * n;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int r = (int)threadIdx.x + k * NT;
+            if (r < nr) {
+                if (r & 1) sim[r >> 1] = v[k];
+                else sre[r >> 1] = v[k];
+            }
+        }
+    }
+};
+
+template <int NT, int K, typename R>
+struct Pref<NT, K, R, false> {
+    const R *f = nullptr;
+    long l = 0;
+    __device__ inline void fetch(const R *__restrict__ f_, long l_, int) { f = f_, l = l_; }
+    __device__ inline void put(R *sre, R *sim, int n) const { ld_link<NT>(sre, sim, f, l, n); }
+};
+
 // Owned outputs: element index, row, column, validity.
 template <int EPL, int NT>
 struct Own {
@@ -199,27 +243,45 @@ __global__ __launch_bounds__(NT) void k_kick(KickArgs a)
     R wr[EPL], wi[EPL], tr[EPL], ti[EPL];
 #pragma unroll
     for (int q = 0; q < EPL; q++) wr[q] = wi[q] = R(0);
-    for (int nu = 0; nu < g.D; nu++) {
-        if (nu == mu) continue;
+    // the staple operands in the order the products consume them; pa / pb
+    // hold the next A / B operand (see Pref)
+    constexpr int K = (2 * NMAX * NMAX + NT - 1) / NT;
+    Pref<NT, K, R> pa, pb;
+    int nu = mu == 0 ? 1 : 0;
+    if (nu < g.D) {
+        if constexpr (Exact) pa.fetch(U, mu * V + s, n);
+        else pa.fetch(U, nu * V + xp, n);
+        if constexpr (Exact) pb.fetch(U, nu * V + xp, n);
+        else pb.fetch(U, mu * V + g.nbr[(2 * nu) * V + s], n);
+    } else if constexpr (!Exact) {
+        pb.fetch(U, mu * V + s, n);
+    }
+    while (nu < g.D) {
+        int nx = nu + 1;
+        if (nx == mu) nx++;
         const long xpn = g.nbr[(2 * nu) * V + s];
         const long xmn = g.nbr[(2 * nu + 1) * V + s];
         const long xpmn = g.nbr[(2 * nu + 1) * V + xp];
         if constexpr (Exact) {
             // ((U_mu(x) U_nu(x+mu)) U_mu(x+nu)^+) U_nu(x)^+
             __syncthreads();
-            ld_link<NT>(Are, Aim, U, mu * V + s, n);
-            ld_link<NT>(Bre, Bim, U, nu * V + xp, n);
+            pa.put(Are, Aim, n);
+            pb.put(Bre, Bim, n);
             __syncthreads();
+            pb.fetch(U, mu * V + xpn, n);
             mul<false, false, false, true>(o, Are, Aim, Bre, Bim, n, R(1), tr, ti);
             __syncthreads();
             st_regs(o, Are, Aim, n, tr, ti);
-            ld_link<NT>(Bre, Bim, U, mu * V + xpn, n);
+            pb.put(Bre, Bim, n);
             __syncthreads();
+            pb.fetch(U, nu * V + s, n);
             mul<false, true, false, true>(o, Are, Aim, Bre, Bim, n, R(1), tr, ti);
             __syncthreads();
             st_regs(o, Are, Aim, n, tr, ti);
-            ld_link<NT>(Bre, Bim, U, nu * V + s, n);
+            pb.put(Bre, Bim, n);
             __syncthreads();
+            pa.fetch(U, mu * V + s, n);
+            pb.fetch(U, nu * V + xpmn, n);
             mul<false, true, false, true>(o, Are, Aim, Bre, Bim, n, R(1), tr, ti);
             {
 #pragma clang fp contract(off)
@@ -231,19 +293,25 @@ __global__ __launch_bounds__(NT) void k_kick(KickArgs a)
             }
             // ((U_mu(x) U_nu(x+mu-nu)^+) U_mu(x-nu)^+) U_nu(x-nu)
             __syncthreads();
-            ld_link<NT>(Are, Aim, U, mu * V + s, n);
-            ld_link<NT>(Bre, Bim, U, nu * V + xpmn, n);
+            pa.put(Are, Aim, n);
+            pb.put(Bre, Bim, n);
             __syncthreads();
+            pb.fetch(U, mu * V + xmn, n);
             mul<false, true, false, true>(o, Are, Aim, Bre, Bim, n, R(1), tr, ti);
             __syncthreads();
             st_regs(o, Are, Aim, n, tr, ti);
-            ld_link<NT>(Bre, Bim, U, mu * V + xmn, n);
+            pb.put(Bre, Bim, n);
             __syncthreads();
+            pb.fetch(U, nu * V + xmn, n);
             mul<false, true, false, true>(o, Are, Aim, Bre, Bim, n, R(1), tr, ti);
             __syncthreads();
             st_regs(o, Are, Aim, n, tr, ti);
-            ld_link<NT>(Bre, Bim, U, nu * V + xmn, n);
+            pb.put(Bre, Bim, n);
             __syncthreads();
+            if (nx < g.D) {
+                pa.fetch(U, mu * V + s, n);
+                pb.fetch(U, nx * V + xp, n);
+            }
             mul<false, false, false, true>(o, Are, Aim, Bre, Bim, n, R(1), tr, ti);
             {
 #pragma clang fp contract(off)
@@ -256,33 +324,44 @@ __global__ __launch_bounds__(NT) void k_kick(KickArgs a)
         } else {
             // S += U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+
             __syncthreads();
-            ld_link<NT>(Are, Aim, U, nu * V + xp, n);
-            ld_link<NT>(Bre, Bim, U, mu * V + xpn, n);
+            pa.put(Are, Aim, n);
+            pb.put(Bre, Bim, n);
             __syncthreads();
+            pb.fetch(U, nu * V + s, n);
             mul<false, true, false, false>(o, Are, Aim, Bre, Bim, n, R(1), tr, ti);
             __syncthreads();
             st_regs(o, Are, Aim, n, tr, ti);
-            ld_link<NT>(Bre, Bim, U, nu * V + s, n);
+            pb.put(Bre, Bim, n);
             __syncthreads();
+            pa.fetch(U, nu * V + xpmn, n);
+            pb.fetch(U, mu * V + xmn, n);
             mul<false, true, true, false>(o, Are, Aim, Bre, Bim, n, R(1), wr, wi);
             // S += U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu)
             __syncthreads();
-            ld_link<NT>(Are, Aim, U, nu * V + xpmn, n);
-            ld_link<NT>(Bre, Bim, U, mu * V + xmn, n);
+            pa.put(Are, Aim, n);
+            pb.put(Bre, Bim, n);
             __syncthreads();
+            pb.fetch(U, nu * V + xmn, n);
             mul<true, true, false, false>(o, Are, Aim, Bre, Bim, n, R(1), tr, ti);
             __syncthreads();
             st_regs(o, Are, Aim, n, tr, ti);
-            ld_link<NT>(Bre, Bim, U, nu * V + xmn, n);
+            pb.put(Bre, Bim, n);
             __syncthreads();
+            if (nx < g.D) {
+                pa.fetch(U, nx * V + xp, n);
+                pb.fetch(U, mu * V + g.nbr[(2 * nx) * V + s], n);
+            } else {
+                pb.fetch(U, mu * V + s, n);  // U_mu(x) for W = U_mu(x) S
+            }
             mul<false, false, true, false>(o, Are, Aim, Bre, Bim, n, R(1), wr, wi);
         }
+        nu = nx;
     }
     if constexpr (!Exact) {
         // W = U_mu(x) S
         __syncthreads();
         st_regs(o, Are, Aim, n, wr, wi);
-        ld_link<NT>(Bre, Bim, U, mu * V + s, n);
+        pb.put(Bre, Bim, n);
         __syncthreads();
         mul<false, false, false, false>(o, Bre, Bim, Are, Aim, n, R(1), tr, ti);
 #pragma unroll
