@@ -25,7 +25,9 @@ hipError_t small_project(const Geom &g, void *U, int mode, int haar, uint64_t se
                                int mode, hipStream_t st);                                                     \
     hipError_t team_project_##NM(const Geom &g, void *U, int mode, int haar, uint64_t seed, uint64_t stream,    \
                                  hipStream_t st);                                                             \
-    hipError_t team_plaq_##NM(const Geom &g, const void *U, int mode, double inv_n, double *part, hipStream_t st);
+    hipError_t team_plaq_##NM(const Geom &g, const void *U, int mode, double inv_n, double *part, hipStream_t st); \
+    hipError_t team_lines_##NM(const Geom &g, const void *U, int dir, long nlines, long lex1, int mode,         \
+                               double *traces, double *wfull, hipStream_t st);
 ASKJA_TEAM_DECL(8)
 ASKJA_TEAM_DECL(16)
 ASKJA_TEAM_DECL(32)
@@ -473,40 +475,9 @@ __global__ __launch_bounds__(kB) void k_gauss_g(Geom g, const R *U, const R *P, 
     }
 }
 
-template <typename R>
-__global__ void k_wlines_g(Geom g, const R *U, int dir, long q0, long nq, double *scratch, double *out)
-{
-    const long t = (long)blockIdx.x * kB + threadIdx.x;
-    if (t >= nq) return;
-    const long q = q0 + t;
-    const int n = g.N, nr = 2 * n * n;
-    double *W = scratch + (size_t)t * 3 * nr, *B = W + nr, *T = B + nr;
-    long rem = q, lex = 0;
-    for (int d = g.D - 1; d >= 0; d--) {
-        if (d == dir) continue;
-        const long x = rem % g.ext[d];
-        rem /= g.ext[d];
-        lex += x * g.helparr[d];
-    }
-    long s = g.lex2dev[lex];
-    gload(U, g, dir * g.V + s, W);
-    for (int k = 1; k < g.ext[dir]; k++) {
-        s = g.nbr[(2 * dir) * g.V + s];
-        gload(U, g, dir * g.V + s, B);
-        gmul(W, false, B, false, T, n);
-        for (int e = 0; e < nr; e++) W[e] = T[e];
-    }
-    double re = 0.0, im = 0.0;
-    for (int i = 0; i < n; i++) {
-        re = re + W[2 * (i * n + i)];
-        im = im + W[2 * (i * n + i) + 1];
-    }
-    out[2 * q] = re / n;
-    out[2 * q + 1] = im / n;
-}
-
-// Full product matrix of one straight line (the matrix W of WilsonLoop,
-// sun.cpp:307-345), one thread, FP64 in the reference's product order.
+// Full product matrix of one straight line for N <= 3 (the matrix W of
+// WilsonLoop, sun.cpp:307-345), one thread, FP64 in the reference's product
+// order.
 template <typename R>
 __global__ void k_line_matrix(Geom g, const R *U, int dir, long lex, double *scratch, double *out)
 {
@@ -675,29 +646,27 @@ hipError_t launch_kinetic(const Geom &g, const void *P, const void *Q, int mode,
     return hipGetLastError();
 }
 
+// straight-line products of 4 <= N <= 64: one workgroup per line
+static hipError_t team_lines(const Geom &g, const void *U, int dir, long nlines, long lex1, int mode, double *traces,
+                             double *wfull, hipStream_t st)
+{
+    if (g.N <= 8) return team_lines_8(g, U, dir, nlines, lex1, mode, traces, wfull, st);
+    if (g.N <= 16) return team_lines_16(g, U, dir, nlines, lex1, mode, traces, wfull, st);
+    if (g.N <= 32) return team_lines_32(g, U, dir, nlines, lex1, mode, traces, wfull, st);
+    return team_lines_64(g, U, dir, nlines, lex1, mode, traces, wfull, st);
+}
+
 hipError_t launch_wlines(const Geom &g, const void *U, int dir, int mode, double *out, double *scratch,
                          long scratch_doubles, hipStream_t st)
 {
     if (g.N <= 3) return small_wlines(g, U, dir, mode, out, st);
-    const long nlines = g.V / g.ext[dir];
-    const long chunk = threads_for(nlines, 3L * 2 * g.N * g.N, scratch_doubles);
-    for (long q0 = 0; q0 < nlines; q0 += chunk) {
-        const long nq = nlines - q0 < chunk ? nlines - q0 : chunk;
-        if (f32_mode(mode))
-            hipLaunchKernelGGL(k_wlines_g<float>, dim3(nblk(nq)), dim3(kB), 0, st, g, (const float *)U, dir, q0, nq,
-                               scratch, out);
-        else
-            hipLaunchKernelGGL(k_wlines_g<double>, dim3(nblk(nq)), dim3(kB), 0, st, g, (const double *)U, dir, q0, nq,
-                               scratch, out);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-    }
-    return hipSuccess;
+    return team_lines(g, U, dir, g.V / g.ext[dir], -1, mode, out, nullptr, st);
 }
 
 hipError_t launch_line_matrix(const Geom &g, const void *U, int dir, long lex, int mode, double *scratch, double *out,
                               hipStream_t st)
 {
+    if (g.N >= 4) return team_lines(g, U, dir, 1, lex, mode, nullptr, out, st);
     if (f32_mode(mode))
         hipLaunchKernelGGL(k_line_matrix<float>, dim3(1), dim3(64), 0, st, g, (const float *)U, dir, lex, scratch, out);
     else hipLaunchKernelGGL(k_line_matrix<double>, dim3(1), dim3(64), 0, st, g, (const double *)U, dir, lex, scratch, out);

@@ -25,4 +25,10 @@ hipError_t team_plaq_64(const Geom &g, const void *U, int mode, double inv_n, do
     return team::plaq<64, 256>(g, U, mode, inv_n, part, st);
 }
 
+hipError_t team_lines_64(const Geom &g, const void *U, int dir, long nlines, long lex1, int mode, double *traces,
+                      double *wfull, hipStream_t st)
+{
+    return team::lines<64, 256>(g, U, dir, nlines, lex1, mode, traces, wfull, st);
+}
+
 }  // namespace askja

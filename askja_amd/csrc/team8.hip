@@ -25,4 +25,10 @@ hipError_t team_plaq_8(const Geom &g, const void *U, int mode, double inv_n, dou
     return team::plaq<8, 64>(g, U, mode, inv_n, part, st);
 }
 
+hipError_t team_lines_8(const Geom &g, const void *U, int dir, long nlines, long lex1, int mode, double *traces,
+                      double *wfull, hipStream_t st)
+{
+    return team::lines<8, 64>(g, U, dir, nlines, lex1, mode, traces, wfull, st);
+}
+
 }  // namespace askja

@@ -720,6 +720,75 @@ __global__ __launch_bounds__(NT) void k_plaq(Geom g, const LS *U, double inv_n, 
     block_reduce_store<2>(s, c, part + 4 * (size_t)blockIdx.x);
 }
 
+// Straight-line products W = U_dir(x) U_dir(x + dir) ... U_dir(x + (L-1) dir)
+// (WilsonLoop sun.cpp:307-345, PolyakovLoop measurements4d.cpp:410-466), one
+// workgroup per line, FP64 in the reference's product order (GSL zgemm: k
+// ascending, no contraction).  Line q = blockIdx.x starts at the site with
+// x_dir = 0 and the other coordinates of q in lexicographic order, or, with
+// lex1 >= 0, at lexicographic site lex1.  traces (may be null): Tr W / N as
+// (re, im) at 2q; wfull (may be null): W, row-major complex.
+template <int NMAX, int NT, typename LS>
+__global__ __launch_bounds__(NT) void k_line(Geom g, const LS *U, int dir, long lex1, double *traces, double *wfull)
+{
+    using C = Cfg<NMAX, NT, double>;
+    constexpr int EPL = C::EPL;
+    __shared__ double Are[C::SLOT], Aim[C::SLOT], Bre[C::SLOT], Bim[C::SLOT];
+    const int n = g.N;
+    const long V = g.V, q = blockIdx.x;
+    long lex = lex1;
+    if (lex < 0) {
+        long rem = q;
+        lex = 0;
+        for (int d = g.D - 1; d >= 0; d--) {
+            if (d == dir) continue;
+            lex += (rem % g.ext[d]) * g.helparr[d];
+            rem /= g.ext[d];
+        }
+    }
+    long s = g.lex2dev[lex];
+    Own<EPL, NT> o;
+    o.init(n);
+    ld_link_f64<NT>(Are, Aim, U, dir * V + s, n);
+    double wr[EPL], wi[EPL];
+    for (int k = 1; k < g.ext[dir]; k++) {
+        s = g.nbr[(2 * dir) * V + s];
+        __syncthreads();
+        ld_link_f64<NT>(Bre, Bim, U, dir * V + s, n);
+        __syncthreads();
+        mul<false, false, false, true>(o, Are, Aim, Bre, Bim, n, 1.0, wr, wi);
+        __syncthreads();
+        st_regs(o, Are, Aim, n, wr, wi);
+    }
+    __syncthreads();
+    if (wfull)
+        for (int e = threadIdx.x; e < n * n; e += NT) {
+            wfull[2 * e] = Are[e];
+            wfull[2 * e + 1] = Aim[e];
+        }
+    if (traces && threadIdx.x == 0) {
+        double re = 0.0, im = 0.0;
+        for (int i = 0; i < n; i++) {
+            re = re + Are[i * n + i];
+            im = im + Aim[i * n + i];
+        }
+        traces[2 * q] = re / n;
+        traces[2 * q + 1] = im / n;
+    }
+}
+
+template <int NMAX, int NT>
+hipError_t lines(const Geom &g, const void *U, int dir, long nlines, long lex1, int mode, double *traces,
+                 double *wfull, hipStream_t st)
+{
+    if (f32_mode(mode))
+        hipLaunchKernelGGL((k_line<NMAX, NT, float>), dim3((unsigned)nlines), dim3(NT), 0, st, g, (const float *)U, dir,
+                           lex1, traces, wfull);
+    else
+        hipLaunchKernelGGL((k_line<NMAX, NT, double>), dim3((unsigned)nlines), dim3(NT), 0, st, g, (const double *)U,
+                           dir, lex1, traces, wfull);
+    return hipGetLastError();
+}
+
 template <int NMAX, int NT>
 hipError_t plaq(const Geom &g, const void *U, int mode, double inv_n, double *part, hipStream_t st)
 {
