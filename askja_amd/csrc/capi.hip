@@ -768,7 +768,7 @@ int askja_gauss_law(askja_ctx *c, double *out)
     if (rc) return rc;
     const int n = c->nc, nr = 2 * n * n;
     const size_t small_need = (size_t)((c->NL + 255) / 256) * 2 * nr;
-    const size_t gen_need = c->scratch_doubles / 3 + 2 * (size_t)nr;
+    const size_t gen_need = (size_t)gauss_rows(c->geom) * 2 * nr;
     if ((rc = ensure_part(c, n <= 3 ? small_need : gen_need))) return rc;
     double *dout = nullptr;
     HIPCHK(c, hipMallocAsync((void **)&dout, (size_t)2 * nr * sizeof(double), c->st));

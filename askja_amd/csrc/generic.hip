@@ -27,7 +27,10 @@ hipError_t small_project(const Geom &g, void *U, int mode, int haar, uint64_t se
                                  hipStream_t st);                                                             \
     hipError_t team_plaq_##NM(const Geom &g, const void *U, int mode, double inv_n, double *part, hipStream_t st); \
     hipError_t team_lines_##NM(const Geom &g, const void *U, int dir, long nlines, long lex1, int mode,         \
-                               double *traces, double *wfull, hipStream_t st);
+                               double *traces, double *wfull, hipStream_t st);                                \
+    hipError_t team_uviol_##NM(const Geom &g, const void *U, int mode, long nwg, double *part, hipStream_t st); \
+    hipError_t team_gauss_##NM(const Geom &g, const void *U, const void *P, int mode, long nwg, double *part,   \
+                               hipStream_t st);
 ASKJA_TEAM_DECL(8)
 ASKJA_TEAM_DECL(16)
 ASKJA_TEAM_DECL(32)
@@ -419,62 +422,6 @@ __device__ inline void gload(const R *U, const Geom &g, long l, double *A)
     for (int e = 0; e < nr; e++) A[e] = (double)lkc(U, g, l, e);
 }
 
-template <typename R>
-__global__ __launch_bounds__(kB) void k_uviol_g(Geom g, const R *U, double *scratch, long nthr, double *part)
-{
-    const long t = (long)blockIdx.x * kB + threadIdx.x;
-    double s[1] = {0.0}, c[1] = {0.0};
-    if (t < nthr) {
-        const int n = g.N, nr = 2 * n * n;
-        double *A = scratch + (size_t)t * 4 * nr, *T = A + nr, *Q = T + nr;
-        for (long l = t; l < g.V * g.D; l += nthr) {
-            gload(U, g, l, A);
-            gmul(A, false, A, true, T, n);
-            for (int i = 0; i < n; i++) T[2 * (i * n + i)] -= 1.0;
-            gmul(T, false, T, false, Q, n);
-            double tr = 0.0;
-            for (int i = 0; i < n; i++) tr = tr + Q[2 * (i * n + i)];
-            kahan_add(s[0], c[0], tr);
-        }
-    }
-    block_reduce_store<1>(s, c, part + 2 * (size_t)blockIdx.x);
-}
-
-// Gauss law: per thread compensated matrix accumulator of P - U^+ P U in its
-// scratch slice (2N^2 (s, c) pairs, then 4 work matrices), then column sums
-// over threads.
-template <typename R>
-__global__ __launch_bounds__(kB) void k_gauss_g(Geom g, const R *U, const R *P, double *scratch, long nthr)
-{
-    const long t = (long)blockIdx.x * kB + threadIdx.x;
-    if (t >= nthr) return;
-    const int n = g.N, nr = 2 * n * n;
-    double *acc = scratch + (size_t)t * 6 * nr;
-    double *A = acc + 2 * nr, *Pm = A + nr, *T = Pm + nr, *W = T + nr;
-    for (int e = 0; e < 2 * nr; e++) acc[e] = 0.0;
-    for (long l = t; l < g.V * g.D; l += nthr) {
-        gload(U, g, l, A);
-        for (int a = 0; a < n; a++)
-            for (int b = 0; b < n; b++) {
-                double re, im;
-                if (a == b) {
-                    re = (double)mk(const_cast<R *>(P), g, l, a);
-                    im = 0.0;
-                } else {
-                    const int h = hup(n, a < b ? a : b, a < b ? b : a);
-                    re = (double)mk(const_cast<R *>(P), g, l, h);
-                    im = (double)mk(const_cast<R *>(P), g, l, h + 1);
-                    if (a > b) im = -im;
-                }
-                Pm[2 * (a * n + b)] = re;
-                Pm[2 * (a * n + b) + 1] = im;
-            }
-        gmul(A, true, Pm, false, T, n);
-        gmul(T, false, A, false, W, n);
-        for (int e = 0; e < nr; e++) kahan_add(acc[2 * e], acc[2 * e + 1], Pm[e] - W[e]);
-    }
-}
-
 // Full product matrix of one straight line for N <= 3 (the matrix W of
 // WilsonLoop, sun.cpp:307-345), one thread, FP64 in the reference's product
 // order.
@@ -597,40 +544,41 @@ hipError_t launch_plaq(const Geom &g, const void *U, int mode, double inv_n, dou
     return team_plaq_64(g, U, mode, inv_n, part, st);
 }
 
+long gauss_rows(const Geom &g)
+{
+    const long nl = g.V * g.D;
+    return nl < 256 ? nl : 256;
+}
+
 hipError_t launch_uviol(const Geom &g, const void *U, int mode, double *part, long *np, double *scratch,
                         long scratch_doubles, hipStream_t st)
 {
     if (g.N <= 3) return small_uviol(g, U, mode, part, np, st);
-    const long nthr = threads_for(g.V * g.D, 4L * 2 * g.N * g.N, scratch_doubles);
-    *np = nblk(nthr);
-    if (f32_mode(mode))
-        hipLaunchKernelGGL(k_uviol_g<float>, dim3(nblk(nthr)), dim3(kB), 0, st, g, (const float *)U, scratch, nthr, part);
-    else
-        hipLaunchKernelGGL(k_uviol_g<double>, dim3(nblk(nthr)), dim3(kB), 0, st, g, (const double *)U, scratch, nthr, part);
-    return hipGetLastError();
+    // workgroups over the links (grid-stride), one (s, c) pair each
+    const long nl = g.V * g.D, nwg = nl < 1024 ? nl : 1024;
+    *np = nwg;
+    if (g.N <= 8) return team_uviol_8(g, U, mode, nwg, part, st);
+    if (g.N <= 16) return team_uviol_16(g, U, mode, nwg, part, st);
+    if (g.N <= 32) return team_uviol_32(g, U, mode, nwg, part, st);
+    return team_uviol_64(g, U, mode, nwg, part, st);
 }
 
 hipError_t launch_gauss(const Geom &g, const void *U, const void *P, int mode, double *part, long *np,
                         double *scratch, long scratch_doubles, double *out, hipStream_t st)
 {
     if (g.N <= 3) return small_gauss(g, U, P, mode, part, np, out, st);
+    // gauss_rows(g) workgroups over the links, one row of 2N^2 (s, c) pairs
+    // each, then the column sums
     const int nr = 2 * g.N * g.N;
-    const long nthr = threads_for(g.V * g.D, 6L * nr, scratch_doubles);
-    *np = nthr;
-    if (f32_mode(mode))
-        hipLaunchKernelGGL(k_gauss_g<float>, dim3(nblk(nthr)), dim3(kB), 0, st, g, (const float *)U, (const float *)P,
-                           scratch, nthr);
-    else
-        hipLaunchKernelGGL(k_gauss_g<double>, dim3(nblk(nthr)), dim3(kB), 0, st, g, (const double *)U,
-                           (const double *)P, scratch, nthr);
-    hipError_t e = hipGetLastError();
+    const long nwg = gauss_rows(g);
+    *np = nwg;
+    hipError_t e;
+    if (g.N <= 8) e = team_gauss_8(g, U, P, mode, nwg, part, st);
+    else if (g.N <= 16) e = team_gauss_16(g, U, P, mode, nwg, part, st);
+    else if (g.N <= 32) e = team_gauss_32(g, U, P, mode, nwg, part, st);
+    else e = team_gauss_64(g, U, P, mode, nwg, part, st);
     if (e != hipSuccess) return e;
-    // accumulators are strided by 5*nr doubles: gather the (s, c) pairs into
-    // `part` as np rows of nr pairs, then column-reduce.
-    hipError_t e2 = hipMemcpy2DAsync(part, (size_t)2 * nr * sizeof(double), scratch, (size_t)6 * nr * sizeof(double),
-                                     (size_t)2 * nr * sizeof(double), (size_t)nthr, hipMemcpyDeviceToDevice, st);
-    if (e2 != hipSuccess) return e2;
-    return launch_finalize(part, nthr, nr, out, st);
+    return launch_finalize(part, nwg, nr, out, st);
 }
 
 hipError_t launch_kinetic(const Geom &g, const void *P, const void *Q, int mode, double *part, long *np,

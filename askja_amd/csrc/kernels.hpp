@@ -98,6 +98,8 @@ hipError_t launch_finalize(const double *part, long nparts, int K, double *out, 
 long plaq_partials(const Geom &g);
 hipError_t launch_plaq(const Geom &g, const void *U, int mode, double inv_n, double *part,
                        long *nparts, double *scratch, long scratch_doubles, hipStream_t st);
+// Rows of the Gauss-law partials for N >= 4 (2N^2 (s, c) pairs each).
+long gauss_rows(const Geom &g);
 // sum over links of ||U U^+ - 1||_F^2 (tr2 of U U^+ - 1)
 hipError_t launch_uviol(const Geom &g, const void *U, int mode, double *part, long *nparts,
                         double *scratch, long scratch_doubles, hipStream_t st);

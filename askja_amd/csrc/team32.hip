@@ -31,4 +31,14 @@ hipError_t team_lines_32(const Geom &g, const void *U, int dir, long nlines, lon
     return team::lines<32, 256>(g, U, dir, nlines, lex1, mode, traces, wfull, st);
 }
 
+hipError_t team_uviol_32(const Geom &g, const void *U, int mode, long nwg, double *part, hipStream_t st)
+{
+    return team::uviol<32, 256>(g, U, mode, nwg, part, st);
+}
+
+hipError_t team_gauss_32(const Geom &g, const void *U, const void *P, int mode, long nwg, double *part, hipStream_t st)
+{
+    return team::gauss<32, 256>(g, U, P, mode, nwg, part, st);
+}
+
 }  // namespace askja

@@ -31,4 +31,14 @@ hipError_t team_lines_8(const Geom &g, const void *U, int dir, long nlines, long
     return team::lines<8, 64>(g, U, dir, nlines, lex1, mode, traces, wfull, st);
 }
 
+hipError_t team_uviol_8(const Geom &g, const void *U, int mode, long nwg, double *part, hipStream_t st)
+{
+    return team::uviol<8, 64>(g, U, mode, nwg, part, st);
+}
+
+hipError_t team_gauss_8(const Geom &g, const void *U, const void *P, int mode, long nwg, double *part, hipStream_t st)
+{
+    return team::gauss<8, 64>(g, U, P, mode, nwg, part, st);
+}
+
 }  // namespace askja

@@ -789,6 +789,119 @@ hipError_t lines(const Geom &g, const void *U, int dir, long nlines, long lex1, 
     return hipGetLastError();
 }
 
+// UViol (sun.cpp:441-471): sum over links of tr2(U U^+ - 1) = ReTr(T T) with
+// T = U U^+ - 1, products in the reference's order (FP64).  Workgroups take
+// the links l = blockIdx.x, blockIdx.x + gridDim.x, ...; each thread adds
+// the diagonal elements of T T it owns; one (s, c) pair per workgroup.
+template <int NMAX, int NT, typename LS>
+__global__ __launch_bounds__(NT) void k_uviol(Geom g, const LS *U, double *part)
+{
+    using C = Cfg<NMAX, NT, double>;
+    constexpr int EPL = C::EPL;
+    __shared__ double Are[C::SLOT], Aim[C::SLOT], Bre[C::SLOT], Bim[C::SLOT];
+    const int n = g.N;
+    Own<EPL, NT> o;
+    o.init(n);
+    double s[1] = {0.0}, c[1] = {0.0};
+    for (long l = blockIdx.x; l < g.V * g.D; l += gridDim.x) {
+        double tr[EPL], ti[EPL];
+        __syncthreads();
+        ld_link_f64<NT>(Are, Aim, U, l, n);
+        __syncthreads();
+        mul<false, true, false, true>(o, Are, Aim, Are, Aim, n, 1.0, tr, ti);
+#pragma unroll
+        for (int q = 0; q < EPL; q++)
+            if (o.i[q] == o.j[q]) tr[q] -= 1.0;
+        st_regs(o, Bre, Bim, n, tr, ti);
+        __syncthreads();
+        mul<false, false, false, true>(o, Bre, Bim, Bre, Bim, n, 1.0, tr, ti);
+#pragma unroll
+        for (int q = 0; q < EPL; q++)
+            if (o.ok[q] && o.i[q] == o.j[q]) kahan_add(s[0], c[0], tr[q]);
+    }
+    block_reduce_store<1>(s, c, part + 2 * (size_t)blockIdx.x);
+}
+
+// GaussLaw (sun.cpp:415-438): the matrix sum over links of P_l - U_l^+ P_l U_l
+// (P_l the Hermitian momentum of link l; summed over all sites, the
+// reference's U^+(x-mu) P(x-mu) U(x-mu) runs over every link once), products
+// in the reference's order (FP64).  Workgroups take the links
+// l = blockIdx.x, blockIdx.x + gridDim.x, ...; each thread keeps compensated
+// sums of the elements it owns and writes them to row blockIdx.x of part (2N^2
+// (s, c) pairs: element (i, j) at 2 (i N + j), re then im), for a column
+// reduction over the rows.
+template <int NMAX, int NT, typename LS>
+__global__ __launch_bounds__(NT) void k_gauss(Geom g, const LS *U, const LS *P, double *part)
+{
+    using C = Cfg<NMAX, NT, double>;
+    constexpr int EPL = C::EPL;
+    __shared__ double Are[C::SLOT], Aim[C::SLOT], Bre[C::SLOT], Bim[C::SLOT];
+    const int n = g.N;
+    Own<EPL, NT> o;
+    o.init(n);
+    double sr[EPL], cr[EPL], si[EPL], ci[EPL];
+#pragma unroll
+    for (int q = 0; q < EPL; q++) sr[q] = cr[q] = si[q] = ci[q] = 0.0;
+    for (long l = blockIdx.x; l < g.V * g.D; l += gridDim.x) {
+        double pr[EPL], pi[EPL], tr[EPL], ti[EPL];
+        __syncthreads();
+        ld_link_f64<NT>(Are, Aim, U, l, n);
+#pragma unroll
+        for (int q = 0; q < EPL; q++) {
+            pr[q] = pi[q] = 0.0;
+            if (!o.ok[q]) continue;
+            LS re, im;
+            mom_get(P, l, n, o.i[q], o.j[q], re, im);
+            pr[q] = (double)re;
+            pi[q] = (double)im;
+        }
+        st_regs(o, Bre, Bim, n, pr, pi);
+        __syncthreads();
+        mul<true, false, false, true>(o, Are, Aim, Bre, Bim, n, 1.0, tr, ti);  // U^+ P
+        __syncthreads();
+        st_regs(o, Bre, Bim, n, tr, ti);  // P itself stays in pr, pi
+        __syncthreads();
+        mul<false, false, false, true>(o, Bre, Bim, Are, Aim, n, 1.0, tr, ti);  // (U^+ P) U
+#pragma unroll
+        for (int q = 0; q < EPL; q++) {
+            kahan_add(sr[q], cr[q], pr[q] - tr[q]);
+            kahan_add(si[q], ci[q], pi[q] - ti[q]);
+        }
+    }
+    const int nr = 2 * n * n;
+    double *row = part + (size_t)blockIdx.x * 2 * nr;
+#pragma unroll
+    for (int q = 0; q < EPL; q++)
+        if (o.ok[q]) {
+            const int e = 2 * (o.i[q] * n + o.j[q]);
+            row[2 * e] = sr[q];
+            row[2 * e + 1] = cr[q];
+            row[2 * (e + 1)] = si[q];
+            row[2 * (e + 1) + 1] = ci[q];
+        }
+}
+
+template <int NMAX, int NT>
+hipError_t uviol(const Geom &g, const void *U, int mode, long nwg, double *part, hipStream_t st)
+{
+    if (f32_mode(mode))
+        hipLaunchKernelGGL((k_uviol<NMAX, NT, float>), dim3((unsigned)nwg), dim3(NT), 0, st, g, (const float *)U, part);
+    else hipLaunchKernelGGL((k_uviol<NMAX, NT, double>), dim3((unsigned)nwg), dim3(NT), 0, st, g, (const double *)U, part);
+    return hipGetLastError();
+}
+
+template <int NMAX, int NT>
+hipError_t gauss(const Geom &g, const void *U, const void *P, int mode, long nwg, double *part, hipStream_t st)
+{
+    if (f32_mode(mode))
+        hipLaunchKernelGGL((k_gauss<NMAX, NT, float>), dim3((unsigned)nwg), dim3(NT), 0, st, g, (const float *)U,
+                           (const float *)P, part);
+    else
+        hipLaunchKernelGGL((k_gauss<NMAX, NT, double>), dim3((unsigned)nwg), dim3(NT), 0, st, g, (const double *)U,
+                           (const double *)P, part);
+    return hipGetLastError();
+}
+
 template <int NMAX, int NT>
 hipError_t plaq(const Geom &g, const void *U, int mode, double inv_n, double *part, hipStream_t st)
 {
