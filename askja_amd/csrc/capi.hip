@@ -105,11 +105,8 @@ int ensure_part(askja_ctx *c, size_t nd)
 int ensure_scratch(askja_ctx *c)
 {
     if (c->scratch) return ASKJA_OK;
-    // enough for one FP64 working set per link, capped at 1 GiB
-    size_t want = (size_t)c->NL * (size_t)generic_scratch_per_thread(c->nc);
-    const size_t cap = (size_t)1 << 27, lo = (size_t)1 << 20;
-    if (want > cap) want = cap;
-    if (want < lo) want = lo;
+    // two FP64 N x N work matrices (the line matrix of N <= 3)
+    const size_t want = (size_t)4 * c->nc * c->nc;
     HIPCHK(c, hipMalloc(&c->scratch, want * sizeof(double)));
     c->scratch_doubles = want;
     return ASKJA_OK;
@@ -201,8 +198,7 @@ int exact_action(askja_ctx *c, const void *U, double *s)
     }
     long np = 0;
     if ((rc = ensure_part(c, (size_t)plaq_partials(c->geom) * 4))) return rc;
-    HIPCHK(c, launch_plaq(c->geom, U, c->mode, 1. / (double)c->nc, c->part, &np, c->scratch, (long)c->scratch_doubles,
-                          c->st));
+    HIPCHK(c, launch_plaq(c->geom, U, c->mode, 1. / (double)c->nc, c->part, &np, c->st));
     HIPCHK(c, launch_finalize(c->part, np, 2, c->res, c->st));
     double h[4];
     if ((rc = read_res(c, h, 4))) return rc;
@@ -501,19 +497,14 @@ int askja_links_cold(askja_ctx *c)
 int askja_links_haar(askja_ctx *c, uint64_t seed, uint64_t stream)
 {
     ENTER(c);
-    int rc = ensure_scratch(c);
-    if (rc) return rc;
-    HIPCHK(c, launch_project(c->geom, c->links[c->cur], c->mode, 1, seed, stream, c->scratch, (long)c->scratch_doubles,
-                             c->st));
+    HIPCHK(c, launch_project(c->geom, c->links[c->cur], c->mode, 1, seed, stream, c->st));
     return ASKJA_OK;
 }
 
 int askja_reunitarize(askja_ctx *c)
 {
     ENTER(c);
-    int rc = ensure_scratch(c);
-    if (rc) return rc;
-    HIPCHK(c, launch_project(c->geom, c->links[c->cur], c->mode, 0, 0, 0, c->scratch, (long)c->scratch_doubles, c->st));
+    HIPCHK(c, launch_project(c->geom, c->links[c->cur], c->mode, 0, 0, 0, c->st));
     return ASKJA_OK;
 }
 
@@ -714,8 +705,7 @@ int askja_action(askja_ctx *c, double *s_t, double *s_s)
     long np = 0;
     int rc = ensure_part(c, (size_t)plaq_partials(c->geom) * 4);
     if (rc) return rc;
-    HIPCHK(c, launch_plaq(c->geom, c->links[c->cur], c->mode, 1. / (double)c->nc, c->part, &np, c->scratch,
-                          (long)c->scratch_doubles, c->st));
+    HIPCHK(c, launch_plaq(c->geom, c->links[c->cur], c->mode, 1. / (double)c->nc, c->part, &np, c->st));
     HIPCHK(c, launch_finalize(c->part, np, 2, c->res, c->st));
     double h[4];
     if ((rc = read_res(c, h, 4))) return rc;
@@ -764,8 +754,7 @@ int askja_gauss_law(askja_ctx *c, double *out)
 {
     ENTER(c);
     if (!out) return fail(c, ASKJA_EINVAL, "gauss_law: null output");
-    int rc = ensure_scratch(c);
-    if (rc) return rc;
+    int rc;
     const int n = c->nc, nr = 2 * n * n;
     const size_t small_need = (size_t)((c->NL + 255) / 256) * 2 * nr;
     const size_t gen_need = (size_t)gauss_rows(c->geom) * 2 * nr;
@@ -773,8 +762,7 @@ int askja_gauss_law(askja_ctx *c, double *out)
     double *dout = nullptr;
     HIPCHK(c, hipMallocAsync((void **)&dout, (size_t)2 * nr * sizeof(double), c->st));
     long np = 0;
-    hipError_t e = launch_gauss(c->geom, c->links[c->cur], c->mom, c->mode, c->part, &np, c->scratch,
-                                (long)c->scratch_doubles, dout, c->st);
+    hipError_t e = launch_gauss(c->geom, c->links[c->cur], c->mom, c->mode, c->part, &np, dout, c->st);
     std::vector<double> h((size_t)2 * nr);
     if (e == hipSuccess) e = hipMemcpyAsync(h.data(), dout, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->st);
     if (e == hipSuccess) e = hipFreeAsync(dout, c->st);
@@ -798,11 +786,10 @@ int askja_uviol(askja_ctx *c, double *out)
 {
     ENTER(c);
     if (!out) return fail(c, ASKJA_EINVAL, "uviol: null output");
-    int rc = ensure_scratch(c);
-    if (rc) return rc;
+    int rc;
     if ((rc = ensure_part(c, (size_t)((c->NL + 255) / 256 + 1024) * 2))) return rc;
     long np = 0;
-    HIPCHK(c, launch_uviol(c->geom, c->links[c->cur], c->mode, c->part, &np, c->scratch, (long)c->scratch_doubles, c->st));
+    HIPCHK(c, launch_uviol(c->geom, c->links[c->cur], c->mode, c->part, &np, c->st));
     HIPCHK(c, launch_finalize(c->part, np, 1, c->res, c->st));
     double h[2];
     if ((rc = read_res(c, h, 2))) return rc;
@@ -815,12 +802,10 @@ int askja_wilson_lines(askja_ctx *c, int dir, double *out)
     ENTER(c);
     if (!out) return fail(c, ASKJA_EINVAL, "wilson_lines: null output");
     if (dir < 0 || dir >= c->dim) return fail(c, ASKJA_EINVAL, "wilson_lines: dir %d out of range", dir);
-    int rc = ensure_scratch(c);
-    if (rc) return rc;
     const long nl = c->V / c->ext[dir];
     double *d = nullptr;
     HIPCHK(c, hipMallocAsync((void **)&d, (size_t)2 * nl * sizeof(double), c->st));
-    hipError_t e = launch_wlines(c->geom, c->links[c->cur], dir, c->mode, d, c->scratch, (long)c->scratch_doubles, c->st);
+    hipError_t e = launch_wlines(c->geom, c->links[c->cur], dir, c->mode, d, c->st);
     if (e == hipSuccess) e = hipMemcpyAsync(out, d, (size_t)2 * nl * sizeof(double), hipMemcpyDeviceToHost, c->st);
     if (e == hipSuccess) e = hipFreeAsync(d, c->st);
     if (e == hipSuccess) e = hipStreamSynchronize(c->st);

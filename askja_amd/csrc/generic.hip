@@ -1,9 +1,8 @@
-// generic.hip -- kernels that run for every N: layout conversion, cold and
-// Haar starts, momentum refresh, SU(N) projection, kinetic sums, the final
-// stage of every reduction, and the N >= 4 diagnostics (plaquette split,
-// unitarity violation, Gauss law, Wilson lines) with FP64 working matrices in
-// a per-thread slice of global scratch memory.  Also the top-level launchers
-// that pick the register (N <= 3) or workgroup-per-link (N >= 4) kernels.
+// generic.hip -- kernels that run for every N: layout conversion, cold
+// start, momentum refresh, kinetic sums, the final stage of every reduction,
+// and the line matrix of N <= 3.  Also the top-level launchers that pick the
+// register (N <= 3), row (4 <= N <= 8), workgroup (team_impl.hpp) or MFMA
+// kernels.
 #include "kernels.hpp"
 #include "common.hpp"
 
@@ -270,104 +269,6 @@ __global__ void k_mom_scale(Geom g, R *P, double f)
     mk(P, g, l, c) = (R)((double)mk(P, g, l, c) * f);
 }
 
-// -------------------------------------------------------- SU(N) projection
-// Modified Gram-Schmidt on the rows, then multiplication by exp(-i arg(det)/N)
-// with det from LU with partial pivoting; FP64, no contraction.  haar: the
-// matrix is first filled with a Ginibre matrix (xi + i eta)/sqrt(2) from the
-// Philox stream (purpose 2, counter = stream): Gram-Schmidt with positive
-// diagonal R is the phase-fixed QR of Mezzadri (2007), so the result is Haar
-// distributed on U(N), and the determinant phase maps it to Haar on SU(N).
-template <typename R>
-__global__ void k_project(Geom g, R *U, int haar, uint64_t seed, uint64_t stream, double *scratch, long nthr)
-{
-#pragma clang fp contract(off)
-    const long t = (long)blockIdx.x * kB + threadIdx.x;
-    if (t >= nthr) return;
-    const int n = g.N;
-    const int nn = n * n;
-    double *A = scratch + (size_t)t * 4 * nn;  // 2 complex matrices
-    double *M = A + 2 * nn;
-    for (long l = t; l < g.V * g.D; l += nthr) {
-        if (haar) {
-            const int d = (int)(l / g.V);
-            const uint32_t link = (uint32_t)(d * g.V + g.dev2lex[l - d * g.V]);
-            for (int b = 0; 4 * b < 2 * nn; b++) {
-                double z[4];
-                normals4(philox_block(seed, stream, kPurposeHaar, link, (uint32_t)b), z);
-                for (int q = 0; q < 4 && 4 * b + q < 2 * nn; q++) A[4 * b + q] = z[q] * 0.7071067811865476;
-            }
-        } else {
-            for (int e = 0; e < 2 * nn; e++) A[e] = (double)lkc(U, g, l, e);
-        }
-        for (int r = 0; r < n; r++) {
-            double *a = A + 2 * r * n;
-            for (int q = 0; q < r; q++) {
-                const double *b = A + 2 * q * n;
-                double dr = 0.0, di = 0.0;
-                for (int k = 0; k < n; k++) {
-                    dr += b[2 * k] * a[2 * k] + b[2 * k + 1] * a[2 * k + 1];
-                    di += b[2 * k] * a[2 * k + 1] - b[2 * k + 1] * a[2 * k];
-                }
-                for (int k = 0; k < n; k++) {
-                    a[2 * k] -= dr * b[2 * k] - di * b[2 * k + 1];
-                    a[2 * k + 1] -= dr * b[2 * k + 1] + di * b[2 * k];
-                }
-            }
-            double nrm = 0.0;
-            for (int k = 0; k < 2 * n; k++) nrm += a[k] * a[k];
-            nrm = 1.0 / sqrt(nrm);
-            for (int k = 0; k < 2 * n; k++) a[k] *= nrm;
-        }
-        // determinant by LU with partial pivoting on a copy
-        for (int e = 0; e < 2 * nn; e++) M[e] = A[e];
-        double rr = 1.0, ri = 0.0;
-        for (int cc = 0; cc < n; cc++) {
-            int p = cc;
-            double best = -1.0;
-            for (int r = cc; r < n; r++) {
-                const double v = hypot(M[2 * (r * n + cc)], M[2 * (r * n + cc) + 1]);
-                if (v > best) {
-                    best = v;
-                    p = r;
-                }
-            }
-            if (p != cc) {
-                for (int k = 0; k < 2 * n; k++) {
-                    const double tmp = M[2 * cc * n + k];
-                    M[2 * cc * n + k] = M[2 * p * n + k];
-                    M[2 * p * n + k] = tmp;
-                }
-                rr = -rr;
-                ri = -ri;
-            }
-            const double pr = M[2 * (cc * n + cc)], pim = M[2 * (cc * n + cc) + 1];
-            const double nr = rr * pr - ri * pim, ni = rr * pim + ri * pr;
-            rr = nr;
-            ri = ni;
-            const double den = pr * pr + pim * pim;
-            if (den == 0.0) {
-                rr = ri = 0.0;
-                break;
-            }
-            for (int r = cc + 1; r < n; r++) {
-                const double ar = M[2 * (r * n + cc)], ai = M[2 * (r * n + cc) + 1];
-                const double fr = (ar * pr + ai * pim) / den, fi = (ai * pr - ar * pim) / den;
-                for (int k = cc; k < n; k++) {
-                    const double br = M[2 * (cc * n + k)], bi = M[2 * (cc * n + k) + 1];
-                    M[2 * (r * n + k)] -= fr * br - fi * bi;
-                    M[2 * (r * n + k) + 1] -= fr * bi + fi * br;
-                }
-            }
-        }
-        const double ph = -atan2(ri, rr) / n;
-        const double cr = cos(ph), ci = sin(ph);
-        for (int q = 0; q < nn; q++) {
-            const double ar = A[2 * q], ai = A[2 * q + 1];
-            lk(U, g, l, 2 * q) = (R)(ar * cr - ai * ci);
-            lk(U, g, l, 2 * q + 1) = (R)(ar * ci + ai * cr);
-        }
-    }
-}
 
 // ------------------------------------------------------------- kinetic
 // {Tr P^2, 1/4 Tr (P + Q)^2} per link from the compact Hermitian layout.
@@ -441,17 +342,9 @@ __global__ void k_line_matrix(Geom g, const R *U, int dir, long lex, double *scr
     }
 }
 
-inline long threads_for(long work, long per_thread, long scratch_doubles)
-{
-    long t = scratch_doubles / per_thread;
-    if (t > work) t = work;
-    if (t < 1) t = 1;
-    return t;
-}
 
 }  // namespace
 
-long generic_scratch_per_thread(int n) { return 12L * n * n; }
 
 // ---------------------------------------------------------------- dispatch
 // 4 <= N <= 8: the row kernel puts 64/N links in one single-wavefront
@@ -532,8 +425,7 @@ hipError_t launch_finalize(const double *part, long np, int K, double *out, hipS
     return hipGetLastError();
 }
 
-hipError_t launch_plaq(const Geom &g, const void *U, int mode, double inv_n, double *part, long *np,
-                       double *scratch, long scratch_doubles, hipStream_t st)
+hipError_t launch_plaq(const Geom &g, const void *U, int mode, double inv_n, double *part, long *np, hipStream_t st)
 {
     if (g.N <= 3) return small_plaq(g, U, mode, inv_n, part, np, st);
     // one workgroup per site, products through LDS
@@ -550,8 +442,7 @@ long gauss_rows(const Geom &g)
     return nl < 256 ? nl : 256;
 }
 
-hipError_t launch_uviol(const Geom &g, const void *U, int mode, double *part, long *np, double *scratch,
-                        long scratch_doubles, hipStream_t st)
+hipError_t launch_uviol(const Geom &g, const void *U, int mode, double *part, long *np, hipStream_t st)
 {
     if (g.N <= 3) return small_uviol(g, U, mode, part, np, st);
     // workgroups over the links (grid-stride), one (s, c) pair each
@@ -563,8 +454,8 @@ hipError_t launch_uviol(const Geom &g, const void *U, int mode, double *part, lo
     return team_uviol_64(g, U, mode, nwg, part, st);
 }
 
-hipError_t launch_gauss(const Geom &g, const void *U, const void *P, int mode, double *part, long *np,
-                        double *scratch, long scratch_doubles, double *out, hipStream_t st)
+hipError_t launch_gauss(const Geom &g, const void *U, const void *P, int mode, double *part, long *np, double *out,
+                        hipStream_t st)
 {
     if (g.N <= 3) return small_gauss(g, U, P, mode, part, np, out, st);
     // gauss_rows(g) workgroups over the links, one row of 2N^2 (s, c) pairs
@@ -604,8 +495,7 @@ static hipError_t team_lines(const Geom &g, const void *U, int dir, long nlines,
     return team_lines_64(g, U, dir, nlines, lex1, mode, traces, wfull, st);
 }
 
-hipError_t launch_wlines(const Geom &g, const void *U, int dir, int mode, double *out, double *scratch,
-                         long scratch_doubles, hipStream_t st)
+hipError_t launch_wlines(const Geom &g, const void *U, int dir, int mode, double *out, hipStream_t st)
 {
     if (g.N <= 3) return small_wlines(g, U, dir, mode, out, st);
     return team_lines(g, U, dir, g.V / g.ext[dir], -1, mode, out, nullptr, st);
@@ -679,22 +569,14 @@ hipError_t launch_mom_scale(const Geom &g, void *P, double f, int mode, hipStrea
     return hipGetLastError();
 }
 
-hipError_t launch_project(const Geom &g, void *U, int mode, int haar, uint64_t seed, uint64_t stream, double *scratch,
-                          long scratch_doubles, hipStream_t st)
+hipError_t launch_project(const Geom &g, void *U, int mode, int haar, uint64_t seed, uint64_t stream, hipStream_t st)
 {
     if (g.N <= 3) return small_project(g, U, mode, haar, seed, stream, st);
     if (g.N <= 8) return team_project_8(g, U, mode, haar, seed, stream, st);
     if (g.N <= 16) return team_project_16(g, U, mode, haar, seed, stream, st);
     if (g.N <= 32) return team_project_32(g, U, mode, haar, seed, stream, st);
     if (g.N <= 64) return team_project_64(g, U, mode, haar, seed, stream, st);
-    const long nthr = threads_for(g.V * g.D, 4L * g.N * g.N, scratch_doubles);
-    if (f32_mode(mode))
-        hipLaunchKernelGGL(k_project<float>, dim3(nblk(nthr)), dim3(kB), 0, st, g, (float *)U, haar, seed, stream, scratch,
-                           nthr);
-    else
-        hipLaunchKernelGGL(k_project<double>, dim3(nblk(nthr)), dim3(kB), 0, st, g, (double *)U, haar, seed, stream,
-                           scratch, nthr);
-    return hipGetLastError();
+    return hipErrorInvalidValue;  // askja_create admits N <= 64
 }
 
 }  // namespace askja

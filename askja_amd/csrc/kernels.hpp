@@ -97,26 +97,25 @@ hipError_t launch_finalize(const double *part, long nparts, int K, double *out, 
 // plaq_partials(g) (2 (s, c) pairs each).
 long plaq_partials(const Geom &g);
 hipError_t launch_plaq(const Geom &g, const void *U, int mode, double inv_n, double *part,
-                       long *nparts, double *scratch, long scratch_doubles, hipStream_t st);
+                       long *nparts, hipStream_t st);
 // Rows of the Gauss-law partials for N >= 4 (2N^2 (s, c) pairs each).
 long gauss_rows(const Geom &g);
 // sum over links of ||U U^+ - 1||_F^2 (tr2 of U U^+ - 1)
-hipError_t launch_uviol(const Geom &g, const void *U, int mode, double *part, long *nparts,
-                        double *scratch, long scratch_doubles, hipStream_t st);
+hipError_t launch_uviol(const Geom &g, const void *U, int mode, double *part, long *nparts, hipStream_t st);
 // sum over links of the N x N matrix P - U^+ P U (2 N^2 reals), written to out
 // (device, 2 N^2 (s, c) pairs).
 hipError_t launch_gauss(const Geom &g, const void *U, const void *P, int mode, double *part,
-                        long *nparts, double *scratch, long scratch_doubles, double *out, hipStream_t st);
+                        long *nparts, double *out, hipStream_t st);
 // {sum Tr P^2, sum 1/4 Tr (P + Q)^2} over links, Q may be null
 hipError_t launch_kinetic(const Geom &g, const void *P, const void *Q, int mode, double *part,
                           long *nparts, hipStream_t st);
 // Straight Wilson lines in direction dir starting at x_dir = 0: out[2q..2q+1]
 // = Tr(prod U)/N of line q (lines in lexicographic order of their start).
-hipError_t launch_wlines(const Geom &g, const void *U, int dir, int mode, double *out,
-                         double *scratch, long scratch_doubles, hipStream_t st);
+hipError_t launch_wlines(const Geom &g, const void *U, int dir, int mode, double *out, hipStream_t st);
 
 // Product matrix (2 N^2 doubles, device) of the straight line in direction
-// dir starting at lexicographic site lex; scratch >= 4 N^2 doubles.
+// dir starting at lexicographic site lex; scratch >= 4 N^2 doubles (used
+// for N <= 3).
 hipError_t launch_line_matrix(const Geom &g, const void *U, int dir, long lex, int mode, double *scratch, double *out,
                               hipStream_t st);
 
@@ -131,13 +130,12 @@ hipError_t launch_mom_refresh(const Geom &g, void *P, uint64_t seed, uint64_t co
                               int mode, hipStream_t st);
 // Doubles of FP64 working memory each generic (N >= 4) diagnostic or
 // projection thread needs.
-long generic_scratch_per_thread(int n);
 hipError_t launch_mom_scale(const Geom &g, void *P, double f, int mode, hipStream_t st);
-// SU(N) projection (Gram-Schmidt on rows + determinant phase), FP64 working
-// copy in `scratch` (>= projection_scratch_doubles(...)).  haar != 0 first
-// fills every link with a Ginibre matrix from Philox (seed, stream).
+// SU(N) projection (Gram-Schmidt on rows + determinant phase) in FP64.
+// haar != 0 first fills every link with a Ginibre matrix from Philox (seed,
+// stream).
 hipError_t launch_project(const Geom &g, void *U, int mode, int haar, uint64_t seed, uint64_t stream,
-                          double *scratch, long scratch_doubles, hipStream_t st);
+                          hipStream_t st);
 
 // ---- measurement (measure.hip) ---------------------------------------------
 // blocks per time slice of the slice-trace partials
