@@ -15,6 +15,12 @@ pytestmark = pytest.mark.gpu
 
 EXT = [4, 4, 2, 2]
 SUBS = [(0, 0, 0), (0, 1, 0), (1, 0, 1), (2, 1, 1), (3, 0, 0), (1, 1, 1), (3, 1, 0), (2, 0, 1), (0, 0, 1), (1, 0, 0)]
+# D = 3: one plane without mu per link, so the forward and backward kernels'
+# two wavefronts each take that plane's loops (and the staples) alone
+EXT3 = [4, 4, 4]
+SUBS3 = [(0, 0, 0), (1, 1, 0), (2, 0, 1), (0, 1, 1), (1, 0, 0), (2, 1, 1)]
+GEOMS = [(EXT, SUBS), (EXT3, SUBS3)]
+GEOM_IDS = ["D4", "D3"]
 RHO, DELTA, BETA = 0.3, 1e-3, 5.7
 
 
@@ -24,32 +30,34 @@ def fo():
     return flow_oracle
 
 
-def _setup(oracle, seed, scale=0.5, prec=A.FP64):
-    o = oracle.Oracle(3, EXT)
+def _setup(oracle, seed, scale=0.5, prec=A.FP64, ext=EXT, subs=SUBS):
+    o = oracle.Oracle(3, ext)
     rng = np.random.default_rng(seed)
     V = random_su(oracle, rng, (o.V, o.dim), 3)
-    lat = A.Lattice(3, EXT, prec)
+    lat = A.Lattice(3, ext, prec)
     lat.set_links(V)
-    bsize = lat.flow_define([s[0] for s in SUBS], [s[1] for s in SUBS], [s[2] for s in SUBS], 2, RHO, DELTA)
+    bsize = lat.flow_define([s[0] for s in subs], [s[1] for s in subs], [s[2] for s in subs], 2, RHO, DELTA)
     w = (rng.standard_normal(2 * bsize) * scale).astype(np.float32)
     lat.flow_set_weights(w)
     nf = 3 * (o.dim - 1) * (o.dim - 2) // 2  # plaquette and two rectangles per plane
     return o, rng, V, lat, w, bsize, nf
 
 
-def test_forward_matches_oracle(oracle, fo):
-    o, rng, V, lat, w, bsize, nf = _setup(oracle, 1)
+@pytest.mark.parametrize("ext,subs", GEOMS, ids=GEOM_IDS)
+def test_forward_matches_oracle(oracle, fo, ext, subs):
+    o, rng, V, lat, w, bsize, nf = _setup(oracle, 1, ext=ext, subs=subs)
     lj = lat.flow_apply()
     U = lat.flow_output()
-    Uo, ljo = fo.forward(o, V, SUBS, w, bsize, nf, RHO, DELTA)
+    Uo, ljo = fo.forward(o, V, subs, w, bsize, nf, RHO, DELTA)
     assert np.max(np.abs(U - Uo)) < 1e-12
     assert abs(lj - ljo) < 1e-11 * max(1.0, abs(ljo))
     assert abs(ljo) > 1e-2
     assert np.max(np.abs(U - V)) > 1e-3  # the map is not the identity
 
 
-def test_link_gradient_vs_finite_differences(oracle, fo):
-    o, rng, V, lat, w, bsize, nf = _setup(oracle, 2)
+@pytest.mark.parametrize("ext,subs", GEOMS, ids=GEOM_IDS)
+def test_link_gradient_vs_finite_differences(oracle, fo, ext, subs):
+    o, rng, V, lat, w, bsize, nf = _setup(oracle, 2, ext=ext, subs=subs)
     s, lj = lat.flow_gradient(BETA)
     g = lat.flow_link_gradient()
     gens = oracle.generators(3)
@@ -61,7 +69,7 @@ def test_link_gradient_vs_finite_differences(oracle, fo):
         for sgn in (1, -1):
             W = V.copy()
             W[x, d] = scipy.linalg.expm(1j * sgn * eps * X) @ V[x, d]
-            vals.append(fo.phi(o, W, SUBS, w, bsize, nf, RHO, DELTA, BETA))
+            vals.append(fo.phi(o, W, subs, w, bsize, nf, RHO, DELTA, BETA))
         fd = (vals[0] - vals[1]) / (2 * eps)
         an = 2 * np.trace(g[x, d] @ X).real
         assert abs(fd - an) < 2e-6 * max(1.0, abs(an)), (x, d, fd, an)
@@ -70,8 +78,9 @@ def test_link_gradient_vs_finite_differences(oracle, fo):
     assert np.allclose(np.trace(g, axis1=-2, axis2=-1), 0, atol=1e-12)
 
 
-def test_weight_gradient_vs_finite_differences(oracle, fo):
-    o, rng, V, lat, w, bsize, nf = _setup(oracle, 3)
+@pytest.mark.parametrize("ext,subs", GEOMS, ids=GEOM_IDS)
+def test_weight_gradient_vs_finite_differences(oracle, fo, ext, subs):
+    o, rng, V, lat, w, bsize, nf = _setup(oracle, 3, ext=ext, subs=subs)
     lat.flow_gradient(BETA, wgrad=True)
     gw = lat.flow_weight_gradient()
     h = np.float32(2.0 ** -9)
@@ -80,7 +89,7 @@ def test_weight_gradient_vs_finite_differences(oracle, fo):
         for sgn in (1, -1):
             ww = w.copy()
             ww[idx] = np.float32(w[idx] + sgn * h)
-            vals.append(fo.phi(o, V, SUBS, ww, bsize, nf, RHO, DELTA, BETA))
+            vals.append(fo.phi(o, V, subs, ww, bsize, nf, RHO, DELTA, BETA))
         fd = (vals[0] - vals[1]) / (2 * float(h))
         assert abs(fd - gw[idx]) < 1e-4 * max(1.0, abs(fd)), (idx, fd, gw[idx])
 
