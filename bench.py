@@ -54,11 +54,11 @@ def tau_int_wolff(x, S=1.5):
     return A.tau_int(x, S)
 
 
-def cpu_baseline(steps_sample):
+def cpu_baseline(steps_sample, ndrift=NDRIFT):
     """The oracle (FP64 C restatement of the reference, OpenMP over sites in
     the kick and drift like sun.cpp:679,777) on the host cores, same lattice:
     `steps_sample` MD steps timed and extrapolated to one trajectory
-    (NDRIFT steps + the two Hamiltonian evaluations)."""
+    (ndrift steps + the two Hamiltonian evaluations)."""
     from oracle import oracle as O
 
     o = O.Oracle(NC, EXT, beta=BETA, level=LEVEL)
@@ -73,11 +73,11 @@ def cpu_baseline(steps_sample):
         o.updateUandcopyDown(DT)
         o.updateE(DT)
     t_step = (time.perf_counter() - t0) / steps_sample
-    t_traj = NDRIFT * t_step + 2 * t_h
+    t_traj = ndrift * t_step + 2 * t_h
     assert math.isfinite(h)
     return {"value": 1.0 / t_traj, "unit": "traj/s", "cores": O.num_threads(), "kind": "port",
             "sample": f"{steps_sample} FP64 MD steps (kick+drift) of the oracle on 8x32^3 SU(3), "
-                      f"{t_step * 1e3:.0f} ms/step, extrapolated to {NDRIFT} steps + 2 H evaluations"}
+                      f"{t_step * 1e3:.0f} ms/step, extrapolated to {ndrift} steps + 2 H evaluations"}
 
 
 def copy_probe(gib=1, reps=5):
@@ -285,9 +285,13 @@ def example_study(reps=200, ntraj=5):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
+    # 3000 trajectories (about 40 s): tau_int(|P|) is about 20 here, and
+    # Wolff's window underestimates it on runs much shorter than 100 tau
+    # (1000 trajectories gave 13.8)
+    ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--warmup", type=int, default=300)
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--md-steps", type=int, default=NDRIFT, help="MD steps per trajectory (length = steps x DT)")
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--roofline-launches", type=int, default=100)
@@ -328,7 +332,7 @@ def main():
     lat.haar(SEED, 2)
     t_w = time.perf_counter()
     for t in range(args.warmup):
-        lat.hmc_trajectory(BETA, DT, NDRIFT, LEVEL, SEED, t, reunit=True)
+        lat.hmc_trajectory(BETA, DT, args.md_steps, LEVEL, SEED, t, reunit=True)
         if (t + 1) % 100 == 0:
             print(f"[bench] warmup {t + 1}/{args.warmup} ({time.perf_counter() - t_w:.1f} s)", file=sys.stderr,
                   flush=True)
@@ -337,7 +341,7 @@ def main():
     t0 = time.perf_counter()
     last = t0
     for k in range(args.steps):
-        r = lat.hmc_trajectory(BETA, DT, NDRIFT, LEVEL, SEED, args.warmup + k, reunit=True)
+        r = lat.hmc_trajectory(BETA, DT, args.md_steps, LEVEL, SEED, args.warmup + k, reunit=True)
         poly.append(abs(lat.polyakov()))
         dh.append(r.dh)
         acc.append(r.accepted)
@@ -382,11 +386,12 @@ def main():
         "dtype": "f64" if prec == A.FP64 else "f32",
         "data": "synthetic (Haar-random SU(3) start, seed 120351616)",
         "config": {"workload": "plain HMC, SU(3) D=4 8x32^3 beta=5.9 DT=0.02 LEVEL=6 hot start, "
-                               "50 MD steps/trajectory", "nc": NC, "extents": EXT, "beta": BETA, "dt": DT,
-                   "level": LEVEL, "md_steps_per_traj": NDRIFT, "precision": args.precision},
+                               f"{args.md_steps} MD steps/trajectory", "nc": NC, "extents": EXT, "beta": BETA, "dt": DT,
+                   "level": LEVEL, "md_steps_per_traj": args.md_steps, "precision": args.precision},
         "traj_per_s": traj_s,
-        "md_steps_per_s": traj_s * NDRIFT,
+        "md_steps_per_s": traj_s * args.md_steps,
         "tau_int_polyakov": tau,
+        "tau_int_err": tau * math.sqrt(2.0 * (2 * window + 1) / args.steps),
         "tau_window": window,
         "acceptance": float(np.mean(acc)),
         "mean_plaquette": float(np.mean(plaq)),
@@ -401,7 +406,7 @@ def main():
     }
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_steps, args.md_steps)
     print(json.dumps(out))
 
 
