@@ -282,6 +282,38 @@ def example_study(reps=200, ntraj=5):
         lat.close()
 
 
+def run_chain(lat, args, tag):
+    """Haar start, --warmup untimed trajectories, then --steps timed ones with
+    the Polyakov loop after each; returns (traj/s, tau, window, value,
+    acceptances, dH, plaquettes) and the timed seconds."""
+    lat.haar(SEED, 2)
+    t_w = time.perf_counter()
+    for t in range(args.warmup):
+        lat.hmc_trajectory(BETA, DT, args.md_steps, LEVEL, SEED, t, reunit=True)
+        if (t + 1) % 100 == 0:
+            print(f"[{tag}] warmup {t + 1}/{args.warmup} ({time.perf_counter() - t_w:.1f} s)", file=sys.stderr,
+                  flush=True)
+    lat.sync()
+    poly, dh, acc, plaq = [], [], [], []
+    t0 = time.perf_counter()
+    last = t0
+    for k in range(args.steps):
+        r = lat.hmc_trajectory(BETA, DT, args.md_steps, LEVEL, SEED, args.warmup + k, reunit=True)
+        poly.append(abs(lat.polyakov()))
+        dh.append(r.dh)
+        acc.append(r.accepted)
+        plaq.append(1.0 - r.action / lat.n_plaquettes())
+        now = time.perf_counter()
+        if now - last > 30:
+            print(f"[{tag}] timed {k + 1}/{args.steps}", file=sys.stderr, flush=True)
+            last = now
+    lat.sync()
+    T = time.perf_counter() - t0
+    traj_s = args.steps / T
+    tau, window = tau_int_wolff(poly)
+    return (traj_s, tau, window, traj_s / tau, acc, dh, plaq), T
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -294,6 +326,7 @@ def main():
     ap.add_argument("--md-steps", type=int, default=NDRIFT, help="MD steps per trajectory (length = steps x DT)")
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the FP32 mixed-precision chain")
     ap.add_argument("--roofline-launches", type=int, default=100)
     ap.add_argument("--copy-probe", action="store_true", help="only run a 1 GiB device copy (counter calibration)")
     ap.add_argument("--study", choices=["precision", "c3scan", "c5exact", "kernels", "example"],
@@ -329,33 +362,7 @@ def main():
 
     prec = A.FP64 if args.precision == "fp64" else A.FP32
     lat = A.Lattice(NC, EXT, prec)
-    lat.haar(SEED, 2)
-    t_w = time.perf_counter()
-    for t in range(args.warmup):
-        lat.hmc_trajectory(BETA, DT, args.md_steps, LEVEL, SEED, t, reunit=True)
-        if (t + 1) % 100 == 0:
-            print(f"[bench] warmup {t + 1}/{args.warmup} ({time.perf_counter() - t_w:.1f} s)", file=sys.stderr,
-                  flush=True)
-    lat.sync()
-    poly, dh, acc, plaq = [], [], [], []
-    t0 = time.perf_counter()
-    last = t0
-    for k in range(args.steps):
-        r = lat.hmc_trajectory(BETA, DT, args.md_steps, LEVEL, SEED, args.warmup + k, reunit=True)
-        poly.append(abs(lat.polyakov()))
-        dh.append(r.dh)
-        acc.append(r.accepted)
-        plaq.append(1.0 - r.action / lat.n_plaquettes())
-        now = time.perf_counter()
-        if now - last > 30:
-            print(f"[bench] timed {k + 1}/{args.steps}", file=sys.stderr, flush=True)
-            last = now
-    lat.sync()
-    T = time.perf_counter() - t0
-    traj_s = args.steps / T
-    tau, window = tau_int_wolff(poly)
-    value = traj_s / tau
-
+    (traj_s, tau, window, value, acc, dh, plaq), T = run_chain(lat, args, "bench")
     # roofline of the dominant kernel: the fused kick+drift launch, timed
     # with HIP events on the library's stream
     nl = lat.V * lat.dim
@@ -404,6 +411,18 @@ def main():
                      "kernel": "fused kick+drift", "ms_per_launch": ms, "bytes_per_link": bytes_per_link,
                      "links_per_launch": nl, "tflops": tflops},
     }
+    if prec == A.FP64 and not args.no_mixed:
+        # the same chain with FP32 links and MD (FP64 sums, reunitarisation and
+        # Metropolis energies of the stored links): the mixed-precision side
+        # of this workload's FP64 / FP32 comparison
+        lat32 = A.Lattice(NC, EXT, A.FP32)
+        (t32, tau32, w32, v32, acc32, dh32, plaq32), _ = run_chain(lat32, args, "bench fp32")
+        out["mixed_precision_fp32"] = {
+            "value": v32, "unit": "configs/s", "traj_per_s": t32, "tau_int_polyakov": tau32,
+            "tau_int_err": tau32 * math.sqrt(2.0 * (2 * w32 + 1) / args.steps), "tau_window": w32,
+            "acceptance": float(np.mean(acc32)), "mean_plaquette": float(np.mean(plaq32)),
+            "exp_minus_dH": float(np.mean(np.exp(-np.asarray(dh32))))}
+        lat32.close()
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.cpu_steps, args.md_steps)
