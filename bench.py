@@ -33,6 +33,7 @@ sys.path.insert(0, ROOT)
 NC, EXT, BETA, DT, LEVEL, NDRIFT = 3, [8, 32, 32, 32], 5.9, 0.02, 6, 50
 SEED = 120351616
 HBM_PEAK_GBS = 8000.0
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}  # MI355X dense
 
 
 # HBM bytes per fused kick+drift launch of this workload, measured with two
@@ -244,9 +245,16 @@ def kernel_study(reps=None):
             lat.reunitarize()
             ms_re = lat.timer_stop()
             flops = (4 * (lat.dim - 1) + 1 + level) * 8 * n ** 3 * lat.V * lat.dim
+            tf = flops / ms / 1e9
+            # the peak of the unit the arithmetic runs on: FP64 / FP32 vector or
+            # matrix (the same figures on MI355X), or for BF16x3 the dense BF16
+            # MFMA peak against the three BF16 products per FP32 product
+            peak, work = (PEAK_TFLOPS["bf16"], 3.0) if (pn == "bf16x3" and n > 8) else \
+                (PEAK_TFLOPS["fp32"], 1.0) if pn in ("fp32", "bf16x3") else (PEAK_TFLOPS["fp64"], 1.0)
             print(json.dumps({"study": "kernels", "config": name, "nc": n, "extents": ext, "precision": pn,
-                              "ms_kick_drift": round(ms, 4), "tflops": round(flops / ms / 1e9, 2),
-                              "ms_reunitarize": round(ms_re, 3)}), flush=True)
+                              "ms_kick_drift": round(ms, 4), "tflops": round(tf, 2),
+                              "roofline_frac": round(work * tf / peak, 3), "ms_reunitarize": round(ms_re, 3)}),
+                  flush=True)
             lat.close()
 
 
