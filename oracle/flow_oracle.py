@@ -1,8 +1,8 @@
 """CPU restatement (numpy, FP64) of the learned field transformation F(V).
 
 TEST INFRASTRUCTURE ONLY: the checker for the HIP kernels in
-askja_amd/csrc/flow.hip.  The reference has no counterpart (BASELINE.json's
-learned-proposal addition); the definitions follow the sub-layer stated there:
+askja_amd/csrc/flow.hip.  The reference has no counterpart (the learned
+proposal is an addition to its method); the definitions are these:
 
   for every link l = (x, mu) of the sub-layer's mask (direction mu, sites of
   one parity), with every other link frozen:
