@@ -89,6 +89,7 @@ def test_run_files_and_first_trajectory(tmp_path, oracle):
     r = _run(["p.txt", "--precision", "fp64-exact"], tmp_path)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "Finished successfully" in r.stdout
+    assert "tau_int(|P|) needs at least 8 trajectories (3)" in r.stdout
     out = tmp_path / "output-DIM4"
     cons = (out / f"constraints-{TAG}.dat").read_text().splitlines()
     assert cons[0] == "#Time\t Gauss-law\t Total Energy\t E_L\t E_M\t Unitary-Violation"
@@ -232,3 +233,19 @@ def test_measure_program_vs_oracle(tmp_path):
     for row, h in zip(hist, want["hist"]):
         assert all(close(a, b) for a, b in zip(row, h)), (row, h)
     assert abs(sum(row[1] for row in hist) - 3) < 1e-9 and abs(sum(row[2] for row in hist) - 3) < 1e-9
+
+
+@pytest.mark.gpu
+def test_run_reports_tau_int(tmp_path):
+    """Twelve trajectories: the run ends with tau_int of |P| (Wolff's window,
+    askja_tau_int) and the independent configurations per second."""
+    import re
+
+    (tmp_path / "p.txt").write_text(PARAMS.format(steps=240))
+    r = _run(["p.txt"], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    m = re.search(r"tau_int\(\|P\|\) = ([0-9.eE+-]+) \(window (\d+)\) over (\d+) trajectories, "
+                  r"([0-9.eE+-]+) independent configurations/s", r.stdout)
+    assert m, r.stdout[-2000:]
+    tau, window, ntraj, rate = float(m.group(1)), int(m.group(2)), int(m.group(3)), float(m.group(4))
+    assert ntraj == 12 and 0 < tau < 12 and 0 <= window < 6 and rate > 0
