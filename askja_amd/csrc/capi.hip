@@ -189,7 +189,7 @@ int exact_action(askja_ctx *c, const void *U, double *s)
         const long np = kick_partials(c->geom);
         if ((rc = ensure_part(c, (size_t)np * 6))) return rc;
         const KickArgs a = kick_args(c, U, nullptr, nullptr, 0.0, 0.0, 0);
-        HIPCHK(c, launch_action_f64(a, c->st));
+        HIPCHK(c, launch_action_f64(a, c->mode, c->st));
         HIPCHK(c, launch_finalize(c->part, np, 3, c->res, c->st));
         double h[6];
         if ((rc = read_res(c, h, 6))) return rc;
@@ -703,8 +703,32 @@ int askja_action(askja_ctx *c, double *s_t, double *s_s)
 {
     ENTER(c);
     long np = 0;
-    int rc = ensure_part(c, (size_t)plaq_partials(c->geom) * 4);
-    if (rc) return rc;
+    int rc;
+    if (c->nc > 8 && c->mode != kModeF64Exact) {
+        // MFMA pass (FP64 arithmetic) of ReTr W = sum of the 2(D-1)
+        // plaquettes through each link: all links give 4 sum_p ReTr P, the
+        // links of direction 0 twice the sum over the temporal plaquettes
+        // (each plane (0, k) plaquette holds two of them)
+        np = kick_partials(c->geom);
+        if ((rc = ensure_part(c, (size_t)np * 6))) return rc;
+        const KickArgs a = kick_args(c, c->links[c->cur], nullptr, nullptr, 0.0, 0.0, 0);
+        HIPCHK(c, launch_action_f64(a, c->mode, c->st));
+        HIPCHK(c, launch_action_split(c->part, np, c->dim, c->res, c->st));
+        double h[4];
+        if ((rc = read_res(c, h, 4))) return rc;
+        const double n = (double)c->nc;
+        const double tot = (double)n_plaquettes(c) - pair_val(h, 0) / (4.0 * n);
+        if (c->dim == 2) {  // one plane, (0, 1): every plaquette is temporal
+            if (s_t) *s_t = tot;
+            if (s_s) *s_s = 0.0;
+            return ASKJA_OK;
+        }
+        const double st_ = (double)(c->V * (c->dim - 1)) - pair_val(h, 1) / (2.0 * n);
+        if (s_t) *s_t = st_;
+        if (s_s) *s_s = tot - st_;
+        return ASKJA_OK;
+    }
+    if ((rc = ensure_part(c, (size_t)plaq_partials(c->geom) * 4))) return rc;
     HIPCHK(c, launch_plaq(c->geom, c->links[c->cur], c->mode, 1. / (double)c->nc, c->part, &np, c->st));
     HIPCHK(c, launch_finalize(c->part, np, 2, c->res, c->st));
     double h[4];

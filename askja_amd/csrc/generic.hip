@@ -40,9 +40,9 @@ hipError_t mfma_kick_32(const KickArgs &a, int mode, bool drift, hipStream_t st)
 hipError_t mfma_kick_64(const KickArgs &a, int mode, bool drift, hipStream_t st);
 hipError_t rows_kick(const KickArgs &a, int mode, bool drift, hipStream_t st);
 long rows_kick_blocks(const Geom &g);
-hipError_t mfma_action_16(const KickArgs &a, hipStream_t st);
-hipError_t mfma_action_32(const KickArgs &a, hipStream_t st);
-hipError_t mfma_action_64(const KickArgs &a, hipStream_t st);
+hipError_t mfma_action_16(const KickArgs &a, bool f32_links, hipStream_t st);
+hipError_t mfma_action_32(const KickArgs &a, bool f32_links, hipStream_t st);
+hipError_t mfma_action_64(const KickArgs &a, bool f32_links, hipStream_t st);
 
 namespace {
 
@@ -393,13 +393,34 @@ hipError_t launch_kick(const KickArgs &a, int mode, bool drift, hipStream_t st)
     return team_kick_64(a, mode, drift, st);
 }
 
-hipError_t launch_action_f64(const KickArgs &a, hipStream_t st)
+hipError_t launch_action_f64(const KickArgs &a, int mode, hipStream_t st)
 {
     const int n = a.g.N;
     if (n <= 8) return hipErrorInvalidValue;
-    if (n <= 16) return mfma_action_16(a, st);
-    if (n <= 32) return mfma_action_32(a, st);
-    return mfma_action_64(a, st);
+    const bool f32 = f32_mode(mode);
+    if (n <= 16) return mfma_action_16(a, f32, st);
+    if (n <= 32) return mfma_action_32(a, f32, st);
+    return mfma_action_64(a, f32, st);
+}
+
+// The action pass's partials (one per link workgroup lb, direction lb % D,
+// first (s, c) pair = ReTr W of the link) -> out = {sum over every link, sum
+// over the links of direction 0}, compensated.
+__global__ __launch_bounds__(1024) void k_action_split(const double *part, long np, int D, double *out)
+{
+    double s[2] = {0.0, 0.0}, c[2] = {0.0, 0.0};
+    for (long p = threadIdx.x; p < np; p += 1024) {
+        const double v = part[(size_t)p * 6], w = part[(size_t)p * 6 + 1];
+        pair_add(s[0], c[0], v, w);
+        if (p % D == 0) pair_add(s[1], c[1], v, w);
+    }
+    block_reduce_store<2>(s, c, out);
+}
+
+hipError_t launch_action_split(const double *part, long np, int D, double *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_action_split, dim3(1), dim3(1024), 0, st, part, np, D, out);
+    return hipGetLastError();
 }
 
 hipError_t launch_drift(const Geom &g, void *U, const void *P, double eps, int level, const double *cinv, int mode,

@@ -84,7 +84,10 @@ inline long kick_site_blocks(const Geom &g)
 hipError_t launch_kick(const KickArgs &a, int mode, bool drift, hipStream_t st);
 // N >= 9, FP32 links: sum ReTr W in FP64 arithmetic into the kick partials
 // (the exact action of the stored links; no momentum is touched)
-hipError_t launch_action_f64(const KickArgs &a, hipStream_t st);
+hipError_t launch_action_f64(const KickArgs &a, int mode, hipStream_t st);
+// Sums of the action pass's partials: {every link, the links of direction 0}
+// (2 (s, c) pairs into out).
+hipError_t launch_action_split(const double *part, long np, int D, double *out, hipStream_t st);
 hipError_t launch_drift(const Geom &g, void *U, const void *P, double eps, int level,
                         const double *cinv, int mode, hipStream_t st);
 

@@ -12,6 +12,9 @@ hipError_t mfma_kick_32(const KickArgs &a, int mode, bool drift, hipStream_t st)
     }
 }
 
-hipError_t mfma_action_32(const KickArgs &a, hipStream_t st) { return mfma::action_f64<32>(a, st); }
+hipError_t mfma_action_32(const KickArgs &a, bool f32_links, hipStream_t st)
+{
+    return mfma::action_f64<32>(a, f32_links, st);
+}
 
 }  // namespace askja

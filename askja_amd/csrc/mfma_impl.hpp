@@ -667,13 +667,15 @@ __global__ __launch_bounds__((Cfg<P, NMAX>::NT)) void k_kick(KickArgs a)
     block_reduce_store<3>(sum, cmp, a.part + 6 * (size_t)lb);
 }
 
-// sum ReTr W in FP64 arithmetic over FP32 links (partials as the kick's)
+// sum ReTr W in FP64 arithmetic over FP32 (f32_links) or FP64 links
+// (partials as the kick's)
 template <int NMAX>
-hipError_t action_f64(const KickArgs &a, hipStream_t st)
+hipError_t action_f64(const KickArgs &a, bool f32_links, hipStream_t st)
 {
     const unsigned nb = (unsigned)(a.g.V * a.g.D);
     constexpr int NT = Cfg<kF64, NMAX>::NT;
-    hipLaunchKernelGGL((k_kick<kF64, NMAX, false, float, true>), dim3(nb), dim3(NT), 0, st, a);
+    if (f32_links) hipLaunchKernelGGL((k_kick<kF64, NMAX, false, float, true>), dim3(nb), dim3(NT), 0, st, a);
+    else hipLaunchKernelGGL((k_kick<kF64, NMAX, false, double, true>), dim3(nb), dim3(NT), 0, st, a);
     return hipGetLastError();
 }
 
