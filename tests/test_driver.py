@@ -249,3 +249,24 @@ def test_run_reports_tau_int(tmp_path):
     assert m, r.stdout[-2000:]
     tau, window, ntraj, rate = float(m.group(1)), int(m.group(2)), int(m.group(3)), float(m.group(4))
     assert ntraj == 12 and 0 < tau < 12 and 0 <= window < 6 and rate > 0
+
+
+@pytest.mark.gpu
+def test_reference_example_params_end_to_end(tmp_path):
+    """The reference's own params file (params_N8S10T10B20k.txt, byte for
+    byte: NC 8, DIM 10, 100000 MD steps, RANDOM 1) runs to the end: 100
+    trajectories, 10 stored configurations of the U-*.conf size, the
+    constraints/Wilson rows every UPDATE, the closing tau_int line."""
+    import shutil
+
+    shutil.copy(os.path.join(ROOT, "tests", "golden", "params_N8S10T10B20k.txt"), tmp_path / "params.txt")
+    r = _run(["params.txt"], tmp_path, timeout=600)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "Finished successfully" in r.stdout and "over 100 trajectories" in r.stdout
+    out = tmp_path / "output-DIM10"
+    tag = "N8-Nt10-Ns10-BETA20000.00-DT0.0100-U10.0000-CT10"
+    confs = sorted(glob.glob(str(out / f"U-{tag}-*.conf")))
+    assert len(confs) == 10
+    assert all(os.path.getsize(c) == 100 * 10 * 8 * 8 * 16 for c in confs)
+    rows = [l for l in open(out / f"constraints-{tag}.dat").read().splitlines() if not l.startswith("#")]
+    assert len(rows) == 1 + 100000 // 500
