@@ -133,11 +133,13 @@ __device__ inline void put(R *Sr, R *Si, int r, int c, R re, R im)
     }
 }
 
-// One half of the stacked product on the 16x16x4 forms: IM = false takes
-// k < NMAX ([Ar] against [Br ; Bi] rows), IM = true the rest ([Ai] against
-// [-Bi ; Br]).  Lane l holds A[row][k] and B[k][col], k = 4 ks + l/16.
-template <int P, int NMAX, bool DA, bool DB, bool IM, typename R>
-__device__ inline void cmul_half(const R *A, const R *Br, const R *Bi, Acc<P, NMAX> &acc)
+// The stacked product on the 16x16x4 forms, both halves in one pass over k:
+// lane l holds A[row][k] and B[k][col], k = 4 ks + l/16, and each pair
+// (Br, Bi) it reads from LDS feeds four products,
+//   Re += Ar Br + (-Ai) Bi,   Im += Ar Bi + Ai Br
+// (Ai, Bi with the sign of the conjugate for a daggered operand).
+template <int P, int NMAX, bool DA, bool DB, typename R>
+__device__ inline void cmul_vec(const R *Ar, const R *Ai, const R *Br, const R *Bi, Acc<P, NMAX> &acc)
 {
     constexpr int NCT = Cfg<P, NMAX>::NCT, LD = Cfg<P, NMAX>::LD;
     const int lane = threadIdx.x & 63;
@@ -146,20 +148,24 @@ __device__ inline void cmul_half(const R *A, const R *Br, const R *Bi, Acc<P, NM
 #pragma unroll 4
     for (int ks = 0; ks < NMAX / 4; ks++) {
         const int k = 4 * ks + (lane >> 4);
-        const R a = DA ? A[k * LD + row] : A[row * LD + k];
-        const R av = (IM && DA) ? -a : a;
+        const R ar = DA ? Ar[k * LD + row] : Ar[row * LD + k];
+        const R ai = DA ? -Ai[k * LD + row] : Ai[row * LD + k];
+        const R nai = -ai;
 #pragma unroll
         for (int t = 0; t < NCT; t++) {
             const int col = 16 * (t0 + t) + (lane & 15);
-            const R bre = DB ? Br[col * LD + k] : Br[k * LD + col];
-            const R bim = DB ? -Bi[col * LD + k] : Bi[k * LD + col];
-            const R br = IM ? -bim : bre, bi = IM ? bre : bim;
+            const R br = DB ? Br[col * LD + k] : Br[k * LD + col];
+            const R bi = DB ? -Bi[col * LD + k] : Bi[k * LD + col];
             if constexpr (P == kF64) {
-                acc.r[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, br, acc.r[t], 0, 0, 0);
-                acc.i[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bi, acc.i[t], 0, 0, 0);
+                acc.r[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, acc.r[t], 0, 0, 0);
+                acc.i[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, acc.i[t], 0, 0, 0);
+                acc.r[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai, bi, acc.r[t], 0, 0, 0);
+                acc.i[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, acc.i[t], 0, 0, 0);
             } else {
-                acc.r[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, br, acc.r[t], 0, 0, 0);
-                acc.i[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bi, acc.i[t], 0, 0, 0);
+                acc.r[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, br, acc.r[t], 0, 0, 0);
+                acc.i[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, bi, acc.i[t], 0, 0, 0);
+                acc.r[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(nai, bi, acc.r[t], 0, 0, 0);
+                acc.i[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, br, acc.i[t], 0, 0, 0);
             }
         }
     }
@@ -226,8 +232,7 @@ __device__ inline void cmul(const R *Ar, const R *Ai, const R *Br, const R *Bi, 
             bf_steps<NMAX, DA, DB, -1>(arh, aih, brh, bih, acc);
         }
     } else {
-        cmul_half<P, NMAX, DA, DB, false>(Ar, Br, Bi, acc);
-        cmul_half<P, NMAX, DA, DB, true>(Ai, Br, Bi, acc);
+        cmul_vec<P, NMAX, DA, DB>(Ar, Ai, Br, Bi, acc);
     }
 }
 
