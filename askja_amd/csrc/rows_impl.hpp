@@ -113,9 +113,10 @@ __device__ inline void mom_el(const R *__restrict__ b, int n, int i, int j, R &r
 
 // Link of logical index t = (site block, direction): the D links of one site
 // are consecutive, sites interleaved even / odd (parity_block with blocks of
-// one site), workgroups XCD-contiguous.
-template <int N, typename R, bool Exact, bool Drift>
-__global__ __launch_bounds__(kT) void k_kick(KickArgs a)
+// one site), workgroups XCD-contiguous.  MW = the wavefronts per SIMD the
+// register allocation must allow (launch-bounds minimum).
+template <int N, typename R, bool Exact, bool Drift, int MW>
+__global__ __launch_bounds__(kT, MW) void k_kick(KickArgs a)
 {
     constexpr int LPB = Cfg<N>::LPB;
     constexpr int SS = Cfg<N>::SS;
@@ -372,14 +373,32 @@ __global__ __launch_bounds__(kT) void k_kick(KickArgs a)
     block_reduce_store<3>(sum, cmp, a.part + 6 * (size_t)lb);
 }
 
+template <int N, typename R, bool Exact, int MW>
+hipError_t kick_mw(const KickArgs &a, bool drift, unsigned nb, hipStream_t st)
+{
+    if (drift) hipLaunchKernelGGL((k_kick<N, R, Exact, true, MW>), dim3(nb), dim3(kT), 0, st, a);
+    else hipLaunchKernelGGL((k_kick<N, R, Exact, false, MW>), dim3(nb), dim3(kT), 0, st, a);
+    return hipGetLastError();
+}
+
+// The FP64 kick at N >= 7 keeps more values live than two wavefronts per
+// SIMD allow (256 VGPRs plus spill AGPRs: one wavefront).  Where the grid
+// gives each of the 1024 SIMDs more than about 1.5 workgroups, the variant
+// compiled for two wavefronts (a few registers spilled to scratch) hides
+// more latency: kick+drift SU(8) 10^4 245 -> 212 us, SU(7) 8^4 73.6 ->
+// 58.8 us (1821 workgroups); with 1024 workgroups (SU(8) 8^3 x 4) the
+// one-wavefront variant is faster (52.7 against 57.0 us).  The SU(8) kick
+// without the drift already fits two wavefronts (242 VGPRs) and keeps it.
+constexpr unsigned kTwoWaveMinBlocks = 1536;
+
 template <int N, typename R, bool Exact>
 hipError_t kick(const KickArgs &a, bool drift, hipStream_t st)
 {
     const long nl = a.g.V * a.g.D;
     const unsigned nb = (unsigned)((nl + Cfg<N>::LPB - 1) / Cfg<N>::LPB);
-    if (drift) hipLaunchKernelGGL((k_kick<N, R, Exact, true>), dim3(nb), dim3(kT), 0, st, a);
-    else hipLaunchKernelGGL((k_kick<N, R, Exact, false>), dim3(nb), dim3(kT), 0, st, a);
-    return hipGetLastError();
+    if constexpr (N >= 7 && !Exact && sizeof(R) == 8)
+        if (nb >= kTwoWaveMinBlocks && (N == 7 || drift)) return kick_mw<N, R, Exact, 2>(a, drift, nb, st);
+    return kick_mw<N, R, Exact, 1>(a, drift, nb, st);
 }
 
 }  // namespace rows

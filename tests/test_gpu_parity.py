@@ -17,8 +17,9 @@ pytestmark = pytest.mark.gpu
 # (N, extents): register path N <= 3 (incl. odd extents, size-1 directions
 # and a volume that is not a multiple of the 64-link tile), workgroup path
 # N >= 4 for every size class.  For 4 <= N <= 8 the small lattices run the
-# workgroup-per-link kernel and the last two cases (at least 2560 links) the
-# row-per-thread kernel.
+# workgroup-per-link kernel and the last three cases (at least 2560 links) the
+# row-per-thread kernel; the last one has enough workgroups (1821) for the
+# FP64 row kernel's two-wavefront variant.
 CASES = [
     (2, [4, 3, 2, 2]),
     (3, [4, 4, 4, 4]),
@@ -30,6 +31,7 @@ CASES = [
     (64, [2, 2]),
     (4, [8, 8, 8, 8]),
     (8, [8, 8, 8, 4]),
+    (7, [8, 8, 8, 8]),
 ]
 IDS = [f"N{n}-{'x'.join(map(str, e))}" for n, e in CASES]
 
