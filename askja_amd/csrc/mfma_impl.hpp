@@ -453,12 +453,15 @@ __device__ inline R mom_el(const R *__restrict__ P, long l, int n, int i, int j,
 // LS: link storage type (FP32 links with FP64 arithmetic for the exact
 // action of the reduced-precision modes).  ActionOnly: only sum ReTr W
 // (= 4 sum of the plaquettes touching the link) into the partials.
-// FP64 at NMAX = 32 needs 236 VGPRs plus 64 AGPRs, one wavefront per SIMD;
-// compiled for two (a few registers spilled to scratch) the kick+drift runs
-// 22-25 % faster (SU(32) 8^4 2.18 -> 1.67 ms, SU(24) 8^4 2.11 -> 1.64 ms,
-// SU(32) 96^2 1.63 -> 1.22 ms).
+// Register budgets (launch-bounds minimum of wavefronts per SIMD).  FP64 at
+// NMAX = 32 needs 236 VGPRs plus 64 AGPRs, one wavefront per SIMD; compiled
+// for two (a few registers spilled to scratch) the kick+drift runs 22-25 %
+// faster (SU(32) 8^4 2.18 -> 1.67 ms, SU(24) 8^4 2.11 -> 1.64 ms, SU(32)
+// 96^2 1.63 -> 1.22 ms).  FP64 at NMAX = 16: four instead of three
+// (126 VGPRs + 32 AGPRs before, 16-24 bytes spilled now): SU(16) 6x24^3
+// 4.61 -> 4.46 ms, its action pass over FP32 links 3.12 -> 2.92 ms.
 template <int P, int NMAX>
-constexpr int kMinWaves = (P == kF64 && NMAX == 32) ? 2 : 1;
+constexpr int kMinWaves = (P == kF64 && NMAX == 32) ? 2 : (P == kF64 && NMAX == 16) ? 4 : 1;
 
 template <int P, int NMAX, bool Drift, typename LS = typename Pol<P>::R, bool ActionOnly = false>
 __global__ __launch_bounds__((Cfg<P, NMAX>::NT), (kMinWaves<P, NMAX>)) void k_kick(KickArgs a)
