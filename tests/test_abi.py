@@ -75,3 +75,11 @@ def test_create_rejects_bad_arguments_before_any_device_call():
         assert out is None, args
     null_out = lib.askja_create(3, 2, (ctypes.c_int * 2)(4, 4), 0, None)
     assert null_out == EINVAL
+
+
+def test_integration_guide_lists_every_entry_point():
+    """INTEGRATION.md's entry-point index names every function the header
+    declares, so the guide cannot fall behind the ABI."""
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    missing = [n for n in declared_functions() if f"`{n}`" not in text]
+    assert not missing, missing
