@@ -325,16 +325,18 @@ def run_chain(lat, args, tag):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    # 3000 trajectories (about 40 s): tau_int(|P|) is about 20 here, and
-    # Wolff's window underestimates it on runs much shorter than 100 tau
-    # (1000 trajectories gave 13.8)
-    ap.add_argument("--steps", type=int, default=3000)
+    # 12000 trajectories (about 160 s).  tau_int(|P|) is about 45-50 here and
+    # Wolff's window underestimates it on shorter runs of this chain: 13.8
+    # after 1000 trajectories, 20.2 after 3000, 34.6 after 8000, 44.4 after
+    # 10000, 44.6 after 12000, 50.1 +- 8.5 after 20000 (window 143)
+    ap.add_argument("--steps", type=int, default=12000)
     ap.add_argument("--warmup", type=int, default=300)
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--md-steps", type=int, default=NDRIFT, help="MD steps per trajectory (length = steps x DT)")
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-mixed", action="store_true", help="skip the FP32 mixed-precision chain")
+    ap.add_argument("--mixed", action="store_true",
+                    help="also run the chain in the FP32 mode (another --steps trajectories)")
     ap.add_argument("--roofline-launches", type=int, default=100)
     ap.add_argument("--copy-probe", action="store_true", help="only run a 1 GiB device copy (counter calibration)")
     ap.add_argument("--study", choices=["precision", "c3scan", "c5exact", "kernels", "example"],
@@ -419,7 +421,7 @@ def main():
                      "kernel": "fused kick+drift", "ms_per_launch": ms, "bytes_per_link": bytes_per_link,
                      "links_per_launch": nl, "tflops": tflops},
     }
-    if prec == A.FP64 and not args.no_mixed:
+    if prec == A.FP64 and args.mixed:
         # the same chain with FP32 links and MD (FP64 sums, reunitarisation and
         # Metropolis energies of the stored links): the mixed-precision side
         # of this workload's FP64 / FP32 comparison
