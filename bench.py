@@ -2,12 +2,12 @@
 
 Workload (the configuration the headline metric is quoted on): plain HMC for
 SU(3) in D=4 on an 8x32^3 lattice (N0 = 8 temporal), beta = 5.9, DT = 0.02,
-LEVEL = 6, hot (Haar-random) start, 50 MD steps per trajectory (tau = 1),
+LEVEL = 6, hot (Haar-random) start, 100 MD steps per trajectory (length 2),
 counter-based momenta, FP64 (or --precision fp32: FP32 links and momenta with
 FP64 sums and Metropolis).  Data are synthetic: the links start Haar random
 from a fixed seed.
 
-One step = one HMC trajectory (momentum refresh, 50 fused kick+drift MD
+One step = one HMC trajectory (momentum refresh, 100 fused kick+drift MD
 steps, the final half kick, H_old/H_new sums, Metropolis) plus the
 Polyakov-loop measurement that feeds the autocorrelation time.  --warmup
 trajectories thermalise the chain untimed; --steps trajectories are timed.
@@ -30,7 +30,13 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-NC, EXT, BETA, DT, LEVEL, NDRIFT = 3, [8, 32, 32, 32], 5.9, 0.02, 6, 50
+# Trajectory length (the workload fixes DT, not the number of MD steps per
+# trajectory), measured on this chain with long runs, independent
+# configurations/s = traj/s / tau_int(|P|): 25 steps 0.37 (tau about 400),
+# 50 steps 1.52 (tau 50.1 +- 8.5), 100 steps 1.89 (tau 20.4 +- 2.9), 150 steps
+# 1.99 (tau 13.0 +- 1.8).  100 steps: near the best, and tau settles within
+# a few minutes of GPU time.
+NC, EXT, BETA, DT, LEVEL, NDRIFT = 3, [8, 32, 32, 32], 5.9, 0.02, 6, 100
 SEED = 120351616
 HBM_PEAK_GBS = 8000.0
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}  # MI355X dense
@@ -325,11 +331,12 @@ def run_chain(lat, args, tag):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    # 12000 trajectories (about 160 s).  tau_int(|P|) is about 45-50 here and
-    # Wolff's window underestimates it on shorter runs of this chain: 13.8
-    # after 1000 trajectories, 20.2 after 3000, 34.6 after 8000, 44.4 after
-    # 10000, 44.6 after 12000, 50.1 +- 8.5 after 20000 (window 143)
-    ap.add_argument("--steps", type=int, default=12000)
+    # 6000 trajectories of 100 MD steps (about 160 s).  Wolff's window
+    # underestimates tau_int(|P|) on runs too short to show the slow tail of
+    # the autocorrelation (50-step trajectories: 20.2 after 3000, 50.1 +- 8.5
+    # after 20000); with 100 steps the estimate is settled from about 6000
+    # trajectories on (20.6, 20.7, 21.2, 20.4 after 6000, 8000, 10000, 12000)
+    ap.add_argument("--steps", type=int, default=6000)
     ap.add_argument("--warmup", type=int, default=300)
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--md-steps", type=int, default=NDRIFT, help="MD steps per trajectory (length = steps x DT)")
