@@ -327,6 +327,12 @@ class Lattice:
         return float(np.mean(np.abs(lines[:: stride][: self.ext[0]])))
 
     # -- HMC --------------------------------------------------------------------------
+    def uniform(self, seed, counter):
+        """The Metropolis uniform of trajectory `counter` (askja_uniform)."""
+        v = C.c_double()
+        self._ck(lib().askja_uniform(self._h, seed, counter, C.byref(v)), "uniform")
+        return v.value
+
     def hmc_trajectory(self, beta, dt, ndrift, level, seed, counter, reunit=True):
         p = HmcParams(beta, dt, ndrift, level, 1 if reunit else 0, seed)
         r = HmcResult()

@@ -59,6 +59,7 @@ struct askja_ctx {
     void *flow_work = nullptr;   // backward working field
     void *flow_log = nullptr;    // input links of every sub-layer
     double *flow_contrib = nullptr;  // backward neighbour contributions
+    void *flow_save = nullptr;       // chain state at the start of a flow trajectory
 };
 
 namespace {
@@ -378,7 +379,7 @@ int askja_destroy(askja_ctx *c)
         if (c->links[b]) (void)hipFree(c->links[b]);
     void *bufs[] = {c->snap, c->mom, c->mom_prev, c->part, c->res, c->scratch, c->staging,
                     c->d_nbr, c->d_dev2lex, c->d_lex2dev, c->flow_w, c->flow_gw, c->flow_g, c->flow_lj,
-                    c->flow_out, c->flow_work, c->flow_log, c->flow_contrib};
+                    c->flow_out, c->flow_work, c->flow_log, c->flow_contrib, c->flow_save};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     if (c->cst) (void)hipStreamSynchronize(c->cst);
@@ -1098,12 +1099,13 @@ int askja_flow_define(askja_ctx *c, int nsub, const int *mu, const int *parity, 
             return fail(c, ASKJA_EINVAL, "flow_define: sub-layer %d out of range", k);
     const int nf = 3 * (c->dim - 1) * (c->dim - 2) / 2;  // plaquette and two rectangles per plane
     // (re)allocate
-    void *old[] = {c->flow_w, c->flow_gw, c->flow_g, c->flow_lj, c->flow_out, c->flow_work, c->flow_log, c->flow_contrib};
+    void *old[] = {c->flow_w,    c->flow_gw,   c->flow_g,        c->flow_lj, c->flow_out,
+                   c->flow_work, c->flow_log,  c->flow_contrib, c->flow_save};
     for (void *p : old)
         if (p) (void)hipFree(p);
     c->flow_w = nullptr;
     c->flow_gw = c->flow_g = c->flow_lj = nullptr;
-    c->flow_out = c->flow_work = c->flow_log = nullptr;
+    c->flow_out = c->flow_work = c->flow_log = c->flow_save = nullptr;
     c->flow_contrib = nullptr;
     c->flow_nsub = 0;
     c->flow_bsize = 32 * nf + 2 * 32 + 1;
@@ -1117,6 +1119,7 @@ int askja_flow_define(askja_ctx *c, int nsub, const int *mu, const int *parity, 
     HIPCHK(c, hipMalloc(&c->flow_work, link_bytes(c)));
     HIPCHK(c, hipMalloc(&c->flow_log, (size_t)nsub * flow_log_reals(c->geom) * elem_bytes(c)));
     HIPCHK(c, hipMalloc((void **)&c->flow_contrib, flow_contrib_doubles(c->geom) * sizeof(double)));
+    HIPCHK(c, hipMalloc(&c->flow_save, link_bytes(c)));
     c->flow_subs.resize(nsub);
     c->flow_block.assign(block, block + nsub);
     for (int k = 0; k < nsub; k++)
@@ -1231,7 +1234,9 @@ int askja_flow_hmc_trajectory(askja_ctx *c, const askja_hmc_params *p, uint64_t 
     if (!(p->beta > 0) || !(p->dt > 0) || p->ndrift < 1)
         return fail(c, ASKJA_EINVAL, "flow_hmc_trajectory: beta, dt and ndrift must be positive");
     if (p->reunit && (rc = askja_reunitarize(c))) return rc;
-    if ((rc = askja_snapshot(c, 0))) return rc;
+    // the chain state goes to the flow's own buffer (askja_snapshot's slot
+    // stays the caller's)
+    HIPCHK(c, hipMemcpyAsync(c->flow_save, c->links[c->cur], link_bytes(c), hipMemcpyDeviceToDevice, c->st));
     if ((rc = askja_mom_refresh(c, p->seed, traj_counter, p->beta))) return rc;
     const double cf = 2.0 * c->nc / p->beta;
     double s0, lj0, k0, s1, lj1, k1;
@@ -1258,7 +1263,8 @@ int askja_flow_hmc_trajectory(askja_ctx *c, const askja_hmc_params *p, uint64_t 
     else if (r->uniform < std::exp(r->h_old - r->h_new)) r->accept_case = 2;
     else r->accept_case = 0;
     r->accepted = r->accept_case != 0;
-    if (!r->accepted && (rc = askja_snapshot(c, 1))) return rc;
+    if (!r->accepted)
+        HIPCHK(c, hipMemcpyAsync(c->links[c->cur], c->flow_save, link_bytes(c), hipMemcpyDeviceToDevice, c->st));
     r->action = r->accepted ? s1 : s0;
     return ASKJA_OK;
 }

@@ -66,17 +66,107 @@ __host__ __device__ inline U4 philox_block(uint64_t seed, uint64_t counter, uint
 
 __host__ __device__ inline double u01(uint32_t x) { return ((double)x + 0.5) * 2.3283064365386963e-10; }
 
+// ---------------------------------------------- reproducible Box-Muller
+// The logarithm and the sine/cosine of the Box-Muller transform are written
+// here as fixed sequences of IEEE-754 double operations (exact scalings and
+// subtractions, one correctly rounded division, correctly rounded sqrt, FMA
+// Horner steps, no contraction), so that any IEEE machine that performs the
+// same steps (the host C library's fma and sqrt are correctly rounded too)
+// gets the same bits: the Philox momenta of the device equal a host
+// computation bit for bit.  Accuracy: within 2 ulp of the true values.
+
+// log x for normal x > 0: x = m 2^e with m in [sqrt(1/2), sqrt(2)),
+// log m = 2 atanh(s) = 2 s sum_k s^2k / (2k + 1), s = (m - 1) / (m + 1),
+// |s| <= 0.1716 so 13 terms reach 2^-70; e log 2 with log 2 split in a
+// 32-bit head (e ln2_hi exact) and a tail.
+__device__ inline double repro_log(double x)
+{
+#pragma clang fp contract(off)
+    const long long b = __double_as_longlong(x);
+    int e = (int)((b >> 52) & 0x7ff) - 1023;
+    double m = __longlong_as_double((b & 0x000fffffffffffffLL) | 0x3ff0000000000000LL);
+    if (m > 1.4142135623730951) {
+        m = m * 0.5;
+        e = e + 1;
+    }
+    const double f = m - 1.0;  // exact (Sterbenz)
+    const double s = f / (m + 1.0);
+    const double z = s * s;
+    double p = 0.04;
+    p = fma(p, z, 0.043478260869565216);
+    p = fma(p, z, 0.047619047619047616);
+    p = fma(p, z, 0.05263157894736842);
+    p = fma(p, z, 0.058823529411764705);
+    p = fma(p, z, 0.06666666666666667);
+    p = fma(p, z, 0.07692307692307693);
+    p = fma(p, z, 0.09090909090909091);
+    p = fma(p, z, 0.1111111111111111);
+    p = fma(p, z, 0.14285714285714285);
+    p = fma(p, z, 0.2);
+    p = fma(p, z, 0.3333333333333333);
+    p = fma(p, z, 1.0);
+    const double lm = (2.0 * s) * p;
+    const double de = (double)e;
+    return de * 0.6931471803691238 + fma(de, 1.9082149292705877e-10, lm);
+}
+
+// (sin, cos) of 2 pi u for u in [0, 1): t = 4u, quadrant q = floor(t + 1/2),
+// theta = (t - q) pi / 2 in [-pi/4, pi/4] (t - q exact), Taylor polynomials
+// to theta^19 / theta^20, then the exact quadrant rotation.
+__device__ inline void repro_sincos2pi(double u, double *sn, double *cs)
+{
+#pragma clang fp contract(off)
+    const double t = 4.0 * u;
+    const double q = floor(t + 0.5);
+    const double th = (t - q) * 1.5707963267948966;
+    const double z = th * th;
+    double ps = -8.22063524662433e-18;
+    ps = fma(ps, z, 2.8114572543455206e-15);
+    ps = fma(ps, z, -7.647163731819816e-13);
+    ps = fma(ps, z, 1.6059043836821613e-10);
+    ps = fma(ps, z, -2.505210838544172e-08);
+    ps = fma(ps, z, 2.7557319223985893e-06);
+    ps = fma(ps, z, -0.0001984126984126984);
+    ps = fma(ps, z, 0.008333333333333333);
+    ps = fma(ps, z, -0.16666666666666666);
+    ps = fma(ps, z, 1.0);
+    double pc = 4.110317623312165e-19;
+    pc = fma(pc, z, -1.5619206968586225e-16);
+    pc = fma(pc, z, 4.779477332387385e-14);
+    pc = fma(pc, z, -1.1470745597729725e-11);
+    pc = fma(pc, z, 2.08767569878681e-09);
+    pc = fma(pc, z, -2.755731922398589e-07);
+    pc = fma(pc, z, 2.48015873015873e-05);
+    pc = fma(pc, z, -0.001388888888888889);
+    pc = fma(pc, z, 0.041666666666666664);
+    pc = fma(pc, z, -0.5);
+    pc = fma(pc, z, 1.0);
+    const double s0 = th * ps, c0 = pc;
+    switch ((int)q & 3) {
+    case 0: *sn = s0; *cs = c0; break;
+    case 1: *sn = c0; *cs = -s0; break;
+    case 2: *sn = -s0; *cs = -c0; break;
+    default: *sn = -c0; *cs = s0; break;
+    }
+}
+
+// Box-Muller pair of two uniforms words: r = sqrt(-2 log u0),
+// (r cos 2 pi u1, r sin 2 pi u1).
+__device__ inline void box_muller(uint32_t w0, uint32_t w1, double *zc, double *zs)
+{
+#pragma clang fp contract(off)
+    const double rad = sqrt(-2.0 * repro_log(u01(w0)));
+    double sn, cs;
+    repro_sincos2pi(u01(w1), &sn, &cs);
+    *zc = rad * cs;
+    *zs = rad * sn;
+}
+
 // 4 normals from one Philox block: two Box-Muller pairs.
 __device__ inline void normals4(U4 r, double z[4])
 {
-    double rad = sqrt(-2.0 * log(u01(r.x)));
-    double ang = 6.283185307179586 * u01(r.y);
-    z[0] = rad * cos(ang);
-    z[1] = rad * sin(ang);
-    rad = sqrt(-2.0 * log(u01(r.z)));
-    ang = 6.283185307179586 * u01(r.w);
-    z[2] = rad * cos(ang);
-    z[3] = rad * sin(ang);
+    box_muller(r.x, r.y, &z[0], &z[1]);
+    box_muller(r.z, r.w, &z[2], &z[3]);
 }
 
 // ------------------------------------------------------------ XCD mapping

@@ -212,20 +212,6 @@ __device__ inline void st(R *U, long l, const C3 &m)
 
 // gradient buffer: 9 doubles per link (compact Hermitian: diag, then
 // (0,1), (0,2), (1,2) as re, im)
-__device__ inline void gadd(double *g, long l, const C3 &h)
-{
-    double *b = g + 9 * (size_t)l;
-    atomicAdd(b + 0, h.r[0][0]);
-    atomicAdd(b + 1, h.r[1][1]);
-    atomicAdd(b + 2, h.r[2][2]);
-    atomicAdd(b + 3, h.r[0][1]);
-    atomicAdd(b + 4, h.i[0][1]);
-    atomicAdd(b + 5, h.r[0][2]);
-    atomicAdd(b + 6, h.i[0][2]);
-    atomicAdd(b + 7, h.r[1][2]);
-    atomicAdd(b + 8, h.i[1][2]);
-}
-
 __device__ inline C3 gget(const double *g, long l)
 {
     const double *b = g + 9 * (size_t)l;

@@ -208,10 +208,9 @@ __device__ inline double normal_at(uint64_t seed, uint64_t counter, uint32_t pur
 {
     const U4 r = philox_block(seed, counter, purpose, link, (uint32_t)(a >> 2));
     const int h = (a >> 1) & 1;
-    const uint32_t x0 = h ? r.z : r.x, x1 = h ? r.w : r.y;
-    const double rad = sqrt(-2.0 * log(u01(x0)));
-    const double ang = 6.283185307179586 * u01(x1);
-    return (a & 1) ? rad * sin(ang) : rad * cos(ang);
+    double zc, zs;
+    box_muller(h ? r.z : r.x, h ? r.w : r.y, &zc, &zs);
+    return (a & 1) ? zs : zc;
 }
 
 // pi = c sum_a xi_a T^a with the reference's generator order (sun.cpp:142-189,
@@ -221,6 +220,7 @@ __device__ inline double normal_at(uint64_t seed, uint64_t counter, uint32_t pur
 template <typename R>
 __global__ void k_mom_refresh(Geom g, R *P, uint64_t seed, uint64_t counter, double c)
 {
+#pragma clang fp contract(off)
     const long l = (long)blockIdx.x * kB + threadIdx.x;
     if (l >= g.V * g.D) return;
     const int n = g.N;

@@ -108,13 +108,24 @@ bool store_parameters(const char *infile, const char *outfile, const Params &p, 
     return (bool)myout;
 }
 
-// Configuration output on a host thread: two page-locked buffers, one
-// writer at a time (joined before the next starts and before exit).
+// Configuration output: the device-to-host copy of a stored configuration
+// runs on the library's copy stream while the chain goes on; the copy is
+// waited for at the end of the next trajectory (or before the next store, or
+// at exit), and the file and master file are then written on a host thread.
+// Two page-locked buffers, one writer at a time (joined before the next
+// starts and before exit); only the main thread calls the library.
 struct ConfWriter {
     double *pinned[2] = {nullptr, nullptr};
     int next = 0;
     std::thread t;
     std::atomic<bool> ok{true};
+    // the configuration whose copy is in flight
+    bool pending = false;
+    std::string conf, master;
+    Params snap;
+    int tk = 0;
+    double *data = nullptr;
+    size_t nd = 0;
     bool join()
     {
         if (t.joinable()) t.join();
@@ -345,6 +356,28 @@ int main(int argc, char *argv[])
     const time_t w0 = time(NULL);
     const auto t0 = std::chrono::steady_clock::now();
     ConfWriter out;
+    // wait for the in-flight copy, then hand the buffer to the writer thread
+    auto start_writer = [&]() -> int {
+        if (!out.pending) return 0;
+        out.pending = false;
+        int rc = askja_download_wait(ctx);
+        if (rc != ASKJA_OK) return die(ctx, rc, "askja_download_wait(ctx)");
+        if (!out.join()) return 1;
+        out.t = std::thread([&out = out, conf = out.conf, master = out.master, snap = out.snap, tk = out.tk,
+                             data = out.data, nd = out.nd, paramsfile]() {
+            FILE *f = fopen(conf.c_str(), "wb");
+            bool ok = f && fwrite(data, sizeof(double), nd, f) == nd;
+            if (f) fclose(f);
+            if (!ok) fprintf(stderr, "askja: writing %s failed\n", conf.c_str());
+            else ok = store_parameters(paramsfile, master.c_str(), snap, tk, snap.TAKEN);
+            if (!ok) out.ok = false;
+        });
+        return 0;
+    };
+#define CKW()                          \
+    do {                               \
+        if (int w_ = start_writer()) return w_; \
+    } while (0)
     int timekeep;
     for (timekeep = P.START + 1; timekeep < P.STEPS + P.START + 1; timekeep++) {
         CK(drift());
@@ -375,6 +408,7 @@ int main(int argc, char *argv[])
             double myenergy;
             CK(hamiltonian(myenergy));
             printf("\t INFO: Final energy =%f\n", myenergy);
+            CKW();  // the last stored configuration's copy overlapped this trajectory
             P.NTOT++;
             // Metropolis uniform: Philox (key = seed, purpose 3, counter)
             bool accept;
@@ -424,35 +458,33 @@ int main(int argc, char *argv[])
                 CK(flush());
                 snprintf(buffer, sizeof buffer, "%s/U-%s-%i.conf", outdir, tag, P.TAKEN);
                 printf("==>Info: storing configuration %s\n", buffer);
-                // device -> pinned buffer, then the file and the master file
-                // are written on a host thread while the chain continues; the
-                // writes stay in order (one writer at a time)
+                // device -> pinned buffer on the copy stream; the file and
+                // the master file are written once the copy has landed (at
+                // the end of the next trajectory), in order, one writer at a
+                // time.  The buffer refilled here was last written by the
+                // writer before the one just started, which start_writer
+                // has joined.
+                CKW();
                 const size_t nd = askja_field_doubles(ctx);
                 if (!out.pinned[out.next]) CK(askja_host_alloc(nd, &out.pinned[out.next]));
                 CK(askja_links_download_async(ctx, out.pinned[out.next]));
-                CK(askja_download_wait(ctx));
-                if (!out.join()) return 1;
                 P.TAKEN++;
-                const std::string conf = buffer;
+                out.conf = buffer;
                 snprintf(buffer, sizeof buffer, "%s/master-%s.params", outdir, tag);
-                const std::string master = buffer;
-                const Params snap = P;
-                const int tk = timekeep;
-                double *data = out.pinned[out.next];
-                out.t = std::thread([&out = out, conf, master, snap, tk, data, nd, paramsfile]() {
-                    FILE *f = fopen(conf.c_str(), "wb");
-                    bool ok = f && fwrite(data, sizeof(double), nd, f) == nd;
-                    if (f) fclose(f);
-                    if (!ok) fprintf(stderr, "askja: writing %s failed\n", conf.c_str());
-                    else ok = store_parameters(paramsfile, master.c_str(), snap, tk, snap.TAKEN);
-                    if (!ok) out.ok = false;
-                });
+                out.master = buffer;
+                out.snap = P;
+                out.tk = timekeep;
+                out.data = out.pinned[out.next];
+                out.nd = nd;
+                out.pending = true;
                 out.next ^= 1;
             }
         }
     }
     CK(flush());
     CK(askja_sync(ctx));
+    CKW();
+#undef CKW
     if (!out.join()) return 1;
     fclose(constraints);
     fclose(wilson);

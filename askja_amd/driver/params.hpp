@@ -145,7 +145,10 @@ inline std::vector<double> unitary_phases(const std::vector<std::complex<double>
         }
         out[k] = std::arg(r);
     }
-    std::sort(out.begin(), out.end());
+    // NaN phases (links overflowed by a diverging trajectory) sort last: a
+    // strict weak order whatever the values
+    std::sort(out.begin(), out.end(),
+              [](double a, double b) { return a < b || (!std::isnan(a) && std::isnan(b)); });
     return out;
 }
 

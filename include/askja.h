@@ -122,7 +122,9 @@ int askja_mom_download_prev(askja_ctx *ctx, double *aos);
 int askja_mom_gsl(askja_ctx *ctx, unsigned long seed, double beta);
 /* initializeE, counter-based form (used for every trajectory after the
  * first): xi from Philox-4x32-10 (key = seed, counter = counter) and
- * Box-Muller, generated on the device. */
+ * Box-Muller, generated on the device.  The Box-Muller logarithm and
+ * sine/cosine are fixed sequences of IEEE double operations, so the momenta
+ * are reproducible bit for bit by any IEEE host performing the same steps. */
 int askja_mom_refresh(askja_ctx *ctx, uint64_t seed, uint64_t counter, double beta);
 /* Flip the sign of the momenta (reversibility checks). */
 int askja_mom_negate(askja_ctx *ctx);
@@ -213,7 +215,10 @@ typedef struct {
  * the state before the trajectory.  NaN energies reject, as in the reference.
  * In the FP32 and BF16x3 modes H_old and H_new are the FP64-arithmetic
  * energies of the stored FP32 fields (an extra action pass at each end), so
- * the chain is exact for them whatever the MD precision. */
+ * the accept test itself carries no reduced-precision error; the chain is
+ * still not exact there, because FP32/BF16x3 MD is reversible and
+ * area-preserving only to its rounding (and reunit projects the FP32 links
+ * outside the accept test). */
 int askja_hmc_trajectory(askja_ctx *ctx, const askja_hmc_params *p, uint64_t traj_counter,
                          askja_hmc_result *res);
 
@@ -269,7 +274,8 @@ int askja_flow_gradient_download(askja_ctx *ctx, double *aos);
 int askja_flow_weight_gradient(askja_ctx *ctx, double *out);
 /* One field-transformation HMC trajectory in V: H = beta S_W(F(V)) - log J(V)
  * + beta/(2N) sum Tr P^2, kick-drift-kick with the flow force, exact
- * Metropolis; on reject V returns to its value before the trajectory. */
+ * Metropolis; on reject V returns to its value before the trajectory (kept
+ * in a buffer of the flow's own: the askja_snapshot slot is not touched). */
 int askja_flow_hmc_trajectory(askja_ctx *ctx, const askja_hmc_params *p, uint64_t traj_counter,
                               askja_hmc_result *res);
 /* askja_wilson_lines of U = F(V). */

@@ -812,6 +812,60 @@ void oracle_philox4x32_10(const uint32_t *ctr_in, const uint32_t *key_in, uint32
 /* uniform in (0,1) from 32 bits: (x + 0.5) / 2^32 */
 static double u01(uint32_t x) { return ((double)x + 0.5) * 2.3283064365386963e-10; }
 
+/* Box-Muller with the logarithm and the sine/cosine written as fixed
+ * sequences of IEEE double operations (exact scalings, one correctly rounded
+ * division, correctly rounded sqrt and fma from the C library, Horner steps),
+ * so that a device performing the same steps produces the same bits.
+ * log x: x = m 2^e, m in [sqrt(1/2), sqrt(2)), log m = 2 atanh s =
+ * 2 s sum_{k=0}^{12} s^2k/(2k+1), s = (m-1)/(m+1); e log 2 with log 2 split
+ * as a 32-bit head and a tail.  (sin, cos)(2 pi u): t = 4u, q = floor(t+1/2),
+ * theta = (t-q) pi/2, Taylor polynomials to theta^19 / theta^20, exact
+ * quadrant rotation.  Both are checked against the C library to 2 ulp in the
+ * tests. */
+double oracle_repro_log(double x)
+{
+    uint64_t b;
+    memcpy(&b, &x, sizeof b);
+    int e = (int)((b >> 52) & 0x7ff) - 1023;
+    uint64_t mb = (b & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL;
+    double m;
+    memcpy(&m, &mb, sizeof m);
+    if (m > 1.4142135623730951) {
+        m = m * 0.5;
+        e = e + 1;
+    }
+    double f = m - 1.0, s = f / (m + 1.0), z = s * s;
+    static const double c[13] = {1.0, 1.0 / 3, 1.0 / 5, 1.0 / 7, 1.0 / 9, 1.0 / 11, 1.0 / 13,
+                                 1.0 / 15, 1.0 / 17, 1.0 / 19, 1.0 / 21, 1.0 / 23, 1.0 / 25};
+    double p = c[12];
+    for (int k = 11; k >= 0; k--) p = fma(p, z, c[k]);
+    double lm = (2.0 * s) * p, de = (double)e;
+    double hi = de * 0.6931471803691238;
+    return hi + fma(de, 1.9082149292705877e-10, lm);
+}
+
+void oracle_repro_sincos2pi(double u, double *sn, double *cs)
+{
+    double t = 4.0 * u, q = floor(t + 0.5), th = (t - q) * 1.5707963267948966, z = th * th;
+    /* (-1)^k / (2k+1)! and (-1)^k / (2k)!, nearest doubles */
+    static const double ks[10] = {1.0, -0.16666666666666666, 0.008333333333333333, -0.0001984126984126984,
+                                  2.7557319223985893e-06, -2.505210838544172e-08, 1.6059043836821613e-10,
+                                  -7.647163731819816e-13, 2.8114572543455206e-15, -8.22063524662433e-18};
+    static const double kc[11] = {1.0, -0.5, 0.041666666666666664, -0.001388888888888889, 2.48015873015873e-05,
+                                  -2.755731922398589e-07, 2.08767569878681e-09, -1.1470745597729725e-11,
+                                  4.779477332387385e-14, -1.5619206968586225e-16, 4.110317623312165e-19};
+    double ps = ks[9], pc = kc[10];
+    for (int k = 8; k >= 0; k--) ps = fma(ps, z, ks[k]);
+    for (int k = 9; k >= 0; k--) pc = fma(pc, z, kc[k]);
+    double s0 = th * ps, c0 = pc;
+    switch ((int)q & 3) {
+    case 0: *sn = s0; *cs = c0; break;
+    case 1: *sn = c0; *cs = -s0; break;
+    case 2: *sn = -s0; *cs = -c0; break;
+    default: *sn = -c0; *cs = s0; break;
+    }
+}
+
 /* Normal deviates for stream `purpose` of link `link` at counter `counter`:
  * Philox(ctr = {block, link, counter_lo, counter_hi ^ purpose<<24}, key =
  * seed) -> 4 uniforms -> 2 Box-Muller pairs (r = sqrt(-2 ln u0),
@@ -826,11 +880,11 @@ void oracle_philox_normals(uint64_t seed, uint64_t counter, uint32_t purpose, ui
         uint32_t r[4];
         oracle_philox4x32_10(ctr, key, r);
         for (int h = 0; h < 2; h++) {
-            double rad = sqrt(-2.0 * log(u01(r[2 * h])));
-            double ang = 6.283185307179586 * u01(r[2 * h + 1]);
+            double rad = sqrt(-2.0 * oracle_repro_log(u01(r[2 * h]))), sn, cs;
+            oracle_repro_sincos2pi(u01(r[2 * h + 1]), &sn, &cs);
             int q = 4 * b + 2 * h;
-            if (q < n) out[q] = rad * cos(ang);
-            if (q + 1 < n) out[q + 1] = rad * sin(ang);
+            if (q < n) out[q] = rad * cs;
+            if (q + 1 < n) out[q + 1] = rad * sn;
         }
     }
 }
@@ -859,11 +913,20 @@ void oracle_mom_philox(oracle_t *o, uint64_t seed, uint64_t counter)
                     p[2 * (s * nc + r) + 1] = im;
                     a += 2;
                 }
-            for (int m = 0; m < nc - 1; m++) {
-                double h = c * xi[a] * sqrt(0.5 / ((double)(m + 1) * (m + 2)));
-                for (int r = 0; r <= m; r++) p[2 * (r * nc + r)] += h;
-                p[2 * ((m + 1) * nc + m + 1)] -= (m + 1) * h;
-                a++;
+            /* Cartan part: h_m = c xi_m / sqrt(2 (m+1)(m+2)) multiplies
+             * diag(1, .., 1, -(m+1), 0, ..), so pi_rr = sum_{m >= r} h_m -
+             * r h_{r-1}; summed from the top (m descending), then the
+             * subtraction */
+            double suffix = 0.0;
+            for (int r = nc - 1; r >= 0; r--) {
+                double v = suffix;
+                if (r >= 1) {
+                    int m = r - 1;
+                    double h = c * xi[a + m] * sqrt(0.5 / ((double)(m + 1) * (m + 2)));
+                    v -= (double)r * h;
+                    suffix += h;
+                }
+                p[2 * (r * nc + r)] = v;
             }
         }
     free(xi);

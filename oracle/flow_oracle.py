@@ -121,3 +121,119 @@ def phi(o, V, subs, weights, bsize, nf, rho_max, delta, beta):
     """Phi = beta S_W(F(V)) - log J(V)."""
     U, lj = forward(o, V, subs, weights, bsize, nf, rho_max, delta)
     return beta * energy_mag(o, U) - lj
+
+
+# -- batched restatement (whole lattice at once) --------------------------------
+# The same map, evaluated for every masked link of a sub-layer at once with
+# batched 3x3 products, so that lattices of the 16^4 size are checked in
+# seconds; the exponential of the Hermitian i theta Lhat is taken from the
+# eigendecomposition (agrees with expm to rounding).  Pinned against the
+# per-link restatement above on small lattices.
+
+def neighbours(ext):
+    """(up[d][x], down[d][x], parity[x]) for lexicographic sites (x[D-1]
+    fastest), periodic +-1 steps (structs.h:169-218)."""
+    ext = list(ext)
+    V = int(np.prod(ext))
+    c = np.array(np.unravel_index(np.arange(V), ext))
+    up, dn = [], []
+    for d in range(len(ext)):
+        cu = c.copy()
+        cu[d] = (cu[d] + 1) % ext[d]
+        up.append(np.ravel_multi_index(cu, ext))
+        cd = c.copy()
+        cd[d] = (cd[d] - 1) % ext[d]
+        dn.append(np.ravel_multi_index(cd, ext))
+    return np.array(up), np.array(dn), c.sum(axis=0) % 2
+
+
+def _dag(a):
+    return np.conj(np.swapaxes(a, -1, -2))
+
+
+def _tr(a):
+    return np.trace(a, axis1=-2, axis2=-1)
+
+
+def _tah_b(c):
+    d = c - _dag(c)
+    d = d - (_tr(d) / 3)[..., None, None] * np.eye(3)
+    return -0.5j * d
+
+
+def link_inputs_batched(up, dn, U, xs, mu):
+    """S, C and the features of the links (xs, mu), as link_inputs."""
+    D = U.shape[1]
+    xp = up[mu][xs]
+    S = np.zeros((len(xs), 3, 3), complex)
+    C = np.zeros_like(S)
+    f, fr = [], []
+    for nu in range(D):
+        if nu == mu:
+            continue
+        xpn, xmn = up[nu][xs], dn[nu][xs]
+        xpmn = dn[nu][xp]
+        S += U[xp, nu] @ _dag(U[xpn, mu]) @ _dag(U[xs, nu])
+        S += _dag(U[xpmn, nu]) @ _dag(U[xmn, mu]) @ U[xmn, nu]
+        for la in range(nu + 1, D):
+            if la == mu:
+                continue
+            xla = up[la][xs]
+            P = U[xs, nu] @ U[xpn, la] @ _dag(U[xla, nu]) @ _dag(U[xs, la])
+            f.append(_tr(P).real / 3)
+            C += P
+            x2n, xnl, x2l = up[nu][xpn], up[la][xpn], up[la][xla]
+            R1 = U[xs, nu] @ U[xpn, nu] @ U[x2n, la] @ _dag(U[xnl, nu]) @ _dag(U[xla, nu]) @ _dag(U[xs, la])
+            R2 = U[xs, nu] @ U[xpn, la] @ U[xnl, la] @ _dag(U[x2l, nu]) @ _dag(U[xla, la]) @ _dag(U[xs, la])
+            fr += [_tr(R1).real / 3, _tr(R2).real / 3]
+    return S, C, np.stack(f + fr, axis=1)
+
+
+def _expi_herm(h, theta):
+    """exp(i theta h) for Hermitian h, batched."""
+    w, v = np.linalg.eigh(h)
+    return (v * np.exp(1j * theta[:, None] * w)[:, None, :]) @ _dag(v)
+
+
+def sub_layer_batched(up, dn, par_of, U, mu, par, w, nf, rho_max, delta):
+    xs = np.nonzero(par_of == par)[0]
+    W1 = w[: HID * nf].reshape(HID, nf).astype(np.float64)
+    b1 = w[HID * nf: HID * nf + HID].astype(np.float64)
+    w2 = w[HID * nf + HID: HID * nf + 2 * HID].astype(np.float64)
+    b2 = float(w[HID * nf + 2 * HID])
+    S, C, f = link_inputs_batched(up, dn, U, xs, mu)
+    Wl = U[xs, mu] @ S
+    Lam = _tah_b(C)
+    n = np.sqrt(2 * _tr(Lam @ Lam).real + delta * delta)
+    Lh = Lam / n[:, None, None]
+    z = b2 + np.tanh(f @ W1.T + b1) @ w2
+    rho = rho_max * np.tanh(z)
+    theta = rho * _tr(Lh @ Wl).imag
+    K = _tr(Lh @ Lh @ Wl).real
+    out = U.copy()
+    out[xs, mu] = _expi_herm(Lh, theta) @ U[xs, mu]
+    return out, float(np.sum(np.log1p(rho * K)))
+
+
+def forward_batched(ext, V, subs, weights, bsize, nf, rho_max, delta):
+    """forward() for the whole lattice at once: (U = F(V), log J)."""
+    up, dn, par_of = neighbours(ext)
+    U = V.copy()
+    lj = 0.0
+    for mu, par, blk in subs:
+        U, l = sub_layer_batched(up, dn, par_of, U, mu, par, weights[blk * bsize:(blk + 1) * bsize], nf, rho_max,
+                                 delta)
+        lj += l
+    return U, lj
+
+
+def energy_mag_batched(ext, U):
+    """EnergyMag (sun.cpp:507-537), batched over sites."""
+    up, _, _ = neighbours(ext)
+    D = U.shape[1]
+    s = 0.0
+    for j in range(D):
+        for k in range(j + 1, D):
+            P = U[:, j] @ U[up[j], k] @ _dag(U[up[k], j]) @ _dag(U[:, k])
+            s += float(np.sum(1 - _tr(P).real / 3))
+    return s

@@ -67,6 +67,8 @@ def lib():
             "oracle_polyakov_field": (None, [P, dp]),
             "oracle_philox4x32_10": (None, [up, up, up]),
             "oracle_philox_normals": (None, [U64, U64, U32, U32, I, dp]),
+            "oracle_repro_log": (D, [D]),
+            "oracle_repro_sincos2pi": (None, [D, C.POINTER(D), C.POINTER(D)]),
             "oracle_mom_philox": (None, [P, U64, U64]),
             "oracle_project_sun": (None, [I, dp]),
             "oracle_det": (None, [I, dp, dp]),
@@ -295,3 +297,13 @@ def philox_normals(seed, counter, purpose, link, n):
 
 def num_threads():
     return lib().oracle_num_threads()
+
+
+def repro_log(x):
+    return lib().oracle_repro_log(float(x))
+
+
+def repro_sincos2pi(u):
+    s, c = C.c_double(), C.c_double()
+    lib().oracle_repro_sincos2pi(float(u), C.byref(s), C.byref(c))
+    return s.value, c.value

@@ -164,6 +164,96 @@ def test_resume_continues_numbering(tmp_path):
     assert open(master).read().split()[-2] == "SEED"
 
 
+@pytest.mark.gpu
+def test_resume_reproduces_uninterrupted_run(tmp_path):
+    """fp64-exact, CONFTAKE 1: a 4-trajectory run and a 2 + 2 run resumed from
+    the first run's master file (sun.cpp:890-919, 1158-1171) write the same
+    bytes: configurations U-*-2.conf and U-*-3.conf, the constraints and
+    Wilson rows, and the master file's counters.  The resumed run continues
+    the Philox stream at counter NTOT (the seed travels after NTOT), so the
+    trajectories after the break draw the momenta and uniforms the
+    uninterrupted run draws.  (With CONFTAKE > 1 the reference rounds NACC
+    down to a multiple of CONFTAKE on resume, sun.cpp:904, and the
+    trajectories after the last stored configuration are run again, so only
+    the chain from that configuration on is reproduced.)"""
+    whole, split = tmp_path / "whole", tmp_path / "split"
+    whole.mkdir()
+    split.mkdir()
+    (whole / "p.txt").write_text(PARAMS.format(steps=80))
+    (split / "p.txt").write_text(PARAMS.format(steps=40))
+    r = _run(["p.txt", "--precision", "fp64-exact"], whole)
+    assert r.returncode == 0, r.stdout + r.stderr
+    r = _run(["p.txt", "--precision", "fp64-exact"], split)
+    assert r.returncode == 0, r.stdout + r.stderr
+    master = split / "output-DIM4" / f"master-{TAG}.params"
+    r = _run([str(master), "--precision", "fp64-exact"], split)
+    assert r.returncode == 0, r.stdout + r.stderr
+    ow, os_ = whole / "output-DIM4", split / "output-DIM4"
+    for k in range(4):
+        a = (ow / f"U-{TAG}-{k}.conf").read_bytes()
+        b = (os_ / f"U-{TAG}-{k}.conf").read_bytes()
+        assert a == b, k
+    for name in (f"constraints-{TAG}.dat", f"Wilson-{TAG}.dat"):
+        assert (ow / name).read_text() == (os_ / name).read_text(), name
+    mw, ms = read_master(ow / f"master-{TAG}.params", 4), read_master(master, 4)
+    for k in ("TIME-STEP", "TAKEN", "NACC", "NAC1", "NREC"):
+        assert mw[k] == ms[k], (k, mw[k], ms[k])
+    # the whole master file agrees but for STEPS, which is the split run's
+    # own (40 per invocation against 80)
+    tw, ts = open(ow / f"master-{TAG}.params").read().split(), open(master).read().split()
+    k = tw.index("STEPS") + 1
+    assert (tw[k], ts[k]) == ("80", "40")
+    assert tw[:k] + tw[k + 1:] == ts[:k] + ts[k + 1:]
+
+
+@pytest.mark.gpu
+def test_truncated_configuration_ends_resume(tmp_path):
+    """A resumed run whose last configuration file is short (an interrupted
+    write; the reference's loadconf reads it with gsl_matrix_complex_fread,
+    sun.cpp:751-763) stops with a non-zero status before any output: the
+    constraints and Wilson files and the master file are unchanged and no new
+    configuration appears."""
+    (tmp_path / "p.txt").write_text(PARAMS.format(steps=40))
+    r = _run(["p.txt"], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = tmp_path / "output-DIM4"
+    conf = out / f"U-{TAG}-1.conf"
+    data = conf.read_bytes()
+    conf.write_bytes(data[: len(data) // 2])
+    before = {p.name: p.read_bytes() for p in out.iterdir()}
+    master = out / f"master-{TAG}.params"
+    r = _run([str(master)], tmp_path)
+    assert r.returncode != 0
+    assert "Error loading fields file" in r.stdout and "Finished with errors" in r.stdout
+    after = {p.name: p.read_bytes() for p in out.iterdir()}
+    assert after == before
+
+
+@pytest.mark.gpu
+def test_nan_energies_reject_and_restore(tmp_path):
+    """DT 1000 overflows the MD within a trajectory (links and momenta become
+    inf / NaN): every trajectory's final energy is not finite, so it is
+    rejected as in the reference (a NaN or +inf H fails both accept tests,
+    sun.cpp:1258, 1288, 1304-1310), NREC counts every trajectory, and every
+    stored configuration is the restored cold start, bit for bit."""
+    params = PARAMS.format(steps=60).replace("DT              0.05", "DT              1000")
+    (tmp_path / "p.txt").write_text(params)
+    r = _run(["p.txt"], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.count("REJECT") == 3 and "ACCEPT" not in r.stdout
+    tag = TAG.replace("DT0.0500-U1.0000", "DT1000.0000-U20000.0000")
+    out = tmp_path / "output-DIM4"
+    m = read_master(out / f"master-{tag}.params", 4)
+    assert m["NREC"] == 3 and m["NACC"] == 0 and m["TAKEN"] == 3
+    V, D, N = 256, 4, 3
+    cold = np.broadcast_to(np.eye(N, dtype=np.complex128), (V, D, N, N))
+    for k in range(3):
+        conf = np.fromfile(out / f"U-{tag}-{k}.conf", dtype=np.complex128).reshape(V, D, N, N)
+        assert np.array_equal(conf, cold), k
+    finals = [l for l in r.stdout.splitlines() if "Final energy" in l]
+    assert len(finals) == 3 and all(("nan" in l or "inf" in l) for l in finals), finals
+
+
 MPARAMS = """#Parameters_file
 NC              3
 DIM             4

@@ -108,3 +108,28 @@ def test_features_closed_form_and_invariance(oracle, fo):
         for d in range(o.dim):
             Vg[y, d] = g[y] @ V[y, d] @ g[o.getplus(y, d)].conj().T
     assert np.allclose(fo.link_inputs(o, V, x, mu)[2], fo.link_inputs(o, Vg, x, mu)[2], atol=1e-13)
+
+
+def test_batched_restatement_matches_per_link(oracle):
+    """The batched restatement (used at the 16^4 size) equals the per-link
+    one: same links to rounding, same log J, same action."""
+    from oracle import flow_oracle as fo
+
+    for ext, subs in (([4, 4, 2, 2], [(0, 0, 0), (1, 1, 1), (2, 0, 0), (3, 1, 1), (0, 1, 0)]),
+                      ([4, 4, 4], [(0, 0, 0), (2, 1, 1), (1, 0, 0)])):
+        o = oracle.Oracle(3, ext)
+        rng = np.random.default_rng(3)
+        V = random_su(oracle, rng, (o.V, o.dim), 3)
+        nf = 3 * (o.dim - 1) * (o.dim - 2) // 2
+        bsize = 32 * nf + 65
+        w = (rng.standard_normal(2 * bsize) * 0.5).astype(np.float32)
+        U1, l1 = fo.forward(o, V, subs, w, bsize, nf, 0.3, 1e-3)
+        U2, l2 = fo.forward_batched(ext, V, subs, w, bsize, nf, 0.3, 1e-3)
+        assert np.max(np.abs(U1 - U2)) < 1e-13
+        assert abs(l1 - l2) < 1e-12 * max(1.0, abs(l1)) and abs(l1) > 1e-3
+        assert abs(fo.energy_mag(o, U1) - fo.energy_mag_batched(ext, U2)) < 1e-10
+        up, dn, par = fo.neighbours(ext)
+        for x in range(o.V):
+            for d in range(o.dim):
+                assert up[d][x] == o.getplus(x, d) and dn[d][x] == o.getminus(x, d)
+

@@ -55,6 +55,30 @@ def test_c1_trajectory_fast_link_by_link():
     assert abs(beta * (e[2] + e[3] / n2) - GOLD["h_new"]) < 1e-6
 
 
+PHILOX = json.load(open(os.path.join(ROOT, "tests", "golden", "c1_trajectories23.json")))
+
+
+def test_c1_trajectories_2_3_exact_bitwise():
+    """Trajectories 2 and 3 of the same run (tests/golden/c1_trajectories23.json,
+    written by make_c1_philox.py): Philox momenta (counter = trajectory - 1),
+    1000 drifts, the Metropolis draw.  The device's momenta, proposed links and
+    decisions equal the oracle's bit for bit (the Box-Muller steps are
+    reproducible IEEE sequences); every trajectory is rejected, so each starts
+    from the cold links again."""
+    lat = A.Lattice(PHILOX["nc"], PHILOX["extents"], A.FP64_EXACT)
+    beta, n2 = PHILOX["beta"], 2.0 * PHILOX["nc"]
+    for tr in PHILOX["trajectories"]:
+        lat.cold()
+        lat.mom_refresh(PHILOX["seed"], tr["counter"], beta)
+        e = lat.md_trajectory(PHILOX["dt"], PHILOX["ndrift"], PHILOX["level"])
+        assert hashlib.sha256(np.ascontiguousarray(lat.links()).tobytes()).hexdigest() == tr["links_sha256"]
+        assert hashlib.sha256(np.ascontiguousarray(lat.momenta()).tobytes()).hexdigest() == tr["momenta_sha256"]
+        h0, h1 = beta * (e[0] + e[1] / n2), beta * (e[2] + e[3] / n2)
+        assert abs(h0 - tr["h_old"]) < 1e-9 * tr["h_old"] and abs(h1 - tr["h_new"]) < 1e-9 * tr["h_old"]
+        assert lat.uniform(PHILOX["seed"], tr["counter"]) == tr["uniform"]
+        assert (h1 < h0 or tr["uniform"] < np.exp(h0 - h1)) == tr["accepted"]
+
+
 EXAMPLE = json.load(open(os.path.join(ROOT, "tests", "golden", "example_dim10_trajectory1.json")))
 
 

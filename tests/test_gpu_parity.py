@@ -220,7 +220,12 @@ def test_mom_refresh_and_haar_vs_oracle(oracle, n, ext):
     lat = _gpu(n, ext, A.FP64)
     lat.mom_refresh(11, 3, beta)
     lat.haar(12, 1)
-    assert np.max(np.abs(lat.momenta() - o.field("pi"))) < 1e-13
+    # the Box-Muller steps are IEEE operations in a fixed order on both
+    # sides: the device's Philox momenta equal the oracle's bit for bit
+    assert np.array_equal(lat.momenta(), o.field("pi"))
+    ex = _gpu(n, ext, A.FP64_EXACT)
+    ex.mom_refresh(11, 3, beta)
+    assert np.array_equal(ex.momenta(), o.field("pi"))
     assert np.max(np.abs(lat.links() - o.field("u"))) < 1e-12
     # reunitarisation of perturbed links
     rng = np.random.default_rng(0)
@@ -275,6 +280,42 @@ def test_hmc_chain_vs_oracle(oracle):
     assert np.max(np.abs(lat.links() - o.field("u"))) < 1e-10
     pl = lat.polyakov()
     assert abs(pl - complex(rec[-1, 3], rec[-1, 4])) < 1e-12
+
+
+def test_hmc_chain_exact_bitwise(oracle):
+    """Six trajectories of the exact mode from the oracle's Haar links: the
+    Philox momenta, the MD (the reference's operation order) and the
+    Metropolis decisions reproduce the oracle's chain, and the final links
+    are bit-identical (no reunitarisation, which rounds differently)."""
+    n, ext, beta, dt, nd, seed = 3, [4, 4, 4, 4], 5.7, 0.05, 8, 17
+    o = oracle.Oracle(n, ext, beta=beta, level=8)
+    o.haar(21, 0)
+    u0 = o.field("u").copy()
+    rec = o.hmc(dt, nd, 6, seed, first_counter=0, reunit=False)
+    lat = _gpu(n, ext, A.FP64_EXACT, u0)
+    for t in range(6):
+        r = lat.hmc_trajectory(beta, dt, nd, 8, seed, t, reunit=False)
+        assert r.accepted == int(rec[t, 1]), t
+        assert abs(r.dh - rec[t, 0]) < 1e-8, (t, r.dh, rec[t, 0])
+    assert rec[:, 1].sum() >= 1
+    assert np.array_equal(lat.links(), o.field("u"))
+
+
+def test_nan_trajectory_rejected_links_restored():
+    """A trajectory whose energies overflow (dt = 1000: the links reach inf
+    and NaN within a few steps) is rejected, as the reference's accept tests
+    both fail for a NaN H (sun.cpp:1258, 1288, 1304-1310), and the chain
+    state is the one before it, bit for bit; the next sane trajectory runs
+    normally."""
+    for prec in (A.FP64, A.FP32):
+        lat = _gpu(3, [4, 4, 4, 4], prec)
+        lat.haar(3, 0)
+        u0 = lat.links()
+        r = lat.hmc_trajectory(5.9, 1000.0, 10, 6, 5, 0, reunit=False)
+        assert not np.isfinite(r.dh) and not r.accepted and r.accept_case == 0
+        assert np.array_equal(lat.links(), u0)
+        r = lat.hmc_trajectory(5.9, 0.02, 10, 6, 5, 1, reunit=False)
+        assert np.isfinite(r.dh) and abs(r.dh) < 1.0
 
 
 def test_errors_are_reported(oracle):

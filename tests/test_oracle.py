@@ -51,6 +51,44 @@ def test_philox_known_answers(oracle):
         assert [int(x) for x in oracle.philox4x32_10(ctr, key)] == want
 
 
+def _ulp_err(got, want):
+    return abs(got - want) / math.ulp(want) if want != 0 else abs(got) / math.ulp(1.0)
+
+
+def test_repro_log_and_sincos_within_2ulp(oracle):
+    """The Box-Muller logarithm and sine/cosine (fixed IEEE operation
+    sequences, so that the device reproduces them bit for bit) against the C
+    library (math.log, math.sin, math.cos): within 2 ulp over the whole range
+    the Philox uniforms take, including the end points and the quadrant
+    boundaries of 2 pi u."""
+    rng = np.random.default_rng(5)
+    words = np.concatenate([rng.integers(0, 2**32, 20000), [0, 1, 2, 2**31, 2**32 - 2, 2**32 - 1],
+                            np.arange(0, 2**32, 2**28) - 1, np.arange(0, 2**32, 2**30)]).astype(np.uint64) % 2**32
+    import mpmath
+
+    mpmath.mp.prec = 120
+    worst_l = worst_s = 0.0
+    for k, w in enumerate(words):
+        u = (float(w) + 0.5) * 2.3283064365386963e-10
+        worst_l = max(worst_l, _ulp_err(oracle.repro_log(u), math.log(u)))
+        if k % 8:
+            continue
+        s, c = oracle.repro_sincos2pi(u)
+        # true values of the exact angle 2 pi u (u is exact), to 120 bits; the
+        # error is counted in ulp of the value, or of 2^-53 where the value is
+        # below that scale (near the zeros)
+        ang = 2 * mpmath.pi * mpmath.mpf(u)
+        for got, want in ((s, float(mpmath.sin(ang))), (c, float(mpmath.cos(ang)))):
+            worst_s = max(worst_s, abs(got - want) / max(math.ulp(want), 2.0**-53))
+    assert worst_l <= 2.0, worst_l
+    assert worst_s <= 2.0, worst_s
+    # exact special values
+    assert oracle.repro_log(1.0) == 0.0
+    assert oracle.repro_log(0.5) == -math.log(2.0) or _ulp_err(oracle.repro_log(0.5), -math.log(2.0)) <= 1
+    assert oracle.repro_sincos2pi(0.0) == (0.0, 1.0)
+    assert oracle.repro_sincos2pi(0.25) == (1.0, -0.0) or oracle.repro_sincos2pi(0.25) == (1.0, 0.0)
+
+
 def test_philox_normals_statistics(oracle):
     z = np.concatenate([oracle.philox_normals(7, 3, 1, link, 64) for link in range(2000)])
     assert abs(z.mean()) < 0.01 and abs(z.var() - 1) < 0.01
