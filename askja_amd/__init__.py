@@ -90,6 +90,7 @@ SIGNATURES = {
     "askja_slice_traces": (_I, [_P, _I, _DP]),
     "askja_line_phases": (_I, [_P, _I, _I, C.POINTER(C.c_int64), _DP]),
     "askja_hmc_trajectory": (_I, [_P, C.POINTER(HmcParams), _U64, C.POINTER(HmcResult)]),
+    "askja_hmc_trajectory_noalloc": (_I, [_P, C.POINTER(HmcParams), _U64, C.POINTER(HmcResult)]),
     "askja_uniform": (_I, [_P, _U64, _U64, _DP]),
     "askja_tau_int": (_I, [_DP, _L, _D, _DP, C.POINTER(C.c_long)]),
     "askja_flow_define": (_I, [_P, _I, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), _I, _D, _D]),

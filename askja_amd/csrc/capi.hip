@@ -11,13 +11,30 @@
 #include <cstdio>
 #include <cstring>
 #include <random>
+#include <mutex>
 #include <string>
 #include <vector>
 
 using namespace askja;
 
+// Allocation-free partial sums (askja_hmc_trajectory_noalloc): a file-scope
+// device array of a fixed size, one copy per GPU, held until the process
+// ends.  Its device address is looked up once per device, with that device
+// current, by askja_create; host code uses only that address (the array's
+// name is a host-side stand-in).  Every call on a GPU shares that GPU's
+// copy: calls that overlap on one GPU race.
+namespace {
+constexpr size_t kScratchResDoubles = 64;                 // finalised sums
+constexpr size_t kScratchPartDoubles = (size_t)1 << 21;   // block partials (16 MiB)
+__device__ __align__(256) double scratch[kScratchResDoubles + kScratchPartDoubles];
+constexpr int kMaxDevices = 64;
+double *g_scratch[kMaxDevices] = {nullptr};
+std::mutex g_scratch_mu;
+}  // namespace
+
 struct askja_ctx {
     int nc = 0, dim = 0, prec = 0, mode = 0;
+    int dev = 0;  // the HIP device current at askja_create
     int ext[32] = {0};
     long V = 0, NL = 0;
     long helparr[32] = {0};
@@ -32,9 +49,10 @@ struct askja_ctx {
     void *mom = nullptr, *mom_prev = nullptr;
     double *part = nullptr;
     size_t part_doubles = 0;
+    bool part_static = false;  // part/res point into the static buffer
     double *res = nullptr;      // device, 64 doubles
-    double *scratch = nullptr;  // FP64 working memory for generic kernels
-    size_t scratch_doubles = 0;
+    double *matwork = nullptr;  // FP64 working memory: two N x N matrices
+    size_t matwork_doubles = 0;
     double *staging = nullptr;  // host<->device AoS staging
     size_t staging_doubles = 0;
     hipStream_t st = nullptr;
@@ -95,6 +113,8 @@ size_t elem_bytes(const askja_ctx *c) { return f32_mode(c->mode) ? 4 : 8; }
 int ensure_part(askja_ctx *c, size_t nd)
 {
     if (nd <= c->part_doubles) return ASKJA_OK;
+    if (c->part_static)  // never grow (or free) the static buffer
+        return fail(c, ASKJA_EINVAL, "partial sums of %zu doubles exceed the static buffer", nd);
     if (c->part) (void)hipFree(c->part);
     c->part = nullptr;
     c->part_doubles = 0;
@@ -103,13 +123,13 @@ int ensure_part(askja_ctx *c, size_t nd)
     return ASKJA_OK;
 }
 
-int ensure_scratch(askja_ctx *c)
+int ensure_matwork(askja_ctx *c)
 {
-    if (c->scratch) return ASKJA_OK;
+    if (c->matwork) return ASKJA_OK;
     // two FP64 N x N work matrices (the line matrix of N <= 3)
     const size_t want = (size_t)4 * c->nc * c->nc;
-    HIPCHK(c, hipMalloc(&c->scratch, want * sizeof(double)));
-    c->scratch_doubles = want;
+    HIPCHK(c, hipMalloc(&c->matwork, want * sizeof(double)));
+    c->matwork_doubles = want;
     return ASKJA_OK;
 }
 
@@ -330,6 +350,15 @@ int askja_create(int nc, int dim, const int *extents, int precision, askja_ctx *
             return e_ == hipErrorOutOfMemory ? ASKJA_ENOMEM : ASKJA_EHIP;       \
         }                                                                       \
     } while (0)
+    CREATE_CHK(hipGetDevice(&c->dev));
+    if (c->dev >= 0 && c->dev < kMaxDevices) {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (!g_scratch[c->dev]) {
+            void *p = nullptr;
+            CREATE_CHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+            g_scratch[c->dev] = (double *)p;
+        }
+    }
     CREATE_CHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
     CREATE_CHK(hipEventCreate(&c->ev0));
     CREATE_CHK(hipEventCreate(&c->ev1));
@@ -377,7 +406,7 @@ int askja_destroy(askja_ctx *c)
     if (c->st) (void)hipStreamSynchronize(c->st);
     for (int b = 0; b < 3; b++)
         if (c->links[b]) (void)hipFree(c->links[b]);
-    void *bufs[] = {c->snap, c->mom, c->mom_prev, c->part, c->res, c->scratch, c->staging,
+    void *bufs[] = {c->snap, c->mom, c->mom_prev, c->part, c->res, c->matwork, c->staging,
                     c->d_nbr, c->d_dev2lex, c->d_lex2dev, c->flow_w, c->flow_gw, c->flow_g, c->flow_lj,
                     c->flow_out, c->flow_work, c->flow_log, c->flow_contrib, c->flow_save};
     for (void *p : bufs)
@@ -899,12 +928,12 @@ int askja_line_matrix(askja_ctx *c, int dir, const int *x, double *out)
         if (x[d] < 0 || x[d] >= c->ext[d]) return fail(c, ASKJA_EINVAL, "line_matrix: coordinate %d out of range", d);
         lex += x[d] * c->helparr[d];
     }
-    int rc = ensure_scratch(c);
+    int rc = ensure_matwork(c);
     if (rc) return rc;
     const size_t nr = (size_t)2 * c->nc * c->nc;
     double *d = nullptr;
     HIPCHK(c, hipMallocAsync((void **)&d, nr * sizeof(double), c->st));
-    hipError_t e = launch_line_matrix(c->geom, c->links[c->cur], dir, lex, c->mode, c->scratch, d, c->st);
+    hipError_t e = launch_line_matrix(c->geom, c->links[c->cur], dir, lex, c->mode, c->matwork, d, c->st);
     if (e == hipSuccess) e = hipMemcpyAsync(out, d, nr * sizeof(double), hipMemcpyDeviceToHost, c->st);
     if (e == hipSuccess) e = hipFreeAsync(d, c->st);
     if (e == hipSuccess) e = hipStreamSynchronize(c->st);
@@ -912,11 +941,21 @@ int askja_line_matrix(askja_ctx *c, int dir, const int *x, double *out)
     return ASKJA_OK;
 }
 
-int askja_hmc_trajectory(askja_ctx *c, const askja_hmc_params *p, uint64_t traj_counter, askja_hmc_result *r)
+}  // extern "C"
+
+namespace {
+
+// Doubles of block partials one trajectory's kernels write (the kicks' three
+// sums per block; the FP64 action pass of the FP32 modes).
+size_t trajectory_partials(const askja_ctx *c)
 {
-    ENTER(c);
-    if (!p || !r) return fail(c, ASKJA_EINVAL, "hmc_trajectory: null argument");
-    if (!(p->beta > 0) || !(p->dt > 0)) return fail(c, ASKJA_EINVAL, "hmc_trajectory: beta and dt must be positive");
+    size_t need = (size_t)kick_partials(c->geom) * 6;
+    if (f32_mode(c->mode) && c->nc <= 8) need = std::max(need, (size_t)plaq_partials(c->geom) * 4);
+    return need;
+}
+
+int hmc_trajectory(askja_ctx *c, const askja_hmc_params *p, uint64_t traj_counter, askja_hmc_result *r)
+{
     int rc;
     if (p->reunit && (rc = askja_reunitarize(c))) return rc;
     if ((rc = askja_mom_refresh(c, p->seed, traj_counter, p->beta))) return rc;
@@ -942,6 +981,60 @@ int askja_hmc_trajectory(askja_ctx *c, const askja_hmc_params *p, uint64_t traj_
     if (r->accepted) c->cur = fin;
     r->action = r->accepted ? s1 : s0;
     return ASKJA_OK;
+}
+
+int hmc_check(askja_ctx *c, const askja_hmc_params *p, askja_hmc_result *r, const char *who)
+{
+    if (!p || !r) return fail(c, ASKJA_EINVAL, "%s: null argument", who);
+    if (!(p->beta > 0) || !(p->dt > 0)) return fail(c, ASKJA_EINVAL, "%s: beta and dt must be positive", who);
+    return ASKJA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int askja_hmc_trajectory(askja_ctx *c, const askja_hmc_params *p, uint64_t traj_counter, askja_hmc_result *r)
+{
+    ENTER(c);
+    int rc = hmc_check(c, p, r, "hmc_trajectory");
+    if (rc) return rc;
+    return hmc_trajectory(c, p, traj_counter, r);
+}
+
+int askja_hmc_trajectory_noalloc(askja_ctx *c, const askja_hmc_params *p, uint64_t traj_counter,
+                                 askja_hmc_result *r)
+{
+    ENTER(c);
+    int rc = hmc_check(c, p, r, "hmc_trajectory_noalloc");
+    if (rc) return rc;
+    int cur = -1;
+    HIPCHK(c, hipGetDevice(&cur));
+    if (cur != c->dev)
+        return fail(c, ASKJA_EINVAL, "hmc_trajectory_noalloc: device %d is current, the context lives on device %d",
+                    cur, c->dev);
+    if (c->dev < 0 || c->dev >= kMaxDevices || !g_scratch[c->dev])
+        return fail(c, ASKJA_EUNSUP, "hmc_trajectory_noalloc: no static buffer on device %d", c->dev);
+    const size_t need = trajectory_partials(c);
+    if (need > kScratchPartDoubles)
+        return fail(c, ASKJA_EINVAL,
+                    "hmc_trajectory_noalloc: this lattice needs %zu doubles of partial sums, the static buffer "
+                    "holds %zu (16 MiB); use askja_hmc_trajectory",
+                    need, kScratchPartDoubles);
+    // the trajectory's kernels write their block partials and finalised sums
+    // into the static buffer instead of the context's (grown on demand)
+    double *part = c->part, *res = c->res;
+    const size_t part_doubles = c->part_doubles;
+    c->res = g_scratch[c->dev];
+    c->part = g_scratch[c->dev] + kScratchResDoubles;
+    c->part_doubles = kScratchPartDoubles;
+    c->part_static = true;
+    rc = hmc_trajectory(c, p, traj_counter, r);
+    c->part_static = false;
+    c->part = part;
+    c->res = res;
+    c->part_doubles = part_doubles;
+    return rc;
 }
 
 int askja_uniform(askja_ctx *c, uint64_t seed, uint64_t counter, double *out)

@@ -222,6 +222,21 @@ typedef struct {
 int askja_hmc_trajectory(askja_ctx *ctx, const askja_hmc_params *p, uint64_t traj_counter,
                          askja_hmc_result *res);
 
+/* askja_hmc_trajectory without any device allocation (no counterpart in the
+ * reference): for callers that cannot allocate, such as code under HIP graph
+ * capture or a tight serving loop.  The same kernels and the same results,
+ * bit for bit; their block partial sums and finalised sums go to a static
+ * device buffer of 16 MiB per GPU (2^21 doubles of partials), held until the
+ * process ends, instead of the context's reduction buffer, which
+ * askja_hmc_trajectory grows with hipMalloc on first use.  Lattices whose
+ * partials exceed the buffer are rejected with ASKJA_EINVAL (kick partials:
+ * 6 doubles per 256-site block and direction for N <= 3, per link for
+ * N >= 9: SU(3) 8x32^3, SU(16) 6x24^3 and SU(64) 128^2 fit).  The context's device must be
+ * current.  All calls on one GPU share that GPU's buffer: calls that overlap
+ * on one GPU (two host threads, two contexts) race. */
+int askja_hmc_trajectory_noalloc(askja_ctx *ctx, const askja_hmc_params *p, uint64_t traj_counter,
+                                 askja_hmc_result *res);
+
 /* The Metropolis uniform in (0, 1) that askja_hmc_trajectory draws for
  * trajectory `counter` (replaces gsl_rng_uniform of an mt19937 seeded with
  * time(NULL), sun.cpp:1277-1287). */
