@@ -52,6 +52,23 @@ def init_weights(nblocks, bsize, nf, seed=0, scale=0.1):
     return w
 
 
+def save_weights(path, w, recipe):
+    """The trained MLP weights (float32, as hex of their bits so the file
+    round-trips exactly) with the recipe that produced them."""
+    w = np.ascontiguousarray(np.asarray(w, np.float32))
+    with open(path, "w") as f:
+        json.dump({"recipe": recipe, "n": int(w.size), "float32_hex": w.view(np.uint32).astype(">u4").tobytes().hex()},
+                  f, indent=1)
+
+
+def load_weights(path):
+    """(weights, recipe) written by save_weights."""
+    d = json.load(open(path))
+    w = np.frombuffer(bytes.fromhex(d["float32_hex"]), dtype=">u4").astype(np.uint32).view(np.float32)
+    assert w.size == d["n"]
+    return w.copy(), d["recipe"]
+
+
 def flow_action_function():
     import torch
 
@@ -166,6 +183,8 @@ def main():
     ap.add_argument("--ndrift", type=int, default=50)
     ap.add_argument("--level", type=int, default=8)
     ap.add_argument("--seed", type=int, default=11)
+    ap.add_argument("--save-weights", help="write the trained weights (and the recipe) to this JSON file")
+    ap.add_argument("--no-chains", action="store_true", help="train (and save) only")
     args = ap.parse_args()
     # torch first: the library then binds to torch's HIP runtime, so both
     # share one runtime in this process
@@ -190,6 +209,15 @@ def main():
     t0 = time.perf_counter()
     w, losses = train(lat, configs, args.beta, w0, epochs=args.epochs, log=log)
     t_train = time.perf_counter() - t0
+    if args.save_weights:
+        save_weights(args.save_weights, w, {
+            "nc": 3, "extents": ext, "beta": args.beta, "beta_train": args.beta_train, "layers": args.layers,
+            "sublayers": len(mu), "ntrain": args.ntrain, "epochs": args.epochs, "dt": args.dt, "ndrift": args.ndrift,
+            "level": args.level, "seed": args.seed, "rho_max": 0.3, "delta": 1e-3, "bsize": bsize,
+            "loss": losses, "train_seconds": t_train})
+        log(f"[learned] weights written to {args.save_weights}")
+    if args.no_chains:
+        return
     # chains at beta from a thermalised start
     lat.set_links(configs[-1])
     plain_chain(lat, args.beta, args.dt, args.ndrift, args.level, args.seed, 2 * 10**6, args.therm, measure=False)

@@ -1,22 +1,33 @@
 """Benchmark of the HMC hot path on one MI355X.
 
-Workload (the configuration the headline metric is quoted on): plain HMC for
-SU(3) in D=4 on an 8x32^3 lattice (N0 = 8 temporal), beta = 5.9, DT = 0.02,
-LEVEL = 6, hot (Haar-random) start, 100 MD steps per trajectory (length 2),
-counter-based momenta, FP64 (or --precision fp32: FP32 links and momenta with
-FP64 sums and Metropolis).  Data are synthetic: the links start Haar random
-from a fixed seed.
+Default workload (the configuration the headline metric is quoted on):
+learned-proposal HMC against plain FP64 HMC for SU(3) in D = 4 on a 16^4
+lattice, beta = 6.0, DT = 0.02, 50 MD steps per trajectory (length 1),
+LEVEL 8.  The learned proposal is field-transformation HMC with the map
+U = F(V) of four sweep layers (32 masked sub-layers, MLP of hidden width 32
+over plaquette and rectangle traces), weights trained on 500 plain-HMC
+configurations at beta 5.7 (askja_amd/data/flow_c4_weights.json, written by
+``python -m askja_amd.learned --save-weights``), exact Metropolis with
+log J.  Data are synthetic: a Haar-random start from a fixed seed,
+thermalised by plain HMC (untimed) before either chain.
 
-One step = one HMC trajectory (momentum refresh, 100 fused kick+drift MD
-steps, the final half kick, H_old/H_new sums, Metropolis) plus the
-Polyakov-loop measurement that feeds the autocorrelation time.  --warmup
-trajectories thermalise the chain untimed; --steps trajectories are timed.
+One step = one learned-proposal trajectory: momentum refresh, 50 MD steps
+with the flow force (forward and hand-written backward of all 32
+sub-layers per step), H with log J, Metropolis, the Polyakov loop of U.
+--warmup trajectories of each chain run untimed, then exactly --steps
+learned trajectories are timed; the plain chain is timed over its own
+series.
 
-metric value = (trajectories/s) / tau, with tau = 1 + 2 sum_t rho(t) the
-integrated autocorrelation time of |volume-averaged Polyakov loop| estimated
-over the timed trajectories by Wolff's automatic windowing (S = 1.5); tau = 1
-for uncorrelated data, so the value counts statistically independent
-configurations per second.
+metric value = (independent configurations/s of the learned chain) /
+(independent configurations/s of the plain chain), each = trajectories/s /
+tau with tau the integrated autocorrelation time of |volume-averaged
+Polyakov loop| (Wolff's automatic window, S = 1.5; clamped to >= 1).  Each
+tau comes from a series of at least 100 tau trajectories (extended beyond
+--steps as needed, up to a cap; without it the value is null with the
+reason).
+
+``--workload c2`` runs the earlier plain-HMC line instead: SU(3) 8x32^3,
+beta 5.9, DT 0.02, LEVEL 6, 100 MD steps per trajectory.
 """
 import argparse
 import json
@@ -47,6 +58,25 @@ PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}  # MI355X dense
 # 1 GiB copy's readings): reads 772.7 MB + writes 226.8 MB (FP64),
 # 264.3 MB + 113.5 MB (FP32)
 PMC_TRAFFIC_BYTES = {"fp64": 999.5e6, "fp32": 377.8e6}
+# the same for the 16^4 lattice's FP64 kick+drift (the plain chain of the
+# learned-vs-plain workload): 154.6 MB reads + 56.7 MB writes
+PMC_TRAFFIC_BYTES_C4 = 211.3e6
+
+
+def _finite(o):
+    """The JSON line with every non-finite number written as null (JSON has
+    no NaN)."""
+    if isinstance(o, dict):
+        return {k: _finite(v) for k, v in o.items()}
+    if isinstance(o, (list, tuple)):
+        return [_finite(v) for v in o]
+    if isinstance(o, (float, np.floating)):
+        return float(o) if math.isfinite(o) else None
+    if isinstance(o, np.integer):
+        return int(o)
+    if isinstance(o, np.bool_):
+        return bool(o)
+    return o
 
 
 def tau_int_wolff(x, S=1.5):
@@ -328,24 +358,256 @@ def run_chain(lat, args, tag):
     return (traj_s, tau, window, traj_s / tau, acc, dh, plaq), T
 
 
+# ------------------------------------------------------------------ C4 line
+C4_NC, C4_EXT, C4_BETA, C4_DT, C4_NDRIFT, C4_LEVEL = 3, [16, 16, 16, 16], 6.0, 0.02, 50, 8
+C4_THERM = 300            # plain trajectories from the Haar start, untimed
+C4_WEIGHTS = os.path.join(ROOT, "askja_amd", "data", "flow_c4_weights.json")
+TAU_SPAN = 100            # a tau estimate needs a series of >= 100 tau
+MIN_SERIES = {"plain": 3000, "learned": 300}
+MAX_SERIES = {"plain": 20000, "learned": 2000}
+
+
+def kick_drift_roofline(lat, nc, level, dt, launches):
+    """The fused kick+drift launch timed with HIP events on the library's
+    stream: algorithmic bytes (read + write link, read + write momentum) and
+    flops ((4(D-1) + 1 + LEVEL) 8 N^3 per link) over the average launch."""
+    import askja_amd as A
+
+    nl = lat.V * lat.dim
+    esz = 8 if lat.precision in (A.FP64, A.FP64_EXACT) else 4
+    bytes_per_link = 2 * (2 * nc * nc * esz) + 2 * (nc * nc * esz)
+    flops_per_link = (4 * (lat.dim - 1) + 1 + level) * 8 * nc ** 3
+    lat.mom_refresh(SEED, 10**9, 5.9)
+    for _ in range(5):
+        lat.kick_drift(dt, dt, level)
+    lat.timer_start()
+    for _ in range(launches):
+        lat.kick_drift(dt, dt, level)
+    ms = lat.timer_stop() / launches
+    achieved = bytes_per_link * nl / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "kernel": "fused kick+drift", "ms_per_launch": ms, "bytes_per_link": bytes_per_link,
+            "links_per_launch": nl, "tflops": flops_per_link * nl / (ms * 1e-3) / 1e12}
+
+
+def series_summary(poly, plaq, acc, dh, seconds, ntimed):
+    tau_raw, window = tau_int_wolff(poly)
+    tau = max(tau_raw, 1.0) if math.isfinite(tau_raw) else float("nan")
+    n = len(poly)
+    err = tau_raw * math.sqrt(2.0 * (2 * window + 1) / n) if math.isfinite(tau_raw) else float("nan")
+
+    def mean_err(x):
+        x = np.asarray(x, np.float64)
+        t, _ = tau_int_wolff(x)
+        t = max(t, 1.0) if math.isfinite(t) else 1.0
+        return float(x.mean()), float(x.std() * math.sqrt(t / len(x)))
+
+    return {"series": n, "timed": ntimed, "seconds_timed": seconds, "traj_per_s": ntimed / seconds,
+            "tau_int_polyakov_raw": tau_raw, "tau_int_polyakov": tau, "tau_int_err": err, "tau_window": window,
+            "tau_span_ok": bool(math.isfinite(tau) and window >= 1 and n >= TAU_SPAN * tau),
+            "indep_configs_per_s": (ntimed / seconds) / tau, "acceptance": float(np.mean(acc)),
+            "abs_polyakov": mean_err(poly), "plaquette": mean_err(plaq), "mean_dH": float(np.mean(dh)),
+            "exp_minus_dH": float(np.mean(np.exp(-np.asarray(dh))))}
+
+
+def c4_chain(lat, kind, first, args, log):
+    """--warmup untimed trajectories, then a series whose first `ntimed`
+    trajectories are timed; the series is extended (untimed) until it spans
+    TAU_SPAN tau or reaches the cap."""
+    step = (lambda c: lat.hmc_trajectory(C4_BETA, C4_DT, C4_NDRIFT, C4_LEVEL, SEED, c, reunit=True)) \
+        if kind == "plain" else \
+        (lambda c: lat.flow_hmc_trajectory(C4_BETA, C4_DT, C4_NDRIFT, C4_LEVEL, SEED, c, reunit=True))
+    measure = lat.polyakov if kind == "plain" else lat.flow_polyakov
+    for k in range(args.warmup):
+        step(first + k)
+    first += args.warmup
+    # the learned chain's timed region is exactly --steps trajectories; the
+    # plain chain (about 100 times cheaper per trajectory) is timed over its
+    # whole minimum series
+    ntimed = args.steps if kind == "learned" else max(args.steps, MIN_SERIES["plain"])
+    poly, plaq, acc, dh = [], [], [], []
+
+    def run(n, c0):
+        for k in range(n):
+            r = step(c0 + k)
+            acc.append(r.accepted)
+            dh.append(r.dh)
+            plaq.append(1.0 - r.action / lat.n_plaquettes())
+            poly.append(abs(measure()))
+
+    lat.sync()
+    t0 = time.perf_counter()
+    run(ntimed, first)
+    lat.sync()
+    T = time.perf_counter() - t0
+    log(f"[bench] {kind}: {ntimed} timed trajectories in {T:.1f} s")
+    want = max(ntimed, MIN_SERIES[kind])
+    while True:
+        if len(poly) < want:
+            run(want - len(poly), first + len(poly))
+        tau, _ = tau_int_wolff(poly)
+        need = int(math.ceil(TAU_SPAN * max(tau, 1.0))) if math.isfinite(tau) else MAX_SERIES[kind]
+        if len(poly) >= need or len(poly) >= MAX_SERIES[kind]:
+            break
+        want = min(need, MAX_SERIES[kind])
+        log(f"[bench] {kind}: tau {tau:.2f}, extending the series to {want}")
+    return series_summary(poly, plaq, acc, dh, T, ntimed)
+
+
+def flow_kernel_times(lat, reps=3):
+    """HIP-event time of one flow gradient (forward + backward of every
+    sub-layer, what each FTHMC MD step runs) and of one forward."""
+    lat.flow_gradient(C4_BETA)
+    lat.timer_start()
+    for _ in range(reps):
+        lat.flow_gradient(C4_BETA)
+    ms_grad = lat.timer_stop() / reps
+    lat.timer_start()
+    for _ in range(reps):
+        lat.flow_apply()
+    ms_fwd = lat.timer_stop() / reps
+    return ms_grad, ms_fwd
+
+
+def cpu_baseline_c4(steps_sample):
+    """The oracle (FP64 C restatement of the reference, OpenMP over sites in
+    the kick and drift like sun.cpp:679,777) on the host cores, plain HMC at
+    16^4: `steps_sample` MD steps timed and extrapolated to one trajectory
+    of 50 steps + the two Hamiltonian evaluations.  (The learned proposal has
+    no CPU counterpart: the reference has none.)"""
+    from oracle import oracle as O
+
+    o = O.Oracle(C4_NC, C4_EXT, beta=C4_BETA, level=C4_LEVEL)
+    o.haar(SEED, 2)
+    o.mom_philox(SEED, 0)
+    t0 = time.perf_counter()
+    h = o.hamiltonian("pi")
+    t_h = time.perf_counter() - t0
+    o.updateE(C4_DT / 2)
+    t0 = time.perf_counter()
+    for _ in range(steps_sample):
+        o.updateUandcopyDown(C4_DT)
+        o.updateE(C4_DT)
+    t_step = (time.perf_counter() - t0) / steps_sample
+    t_traj = C4_NDRIFT * t_step + 2 * t_h
+    assert math.isfinite(h)
+    return {"value": 1.0 / t_traj, "unit": "traj/s (plain HMC)", "cores": O.num_threads(), "kind": "port",
+            "sample": f"{steps_sample} FP64 MD steps (kick+drift) of the oracle on 16^4 SU(3), "
+                      f"{t_step * 1e3:.1f} ms/step, extrapolated to {C4_NDRIFT} steps + 2 H evaluations"}
+
+
+def main_c4(args):
+    import askja_amd as A
+    from askja_amd import learned as L
+
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    w, recipe = L.load_weights(args.weights)
+    lat = A.Lattice(C4_NC, C4_EXT, A.FP64)
+    lat.haar(SEED, 3)
+    for t in range(C4_THERM):
+        lat.hmc_trajectory(C4_BETA, C4_DT, C4_NDRIFT, C4_LEVEL, SEED, t, reunit=True)
+    start = lat.links()
+    log(f"[bench] thermalised ({C4_THERM} plain trajectories)")
+    plain = c4_chain(lat, "plain", C4_THERM, args, log)
+    roof = kick_drift_roofline(lat, C4_NC, C4_LEVEL, C4_DT, args.roofline_launches)
+    lat.set_links(start)
+    mu, par, blk = L.sweep_masks(4, recipe["layers"])
+    bsize = lat.flow_define(mu, par, blk, recipe["layers"], recipe["rho_max"], recipe["delta"])
+    assert bsize == recipe["bsize"] and w.size == recipe["layers"] * bsize, "weights do not fit this map"
+    lat.flow_set_weights(w)
+    learned = c4_chain(lat, "learned", 10**7, args, log)
+    ms_grad, ms_fwd = flow_kernel_times(lat)
+    ok = plain["tau_span_ok"] and learned["tau_span_ok"]
+    gain = learned["indep_configs_per_s"] / plain["indep_configs_per_s"]
+    for obs in ("abs_polyakov", "plaquette"):
+        (m, e), (m0, e0) = learned[obs], plain[obs]
+        learned[obs + "_z_vs_plain"] = (m - m0) / math.hypot(e, e0)
+    lat.close()
+    # secondary: the earlier plain-HMC workload's kernel and trajectory rate
+    c2 = A.Lattice(NC, EXT, A.FP64)
+    c2.haar(SEED, 2)
+    for t in range(5):
+        c2.hmc_trajectory(BETA, DT, NDRIFT, LEVEL, SEED, t, reunit=True)
+    c2.sync()
+    t0 = time.perf_counter()
+    nc2 = 20
+    for t in range(nc2):
+        c2.hmc_trajectory(BETA, DT, NDRIFT, LEVEL, SEED, 5 + t, reunit=True)
+    c2.sync()
+    c2_traj_s = nc2 / (time.perf_counter() - t0)
+    roof_c2 = kick_drift_roofline(c2, NC, LEVEL, DT, args.roofline_launches)
+    roof_c2["traffic"] = PMC_TRAFFIC_BYTES["fp64"]
+    roof_c2["traffic_source"] = ("rocprofv3 --pmc FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) of this launch, "
+                                 "profiles/kick_drift_su3_8x32c_pmc.md")
+    c2.close()
+    roof["traffic"] = PMC_TRAFFIC_BYTES_C4
+    roof["traffic_source"] = ("rocprofv3 --pmc FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) of this launch, "
+                              "profiles/kick_drift_su3_16x4_pmc.md")
+    out = {
+        "metric": "indep. configs/s (traj/s ÷ τ_int Polyakov), learned vs plain HMC; HBM GB/s",
+        "value": gain if ok else None,
+        "unit": "x (learned / plain independent configs/s)",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1000.0 * learned["seconds_timed"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": f"synthetic (Haar-random SU(3) start, seed {SEED}, {C4_THERM} untimed plain trajectories)",
+        "config": {"workload": "learned-proposal (FTHMC) vs plain HMC, SU(3) D=4 16^4 beta=6.0 DT=0.02 "
+                               f"{C4_NDRIFT} MD steps LEVEL={C4_LEVEL}, map: {recipe['layers']} sweep layers "
+                               f"({recipe['sublayers']} sub-layers), trained on {recipe['ntrain']} plain configs "
+                               f"at beta {recipe['beta_train']}",
+                   "nc": C4_NC, "extents": C4_EXT, "beta": C4_BETA, "dt": C4_DT, "md_steps_per_traj": C4_NDRIFT,
+                   "level": C4_LEVEL, "precision": "fp64", "weights": os.path.relpath(args.weights, ROOT)},
+        "learned": learned,
+        "plain": plain,
+        "gain": gain,
+        "gain_ceiling_note": (f"plain HMC decorrelates |P| in tau = {plain['tau_int_polyakov']:.2f} trajectories, "
+                              "so a proposal of equal cost per trajectory could gain at most about that factor "
+                              "on this observable"),
+        "matched_acceptance_ok": bool(plain["acceptance"] >= 0.7 and learned["acceptance"] >= 0.7),
+        "flow_ms_per_gradient": ms_grad,
+        "flow_ms_per_forward": ms_fwd,
+        "plain_ms_per_kick_drift": roof["ms_per_launch"],
+        "roofline": roof,
+        "secondary_c2": {"workload": "plain HMC, SU(3) D=4 8x32^3 beta=5.9 DT=0.02 LEVEL=6, 100 MD steps",
+                         "traj_per_s": c2_traj_s, "trajectories_timed": nc2, "roofline": roof_c2},
+    }
+    if not ok:
+        out["value_null_reason"] = "a tau_int series did not span 100 tau within its cap"
+    if not args.no_cpu:
+        log("[bench] cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline_c4(args.cpu_steps)
+    print(json.dumps(_finite(out), allow_nan=False))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    # 6000 trajectories of 100 MD steps (about 160 s).  Wolff's window
-    # underestimates tau_int(|P|) on runs too short to show the slow tail of
-    # the autocorrelation (50-step trajectories: 20.2 after 3000, 50.1 +- 8.5
-    # after 20000); with 100 steps the estimate is settled from about 6000
-    # trajectories on (20.6, 20.7, 21.2, 20.4 after 6000, 8000, 10000, 12000)
-    ap.add_argument("--steps", type=int, default=6000)
-    ap.add_argument("--warmup", type=int, default=300)
+    ap.add_argument("--workload", choices=["c4", "c2"], default="c4")
+    # c4: learned-proposal trajectories timed; c2: plain trajectories timed.
+    # For c2, 6000 trajectories of 100 MD steps (about 160 s): Wolff's window
+    # underestimates tau_int(|P|) of that chain on runs too short to show the
+    # slow tail of its autocorrelation (50-step trajectories: 20.2 after
+    # 3000, 50.1 +- 8.5 after 20000); with 100 steps the estimate is settled
+    # from about 6000 trajectories on (20.6, 20.7, 21.2, 20.4 after 6000,
+    # 8000, 10000, 12000)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--weights", default=C4_WEIGHTS)
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
-    ap.add_argument("--md-steps", type=int, default=NDRIFT, help="MD steps per trajectory (length = steps x DT)")
+    ap.add_argument("--md-steps", type=int, default=NDRIFT, help="c2: MD steps per trajectory (length = steps x DT)")
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mixed", action="store_true",
-                    help="also run the chain in the FP32 mode (another --steps trajectories)")
+                    help="c2: also run the chain in the FP32 mode (another --steps trajectories)")
     ap.add_argument("--roofline-launches", type=int, default=100)
     ap.add_argument("--copy-probe", action="store_true", help="only run a 1 GiB device copy (counter calibration)")
+    ap.add_argument("--kernel-only", choices=["c2", "c4"],
+                    help="only time --roofline-launches fused kick+drift launches of that lattice (for rocprofv3)")
     ap.add_argument("--study", choices=["precision", "c3scan", "c5exact", "kernels", "example"],
                     help="print a precision, beta-scan, exact-2D, per-config kernel or example-params study instead")
     ap.add_argument("--study-configs", default="C2,C3,C5")
@@ -355,6 +617,19 @@ def main():
     if args.copy_probe:
         copy_probe()
         return
+    if args.kernel_only:
+        import askja_amd as A
+
+        n, ext, level, dt = (NC, EXT, LEVEL, DT) if args.kernel_only == "c2" else (C4_NC, C4_EXT, C4_LEVEL, C4_DT)
+        lat = A.Lattice(n, ext, A.FP64 if args.precision == "fp64" else A.FP32)
+        lat.haar(SEED, 2)
+        print(json.dumps({"kernel_only": args.kernel_only, "extents": ext,
+                          "roofline": kick_drift_roofline(lat, n, level, dt, args.roofline_launches)}))
+        return
+    if args.steps is None:
+        args.steps = 100 if args.workload == "c4" else 6000
+    if args.warmup is None:
+        args.warmup = 10 if args.workload == "c4" else 300
     if args.study == "precision":
         precision_study(args.study_configs.split(","))
         return
@@ -374,31 +649,30 @@ def main():
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU path is not built.",
               file=sys.stderr)
         sys.exit(2)
+    if args.steps < 1:
+        print("bench.py: --steps must be at least 1", file=sys.stderr)
+        sys.exit(2)
+    if args.workload == "c4":
+        main_c4(args)
+        return
+    main_c2(args)
 
+
+def main_c2(args):
     import askja_amd as A
 
     prec = A.FP64 if args.precision == "fp64" else A.FP32
     lat = A.Lattice(NC, EXT, prec)
     (traj_s, tau, window, value, acc, dh, plaq), T = run_chain(lat, args, "bench")
-    # roofline of the dominant kernel: the fused kick+drift launch, timed
-    # with HIP events on the library's stream
-    nl = lat.V * lat.dim
-    esz = 8 if prec == A.FP64 else 4
-    bytes_per_link = 2 * (2 * NC * NC * esz) + 2 * (NC * NC * esz)  # read+write link, read+write momentum
-    flops_per_link = (4 * (lat.dim - 1) + 1 + LEVEL) * 8 * NC**3
-    lat.mom_refresh(SEED, 10**9, BETA)
-    for _ in range(5):
-        lat.kick_drift(DT, DT, LEVEL)
-    lat.timer_start()
-    for _ in range(args.roofline_launches):
-        lat.kick_drift(DT, DT, LEVEL)
-    ms = lat.timer_stop() / args.roofline_launches
-    achieved = bytes_per_link * nl / (ms * 1e-3) / 1e9
-    tflops = flops_per_link * nl / (ms * 1e-3) / 1e12
-
+    tau_used = max(tau, 1.0) if math.isfinite(tau) else float("nan")
+    ok = math.isfinite(tau) and window >= 1 and args.steps >= TAU_SPAN * tau_used
+    roof = kick_drift_roofline(lat, NC, LEVEL, DT, args.roofline_launches)
+    roof["traffic"] = PMC_TRAFFIC_BYTES[args.precision]
+    roof["traffic_source"] = ("rocprofv3 --pmc FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) of this launch, "
+                              "profiles/kick_drift_su3_8x32c_pmc.md")
     out = {
         "metric": "indep. configs/s (traj/s / tau_int Polyakov), plain HMC",
-        "value": value,
+        "value": traj_s / tau_used if ok else None,
         "unit": "configs/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -414,28 +688,23 @@ def main():
                    "level": LEVEL, "md_steps_per_traj": args.md_steps, "precision": args.precision},
         "traj_per_s": traj_s,
         "md_steps_per_s": traj_s * args.md_steps,
-        "tau_int_polyakov": tau,
-        "tau_int_err": tau * math.sqrt(2.0 * (2 * window + 1) / args.steps),
+        "tau_int_polyakov_raw": tau,
+        "tau_int_polyakov": tau_used,
+        "tau_int_err": tau * math.sqrt(2.0 * (2 * window + 1) / args.steps) if math.isfinite(tau) else None,
         "tau_window": window,
         "acceptance": float(np.mean(acc)),
         "mean_plaquette": float(np.mean(plaq)),
         "mean_dH": float(np.mean(dh)),
         "exp_minus_dH": float(np.mean(np.exp(-np.asarray(dh)))),
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": PMC_TRAFFIC_BYTES[args.precision],
-                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) of this launch, "
-                                       "profiles/kick_drift_su3_8x32c_pmc.md",
-                     "kernel": "fused kick+drift", "ms_per_launch": ms, "bytes_per_link": bytes_per_link,
-                     "links_per_launch": nl, "tflops": tflops},
+        "roofline": roof,
     }
+    if not ok:
+        out["value_null_reason"] = "the timed series is shorter than 100 tau_int (or tau is undefined)"
     if prec == A.FP64 and args.mixed:
-        # the same chain with FP32 links and MD (FP64 sums, reunitarisation and
-        # Metropolis energies of the stored links): the mixed-precision side
-        # of this workload's FP64 / FP32 comparison
         lat32 = A.Lattice(NC, EXT, A.FP32)
         (t32, tau32, w32, v32, acc32, dh32, plaq32), _ = run_chain(lat32, args, "bench fp32")
         out["mixed_precision_fp32"] = {
-            "value": v32, "unit": "configs/s", "traj_per_s": t32, "tau_int_polyakov": tau32,
+            "traj_per_s": t32, "tau_int_polyakov": tau32,
             "tau_int_err": tau32 * math.sqrt(2.0 * (2 * w32 + 1) / args.steps), "tau_window": w32,
             "acceptance": float(np.mean(acc32)), "mean_plaquette": float(np.mean(plaq32)),
             "exp_minus_dH": float(np.mean(np.exp(-np.asarray(dh32))))}
@@ -443,7 +712,7 @@ def main():
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.cpu_steps, args.md_steps)
-    print(json.dumps(out))
+    print(json.dumps(_finite(out), allow_nan=False))
 
 
 if __name__ == "__main__":

@@ -26,7 +26,10 @@ def test_more_than_one_gpu_is_refused():
 
 @pytest.mark.gpu
 def test_json_line_contract():
-    r = _run(["--steps", "12", "--warmup", "2", "--roofline-launches", "5", "--cpu-steps", "1", "--mixed"], 600)
+    """The line the driver records (--steps 20 --warmup 5): the learned and
+    plain chains' tau_int come from series of at least 100 tau with a Wolff
+    window of at least 1, tau >= 1, so the value is a measurement."""
+    r = _run(["--steps", "20", "--warmup", "5", "--roofline-launches", "5", "--cpu-steps", "2"], 900)
     assert r.returncode == 0, r.stderr[-2000:]
     lines = [l for l in r.stdout.splitlines() if l.strip()]
     assert len(lines) == 1
@@ -34,15 +37,38 @@ def test_json_line_contract():
     for k in ("metric", "value", "unit", "n_gpus", "steps", "warmup", "ms_per_step", "higher_is_better",
               "scaling", "vs_baseline", "dtype", "data", "config", "roofline", "cpu_baseline"):
         assert k in d, k
-    assert d["n_gpus"] == 1 and d["steps"] == 12 and d["warmup"] == 2
-    assert d["value"] > 0 and d["ms_per_step"] > 0 and d["higher_is_better"] is True
+    assert d["n_gpus"] == 1 and d["steps"] == 20 and d["warmup"] == 5
+    assert d["ms_per_step"] > 0 and d["higher_is_better"] is True
     assert d["dtype"] == "f64" and d["vs_baseline"] is None
-    assert "workload" in d["config"] and d["config"]["md_steps_per_traj"] == 100
+    assert "learned vs plain" in d["metric"] and "workload" in d["config"]
+    for chain in ("learned", "plain"):
+        c = d[chain]
+        assert c["tau_int_polyakov"] >= 1.0 and c["tau_window"] >= 1, (chain, c)
+        assert c["series"] >= 100 * c["tau_int_polyakov"], (chain, c)
+        assert c["tau_span_ok"] and c["traj_per_s"] > 0 and 0.0 < c["acceptance"] <= 1.0
+    assert d["learned"]["timed"] == 20
+    assert d["value"] is not None and d["value"] > 0
+    assert abs(d["value"] - d["learned"]["indep_configs_per_s"] / d["plain"]["indep_configs_per_s"]) < 1e-12
+    for obs in ("abs_polyakov_z_vs_plain", "plaquette_z_vs_plain"):
+        assert abs(d["learned"][obs]) < 5.0, (obs, d["learned"][obs])
     rl = d["roofline"]
     assert rl["bound"] == "hbm" and rl["unit"] == "GB/s" and rl["peak"] == 8000.0
     assert 0 < rl["frac"] < 1 and abs(rl["frac"] - rl["achieved"] / rl["peak"]) < 1e-12
     assert rl["traffic"] > 0
+    assert d["secondary_c2"]["roofline"]["traffic"] > 0
     cb = d["cpu_baseline"]
-    assert cb["kind"] == "port" and cb["cores"] >= 1 and cb["value"] > 0 and cb["unit"] == "traj/s"
-    mp = d["mixed_precision_fp32"]
-    assert mp["traj_per_s"] > 0 and mp["tau_int_polyakov"] > 0
+    assert cb["kind"] == "port" and cb["cores"] >= 1 and cb["value"] > 0
+
+
+@pytest.mark.gpu
+def test_short_series_gives_null_value():
+    """The plain 8x32^3 line over 12 trajectories: tau_int (about 20 on this
+    chain) cannot be estimated from a series shorter than 100 tau, so the
+    value is null with the reason; the JSON stays valid (no NaN)."""
+    r = _run(["--workload", "c2", "--steps", "12", "--warmup", "2", "--roofline-launches", "5", "--cpu-steps", "1",
+              "--mixed"], 600)
+    assert r.returncode == 0, r.stderr[-2000:]
+    d = json.loads([l for l in r.stdout.splitlines() if l.strip()][-1])
+    assert d["value"] is None and "100 tau" in d["value_null_reason"]
+    assert d["traj_per_s"] > 0 and d["roofline"]["traffic"] > 0
+    assert d["mixed_precision_fp32"]["traj_per_s"] > 0
