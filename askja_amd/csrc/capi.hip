@@ -78,6 +78,10 @@ struct askja_ctx {
     void *flow_log = nullptr;    // input links of every sub-layer
     double *flow_contrib = nullptr;  // backward neighbour contributions
     void *flow_save = nullptr;       // chain state at the start of a flow trajectory
+    int flow_kind = kFlowRankOne;
+    double *flow_st_wr = nullptr;    // stout: W and rho of every sub-layer's updated links
+    double *flow_st_aux = nullptr;   // stout: inverse Jacobian and eigenbasis (one sub-layer)
+    double *flow_st_lg = nullptr;    // stout: log-det gradient (one sub-layer)
 };
 
 namespace {
@@ -408,7 +412,8 @@ int askja_destroy(askja_ctx *c)
         if (c->links[b]) (void)hipFree(c->links[b]);
     void *bufs[] = {c->snap, c->mom, c->mom_prev, c->part, c->res, c->matwork, c->staging,
                     c->d_nbr, c->d_dev2lex, c->d_lex2dev, c->flow_w, c->flow_gw, c->flow_g, c->flow_lj,
-                    c->flow_out, c->flow_work, c->flow_log, c->flow_contrib, c->flow_save};
+                    c->flow_out, c->flow_work, c->flow_log, c->flow_contrib, c->flow_save,
+                    c->flow_st_wr, c->flow_st_aux, c->flow_st_lg};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     if (c->cst) (void)hipStreamSynchronize(c->cst);
@@ -1099,6 +1104,11 @@ void *flow_log_of(askja_ctx *c, int k)
     return (char *)c->flow_log + (size_t)k * flow_log_reals(c->geom) * elem_bytes(c);
 }
 
+double *flow_wr_of(askja_ctx *c, int k)
+{
+    return c->flow_st_wr ? c->flow_st_wr + (size_t)k * stout_wr_doubles(c->geom) : nullptr;
+}
+
 // U = F(V) into flow_out; with store, the input links of every sub-layer
 // go to flow_log.  Per sub-layer log J into flow_lj.
 int flow_forward(askja_ctx *c, bool store)
@@ -1112,7 +1122,10 @@ int flow_forward(askja_ctx *c, bool store)
     for (int k = 0; k < c->flow_nsub; k++) {
         const FlowSub &f = c->flow_subs[k];
         HIPCHK(c, launch_flow_forward(c->geom, c->flow_out, c->flow_out, f, c->mode, c->part,
-                                      store ? flow_log_of(c, k) : nullptr, c->st));
+                                      store ? flow_log_of(c, k) : nullptr, flow_wr_of(c, k), c->st));
+        // stout: the per-link 8 x 8 log det from the W and rho just written
+        if (f.kind == kFlowStout)
+            HIPCHK(c, launch_stout_logdet(c->geom, f, flow_wr_of(c, k), c->part, nullptr, c->st));
         HIPCHK(c, launch_finalize(c->part, flow_forward_partials(c->geom, f.par), 1, c->flow_lj + 2 * k, c->st));
     }
     return ASKJA_OK;
@@ -1147,9 +1160,16 @@ int flow_gradient(askja_ctx *c, double beta, bool wgrad, double *action)
     // backward launch restores its sub-layer's input links from the log
     HIPCHK(c, hipMemcpyAsync(c->flow_work, c->flow_out, link_bytes(c), hipMemcpyDeviceToDevice, c->st));
     for (int k = c->flow_nsub - 1; k >= 0; k--) {
+        const FlowSub &f = c->flow_subs[k];
         double *gw = wgrad ? c->flow_gw + (size_t)c->flow_block[k] * c->flow_bsize : nullptr;
-        HIPCHK(c, launch_flow_backward(c->geom, c->flow_work, c->flow_subs[k], c->mode, c->flow_g, -1.0, gw,
-                                       flow_log_of(c, k), c->flow_contrib, c->st));
+        if (f.kind == kFlowStout) {
+            // c_logj d log det J of this sub-layer's links (c_logj = -1)
+            HIPCHK(c, launch_stout_logdet(c->geom, f, flow_wr_of(c, k), c->part, c->flow_st_aux, c->st));
+            HIPCHK(c, launch_stout_logdet_grad(c->geom, f, flow_wr_of(c, k), c->flow_st_aux, -1.0, c->flow_st_lg,
+                                               c->st));
+        }
+        HIPCHK(c, launch_flow_backward(c->geom, c->flow_work, f, c->mode, c->flow_g, -1.0, gw, flow_log_of(c, k),
+                                       c->flow_contrib, c->flow_st_lg, c->st));
     }
     if (action) {
         double h[2];
@@ -1174,17 +1194,20 @@ int kinetic_now(askja_ctx *c, double *k)
 
 }  // namespace
 
-int askja_flow_define(askja_ctx *c, int nsub, const int *mu, const int *parity, const int *block, int nblocks,
-                      double rho_max, double delta)
+namespace {
+
+int flow_define(askja_ctx *c, int kind, int nsub, const int *mu, const int *parity, const int *block, int nblocks,
+                double rho_max, double delta)
 {
-    ENTER(c);
     if (c->nc != 3) return fail(c, ASKJA_EUNSUP, "flow: only SU(3) is supported");
     if (c->dim < 3 || c->dim > 4)
         return fail(c, ASKJA_EUNSUP, "flow: D = 3 or 4 (D >= 3: planes orthogonal to every direction)");
     if (nsub < 1 || nsub > 4096 || nblocks < 1 || !mu || !parity || !block)
         return fail(c, ASKJA_EINVAL, "flow_define: bad sizes or null arrays");
-    if (!(rho_max > 0.0) || rho_max * 2.0 * (c->dim - 1) >= 2.0)
+    if (kind == kFlowRankOne && (!(rho_max > 0.0) || rho_max * 2.0 * (c->dim - 1) >= 2.0))
         return fail(c, ASKJA_EINVAL, "flow_define: rho_max must be in (0, 1/(D-1)) for invertibility");
+    if (kind == kFlowStout && (!(rho_max > 0.0) || rho_max * 2.0 * (c->dim - 1) >= 1.0))
+        return fail(c, ASKJA_EINVAL, "flow_define_stout: rho_max must be in (0, 1/(2(D-1))) for invertibility");
     for (int d = 0; d < c->dim; d++)
         if (c->ext[d] % 2) return fail(c, ASKJA_EUNSUP, "flow: every extent must be even (checkerboard masks)");
     for (int k = 0; k < nsub; k++)
@@ -1192,13 +1215,14 @@ int askja_flow_define(askja_ctx *c, int nsub, const int *mu, const int *parity, 
             return fail(c, ASKJA_EINVAL, "flow_define: sub-layer %d out of range", k);
     const int nf = 3 * (c->dim - 1) * (c->dim - 2) / 2;  // plaquette and two rectangles per plane
     // (re)allocate
-    void *old[] = {c->flow_w,    c->flow_gw,   c->flow_g,        c->flow_lj, c->flow_out,
-                   c->flow_work, c->flow_log,  c->flow_contrib, c->flow_save};
+    void *old[] = {c->flow_w,    c->flow_gw,  c->flow_g,       c->flow_lj,    c->flow_out,  c->flow_work,
+                   c->flow_log,  c->flow_contrib, c->flow_save, c->flow_st_wr, c->flow_st_aux, c->flow_st_lg};
     for (void *p : old)
         if (p) (void)hipFree(p);
     c->flow_w = nullptr;
     c->flow_gw = c->flow_g = c->flow_lj = nullptr;
     c->flow_out = c->flow_work = c->flow_log = c->flow_save = nullptr;
+    c->flow_st_wr = c->flow_st_aux = c->flow_st_lg = nullptr;
     c->flow_contrib = nullptr;
     c->flow_nsub = 0;
     c->flow_bsize = 32 * nf + 2 * 32 + 1;
@@ -1213,12 +1237,35 @@ int askja_flow_define(askja_ctx *c, int nsub, const int *mu, const int *parity, 
     HIPCHK(c, hipMalloc(&c->flow_log, (size_t)nsub * flow_log_reals(c->geom) * elem_bytes(c)));
     HIPCHK(c, hipMalloc((void **)&c->flow_contrib, flow_contrib_doubles(c->geom) * sizeof(double)));
     HIPCHK(c, hipMalloc(&c->flow_save, link_bytes(c)));
+    if (kind == kFlowStout) {
+        HIPCHK(c, hipMalloc((void **)&c->flow_st_wr, (size_t)nsub * stout_wr_doubles(c->geom) * sizeof(double)));
+        HIPCHK(c, hipMalloc((void **)&c->flow_st_aux, stout_aux_doubles(c->geom) * sizeof(double)));
+        HIPCHK(c, hipMalloc((void **)&c->flow_st_lg, stout_lg_doubles(c->geom) * sizeof(double)));
+    }
     c->flow_subs.resize(nsub);
     c->flow_block.assign(block, block + nsub);
     for (int k = 0; k < nsub; k++)
-        c->flow_subs[k] = FlowSub{mu[k], parity[k], c->flow_w + (size_t)block[k] * c->flow_bsize, nf, rho_max, delta};
+        c->flow_subs[k] =
+            FlowSub{mu[k], parity[k], c->flow_w + (size_t)block[k] * c->flow_bsize, nf, rho_max, delta, kind};
+    c->flow_kind = kind;
     c->flow_nsub = nsub;
     return ASKJA_OK;
+}
+
+}  // namespace
+
+int askja_flow_define(askja_ctx *c, int nsub, const int *mu, const int *parity, const int *block, int nblocks,
+                      double rho_max, double delta)
+{
+    ENTER(c);
+    return flow_define(c, kFlowRankOne, nsub, mu, parity, block, nblocks, rho_max, delta);
+}
+
+int askja_flow_define_stout(askja_ctx *c, int nsub, const int *mu, const int *parity, const int *block, int nblocks,
+                            double rho_max)
+{
+    ENTER(c);
+    return flow_define(c, kFlowStout, nsub, mu, parity, block, nblocks, rho_max, 0.0);
 }
 
 int askja_flow_block_size(askja_ctx *c, int *bsize)

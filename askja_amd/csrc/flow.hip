@@ -210,6 +210,236 @@ __device__ inline void st(R *U, long l, const C3 &m)
         }
 }
 
+// ---- stout sub-layer: the su(3) Jacobian -------------------------------------
+// The stout update of link l is U' = exp(iQ) U with Q = -rho TAH(W), W = U S
+// (S the frozen staple sum, rho from the MLP of frozen features).  In
+// left-invariant coordinates x_a = 2 Tr(T^a X) (Tr T^a T^b = delta_ab / 2)
+//   J = Ad_{exp(iQ)} (1 + g(ad_{iQ}) M),  g(z) = (1 - e^{-z}) / z,
+//   M X = -rho TAH(i X W)  (the change of Q when U -> e^{iX} U),
+// and det Ad = 1, so log det J = log det(1 + g(ad_{iQ}) M).  Everything is
+// evaluated in the eigenbasis of Q = V diag(lambda) V^+ (the hat frame,
+// Y^ = V^+ Y V): there g(ad_{iQ}) multiplies element (a, b) by
+// g(i (lambda_a - lambda_b)), and M^ X^ = -rho TAH(i X^ W^).  The 8 x 8
+// matrix 1 + g M in the basis T^a (taken in the hat frame; the determinant
+// does not depend on the basis) is inverted in place by Gauss-Jordan
+// elimination, whose pivots give the log det (|g M| < 1 for
+// rho < 1/(2(D-1)), so no pivoting is needed).
+//
+// Gradient of L = log det(1 + g M) with G = (1 + g M)^{-1}, U_b = sum_a G_ba T^a:
+//   through M:  dL = -2 drho ReTr(Lam W) - 2 rho ReTr(Lam dW),
+//               Lam = sum_b g(ad_{-iQ})(U_b) T^b;
+//   through g:  dL = 2 Tr(Z dQ), with (Daleckii-Krein form of the derivative
+//               of g(ad_{iQ}) in the hat frame)
+//               Z^_ab = i sum_{b', l} [D_abl M^_b',al U^_b',lb
+//                                      - conj(D_abl) U^_b',al M^_b',lb],
+//               M_b' = -rho TAH(i T^b' W), D_abl = sum_m c_{m+1} i^m
+//               h_m(lambda_a - lambda_l, lambda_b - lambda_l) with
+//               c_k = (-1)^k / (k + 1)! and h_m the complete homogeneous
+//               polynomial of degree m (the divided difference of g).
+
+// Gell-Mann matrix b / 2, element (p, q), real and imaginary parts
+__device__ inline double gm_re(int b, int p, int q)
+{
+    constexpr double h = 0.5, c8 = 0.28867513459481287, d8 = -0.5773502691896258;
+    constexpr double R[8][3][3] = {
+        {{0, h, 0}, {h, 0, 0}, {0, 0, 0}}, {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}},
+        {{h, 0, 0}, {0, -h, 0}, {0, 0, 0}}, {{0, 0, h}, {0, 0, 0}, {h, 0, 0}},
+        {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, {{0, 0, 0}, {0, 0, h}, {0, h, 0}},
+        {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, {{c8, 0, 0}, {0, c8, 0}, {0, 0, d8}}};
+    return R[b][p][q];
+}
+__device__ inline double gm_im(int b, int p, int q)
+{
+    constexpr double h = 0.5;
+    constexpr double I[8][3][3] = {
+        {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, {{0, -h, 0}, {h, 0, 0}, {0, 0, 0}},
+        {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}},
+        {{0, 0, -h}, {0, 0, 0}, {h, 0, 0}}, {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}},
+        {{0, 0, 0}, {0, 0, -h}, {0, h, 0}}, {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}};
+    return I[b][p][q];
+}
+
+// i T^b Y (b a compile-time constant after unrolling)
+__device__ inline C3 i_gm_mul(int b, const C3 &Y)
+{
+    C3 Z;
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            double zr = 0.0, zi = 0.0;
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                const double tr = gm_re(b, p, r), ti = gm_im(b, p, r);
+                // i (tr + i ti) (yr + i yi) = (-ti yr - tr yi) + i (tr yr - ti yi)
+                zr += -ti * Y.r[r][q] - tr * Y.i[r][q];
+                zi += tr * Y.r[r][q] - ti * Y.i[r][q];
+            }
+            Z.r[p][q] = zr;
+            Z.i[p][q] = zi;
+        }
+    return Z;
+}
+
+// 2 Re Tr(T^a Y)
+__device__ inline double gm_coord(int a, const C3 &Y)
+{
+    double s = 0.0;
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) s += gm_re(a, p, r) * Y.r[r][p] - gm_im(a, p, r) * Y.i[r][p];
+    return 2.0 * s;
+}
+
+// Eigen-decomposition of a Hermitian 3 x 3 matrix by cyclic complex Jacobi
+// rotations: A = V diag(lam) V^+ (columns of V the eigenvectors).
+__device__ inline void herm_eig3(C3 A, double lam[3], C3 &V)
+{
+    eye(V);
+#pragma unroll 1
+    for (int sweep = 0; sweep < 10; sweep++) {
+        const double off = A.r[0][1] * A.r[0][1] + A.i[0][1] * A.i[0][1] + A.r[0][2] * A.r[0][2] +
+                           A.i[0][2] * A.i[0][2] + A.r[1][2] * A.r[1][2] + A.i[1][2] * A.i[1][2];
+        const double dia = A.r[0][0] * A.r[0][0] + A.r[1][1] * A.r[1][1] + A.r[2][2] * A.r[2][2];
+        if (off <= 1e-36 * dia || off < 1e-300) break;
+#pragma unroll
+        for (int pr = 0; pr < 3; pr++) {
+            const int p = pr == 2 ? 1 : 0, q = pr == 0 ? 1 : 2;
+            const double ar = A.r[p][q], ai = A.i[p][q];
+            const double a = sqrt(ar * ar + ai * ai);
+            if (a < 1e-300) continue;
+            const double phr = ar / a, phi = ai / a;  // ph = a_pq / |a_pq|
+            const double tau = (A.r[q][q] - A.r[p][p]) / (2.0 * a);
+            const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+            const double c = 1.0 / sqrt(1.0 + t * t), sn = t * c;
+            // G = 1 but G_pp = G_qq = c, G_pq = s ph, G_qp = -s conj(ph);
+            // A <- G^+ A G (columns, then rows), V <- V G
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const double xr = A.r[k][p], xi = A.i[k][p], yr = A.r[k][q], yi = A.i[k][q];
+                // col p: c x - s conj(ph) y ; col q: s ph x + c y
+                A.r[k][p] = c * xr - sn * (phr * yr + phi * yi);
+                A.i[k][p] = c * xi - sn * (phr * yi - phi * yr);
+                A.r[k][q] = sn * (phr * xr - phi * xi) + c * yr;
+                A.i[k][q] = sn * (phr * xi + phi * xr) + c * yi;
+                const double vxr = V.r[k][p], vxi = V.i[k][p], vyr = V.r[k][q], vyi = V.i[k][q];
+                V.r[k][p] = c * vxr - sn * (phr * vyr + phi * vyi);
+                V.i[k][p] = c * vxi - sn * (phr * vyi - phi * vyr);
+                V.r[k][q] = sn * (phr * vxr - phi * vxi) + c * vyr;
+                V.i[k][q] = sn * (phr * vxi + phi * vxr) + c * vyi;
+            }
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const double xr = A.r[p][k], xi = A.i[p][k], yr = A.r[q][k], yi = A.i[q][k];
+                // row p: c x - s ph y ; row q: s conj(ph) x + c y
+                A.r[p][k] = c * xr - sn * (phr * yr - phi * yi);
+                A.i[p][k] = c * xi - sn * (phr * yi + phi * yr);
+                A.r[q][k] = sn * (phr * xr + phi * xi) + c * yr;
+                A.i[q][k] = sn * (phr * xi - phi * xr) + c * yi;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) lam[k] = A.r[k][k];
+}
+
+// g(i d) = (1 - e^{-id}) / (id) = sin(d)/d - i (1 - cos d)/d, by its series
+// (|d| <= 2 here: 14 terms reach 2^-60)
+__device__ inline void g_of(double d, double &gr, double &gi)
+{
+    const double z = d * d;
+    double fs = 1.0, fc = 0.5, sg = 1.0, sc = 0.0, oc = 0.0;
+#pragma unroll
+    for (int k = 0; k < 14; k++) {
+        sc += sg * fs;
+        oc += sg * fc;
+        sg *= -z;
+        fs /= (double)((2 * k + 2) * (2 * k + 3));
+        fc /= (double)((2 * k + 3) * (2 * k + 4));
+    }
+    gr = sc;
+    gi = -d * oc;
+}
+
+// D(iu, iv) = sum_m c_{m+1} i^m h_m(u, v): the divided difference
+// (g(iu) - g(iv)) / (iu - iv), summed without division (|u|, |v| <= 2)
+__device__ inline void dd_of(double u, double v, double &re, double &im)
+{
+    // c_{m+1} i^m = (-1)^(m+1) i^m / (m+2)!: real -(-1)^(m/2) / (m+2)! for
+    // even m, imaginary (-1)^((m-1)/2) / (m+2)! for odd m
+    double h = 1.0, vm = 1.0, fact = 2.0;
+    re = 0.0;
+    im = 0.0;
+#pragma unroll
+    for (int m = 0; m < 28; m++) {
+        const double s2 = (m & 2) ? -1.0 : 1.0;
+        if (m & 1) im += s2 * h / fact;
+        else re -= s2 * h / fact;
+        vm *= v;
+        h = u * h + vm;
+        fact *= (double)(m + 3);
+    }
+}
+
+// 8 x 8 matrix 1 + g(ad_{iQ}) M in the hat frame (column b = image of T^b)
+__device__ inline void stout_jac(const C3 &Wh, double rho, const double (&gr)[3][3], const double (&gi)[3][3],
+                                 double (&J)[8][8])
+{
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+        const C3 Mb = cscale(tah(i_gm_mul(b, Wh)), -rho, 0.0);
+        C3 Y;
+#pragma unroll
+        for (int p = 0; p < 3; p++)
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                Y.r[p][q] = gr[p][q] * Mb.r[p][q] - gi[p][q] * Mb.i[p][q];
+                Y.i[p][q] = gr[p][q] * Mb.i[p][q] + gi[p][q] * Mb.r[p][q];
+            }
+#pragma unroll
+        for (int a = 0; a < 8; a++) J[a][b] = (a == b ? 1.0 : 0.0) + gm_coord(a, Y);
+    }
+}
+
+// In-place inverse by Gauss-Jordan elimination without pivoting; returns
+// the log of the determinant (sum of the log pivots)
+__device__ inline double gauss_jordan8(double (&A)[8][8])
+{
+    double ld = 0.0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const double piv = A[k][k];
+        ld += log(piv);
+        const double inv = 1.0 / piv;
+        A[k][k] = 1.0;
+#pragma unroll
+        for (int j = 0; j < 8; j++) A[k][j] *= inv;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            if (i == k) continue;
+            const double f = A[i][k];
+            A[i][k] = 0.0;
+#pragma unroll
+            for (int j = 0; j < 8; j++) A[i][j] -= f * A[k][j];
+        }
+    }
+    return ld;
+}
+
+// the hat frame of one link: Q = -rho TAH(W), its eigenbasis, W^ and the
+// factors g(i (lambda_a - lambda_b))
+__device__ inline void stout_frame(const C3 &W, double rho, double (&lam)[3], C3 &V, C3 &Wh, double (&gr)[3][3],
+                                   double (&gi)[3][3])
+{
+    herm_eig3(cscale(tah(W), -rho, 0.0), lam, V);
+    Wh = mm<true, false>(V, mm<false, false>(W, V));
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int q = 0; q < 3; q++) g_of(lam[p] - lam[q], gr[p][q], gi[p][q]);
+}
+
 // gradient buffer: 9 doubles per link (compact Hermitian: diag, then
 // (0,1), (0,2), (1,2) as re, im)
 __device__ inline C3 gget(const double *g, long l)
@@ -368,7 +598,7 @@ __device__ inline C3 xget(const double (*d)[kLinks], int lane)
 // link on entry (ok lanes).  Every thread of the workgroup calls it (it has
 // two barriers).  The loops stay rolled (a fully unrolled stencil hoists
 // every neighbour load and spills).
-template <typename R, int D, int MU>
+template <typename R, int D, int MU, bool ST>
 __device__ inline void forward_link(const Geom &g, const R *U, const Sub &sb, long s, bool ok, int role, int lane,
                                     Xch<D> &x, Fwd<D> &F)
 {
@@ -431,9 +661,11 @@ __device__ inline void forward_link(const Geom &g, const R *U, const Sub &sb, lo
     if (ok) {
 #pragma unroll
         for (int q = 0; q < nf; q++) F.f[q] = x.f[q][lane];
-        const C3 Lam = tah(xget(x.c, lane));
-        F.n = sqrt(2.0 * retr(mm<false, false>(Lam, Lam)) + sb.delta * sb.delta);
-        F.Lh = cscale(Lam, 1.0 / F.n, 0.0);
+        if constexpr (!ST) {
+            const C3 Lam = tah(xget(x.c, lane));
+            F.n = sqrt(2.0 * retr(mm<false, false>(Lam, Lam)) + sb.delta * sb.delta);
+            F.Lh = cscale(Lam, 1.0 / F.n, 0.0);
+        }
         // hidden units [role kHid/2, (role + 1) kHid/2); role 0 adds b2
         double zp = role == 0 ? (double)w[kHid * nf + 2 * kHid] : 0.0;
 #pragma unroll 1
@@ -449,8 +681,10 @@ __device__ inline void forward_link(const Geom &g, const R *U, const Sub &sb, lo
     if (ok) {
         F.rho = sb.rho_max * tanh(x.z[0][lane] + x.z[1][lane]);
         F.W = mm<false, false>(F.U, xget(x.sm, lane));
-        F.theta = F.rho * imtr(mm<false, false>(F.Lh, F.W));
-        F.K = retr(mm<false, false>(mm<false, false>(F.Lh, F.Lh), F.W));
+        if constexpr (!ST) {
+            F.theta = F.rho * imtr(mm<false, false>(F.Lh, F.W));
+            F.K = retr(mm<false, false>(mm<false, false>(F.Lh, F.Lh), F.W));
+        }
     }
 }
 
@@ -465,8 +699,11 @@ __device__ inline bool site_of(const Geom &g, int par, long t, long &s)
 // One sub-layer in place (Uin may equal Uout: no thread reads another
 // thread's updated link).  With log, the sub-layer's input links are kept,
 // link of parity-site t at index t (64-link tiles), for the backward.
-template <typename R, int D, int MU>
-__global__ __launch_bounds__(2 * kLinks) void k_forward(Geom g, const R *Uin, R *Uout, Sub sb, double *part, R *log)
+// ST: the stout update U' = exp(-i rho TAH(W)) U instead, with W and rho of
+// every updated link written to wr (stride T) for k_stout_logdet; no partials.
+template <typename R, int D, int MU, bool ST>
+__global__ __launch_bounds__(2 * kLinks) void k_forward(Geom g, const R *Uin, R *Uout, Sub sb, double *part, R *log,
+                                                        double *wr, long T)
 {
     __shared__ Xch<D> x;
     const int lane = (int)(threadIdx.x % kLinks), role = (int)(threadIdx.x / kLinks);
@@ -479,13 +716,27 @@ __global__ __launch_bounds__(2 * kLinks) void k_forward(Geom g, const R *Uin, R 
         F.U = ld(Uin, MU * g.V + s);
         if (log && role == 0) st(log, t, F.U);
     }
-    forward_link<R, D, MU>(g, Uin, sb, s, ok, role, lane, x, F);
-    if (ok && role == 0) {
-        const C3 E = expi(cscale(F.Lh, F.theta, 0.0));
-        st(Uout, MU * g.V + s, mm<false, false>(E, F.U));
-        sum[0] = log1p(F.rho * F.K);
+    forward_link<R, D, MU, ST>(g, Uin, sb, s, ok, role, lane, x, F);
+    if constexpr (ST) {
+        if (ok && role == 0) {
+            const C3 E = expi(cscale(tah(F.W), -F.rho, 0.0));
+            st(Uout, MU * g.V + s, mm<false, false>(E, F.U));
+        } else if (ok) {
+#pragma unroll
+            for (int e = 0; e < 9; e++) {
+                wr[(size_t)e * T + t] = F.W.r[e / 3][e % 3];
+                wr[(size_t)(9 + e) * T + t] = F.W.i[e / 3][e % 3];
+            }
+            wr[(size_t)18 * T + t] = F.rho;
+        }
+    } else {
+        if (ok && role == 0) {
+            const C3 E = expi(cscale(F.Lh, F.theta, 0.0));
+            st(Uout, MU * g.V + s, mm<false, false>(E, F.U));
+            sum[0] = log1p(F.rho * F.K);
+        }
+        block_reduce_store<1>(sum, cmp, part + 2 * (size_t)blockIdx.x);
     }
-    block_reduce_store<1>(sum, cmp, part + 2 * (size_t)blockIdx.x);
 }
 
 __device__ inline double wave_sum(double v)
@@ -532,12 +783,16 @@ __device__ inline void rect_grad(const R *U, const Rect &r, double c, double *co
     }
 }
 
-template <typename R, int D, int MU>
+// ST: the stout sub-layer's backward; the log det part of the gradient
+// comes from k_stout_logdet_grad in lg (stride T), c_logj already applied.
+template <typename R, int D, int MU, bool ST>
 __global__ __launch_bounds__(2 * kLinks) void k_backward(Geom g, R *U, Sub sb, double *gr, double c_logj, double *gw,
-                                                         const R *log, double *contrib, long T)
+                                                         const R *log, double *contrib, long T, const double *lg)
 {
     constexpr int nf = nfeat(D), np = nplanes(D);
     __shared__ Xch<D> x;
+    __shared__ double sga[18][kLinks];  // stout: Gamma = R^T(G') from role 0
+    __shared__ double sdr[kLinks];      //        and its d / d rho part
     const int lane = (int)(threadIdx.x % kLinks), role = (int)(threadIdx.x / kLinks);
     const long t = (long)blockIdx.x * kLinks + lane;
     long s;
@@ -551,13 +806,48 @@ __global__ __launch_bounds__(2 * kLinks) void k_backward(Geom g, R *U, Sub sb, d
         F.U = ld(log, t);
         if (role == 0) st(U, l, F.U);
     }
-    forward_link<R, D, MU>(g, U, sb, s, ok, role, lane, x, F);
+    forward_link<R, D, MU, ST>(g, U, sb, s, ok, role, lane, x, F);
     // Both roles need dPhi/dz; role 0 then needs dPhi/dC (through Lhat, the
     // dexp adjoint series), role 1 the own link's gradient (through E) and
     // dPhi/dS.
     double dz = 0.0;
     C3 gU, GC, GS;
-    if (ok) {
+    if constexpr (ST) {
+        // U' = E U, E = exp(iQ), Q = -rho TAH(W):
+        //   dPhi = 2 Tr(G' dE E^+) + 2 Tr(E^+ G' E X)     (dU = iXU)
+        //   2 Tr(G' dE E^+) = 2 Tr(Gamma dQ), Gamma = sum_k ad_{-iQ}^k(G') / (k+1)!
+        //   dQ = -drho TAH(W) - rho TAH(dW): dPhi/drho = -2 Im Tr(Gamma W),
+        //   G_W = i rho Gamma (dPhi = 2 ReTr(G_W dW)), then the log det's part
+        if (ok) {
+            const C3 gp = gget(gr, l);
+            const C3 Q = cscale(tah(F.W), -F.rho, 0.0);
+            if (role == 0) {
+                const C3 Ga = avg_ad(Q, gp);
+                xput(sga, lane, Ga);
+                sdr[lane] = -2.0 * imtr(mm<false, false>(Ga, F.W));
+                zero(GC);
+            } else {
+                const C3 E = expi(Q);
+                gU = proj(mm<true, false>(E, mm<false, false>(gp, E)));
+            }
+        }
+        __syncthreads();
+        if (ok) {
+            const double drho = sdr[lane] + lg[(size_t)18 * T + t];
+            const double th = F.rho / sb.rho_max;
+            dz = drho * sb.rho_max * (1.0 - th * th);
+            if (role == 1) {
+                C3 GW = cscale(xget(sga, lane), 0.0, F.rho);
+#pragma unroll
+                for (int e = 0; e < 9; e++) {
+                    GW.r[e / 3][e % 3] += lg[(size_t)e * T + t];
+                    GW.i[e / 3][e % 3] += lg[(size_t)(9 + e) * T + t];
+                }
+                addto(gU, proj(cscale(mm<false, false>(F.W, GW), 0.0, 1.0)));
+                GS = mm<false, false>(GW, F.U);
+            }
+        }
+    } else if (ok) {
         const C3 gp = gget(gr, l);
         const double dtheta = 2.0 * retr(mm<false, false>(gp, F.Lh));
         // theta = rho Im Tr(Lhat W)
@@ -855,6 +1145,191 @@ __global__ void k_kick_g(Geom g, R *P, const double *gr, double eps_c)
     for (int c = 0; c < 9; c++) p[c * 64] = (R)((double)p[c * 64] - eps_c * b[c]);
 }
 
+// ---- stout sub-layer: log det J and its gradient, one thread per link ----
+// wr holds W = U S (18 doubles, re then im, row-major) and rho of every
+// updated link t (parity-site index), structure of arrays with stride T.
+
+__device__ inline C3 wr_get(const double *wr, long t, long T, double &rho)
+{
+    C3 W;
+#pragma unroll
+    for (int e = 0; e < 9; e++) {
+        W.r[e / 3][e % 3] = wr[(size_t)e * T + t];
+        W.i[e / 3][e % 3] = wr[(size_t)(9 + e) * T + t];
+    }
+    rho = wr[(size_t)18 * T + t];
+    return W;
+}
+
+// log det J per link into per-block partials; with aux also (1 + g M)^{-1}
+// (64 doubles), V (18) and lambda (3) for the gradient kernel
+__global__ __launch_bounds__(kLinks) void k_stout_logdet(Geom g, int par, const double *wr, long T, double *part,
+                                                         double *aux)
+{
+    const long t = (long)blockIdx.x * kLinks + threadIdx.x;
+    double sum[1] = {0.0}, cmp[1] = {0.0};
+    long s;
+    if (site_of(g, par, t, s)) {
+        double rho;
+        const C3 W = wr_get(wr, t, T, rho);
+        double lam[3], gr[3][3], gi[3][3];
+        C3 V, Wh;
+        stout_frame(W, rho, lam, V, Wh, gr, gi);
+        double J[8][8];
+        stout_jac(Wh, rho, gr, gi, J);
+        sum[0] = gauss_jordan8(J);
+        if (aux) {
+#pragma unroll
+            for (int a = 0; a < 8; a++)
+#pragma unroll
+                for (int b = 0; b < 8; b++) aux[(size_t)(a * 8 + b) * T + t] = J[a][b];
+#pragma unroll
+            for (int e = 0; e < 9; e++) {
+                aux[(size_t)(64 + e) * T + t] = V.r[e / 3][e % 3];
+                aux[(size_t)(73 + e) * T + t] = V.i[e / 3][e % 3];
+            }
+#pragma unroll
+            for (int k = 0; k < 3; k++) aux[(size_t)(82 + k) * T + t] = lam[k];
+        }
+    }
+    block_reduce_store<1>(sum, cmp, part + 2 * (size_t)blockIdx.x);
+}
+
+// U_b = sum_a G_ba T^a from row b of the inverse (in aux)
+__device__ inline C3 row_gen(const double *aux, int b, long t, long T)
+{
+    C3 U;
+    zero(U);
+#pragma unroll
+    for (int a = 0; a < 8; a++) {
+        const double gba = aux[(size_t)(b * 8 + a) * T + t];
+#pragma unroll
+        for (int p = 0; p < 3; p++)
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                U.r[p][q] += gba * gm_re(a, p, q);
+                U.i[p][q] += gba * gm_im(a, p, q);
+            }
+    }
+    return U;
+}
+
+// lg = c (i rho Z - rho Lam) (18 doubles: the G_W of c log det J) and
+// c (-2 Im Tr(Z W) - 2 Re Tr(Lam W)) (d / d rho), per updated link
+__global__ __launch_bounds__(kLinks) void k_stout_logdet_grad(Geom g, int par, const double *wr, const double *aux,
+                                                              long T, double c, double *lg)
+{
+    const long t = (long)blockIdx.x * kLinks + threadIdx.x;
+    long s;
+    if (!site_of(g, par, t, s)) return;
+    double rho;
+    const C3 W = wr_get(wr, t, T, rho);
+    C3 V;
+    double lam[3];
+#pragma unroll
+    for (int e = 0; e < 9; e++) {
+        V.r[e / 3][e % 3] = aux[(size_t)(64 + e) * T + t];
+        V.i[e / 3][e % 3] = aux[(size_t)(73 + e) * T + t];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) lam[k] = aux[(size_t)(82 + k) * T + t];
+    const C3 Wh = mm<true, false>(V, mm<false, false>(W, V));
+    double gr[3][3], gi[3][3];
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int q = 0; q < 3; q++) g_of(lam[p] - lam[q], gr[p][q], gi[p][q]);
+    // through M: Lam^ = sum_b g(ad_{-iQ})(U_b) T^b, g(-ix) = conj(g(ix))
+    C3 Lh;
+    zero(Lh);
+#pragma unroll 1
+    for (int bb = 0; bb < 8; bb++) {
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            if (b != bb) continue;
+            const C3 U = row_gen(aux, b, t, T);
+            C3 Y;
+#pragma unroll
+            for (int p = 0; p < 3; p++)
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    Y.r[p][q] = gr[p][q] * U.r[p][q] + gi[p][q] * U.i[p][q];
+                    Y.i[p][q] = gr[p][q] * U.i[p][q] - gi[p][q] * U.r[p][q];
+                }
+#pragma unroll
+            for (int p = 0; p < 3; p++)
+#pragma unroll
+                for (int q = 0; q < 3; q++)
+#pragma unroll
+                    for (int r = 0; r < 3; r++) {
+                        const double tr = gm_re(b, r, q), ti = gm_im(b, r, q);
+                        Lh.r[p][q] += Y.r[p][r] * tr - Y.i[p][r] * ti;
+                        Lh.i[p][q] += Y.r[p][r] * ti + Y.i[p][r] * tr;
+                    }
+        }
+    }
+    // through g: Z^ with the divided differences D_abl (symmetric in a, b),
+    // kept in LDS (one column per thread)
+    __shared__ double sD[2][27][kLinks];
+    const int lane = (int)threadIdx.x;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = a; b < 3; b++)
+#pragma unroll
+            for (int l = 0; l < 3; l++) {
+                double dr, di;
+                dd_of(lam[a] - lam[l], lam[b] - lam[l], dr, di);
+                sD[0][(a * 3 + b) * 3 + l][lane] = sD[0][(b * 3 + a) * 3 + l][lane] = dr;
+                sD[1][(a * 3 + b) * 3 + l][lane] = sD[1][(b * 3 + a) * 3 + l][lane] = di;
+            }
+    C3 Zh;
+    zero(Zh);
+#pragma unroll 1
+    for (int bb = 0; bb < 8; bb++) {
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            if (b != bb) continue;
+            const C3 U = row_gen(aux, b, t, T);
+            const C3 M = cscale(tah(i_gm_mul(b, Wh)), -rho, 0.0);
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    double ar = 0.0, ai = 0.0;
+#pragma unroll
+                    for (int l = 0; l < 3; l++) {
+                        const double dr = sD[0][(a * 3 + q) * 3 + l][lane], di = sD[1][(a * 3 + q) * 3 + l][lane];
+                        // M_al U_lq
+                        const double p1r = M.r[a][l] * U.r[l][q] - M.i[a][l] * U.i[l][q];
+                        const double p1i = M.r[a][l] * U.i[l][q] + M.i[a][l] * U.r[l][q];
+                        // U_al M_lq
+                        const double p2r = U.r[a][l] * M.r[l][q] - U.i[a][l] * M.i[l][q];
+                        const double p2i = U.r[a][l] * M.i[l][q] + U.i[a][l] * M.r[l][q];
+                        // D p1 - conj(D) p2
+                        ar += dr * p1r - di * p1i - (dr * p2r + di * p2i);
+                        ai += dr * p1i + di * p1r - (dr * p2i - di * p2r);
+                    }
+                    // Z^ += i (ar + i ai)
+                    Zh.r[a][q] -= ai;
+                    Zh.i[a][q] += ar;
+                }
+        }
+    }
+    const C3 Lo = mm<false, true>(mm<false, false>(V, Lh), V);
+    const C3 Zo = mm<false, true>(mm<false, false>(V, Zh), V);
+    C3 G = cscale(Zo, 0.0, rho);  // i rho Z
+    addto(G, Lo, -rho);
+    G = cscale(G, c, 0.0);
+    const double dr = c * (-2.0 * imtr(mm<false, false>(Zo, W)) - 2.0 * retr(mm<false, false>(Lo, W)));
+#pragma unroll
+    for (int e = 0; e < 9; e++) {
+        lg[(size_t)e * T + t] = G.r[e / 3][e % 3];
+        lg[(size_t)(9 + e) * T + t] = G.i[e / 3][e % 3];
+    }
+    lg[(size_t)18 * T + t] = dr;
+}
+
 }  // namespace flow
 
 // ---------------------------------------------------------------- launchers
@@ -892,14 +1367,47 @@ static hipError_t flow_dispatch(int D, int mu, unsigned nb, hipStream_t st, Args
 
 template <typename R, int D, int MU>
 struct FwdK {
-    static constexpr auto fn = flow::k_forward<R, D, MU>;
+    static constexpr auto fn = flow::k_forward<R, D, MU, false>;
+    static constexpr int block = 2 * flow::kLinks;
+};
+template <typename R, int D, int MU>
+struct FwdKS {
+    static constexpr auto fn = flow::k_forward<R, D, MU, true>;
     static constexpr int block = 2 * flow::kLinks;
 };
 template <typename R, int D, int MU>
 struct BwdK {
-    static constexpr auto fn = flow::k_backward<R, D, MU>;
+    static constexpr auto fn = flow::k_backward<R, D, MU, false>;
     static constexpr int block = 2 * flow::kLinks;
 };
+template <typename R, int D, int MU>
+struct BwdKS {
+    static constexpr auto fn = flow::k_backward<R, D, MU, true>;
+    static constexpr int block = 2 * flow::kLinks;
+};
+
+// parity-site stride of the per-link structure-of-arrays buffers
+static long sub_stride(const Geom &g) { return (long)(flow_log_reals(g) / 18); }
+
+size_t stout_wr_doubles(const Geom &g) { return (size_t)19 * sub_stride(g); }
+size_t stout_aux_doubles(const Geom &g) { return (size_t)85 * sub_stride(g); }
+size_t stout_lg_doubles(const Geom &g) { return (size_t)19 * sub_stride(g); }
+
+hipError_t launch_stout_logdet(const Geom &g, const FlowSub &f, const double *st_wr, double *part, double *aux,
+                               hipStream_t st)
+{
+    hipLaunchKernelGGL(flow::k_stout_logdet, dim3(par_blocks(g, f.par)), dim3(flow::kLinks), 0, st, g, f.par, st_wr,
+                       sub_stride(g), part, aux);
+    return hipGetLastError();
+}
+
+hipError_t launch_stout_logdet_grad(const Geom &g, const FlowSub &f, const double *st_wr, const double *aux,
+                                    double c_logj, double *lg, hipStream_t st)
+{
+    hipLaunchKernelGGL(flow::k_stout_logdet_grad, dim3(par_blocks(g, f.par)), dim3(flow::kLinks), 0, st, g, f.par,
+                       st_wr, aux, sub_stride(g), c_logj, lg);
+    return hipGetLastError();
+}
 template <typename R, int D, int MU>
 struct GatK {
     static constexpr auto fn = flow::k_gather<R, D, MU>;
@@ -907,30 +1415,46 @@ struct GatK {
 };
 
 hipError_t launch_flow_forward(const Geom &g, const void *Uin, void *Uout, const FlowSub &f, int mode, double *part,
-                               void *log, hipStream_t st)
+                               void *log, double *st_wr, hipStream_t st)
 {
     const unsigned nb = par_blocks(g, f.par);
-    if (f32_mode(mode))
+    const long T = sub_stride(g);
+    const bool stout = f.kind == kFlowStout;
+    if (stout && !st_wr) return hipErrorInvalidValue;
+    if (f32_mode(mode)) {
+        if (stout)
+            return flow_dispatch<FwdKS, float>(g.D, f.mu, nb, st, g, (const float *)Uin, (float *)Uout, to_sub(f), part,
+                                               (float *)log, st_wr, T);
         return flow_dispatch<FwdK, float>(g.D, f.mu, nb, st, g, (const float *)Uin, (float *)Uout, to_sub(f), part,
-                                          (float *)log);
+                                          (float *)log, st_wr, T);
+    }
+    if (stout)
+        return flow_dispatch<FwdKS, double>(g.D, f.mu, nb, st, g, (const double *)Uin, (double *)Uout, to_sub(f),
+                                            part, (double *)log, st_wr, T);
     return flow_dispatch<FwdK, double>(g.D, f.mu, nb, st, g, (const double *)Uin, (double *)Uout, to_sub(f), part,
-                                       (double *)log);
+                                       (double *)log, st_wr, T);
 }
 
 hipError_t launch_flow_backward(const Geom &g, void *U, const FlowSub &f, int mode, double *gr, double c_logj,
-                                double *gw, const void *log, double *contrib, hipStream_t st)
+                                double *gw, const void *log, double *contrib, const double *lg, hipStream_t st)
 {
     const unsigned nb = par_blocks(g, f.par);
-    const long T = (long)(flow_log_reals(g) / 18);
+    const long T = sub_stride(g);
     const unsigned nl = (unsigned)((g.V * g.D + flow::kB - 1) / flow::kB);
+    const bool stout = f.kind == kFlowStout;
+    if (stout && !lg) return hipErrorInvalidValue;
     hipError_t e;
     if (f32_mode(mode)) {
-        e = flow_dispatch<BwdK, float>(g.D, f.mu, nb, st, g, (float *)U, to_sub(f), gr, c_logj, gw, (const float *)log,
-                                       contrib, T);
+        e = stout ? flow_dispatch<BwdKS, float>(g.D, f.mu, nb, st, g, (float *)U, to_sub(f), gr, c_logj, gw,
+                                                (const float *)log, contrib, T, lg)
+                  : flow_dispatch<BwdK, float>(g.D, f.mu, nb, st, g, (float *)U, to_sub(f), gr, c_logj, gw,
+                                               (const float *)log, contrib, T, lg);
         if (e == hipSuccess) e = flow_dispatch<GatK, float>(g.D, f.mu, nl, st, g, gr, (const double *)contrib, f.par, T);
     } else {
-        e = flow_dispatch<BwdK, double>(g.D, f.mu, nb, st, g, (double *)U, to_sub(f), gr, c_logj, gw,
-                                        (const double *)log, contrib, T);
+        e = stout ? flow_dispatch<BwdKS, double>(g.D, f.mu, nb, st, g, (double *)U, to_sub(f), gr, c_logj, gw,
+                                                 (const double *)log, contrib, T, lg)
+                  : flow_dispatch<BwdK, double>(g.D, f.mu, nb, st, g, (double *)U, to_sub(f), gr, c_logj, gw,
+                                                (const double *)log, contrib, T, lg);
         if (e == hipSuccess)
             e = flow_dispatch<GatK, double>(g.D, f.mu, nl, st, g, gr, (const double *)contrib, f.par, T);
     }

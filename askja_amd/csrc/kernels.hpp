@@ -151,20 +151,37 @@ hipError_t launch_line_phases(const Geom &g, const void *U, int dir, int mode, i
                               double *wbuf, double *out, hipStream_t st);
 
 // ---- learned field transformation (flow.hip, N = 3) -------------------------
+enum { kFlowRankOne = 0, kFlowStout = 1 };
 struct FlowSub {
     int mu, par;     // updated links: direction mu on sites of parity par
     const float *w;  // device MLP block of the sub-layer
     int nf;          // features = planes not containing mu
     double rho_max, delta;
+    int kind;        // kFlowRankOne or kFlowStout
 };
 long flow_forward_partials(const Geom &g, int par);
-// one sub-layer in place; log (may be null) receives its input links
+// one sub-layer in place; log (may be null) receives its input links; for
+// the stout kind, st_wr (stout_wr_doubles) receives W = U S and rho of every
+// updated link, which launch_stout_logdet turns into log det J
 hipError_t launch_flow_forward(const Geom &g, const void *Uin, void *Uout, const FlowSub &f, int mode, double *part,
-                               void *log, hipStream_t st);
+                               void *log, double *st_wr, hipStream_t st);
 // backward of one sub-layer; restores its input links from log into U
-// (contrib: flow_contrib_doubles scratch for the neighbour contributions)
+// (contrib: flow_contrib_doubles scratch for the neighbour contributions;
+// lg: for the stout kind, the log-det gradient of launch_stout_logdet_grad)
 hipError_t launch_flow_backward(const Geom &g, void *U, const FlowSub &f, int mode, double *gr, double c_logj,
-                                double *gw, const void *log, double *contrib, hipStream_t st);
+                                double *gw, const void *log, double *contrib, const double *lg, hipStream_t st);
+// stout kind: per updated link W (18 doubles) and rho, structure of arrays
+size_t stout_wr_doubles(const Geom &g);
+size_t stout_aux_doubles(const Geom &g);  // J^{-1}, eigenbasis (85 doubles per link)
+size_t stout_lg_doubles(const Geom &g);   // the log-det gradient (19 per link)
+// log det J of every updated link of a stout sub-layer into per-block
+// partials (flow_forward_partials of its parity); with aux, also J^{-1} and
+// the eigenbasis of Q for launch_stout_logdet_grad
+hipError_t launch_stout_logdet(const Geom &g, const FlowSub &f, const double *st_wr, double *part, double *aux,
+                               hipStream_t st);
+// c_logj d log det J / dW (as G_W: d = 2 ReTr(G_W dW)) and / d rho per link
+hipError_t launch_stout_logdet_grad(const Geom &g, const FlowSub &f, const double *st_wr, const double *aux,
+                                    double c_logj, double *lg, hipStream_t st);
 size_t flow_contrib_doubles(const Geom &g);
 // link reals of one sub-layer's input log (sites of one parity, 64-link tiles)
 size_t flow_log_reals(const Geom &g);

@@ -267,6 +267,17 @@ int askja_tau_int(const double *x, long n, double S, double *tau, long *window);
  * layout: W1[32][nf], b1[32], w2[32], b2 with nf = 3 (D-1)(D-2)/2. */
 int askja_flow_define(askja_ctx *ctx, int nsub, const int *mu, const int *parity, const int *block, int nblocks,
                       double rho_max, double delta);
+/* The stout form of the map (no reference counterpart; SU(3), D = 3 or 4):
+ * the same masks, features, MLP and weight blocks, with sub-layer k updating
+ * its links as
+ *   U'_l = exp(i Q) U_l,  Q = -rho TAH(U_l S_l),  rho = rho_max tanh(MLP(f)),
+ * (S_l the frozen staple sum: a smearing step along the force direction) and
+ * log det J = sum over updated links of the log det of the 8 x 8 Jacobian
+ * 1 + g(ad_{iQ}) M in su(3) coordinates (g(z) = (1 - e^{-z}) / z, M the
+ * change of Q with the link), computed exactly per link.  Requires even
+ * extents and 0 < rho_max < 1/(2(D-1)) (then |g M| < 1: invertible). */
+int askja_flow_define_stout(askja_ctx *ctx, int nsub, const int *mu, const int *parity, const int *block, int nblocks,
+                            double rho_max);
 /* Floats per weight block. */
 int askja_flow_block_size(askja_ctx *ctx, int *bsize);
 /* All nblocks * bsize weights, from host memory or (on_device != 0) device memory. */

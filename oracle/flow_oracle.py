@@ -237,3 +237,99 @@ def energy_mag_batched(ext, U):
             P = U[:, j] @ U[up[j], k] @ _dag(U[up[k], j]) @ _dag(U[:, k])
             s += float(np.sum(1 - _tr(P).real / 3))
     return s
+
+
+# -- stout sub-layer ----------------------------------------------------------------
+# For every masked link l = (x, mu), with S, the features f and rho as above
+# (rho = rho_max tanh(MLP(f)), rho_max < 1/(2(D-1))):
+#   W  = U_l S,  Q = -rho TAH(W)  (Hermitian traceless),  U'_l = exp(i Q) U_l.
+# Jacobian in left-invariant coordinates x_a = 2 Tr(T^a X) (Tr T^a T^b = d/2):
+#   J = Ad_{exp(iQ)} (1 + g(A) M),  g(z) = (1 - e^{-z}) / z,
+#   A_ab = f_abc q_c (the matrix of ad_{iQ}; [T^a, T^b] = i f_abc T^c),
+#   M_ab = -2 rho ReTr(T^a T^b W)   (dQ for dU = i T^b U),
+# so log det J = log det(1 + g(A) M) (det Ad = 1).  Here g(A) is summed as a
+# power series of the 8 x 8 matrix A and the determinant taken by slogdet:
+# an independent route from the device's (eigenbasis of Q).
+
+def su3_basis():
+    """An orthonormal basis T^a of su(3): the Gell-Mann matrices / 2."""
+    l = np.zeros((8, 3, 3), complex)
+    l[0][0, 1] = l[0][1, 0] = 1
+    l[1][0, 1], l[1][1, 0] = -1j, 1j
+    l[2][0, 0], l[2][1, 1] = 1, -1
+    l[3][0, 2] = l[3][2, 0] = 1
+    l[4][0, 2], l[4][2, 0] = -1j, 1j
+    l[5][1, 2] = l[5][2, 1] = 1
+    l[6][1, 2], l[6][2, 1] = -1j, 1j
+    l[7] = np.diag([1, 1, -2]) / np.sqrt(3)
+    return l / 2
+
+
+def structure_constants(T):
+    """f_abc = -2i Tr([T^a, T^b] T^c), real and totally antisymmetric."""
+    comm = np.einsum("aij,bjk->abik", T, T) - np.einsum("bij,ajk->abik", T, T)
+    f = -2j * np.einsum("abij,cji->abc", comm, T)
+    assert np.max(np.abs(f.imag)) < 1e-14
+    return f.real
+
+
+def coords(X, T):
+    """x_a = 2 Re Tr(T^a X) of (batched) Hermitian traceless X."""
+    return 2 * np.einsum("aij,nji->na", T, X).real
+
+
+def stout_link_batched(W, rho, nterms=40):
+    """(exp(iQ), log det J) per link for W = U S, Q = -rho TAH(W)."""
+    T = su3_basis()
+    f = structure_constants(T)
+    Q = -rho[:, None, None] * _tah_b(W)
+    q = coords(Q, T)
+    A = np.einsum("abc,nc->nab", f, q)
+    TT = np.einsum("aij,bjk->abik", T, T)
+    M = -2 * rho[:, None, None] * np.einsum("abij,nji->nab", TT, W).real
+    gA = np.zeros_like(A)
+    P = np.broadcast_to(np.eye(8), A.shape).copy()
+    fact = 1.0
+    for k in range(nterms):
+        fact /= (k + 1)
+        gA += ((-1) ** k) * fact * P
+        P = P @ A
+    J = np.eye(8) + gA @ M
+    sign, ld = np.linalg.slogdet(J)
+    assert np.all(sign > 0)
+    w, v = np.linalg.eigh(Q)
+    E = (v * np.exp(1j * w)[:, None, :]) @ _dag(v)
+    return E, ld
+
+
+def stout_sub_layer_batched(up, dn, par_of, U, mu, par, w, nf, rho_max):
+    xs = np.nonzero(par_of == par)[0]
+    W1 = w[: HID * nf].reshape(HID, nf).astype(np.float64)
+    b1 = w[HID * nf: HID * nf + HID].astype(np.float64)
+    w2 = w[HID * nf + HID: HID * nf + 2 * HID].astype(np.float64)
+    b2 = float(w[HID * nf + 2 * HID])
+    S, C, feat = link_inputs_batched(up, dn, U, xs, mu)
+    Wl = U[xs, mu] @ S
+    rho = rho_max * np.tanh(b2 + np.tanh(feat @ W1.T + b1) @ w2)
+    E, ld = stout_link_batched(Wl, rho)
+    out = U.copy()
+    out[xs, mu] = E @ U[xs, mu]
+    return out, float(np.sum(ld))
+
+
+def stout_forward_batched(ext, V, subs, weights, bsize, nf, rho_max):
+    """The stout map for the whole lattice: (U = F(V), log J)."""
+    up, dn, par_of = neighbours(ext)
+    U = V.copy()
+    lj = 0.0
+    for mu, par, blk in subs:
+        U, l = stout_sub_layer_batched(up, dn, par_of, U, mu, par, weights[blk * bsize:(blk + 1) * bsize], nf,
+                                       rho_max)
+        lj += l
+    return U, lj
+
+
+def stout_phi(ext, V, subs, weights, bsize, nf, rho_max, beta):
+    """Phi = beta S_W(F(V)) - log J(V) of the stout map."""
+    U, lj = stout_forward_batched(ext, V, subs, weights, bsize, nf, rho_max)
+    return beta * energy_mag_batched(ext, U) - lj

@@ -133,3 +133,67 @@ def test_batched_restatement_matches_per_link(oracle):
             for d in range(o.dim):
                 assert up[d][x] == o.getplus(x, d) and dn[d][x] == o.getminus(x, d)
 
+
+
+def _rsu3(rng):
+    a = rng.standard_normal((3, 3)) + 1j * rng.standard_normal((3, 3))
+    q, r = np.linalg.qr(a)
+    q = q @ np.diag(np.diag(r) / np.abs(np.diag(r)))
+    return q / np.linalg.det(q) ** (1 / 3)
+
+
+def test_stout_logdet_vs_numerical_jacobian():
+    """The stout link map U -> exp(-i rho TAH(U S)) U: the oracle's log det
+    (1 + g(ad_iQ) M in su(3) coordinates, g by its power series) against the
+    8 x 8 Jacobian taken by central differences in left-invariant
+    coordinates (matrix log of U'(eps) U'(0)^+), for rho of both signs up to
+    the invertibility bound 1/6 and six unitary staples."""
+    from oracle import flow_oracle as fo
+
+    T = fo.su3_basis()
+    rng = np.random.default_rng(11)
+    for trial in range(6):
+        U = _rsu3(rng)
+        S = sum(_rsu3(rng) for _ in range(6))
+        rho = (0.165 if trial == 0 else 0.16 * rng.uniform(-1, 1))
+
+        def F(V):
+            E, ld = fo.stout_link_batched((V @ S)[None], np.array([rho]))
+            return E[0] @ V, ld[0]
+
+        U1, ld = F(U)
+        eps = 1e-5
+        J = np.zeros((8, 8))
+        for b in range(8):
+            Up, _ = F(scipy.linalg.expm(1j * eps * T[b]) @ U)
+            Um, _ = F(scipy.linalg.expm(-1j * eps * T[b]) @ U)
+            X = (scipy.linalg.logm(Up @ U1.conj().T) - scipy.linalg.logm(Um @ U1.conj().T)) / (2j * eps)
+            J[:, b] = fo.coords(X[None], T)[0]
+        sgn, lnum = np.linalg.slogdet(J)
+        assert sgn > 0 and abs(lnum - ld) < 1e-8, (trial, rho, lnum, ld)
+        # smearing direction: rho > 0 raises ReTr(U S)
+        assert (np.trace(U1 @ S).real > np.trace(U @ S).real) == (rho > 0)
+
+
+def test_stout_map_equivariance_and_identity(oracle):
+    """F(g V g^+) = g F(V) g^+ with the same log J (gauge equivariance), and
+    rho = 0 (zero weights) is the identity with log J = 0."""
+    from oracle import flow_oracle as fo
+
+    ext = [4, 4, 2, 2]
+    o = oracle.Oracle(3, ext)
+    rng = np.random.default_rng(5)
+    V = random_su(oracle, rng, (o.V, o.dim), 3)
+    subs = [(0, 0, 0), (1, 1, 0), (3, 0, 0)]
+    nf, bsize = 9, 32 * 9 + 65
+    w = (rng.standard_normal(bsize) * 0.5).astype(np.float32)
+    U, lj = fo.stout_forward_batched(ext, V, subs, w, bsize, nf, 0.15)
+    assert abs(lj) > 1e-3 and np.max(np.abs(U - V)) > 1e-3
+    g = random_su(oracle, rng, (o.V,), 3)
+    up, dn, _ = fo.neighbours(ext)
+    Vg = np.einsum("xij,xdjk,xdlk->xdil", g, V, np.conj(g[up.T]))
+    Ug, ljg = fo.stout_forward_batched(ext, Vg, subs, w, bsize, nf, 0.15)
+    assert np.max(np.abs(Ug - np.einsum("xij,xdjk,xdlk->xdil", g, U, np.conj(g[up.T])))) < 1e-12
+    assert abs(ljg - lj) < 1e-10
+    U0, lj0 = fo.stout_forward_batched(ext, V, subs, np.zeros(bsize, np.float32), bsize, nf, 0.15)
+    assert np.max(np.abs(U0 - V)) < 1e-14 and lj0 == 0.0
