@@ -80,7 +80,7 @@ struct askja_ctx {
     void *flow_save = nullptr;       // chain state at the start of a flow trajectory
     int flow_kind = kFlowRankOne;
     double *flow_st_wr = nullptr;    // stout: W and rho of every sub-layer's updated links
-    double *flow_st_aux = nullptr;   // stout: inverse Jacobian and eigenbasis (one sub-layer)
+    double *flow_st_aux = nullptr;   // stout: inverse Jacobian and eigenbasis of every sub-layer
     double *flow_st_lg = nullptr;    // stout: log-det gradient (one sub-layer)
 };
 
@@ -1109,6 +1109,11 @@ double *flow_wr_of(askja_ctx *c, int k)
     return c->flow_st_wr ? c->flow_st_wr + (size_t)k * stout_wr_doubles(c->geom) : nullptr;
 }
 
+double *flow_aux_of(askja_ctx *c, int k)
+{
+    return c->flow_st_aux ? c->flow_st_aux + (size_t)k * stout_aux_doubles(c->geom) : nullptr;
+}
+
 // U = F(V) into flow_out; with store, the input links of every sub-layer
 // go to flow_log.  Per sub-layer log J into flow_lj.
 int flow_forward(askja_ctx *c, bool store)
@@ -1124,8 +1129,11 @@ int flow_forward(askja_ctx *c, bool store)
         HIPCHK(c, launch_flow_forward(c->geom, c->flow_out, c->flow_out, f, c->mode, c->part,
                                       store ? flow_log_of(c, k) : nullptr, flow_wr_of(c, k), c->st));
         // stout: the per-link 8 x 8 log det from the W and rho just written
+        // (with store, the inverse Jacobians and eigenbases are kept for the
+        // backward)
         if (f.kind == kFlowStout)
-            HIPCHK(c, launch_stout_logdet(c->geom, f, flow_wr_of(c, k), c->part, nullptr, c->st));
+            HIPCHK(c, launch_stout_logdet(c->geom, f, flow_wr_of(c, k), c->part, store ? flow_aux_of(c, k) : nullptr,
+                                          c->st));
         HIPCHK(c, launch_finalize(c->part, flow_forward_partials(c->geom, f.par), 1, c->flow_lj + 2 * k, c->st));
     }
     return ASKJA_OK;
@@ -1162,12 +1170,9 @@ int flow_gradient(askja_ctx *c, double beta, bool wgrad, double *action)
     for (int k = c->flow_nsub - 1; k >= 0; k--) {
         const FlowSub &f = c->flow_subs[k];
         double *gw = wgrad ? c->flow_gw + (size_t)c->flow_block[k] * c->flow_bsize : nullptr;
-        if (f.kind == kFlowStout) {
-            // c_logj d log det J of this sub-layer's links (c_logj = -1)
-            HIPCHK(c, launch_stout_logdet(c->geom, f, flow_wr_of(c, k), c->part, c->flow_st_aux, c->st));
-            HIPCHK(c, launch_stout_logdet_grad(c->geom, f, flow_wr_of(c, k), c->flow_st_aux, -1.0, c->flow_st_lg,
+        if (f.kind == kFlowStout)  // c_logj d log det J of this sub-layer's links (c_logj = -1)
+            HIPCHK(c, launch_stout_logdet_grad(c->geom, f, flow_wr_of(c, k), flow_aux_of(c, k), -1.0, c->flow_st_lg,
                                                c->st));
-        }
         HIPCHK(c, launch_flow_backward(c->geom, c->flow_work, f, c->mode, c->flow_g, -1.0, gw, flow_log_of(c, k),
                                        c->flow_contrib, c->flow_st_lg, c->st));
     }
@@ -1239,7 +1244,7 @@ int flow_define(askja_ctx *c, int kind, int nsub, const int *mu, const int *pari
     HIPCHK(c, hipMalloc(&c->flow_save, link_bytes(c)));
     if (kind == kFlowStout) {
         HIPCHK(c, hipMalloc((void **)&c->flow_st_wr, (size_t)nsub * stout_wr_doubles(c->geom) * sizeof(double)));
-        HIPCHK(c, hipMalloc((void **)&c->flow_st_aux, stout_aux_doubles(c->geom) * sizeof(double)));
+        HIPCHK(c, hipMalloc((void **)&c->flow_st_aux, (size_t)nsub * stout_aux_doubles(c->geom) * sizeof(double)));
         HIPCHK(c, hipMalloc((void **)&c->flow_st_lg, stout_lg_doubles(c->geom) * sizeof(double)));
     }
     c->flow_subs.resize(nsub);

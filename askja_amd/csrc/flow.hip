@@ -259,6 +259,29 @@ __device__ inline double gm_im(int b, int p, int q)
     return I[b][p][q];
 }
 
+// the same matrices for a runtime generator index (uniform across a
+// wavefront: scalar loads): element (p, q) at [b][3 p + q]
+__constant__ double kGmRe[8][9] = {
+    {0, 0.5, 0, 0.5, 0, 0, 0, 0, 0},  {0, 0, 0, 0, 0, 0, 0, 0, 0},
+    {0.5, 0, 0, 0, -0.5, 0, 0, 0, 0}, {0, 0, 0.5, 0, 0, 0, 0.5, 0, 0},
+    {0, 0, 0, 0, 0, 0, 0, 0, 0},      {0, 0, 0, 0, 0, 0.5, 0, 0.5, 0},
+    {0, 0, 0, 0, 0, 0, 0, 0, 0},      {0.28867513459481287, 0, 0, 0, 0.28867513459481287, 0, 0, 0, -0.5773502691896258}};
+__constant__ double kGmIm[8][9] = {
+    {0, 0, 0, 0, 0, 0, 0, 0, 0},  {0, -0.5, 0, 0.5, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0, 0, 0, 0},
+    {0, 0, 0, 0, 0, 0, 0, 0, 0},  {0, 0, -0.5, 0, 0, 0, 0.5, 0, 0}, {0, 0, 0, 0, 0, 0, 0, 0, 0},
+    {0, 0, 0, 0, 0, -0.5, 0, 0.5, 0}, {0, 0, 0, 0, 0, 0, 0, 0, 0}};
+
+__device__ inline C3 gm_mat(int b)
+{
+    C3 T;
+#pragma unroll
+    for (int e = 0; e < 9; e++) {
+        T.r[e / 3][e % 3] = kGmRe[b][e];
+        T.i[e / 3][e % 3] = kGmIm[b][e];
+    }
+    return T;
+}
+
 // i T^b Y (b a compile-time constant after unrolling)
 __device__ inline C3 i_gm_mul(int b, const C3 &Y)
 {
@@ -1195,30 +1218,32 @@ __global__ __launch_bounds__(kLinks) void k_stout_logdet(Geom g, int par, const 
     block_reduce_store<1>(sum, cmp, part + 2 * (size_t)blockIdx.x);
 }
 
-// U_b = sum_a G_ba T^a from row b of the inverse (in aux)
+// U_b = sum_a G_ba T^a from row b of the inverse (in aux); b runtime
 __device__ inline C3 row_gen(const double *aux, int b, long t, long T)
 {
     C3 U;
     zero(U);
-#pragma unroll
+#pragma unroll 1
     for (int a = 0; a < 8; a++) {
         const double gba = aux[(size_t)(b * 8 + a) * T + t];
 #pragma unroll
-        for (int p = 0; p < 3; p++)
-#pragma unroll
-            for (int q = 0; q < 3; q++) {
-                U.r[p][q] += gba * gm_re(a, p, q);
-                U.i[p][q] += gba * gm_im(a, p, q);
-            }
+        for (int e = 0; e < 9; e++) {
+            U.r[e / 3][e % 3] += gba * kGmRe[a][e];
+            U.i[e / 3][e % 3] += gba * kGmIm[a][e];
+        }
     }
     return U;
 }
 
 // lg = c (i rho Z - rho Lam) (18 doubles: the G_W of c log det J) and
-// c (-2 Im Tr(Z W) - 2 Re Tr(Lam W)) (d / d rho), per updated link
+// c (-2 Im Tr(Z W) - 2 Re Tr(Lam W)) (d / d rho), per updated link.  The
+// generator loops run over a runtime index (matrices from constant memory)
+// to keep the register footprint small.
 __global__ __launch_bounds__(kLinks) void k_stout_logdet_grad(Geom g, int par, const double *wr, const double *aux,
                                                               long T, double c, double *lg)
 {
+    __shared__ double sD[2][27][kLinks];  // divided differences D_abl, one column per thread
+    const int lane = (int)threadIdx.x;
     const long t = (long)blockIdx.x * kLinks + threadIdx.x;
     long s;
     if (!site_of(g, par, t, s)) return;
@@ -1239,39 +1264,6 @@ __global__ __launch_bounds__(kLinks) void k_stout_logdet_grad(Geom g, int par, c
     for (int p = 0; p < 3; p++)
 #pragma unroll
         for (int q = 0; q < 3; q++) g_of(lam[p] - lam[q], gr[p][q], gi[p][q]);
-    // through M: Lam^ = sum_b g(ad_{-iQ})(U_b) T^b, g(-ix) = conj(g(ix))
-    C3 Lh;
-    zero(Lh);
-#pragma unroll 1
-    for (int bb = 0; bb < 8; bb++) {
-#pragma unroll
-        for (int b = 0; b < 8; b++) {
-            if (b != bb) continue;
-            const C3 U = row_gen(aux, b, t, T);
-            C3 Y;
-#pragma unroll
-            for (int p = 0; p < 3; p++)
-#pragma unroll
-                for (int q = 0; q < 3; q++) {
-                    Y.r[p][q] = gr[p][q] * U.r[p][q] + gi[p][q] * U.i[p][q];
-                    Y.i[p][q] = gr[p][q] * U.i[p][q] - gi[p][q] * U.r[p][q];
-                }
-#pragma unroll
-            for (int p = 0; p < 3; p++)
-#pragma unroll
-                for (int q = 0; q < 3; q++)
-#pragma unroll
-                    for (int r = 0; r < 3; r++) {
-                        const double tr = gm_re(b, r, q), ti = gm_im(b, r, q);
-                        Lh.r[p][q] += Y.r[p][r] * tr - Y.i[p][r] * ti;
-                        Lh.i[p][q] += Y.r[p][r] * ti + Y.i[p][r] * tr;
-                    }
-        }
-    }
-    // through g: Z^ with the divided differences D_abl (symmetric in a, b),
-    // kept in LDS (one column per thread)
-    __shared__ double sD[2][27][kLinks];
-    const int lane = (int)threadIdx.x;
 #pragma unroll
     for (int a = 0; a < 3; a++)
 #pragma unroll
@@ -1283,38 +1275,48 @@ __global__ __launch_bounds__(kLinks) void k_stout_logdet_grad(Geom g, int par, c
                 sD[0][(a * 3 + b) * 3 + l][lane] = sD[0][(b * 3 + a) * 3 + l][lane] = dr;
                 sD[1][(a * 3 + b) * 3 + l][lane] = sD[1][(b * 3 + a) * 3 + l][lane] = di;
             }
-    C3 Zh;
+    // Lam^ = sum_b g(ad_{-iQ})(U_b) T^b (g(-ix) = conj(g(ix))) and Z^, one
+    // generator at a time
+    C3 Lh, Zh;
+    zero(Lh);
     zero(Zh);
 #pragma unroll 1
-    for (int bb = 0; bb < 8; bb++) {
+    for (int b = 0; b < 8; b++) {
+        const C3 U = row_gen(aux, b, t, T);
+        const C3 Tb = gm_mat(b);
+        C3 Y;
 #pragma unroll
-        for (int b = 0; b < 8; b++) {
-            if (b != bb) continue;
-            const C3 U = row_gen(aux, b, t, T);
-            const C3 M = cscale(tah(i_gm_mul(b, Wh)), -rho, 0.0);
+        for (int p = 0; p < 3; p++)
 #pragma unroll
-            for (int a = 0; a < 3; a++)
+            for (int q = 0; q < 3; q++) {
+                Y.r[p][q] = gr[p][q] * U.r[p][q] + gi[p][q] * U.i[p][q];
+                Y.i[p][q] = gr[p][q] * U.i[p][q] - gi[p][q] * U.r[p][q];
+            }
+        addto(Lh, mm<false, false>(Y, Tb));
+        // M_b = -rho TAH(i T^b W^)
+        const C3 M = cscale(tah(cscale(mm<false, false>(Tb, Wh), 0.0, 1.0)), -rho, 0.0);
 #pragma unroll
-                for (int q = 0; q < 3; q++) {
-                    double ar = 0.0, ai = 0.0;
+        for (int a = 0; a < 3; a++)
 #pragma unroll
-                    for (int l = 0; l < 3; l++) {
-                        const double dr = sD[0][(a * 3 + q) * 3 + l][lane], di = sD[1][(a * 3 + q) * 3 + l][lane];
-                        // M_al U_lq
-                        const double p1r = M.r[a][l] * U.r[l][q] - M.i[a][l] * U.i[l][q];
-                        const double p1i = M.r[a][l] * U.i[l][q] + M.i[a][l] * U.r[l][q];
-                        // U_al M_lq
-                        const double p2r = U.r[a][l] * M.r[l][q] - U.i[a][l] * M.i[l][q];
-                        const double p2i = U.r[a][l] * M.i[l][q] + U.i[a][l] * M.r[l][q];
-                        // D p1 - conj(D) p2
-                        ar += dr * p1r - di * p1i - (dr * p2r + di * p2i);
-                        ai += dr * p1i + di * p1r - (dr * p2i - di * p2r);
-                    }
-                    // Z^ += i (ar + i ai)
-                    Zh.r[a][q] -= ai;
-                    Zh.i[a][q] += ar;
+            for (int q = 0; q < 3; q++) {
+                double ar = 0.0, ai = 0.0;
+#pragma unroll
+                for (int l = 0; l < 3; l++) {
+                    const double dr = sD[0][(a * 3 + q) * 3 + l][lane], di = sD[1][(a * 3 + q) * 3 + l][lane];
+                    // M_al U_lq
+                    const double p1r = M.r[a][l] * U.r[l][q] - M.i[a][l] * U.i[l][q];
+                    const double p1i = M.r[a][l] * U.i[l][q] + M.i[a][l] * U.r[l][q];
+                    // U_al M_lq
+                    const double p2r = U.r[a][l] * M.r[l][q] - U.i[a][l] * M.i[l][q];
+                    const double p2i = U.r[a][l] * M.i[l][q] + U.i[a][l] * M.r[l][q];
+                    // D p1 - conj(D) p2
+                    ar += dr * p1r - di * p1i - (dr * p2r + di * p2i);
+                    ai += dr * p1i + di * p1r - (dr * p2i - di * p2r);
                 }
-        }
+                // Z^ += i (ar + i ai)
+                Zh.r[a][q] -= ai;
+                Zh.i[a][q] += ar;
+            }
     }
     const C3 Lo = mm<false, true>(mm<false, false>(V, Lh), V);
     const C3 Zo = mm<false, true>(mm<false, false>(V, Zh), V);

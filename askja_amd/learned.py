@@ -183,6 +183,10 @@ def main():
     ap.add_argument("--ndrift", type=int, default=50)
     ap.add_argument("--level", type=int, default=8)
     ap.add_argument("--seed", type=int, default=11)
+    ap.add_argument("--kind", choices=["rank-one", "stout"], default="stout",
+                    help="sub-layer form: stout smearing step (exact 8x8 log det) or rank-one rotation")
+    ap.add_argument("--rho-max", type=float, default=None, help="default 0.15 (stout), 0.3 (rank-one)")
+    ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--save-weights", help="write the trained weights (and the recipe) to this JSON file")
     ap.add_argument("--no-chains", action="store_true", help="train (and save) only")
     args = ap.parse_args()
@@ -203,18 +207,19 @@ def main():
         configs.append(lat.links())
     log(f"[learned] {len(configs)} training configurations at beta {args.beta_train}")
     mu, par, blk = sweep_masks(4, args.layers)
-    bsize = lat.flow_define(mu, par, blk, args.layers, 0.3, 1e-3)
+    rho_max = args.rho_max if args.rho_max is not None else (0.15 if args.kind == "stout" else 0.3)
+    bsize = lat.flow_define(mu, par, blk, args.layers, rho_max, 1e-3, kind=args.kind)
     nf = (bsize - 65) // 32  # features: plaquette and two rectangles per plane
     w0 = init_weights(args.layers, bsize, nf, seed=args.seed)
     t0 = time.perf_counter()
-    w, losses = train(lat, configs, args.beta, w0, epochs=args.epochs, log=log)
+    w, losses = train(lat, configs, args.beta, w0, epochs=args.epochs, lr=args.lr, log=log)
     t_train = time.perf_counter() - t0
     if args.save_weights:
         save_weights(args.save_weights, w, {
             "nc": 3, "extents": ext, "beta": args.beta, "beta_train": args.beta_train, "layers": args.layers,
-            "sublayers": len(mu), "ntrain": args.ntrain, "epochs": args.epochs, "dt": args.dt, "ndrift": args.ndrift,
-            "level": args.level, "seed": args.seed, "rho_max": 0.3, "delta": 1e-3, "bsize": bsize,
-            "loss": losses, "train_seconds": t_train})
+            "sublayers": len(mu), "ntrain": args.ntrain, "epochs": args.epochs, "lr": args.lr, "dt": args.dt,
+            "ndrift": args.ndrift, "level": args.level, "seed": args.seed, "kind": args.kind, "rho_max": rho_max,
+            "delta": 1e-3, "bsize": bsize, "loss": losses, "train_seconds": t_train})
         log(f"[learned] weights written to {args.save_weights}")
     if args.no_chains:
         return
@@ -243,7 +248,7 @@ def main():
             for obs in ("abs_polyakov", "plaquette"):
                 (m, e), (m0, e0) = v[obs], res["plain"][obs]
                 v[obs + "_z_vs_plain"] = (m - m0) / math.hypot(e, e0)
-    out = {"workload": f"SU(3) D=4 {args.extent}^4 beta={args.beta}, FTHMC {args.layers} sweep layer(s) "
+    out = {"workload": f"SU(3) D=4 {args.extent}^4 beta={args.beta}, FTHMC {args.layers} {args.kind} sweep layer(s) "
                        f"({len(mu)} masked sub-layers), trained on {args.ntrain} plain-HMC configs at "
                        f"beta={args.beta_train}, dt={args.dt}, {args.ndrift} MD steps",
            "train_seconds": t_train, "loss": losses, "results": res,

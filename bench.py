@@ -427,6 +427,8 @@ def c4_chain(lat, kind, first, args, log):
     ntimed = args.steps if kind == "learned" else max(args.steps, MIN_SERIES["plain"])
     poly, plaq, acc, dh = [], [], [], []
 
+    last = [time.perf_counter()]
+
     def run(n, c0):
         for k in range(n):
             r = step(c0 + k)
@@ -434,6 +436,10 @@ def c4_chain(lat, kind, first, args, log):
             dh.append(r.dh)
             plaq.append(1.0 - r.action / lat.n_plaquettes())
             poly.append(abs(measure()))
+            now = time.perf_counter()
+            if now - last[0] > 30:
+                log(f"[bench] {kind}: {len(poly)} trajectories")
+                last[0] = now
 
     lat.sync()
     t0 = time.perf_counter()
@@ -441,7 +447,7 @@ def c4_chain(lat, kind, first, args, log):
     lat.sync()
     T = time.perf_counter() - t0
     log(f"[bench] {kind}: {ntimed} timed trajectories in {T:.1f} s")
-    want = max(ntimed, MIN_SERIES[kind])
+    want = max(ntimed, args.learned_min_series if kind == "learned" else MIN_SERIES[kind])
     while True:
         if len(poly) < want:
             run(want - len(poly), first + len(poly))
@@ -467,6 +473,30 @@ def flow_kernel_times(lat, reps=3):
         lat.flow_apply()
     ms_fwd = lat.timer_stop() / reps
     return ms_grad, ms_fwd
+
+
+def flow_study(reps=5):
+    """HIP-event time of one flow gradient (forward and backward of every
+    sub-layer: one FTHMC MD step's force) and of one forward, at 16^4 for
+    the rank-one and the stout sub-layers, one and four sweep layers."""
+    import askja_amd as A
+    from askja_amd import learned as L
+
+    lat = A.Lattice(C4_NC, C4_EXT, A.FP64)
+    lat.haar(SEED, 3)
+    for t in range(20):
+        lat.hmc_trajectory(C4_BETA, C4_DT, C4_NDRIFT, C4_LEVEL, SEED, t, reunit=True)
+    rng = np.random.default_rng(1)
+    for kind, rho in (("rank-one", 0.3), ("stout", 0.15)):
+        for layers in (1, 4):
+            mu, par, blk = L.sweep_masks(4, layers)
+            bsize = lat.flow_define(mu, par, blk, layers, rho, 1e-3, kind=kind)
+            lat.flow_set_weights((rng.standard_normal(layers * bsize) * 0.3).astype(np.float32))
+            ms_grad, ms_fwd = flow_kernel_times(lat, reps)
+            print(json.dumps({"study": "flow", "kind": kind, "layers": layers, "sublayers": len(mu),
+                              "ms_gradient": ms_grad, "ms_forward": ms_fwd,
+                              "us_gradient_per_sublayer": 1e3 * ms_grad / len(mu)}), flush=True)
+    lat.close()
 
 
 def cpu_baseline_c4(steps_sample):
@@ -502,6 +532,11 @@ def main_c4(args):
 
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     w, recipe = L.load_weights(args.weights)
+    if args.layers is not None:  # the first --layers sweep layers of the trained map
+        if not 1 <= args.layers <= recipe["layers"]:
+            raise SystemExit(f"--layers must be in 1..{recipe['layers']}")
+        w = w[: args.layers * recipe["bsize"]]
+        recipe = dict(recipe, layers=args.layers, sublayers=args.layers * 8)
     lat = A.Lattice(C4_NC, C4_EXT, A.FP64)
     lat.haar(SEED, 3)
     for t in range(C4_THERM):
@@ -512,7 +547,8 @@ def main_c4(args):
     roof = kick_drift_roofline(lat, C4_NC, C4_LEVEL, C4_DT, args.roofline_launches)
     lat.set_links(start)
     mu, par, blk = L.sweep_masks(4, recipe["layers"])
-    bsize = lat.flow_define(mu, par, blk, recipe["layers"], recipe["rho_max"], recipe["delta"])
+    kind = recipe.get("kind", "rank-one")
+    bsize = lat.flow_define(mu, par, blk, recipe["layers"], recipe["rho_max"], recipe["delta"], kind=kind)
     assert bsize == recipe["bsize"] and w.size == recipe["layers"] * bsize, "weights do not fit this map"
     lat.flow_set_weights(w)
     learned = c4_chain(lat, "learned", 10**7, args, log)
@@ -557,7 +593,7 @@ def main_c4(args):
         "dtype": "f64",
         "data": f"synthetic (Haar-random SU(3) start, seed {SEED}, {C4_THERM} untimed plain trajectories)",
         "config": {"workload": "learned-proposal (FTHMC) vs plain HMC, SU(3) D=4 16^4 beta=6.0 DT=0.02 "
-                               f"{C4_NDRIFT} MD steps LEVEL={C4_LEVEL}, map: {recipe['layers']} sweep layers "
+                               f"{C4_NDRIFT} MD steps LEVEL={C4_LEVEL}, map: {recipe['layers']} {kind} sweep layers "
                                f"({recipe['sublayers']} sub-layers), trained on {recipe['ntrain']} plain configs "
                                f"at beta {recipe['beta_train']}",
                    "nc": C4_NC, "extents": C4_EXT, "beta": C4_BETA, "dt": C4_DT, "md_steps_per_traj": C4_NDRIFT,
@@ -598,6 +634,10 @@ def main():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--weights", default=C4_WEIGHTS)
+    ap.add_argument("--layers", type=int, default=None,
+                    help="c4: use only the first LAYERS sweep layers of the trained map (default: all)")
+    ap.add_argument("--learned-min-series", type=int, default=MIN_SERIES["learned"],
+                    help="c4: shortest learned tau series (it is extended while shorter than 100 tau)")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--md-steps", type=int, default=NDRIFT, help="c2: MD steps per trajectory (length = steps x DT)")
     ap.add_argument("--cpu-steps", type=int, default=20)
@@ -608,7 +648,7 @@ def main():
     ap.add_argument("--copy-probe", action="store_true", help="only run a 1 GiB device copy (counter calibration)")
     ap.add_argument("--kernel-only", choices=["c2", "c4"],
                     help="only time --roofline-launches fused kick+drift launches of that lattice (for rocprofv3)")
-    ap.add_argument("--study", choices=["precision", "c3scan", "c5exact", "kernels", "example"],
+    ap.add_argument("--study", choices=["precision", "c3scan", "c5exact", "kernels", "example", "flow"],
                     help="print a precision, beta-scan, exact-2D, per-config kernel or example-params study instead")
     ap.add_argument("--study-configs", default="C2,C3,C5")
     ap.add_argument("--study-therm", type=int, default=20, help="thermalising trajectories (c5exact)")
@@ -641,6 +681,9 @@ def main():
         return
     if args.study == "example":
         example_study()
+        return
+    if args.study == "flow":
+        flow_study()
         return
     if args.study == "c5exact":
         c5_exact(max(args.steps, 2), args.study_therm, tuple(args.study_precisions.split(",")))

@@ -161,7 +161,8 @@ print(json.dumps({"err": err, "first": losses[0], "last": losses[-1]}))
     assert out["last"] < out["first"]
 
 
-def test_trained_map_lowers_heldout_objective():
+@pytest.mark.parametrize("kind,rho", [("rank-one", 0.3), ("stout", 0.15)])
+def test_trained_map_lowers_heldout_objective(kind, rho):
     """What the trained network gains over the method without it.  Without
     the network the map is the identity (w2 = b2 = 0: rho = 0), and the
     objective E_q[beta S_W(F(V)) - log J(V)] is then beta E_q[S_W(V)]: plain
@@ -189,7 +190,7 @@ for k in range(12):
     L.plain_chain(lat, 5.7, 0.02, 50, 8, 5, 1000 + 3 * k, 3, measure=False)
     configs.append(lat.links())
 mu, par, blk = L.sweep_masks(4, 1)
-bsize = lat.flow_define(mu, par, blk, 1, 0.3, 1e-3)
+bsize = lat.flow_define(mu, par, blk, 1, %r, 1e-3, kind=%r)
 w0 = L.init_weights(1, bsize, 9, seed=1)
 def objective(w, cs):
     lat.flow_set_weights(w)
@@ -202,12 +203,12 @@ def objective(w, cs):
 wt, losses = L.train(lat, configs[:8], 6.0, w0, epochs=10, lr=2e-3)
 u, t = objective(w0, configs[8:]), objective(wt, configs[8:])
 print(json.dumps({"untrained": u.tolist(), "gain": (u - t).tolist(), "train": [losses[0], losses[-1]]}))
-""" % (str(__import__("conftest").ROOT),)
+""" % (str(__import__("conftest").ROOT), rho, kind)
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stderr[-2000:]
     out = json.loads(r.stdout.strip().splitlines()[-1])
     gain = np.asarray(out["gain"])
-    print(f"held-out objective gain of the trained map: {gain.mean():.3f} nats per configuration "
+    print(f"held-out objective gain of the trained {kind} map: {gain.mean():.3f} nats per configuration "
           f"(per configuration {np.round(gain, 3).tolist()}; untrained objective ~{np.mean(out['untrained']):.0f})")
     assert out["train"][-1] < out["train"][0]
     assert gain.mean() > 0.0
