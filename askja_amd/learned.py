@@ -110,6 +110,15 @@ def plain_chain(lat, beta, dt, ndrift, level, seed, first, n, measure=True, plaq
     return np.array(series), acc / max(n, 1), time.perf_counter() - t0
 
 
+def flow_relax(lat, beta, level, seed, first, n=20, dt=0.005, ndrift=200):
+    """FTHMC trajectories of a small step that bring V, started from a
+    configuration thermalised for plain HMC (atypical for the learned chain's
+    density of V: its first dt = 0.02 trajectories are all rejected), to
+    where the chain's own step size is accepted."""
+    for k in range(n):
+        lat.flow_hmc_trajectory(beta, dt, ndrift, level, seed, first + k, reunit=True)
+
+
 def flow_chain(lat, beta, dt, ndrift, level, seed, first, n, log=None, plaq=None):
     """n FTHMC trajectories; |P| (and the plaquette, into `plaq`) measured on
     the physical configuration U = F(V)."""
@@ -236,6 +245,7 @@ def main():
     for name, weights in (("flow_untrained", w0), ("flow_trained", w)):
         lat.set_links(start)
         lat.flow_set_weights(weights)
+        flow_relax(lat, args.beta, args.level, args.seed, 5 * 10**6)
         log(f"[learned] {name} chain")
         fq = []
         fl, acc_f, t_f = flow_chain(lat, args.beta, args.dt, args.ndrift, args.level, args.seed, 4 * 10**6, args.ntraj,

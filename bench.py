@@ -361,6 +361,12 @@ def run_chain(lat, args, tag):
 # ------------------------------------------------------------------ C4 line
 C4_NC, C4_EXT, C4_BETA, C4_DT, C4_NDRIFT, C4_LEVEL = 3, [16, 16, 16, 16], 6.0, 0.02, 50, 8
 C4_THERM = 300            # plain trajectories from the Haar start, untimed
+# The learned chain then relaxes in V (untimed): a configuration thermalised
+# for the plain chain is atypical for the learned chain's target density of
+# V, and from it every dt = 0.02 trajectory has dH of about +7.7 (measured:
+# no trajectory accepted); 20 trajectories of dt = 0.005 x 200 steps (90 %
+# accepted) bring it to where dt = 0.02 is accepted 70 % of the time.
+C4_RELAX = (20, 0.005, 200)
 C4_WEIGHTS = os.path.join(ROOT, "askja_amd", "data", "flow_c4_weights.json")
 TAU_SPAN = 100            # a tau estimate needs a series of >= 100 tau
 MIN_SERIES = {"plain": 3000, "learned": 300}
@@ -418,6 +424,12 @@ def c4_chain(lat, kind, first, args, log):
         if kind == "plain" else \
         (lambda c: lat.flow_hmc_trajectory(C4_BETA, C4_DT, C4_NDRIFT, C4_LEVEL, SEED, c, reunit=True))
     measure = lat.polyakov if kind == "plain" else lat.flow_polyakov
+    if kind == "learned":
+        n, dt, nd = C4_RELAX
+        for k in range(n):
+            lat.flow_hmc_trajectory(C4_BETA, dt, nd, C4_LEVEL, SEED, first + k, reunit=True)
+        first += n
+        log(f"[bench] learned: relaxed ({n} trajectories of {nd} x {dt})")
     for k in range(args.warmup):
         step(first + k)
     first += args.warmup
@@ -591,7 +603,9 @@ def main_c4(args):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": f"synthetic (Haar-random SU(3) start, seed {SEED}, {C4_THERM} untimed plain trajectories)",
+        "data": f"synthetic (Haar-random SU(3) start, seed {SEED}, {C4_THERM} untimed plain trajectories; "
+                f"the learned chain then relaxes in V for {C4_RELAX[0]} untimed trajectories of "
+                f"{C4_RELAX[2]} x {C4_RELAX[1]})",
         "config": {"workload": "learned-proposal (FTHMC) vs plain HMC, SU(3) D=4 16^4 beta=6.0 DT=0.02 "
                                f"{C4_NDRIFT} MD steps LEVEL={C4_LEVEL}, map: {recipe['layers']} {kind} sweep layers "
                                f"({recipe['sublayers']} sub-layers), trained on {recipe['ntrain']} plain configs "
