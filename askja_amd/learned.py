@@ -198,6 +198,10 @@ def main():
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--save-weights", help="write the trained weights (and the recipe) to this JSON file")
     ap.add_argument("--no-chains", action="store_true", help="train (and save) only")
+    ap.add_argument("--weights", help="load trained weights (save_weights format) instead of training")
+    ap.add_argument("--chains", default="plain,flow_untrained,flow_trained",
+                    help="comma-separated chains to run: plain, flow_untrained, flow_trained")
+    ap.add_argument("--series-out", help="write each chain's |P| and plaquette series to this .npz file")
     args = ap.parse_args()
     # torch first: the library then binds to torch's HIP runtime, so both
     # share one runtime in this process
@@ -211,19 +215,28 @@ def main():
     lat.haar(args.seed, 3)
     plain_chain(lat, args.beta_train, args.dt, args.ndrift, args.level, args.seed, 0, args.therm, measure=False)
     configs = []
-    for k in range(args.ntrain):
-        plain_chain(lat, args.beta_train, args.dt, args.ndrift, args.level, args.seed, 10**6 + 2 * k, 2, measure=False)
+    if args.weights:
+        w, recipe = load_weights(args.weights)
+        args.kind, args.layers, args.rho_max = recipe["kind"], recipe["layers"], recipe["rho_max"]
+        losses, t_train = recipe["loss"], recipe["train_seconds"]
+        plain_chain(lat, args.beta_train, args.dt, args.ndrift, args.level, args.seed, 10**6, 2, measure=False)
         configs.append(lat.links())
-    log(f"[learned] {len(configs)} training configurations at beta {args.beta_train}")
+    else:
+        for k in range(args.ntrain):
+            plain_chain(lat, args.beta_train, args.dt, args.ndrift, args.level, args.seed, 10**6 + 2 * k, 2,
+                        measure=False)
+            configs.append(lat.links())
+        log(f"[learned] {len(configs)} training configurations at beta {args.beta_train}")
     mu, par, blk = sweep_masks(4, args.layers)
     rho_max = args.rho_max if args.rho_max is not None else (0.15 if args.kind == "stout" else 0.3)
     bsize = lat.flow_define(mu, par, blk, args.layers, rho_max, 1e-3, kind=args.kind)
     nf = (bsize - 65) // 32  # features: plaquette and two rectangles per plane
     w0 = init_weights(args.layers, bsize, nf, seed=args.seed)
-    t0 = time.perf_counter()
-    w, losses = train(lat, configs, args.beta, w0, epochs=args.epochs, lr=args.lr, log=log)
-    t_train = time.perf_counter() - t0
-    if args.save_weights:
+    if not args.weights:
+        t0 = time.perf_counter()
+        w, losses = train(lat, configs, args.beta, w0, epochs=args.epochs, lr=args.lr, log=log)
+        t_train = time.perf_counter() - t0
+    if args.save_weights and not args.weights:
         save_weights(args.save_weights, w, {
             "nc": 3, "extents": ext, "beta": args.beta, "beta_train": args.beta_train, "layers": args.layers,
             "sublayers": len(mu), "ntrain": args.ntrain, "epochs": args.epochs, "lr": args.lr, "dt": args.dt,
@@ -236,13 +249,18 @@ def main():
     lat.set_links(configs[-1])
     plain_chain(lat, args.beta, args.dt, args.ndrift, args.level, args.seed, 2 * 10**6, args.therm, measure=False)
     start = lat.links()
-    res = {}
-    pq = []
-    pl, acc_p, t_p = plain_chain(lat, args.beta, args.dt, args.ndrift, args.level, args.seed, 3 * 10**6, args.ntraj,
-                                 plaq=pq)
-    res["plain"] = {"traj_per_s": args.ntraj / t_p, "acceptance": acc_p, "tau": tau_int(pl),
-                    "abs_polyakov": mean_err(pl), "plaquette": mean_err(pq)}
+    res, series = {}, {}
+    chains = args.chains.split(",")
+    if "plain" in chains:
+        pq = []
+        pl, acc_p, t_p = plain_chain(lat, args.beta, args.dt, args.ndrift, args.level, args.seed, 3 * 10**6,
+                                     args.ntraj, plaq=pq)
+        res["plain"] = {"traj_per_s": args.ntraj / t_p, "acceptance": acc_p, "tau": tau_int(pl),
+                        "abs_polyakov": mean_err(pl), "plaquette": mean_err(pq), "ntraj": args.ntraj}
+        series["plain_abs_polyakov"], series["plain_plaquette"] = pl, np.asarray(pq)
     for name, weights in (("flow_untrained", w0), ("flow_trained", w)):
+        if name not in chains:
+            continue
         lat.set_links(start)
         lat.flow_set_weights(weights)
         flow_relax(lat, args.beta, args.level, args.seed, 5 * 10**6)
@@ -251,20 +269,26 @@ def main():
         fl, acc_f, t_f = flow_chain(lat, args.beta, args.dt, args.ndrift, args.level, args.seed, 4 * 10**6, args.ntraj,
                                     log=log, plaq=fq)
         res[name] = {"traj_per_s": args.ntraj / t_f, "acceptance": acc_f, "tau": tau_int(fl),
-                     "abs_polyakov": mean_err(fl), "plaquette": mean_err(fq)}
+                     "abs_polyakov": mean_err(fl), "plaquette": mean_err(fq), "ntraj": args.ntraj}
+        series[name + "_abs_polyakov"], series[name + "_plaquette"] = fl, np.asarray(fq)
+    if args.series_out:
+        np.savez(args.series_out, **series)
     for k, v in res.items():
-        v["indep_configs_per_s"] = v["traj_per_s"] / v["tau"]
-        if k != "plain":  # agreement with the plain chain, in combined standard errors
+        v["tau_err"] = v["tau"] * math.sqrt(2.0 * (2 * A.tau_int(series[k + "_abs_polyakov"], 1.5)[1] + 1) / v["ntraj"])
+        v["indep_configs_per_s"] = v["traj_per_s"] / max(v["tau"], 1.0)
+        if k != "plain" and "plain" in res:  # agreement with the plain chain, in combined standard errors
             for obs in ("abs_polyakov", "plaquette"):
                 (m, e), (m0, e0) = v[obs], res["plain"][obs]
                 v[obs + "_z_vs_plain"] = (m - m0) / math.hypot(e, e0)
     out = {"workload": f"SU(3) D=4 {args.extent}^4 beta={args.beta}, FTHMC {args.layers} {args.kind} sweep layer(s) "
                        f"({len(mu)} masked sub-layers), trained on {args.ntrain} plain-HMC configs at "
                        f"beta={args.beta_train}, dt={args.dt}, {args.ndrift} MD steps",
-           "train_seconds": t_train, "loss": losses, "results": res,
-           "gain_learned_vs_plain": res["flow_trained"]["indep_configs_per_s"] / res["plain"]["indep_configs_per_s"],
-           "gain_trained_vs_untrained": res["flow_trained"]["indep_configs_per_s"]
-           / res["flow_untrained"]["indep_configs_per_s"]}
+           "train_seconds": t_train, "loss": losses, "results": res}
+    if "flow_trained" in res and "plain" in res:
+        out["gain_learned_vs_plain"] = res["flow_trained"]["indep_configs_per_s"] / res["plain"]["indep_configs_per_s"]
+    if "flow_trained" in res and "flow_untrained" in res:
+        out["gain_trained_vs_untrained"] = (res["flow_trained"]["indep_configs_per_s"]
+                                            / res["flow_untrained"]["indep_configs_per_s"])
     print(json.dumps(out))
 
 
