@@ -181,7 +181,14 @@ int main(int argc, char *argv[])
         }
     }
     Params P;
-    if (!load_parameters(paramsfile, P)) {
+    const bool params_ok = load_parameters(paramsfile, P);
+    if (params_ok && precision == ASKJA_BF16X3 && !P.RUNCONTINUE && P.NC > 8)
+        fprintf(stderr,
+                "askja: warning: bf16x3 from the cold start: next to the ordered start the force is small and "
+                "BF16x3's error in the staple sum is not, so most trajectories are rejected until the chain is "
+                "disordered; thermalise with --precision fp32 or fp64 first, then continue (RUNCONTINUE 1) in "
+                "bf16x3\n");
+    if (!params_ok) {
         printf("===>Info: Statistics: acc=%i (%i) rec=%i, rate=%f\n", 0, 0, 0, 0.0 / 0.0);
         printf("===>Took %i configurations\n", 0);
         printf("Finished with errors\n");

@@ -652,7 +652,8 @@ def main():
                     help="c4: use only the first LAYERS sweep layers of the trained map (default: all)")
     ap.add_argument("--learned-min-series", type=int, default=MIN_SERIES["learned"],
                     help="c4: shortest learned tau series (it is extended while shorter than 100 tau)")
-    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "fp32", "bf16x3"], default="fp64",
+                    help="c2 chain: fp64 or fp32; --kernel-only: any")
     ap.add_argument("--md-steps", type=int, default=NDRIFT, help="c2: MD steps per trajectory (length = steps x DT)")
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
@@ -660,7 +661,7 @@ def main():
                     help="c2: also run the chain in the FP32 mode (another --steps trajectories)")
     ap.add_argument("--roofline-launches", type=int, default=100)
     ap.add_argument("--copy-probe", action="store_true", help="only run a 1 GiB device copy (counter calibration)")
-    ap.add_argument("--kernel-only", choices=["c2", "c4"],
+    ap.add_argument("--kernel-only", choices=["c2", "c4", "c3", "c5"],
                     help="only time --roofline-launches fused kick+drift launches of that lattice (for rocprofv3)")
     ap.add_argument("--study", choices=["precision", "c3scan", "c5exact", "kernels", "example", "flow"],
                     help="print a precision, beta-scan, exact-2D, per-config kernel or example-params study instead")
@@ -674,8 +675,9 @@ def main():
     if args.kernel_only:
         import askja_amd as A
 
-        n, ext, level, dt = (NC, EXT, LEVEL, DT) if args.kernel_only == "c2" else (C4_NC, C4_EXT, C4_LEVEL, C4_DT)
-        lat = A.Lattice(n, ext, A.FP64 if args.precision == "fp64" else A.FP32)
+        n, ext, level, dt = {"c2": (NC, EXT, LEVEL, DT), "c4": (C4_NC, C4_EXT, C4_LEVEL, C4_DT),
+                             "c3": (16, [6, 24, 24, 24], 8, 0.01), "c5": (64, [128, 128], 8, 0.005)}[args.kernel_only]
+        lat = A.Lattice(n, ext, {"fp64": A.FP64, "fp32": A.FP32, "bf16x3": A.BF16X3}[args.precision])
         lat.haar(SEED, 2)
         print(json.dumps({"kernel_only": args.kernel_only, "extents": ext,
                           "roofline": kick_drift_roofline(lat, n, level, dt, args.roofline_launches)}))
@@ -712,6 +714,9 @@ def main():
     if args.workload == "c4":
         main_c4(args)
         return
+    if args.precision == "bf16x3":
+        print("bench.py: the c2 chain runs in fp64 or fp32", file=sys.stderr)
+        sys.exit(2)
     main_c2(args)
 
 

@@ -1,9 +1,10 @@
 """Writes tests/golden/ensemble_su3_4x4.json: ensemble averages (plaquette,
 volume-averaged |Polyakov loop|, and measurements4d.cpp's spatial Wilson line
-W_1 and Polyakov loop P-bar) of plain HMC from the CPU oracle (SU(3), D=4, 4^4, beta 5.7, dt 0.05, 10 MD steps, LEVEL 8,
-Haar start, counter-based momenta, 4000 trajectories after 200 for
-thermalisation, reunitarised chain) with their autocorrelation-corrected
-errors.  The GPU chain (independent seed) must agree within these errors.
+W_1 and Polyakov loop P-bar) of plain HMC from the CPU oracle (SU(3), D=4, 4^4, beta 5.7, dt 0.05, 20 MD steps
+(trajectory length 1), LEVEL 8, Haar start, counter-based momenta, 20000
+trajectories after 200 for thermalisation, reunitarised chain) with their
+autocorrelation-corrected errors.  The GPU chain (independent seed) must
+agree within these errors.  Takes about 10 min on 8 cores.
 """
 import json
 import math
@@ -17,8 +18,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 from oracle import oracle as O  # noqa: E402
 
-NC, EXT, BETA, DT, NDRIFT, LEVEL = 3, [4, 4, 4, 4], 5.7, 0.05, 10, 8
-NTHERM, NTRAJ, SEED = 200, 4000, 777
+NC, EXT, BETA, DT, NDRIFT, LEVEL = 3, [4, 4, 4, 4], 5.7, 0.05, 20, 8
+NTHERM, NTRAJ, SEED = 200, 20000, 777
 
 
 def tau_int(x, S=1.5):

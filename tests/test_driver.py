@@ -69,6 +69,18 @@ def read_master(path, dim):
     return vals
 
 
+@pytest.mark.gpu
+def test_bf16x3_cold_start_warns(tmp_path):
+    """--precision bf16x3 on a fresh run (cold start) at N >= 9 prints a
+    warning that such a chain rejects most trajectories until disordered."""
+    params = PARAMS.format(steps=20).replace("NC              3", "NC              12")
+    params = params.replace("N1              4\nN2              4\nN3              4", "N1              2\nN2              2\nN3              2")
+    (tmp_path / "p.txt").write_text(params)
+    r = _run(["p.txt", "--precision", "bf16x3"], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "warning: bf16x3 from the cold start" in r.stderr
+
+
 def test_missing_and_truncated_params(tmp_path):
     r = _run([str(tmp_path / "nope.txt")], tmp_path)
     assert r.returncode != 0
