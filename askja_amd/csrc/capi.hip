@@ -188,6 +188,9 @@ int do_kick(askja_ctx *c, const void *U, void *Uout, void *Pprev, double ek, dou
     int rc = ensure_part(c, (size_t)np * 6);
     if (rc) return rc;
     KickArgs a = kick_args(c, U, Uout, Pprev, ek, ed, level);
+    // the sums of a kick whose energies nobody reads (every one of a
+    // trajectory but the first and the last) are not computed
+    if (res_off < 0) a.part = nullptr;
     HIPCHK(c, launch_kick(a, c->mode, drift, c->st));
     if (res_off >= 0) HIPCHK(c, launch_finalize(c->part, np, 3, c->res + res_off, c->st));
     return ASKJA_OK;

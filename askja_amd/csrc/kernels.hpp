@@ -69,7 +69,8 @@ struct KickArgs {
     int level;
     double cinv[9];    // 1./k, k = 0..8, computed on the host
     double inv_n;      // 1./N computed on the host
-    double *part;      // per-block partial sums: {ReTr W, Tr Pin^2, Tr Pout^2} x (s, c)
+    double *part;      // per-block partial sums: {ReTr W, Tr Pin^2, Tr Pout^2} x (s, c);
+                       // null: not wanted (the kick skips its block reductions)
 };
 
 // Number of partial-sum blocks a kick launch writes (each 3 (s, c) pairs).

@@ -463,7 +463,9 @@ __device__ inline R mom_el(const R *__restrict__ P, long l, int n, int i, int j,
 template <int P, int NMAX>
 constexpr int kMinWaves = (P == kF64 && NMAX == 32) ? 2 : (P == kF64 && NMAX == 16) ? 4 : 1;
 
-template <int P, int NMAX, bool Drift, typename LS = typename Pol<P>::R, bool ActionOnly = false>
+// EN: the link size N equals NMAX (no padding), a compile-time constant, so
+// that every bounds test on the padding folds away.
+template <int P, int NMAX, bool Drift, typename LS = typename Pol<P>::R, bool ActionOnly = false, bool EN = false>
 __global__ __launch_bounds__((Cfg<P, NMAX>::NT), (kMinWaves<P, NMAX>)) void k_kick(KickArgs a)
 {
     typedef typename Pol<P>::R R;
@@ -474,7 +476,7 @@ __global__ __launch_bounds__((Cfg<P, NMAX>::NT), (kMinWaves<P, NMAX>)) void k_ki
     __shared__ __align__(16) R Bre[Cfg<P, NMAX>::SLOT];
     __shared__ __align__(16) R Bim[Cfg<P, NMAX>::SLOT];
     const Geom &g = a.g;
-    const int n = g.N;
+    const int n = EN ? NMAX : g.N;
     const long V = g.V;
     const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
     const int mu = (int)(lb % (unsigned)g.D);
@@ -679,7 +681,7 @@ __global__ __launch_bounds__((Cfg<P, NMAX>::NT), (kMinWaves<P, NMAX>)) void k_ki
                 }
             }
     }
-    block_reduce_store<3>(sum, cmp, a.part + 6 * (size_t)lb);
+    if (a.part) block_reduce_store<3>(sum, cmp, a.part + 6 * (size_t)lb);
 }
 
 // sum ReTr W in FP64 arithmetic over FP32 (f32_links) or FP64 links
@@ -697,9 +699,13 @@ hipError_t action_f64(const KickArgs &a, bool f32_links, hipStream_t st)
 template <int P, int NMAX>
 hipError_t kick(const KickArgs &a, bool drift, hipStream_t st)
 {
+    typedef typename Pol<P>::R R;
     const unsigned nb = (unsigned)(a.g.V * a.g.D);
     constexpr int NT = Cfg<P, NMAX>::NT;
-    if (drift) hipLaunchKernelGGL((k_kick<P, NMAX, true>), dim3(nb), dim3(NT), 0, st, a);
+    const bool en = a.g.N == NMAX;
+    if (drift && en) hipLaunchKernelGGL((k_kick<P, NMAX, true, R, false, true>), dim3(nb), dim3(NT), 0, st, a);
+    else if (drift) hipLaunchKernelGGL((k_kick<P, NMAX, true>), dim3(nb), dim3(NT), 0, st, a);
+    else if (en) hipLaunchKernelGGL((k_kick<P, NMAX, false, R, false, true>), dim3(nb), dim3(NT), 0, st, a);
     else hipLaunchKernelGGL((k_kick<P, NMAX, false>), dim3(nb), dim3(NT), 0, st, a);
     return hipGetLastError();
 }

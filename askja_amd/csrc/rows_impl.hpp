@@ -370,7 +370,7 @@ __global__ __launch_bounds__(kT, MW) void k_kick(KickArgs a)
             }
         }
     }
-    block_reduce_store<3>(sum, cmp, a.part + 6 * (size_t)lb);
+    if (a.part) block_reduce_store<3>(sum, cmp, a.part + 6 * (size_t)lb);
 }
 
 template <int N, typename R, bool Exact, int MW>

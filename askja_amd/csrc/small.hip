@@ -123,7 +123,7 @@ __global__ __launch_bounds__(kBlock) void k_kick(KickArgs a)
         sum[1] = herm_norm2(Pin);
         sum[2] = herm_norm2(Pout);
     }
-    block_reduce_store<3>(sum, cmp, a.part + 6 * (size_t)lb);
+    if (a.part) block_reduce_store<3>(sum, cmp, a.part + 6 * (size_t)lb);
 }
 
 // Standalone drift, in place: U <- T_L(i eps P) U.

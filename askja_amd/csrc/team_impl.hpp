@@ -457,7 +457,7 @@ __global__ __launch_bounds__(NT) void k_kick(KickArgs a)
         __syncthreads();
         drift_core<Exact>(o, Are, Aim, Bre, Bim, n, a.level, a.cinv, U, l, (R *)a.Uout);
     }
-    block_reduce_store<3>(sum, cmp, a.part + 6 * (size_t)lb);
+    if (a.part) block_reduce_store<3>(sum, cmp, a.part + 6 * (size_t)lb);
 }
 
 template <int NMAX, int NT, typename R, bool Exact>
