@@ -95,6 +95,7 @@ SIGNATURES = {
     "askja_tau_int": (_I, [_DP, _L, _D, _DP, C.POINTER(C.c_long)]),
     "askja_flow_define": (_I, [_P, _I, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), _I, _D, _D]),
     "askja_flow_define_stout": (_I, [_P, _I, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), _I, _D]),
+    "askja_flow_define_spectral": (_I, [_P, _I, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), _I, _D]),
     "askja_flow_block_size": (_I, [_P, C.POINTER(C.c_int)]),
     "askja_flow_set_weights": (_I, [_P, _P, _I]),
     "askja_flow_apply": (_I, [_P, _DP]),
@@ -344,11 +345,16 @@ class Lattice:
     # -- learned field transformation ------------------------------------------
     def flow_define(self, mu, parity, block, nblocks, rho_max=0.3, delta=1e-3, kind="rank-one"):
         """kind "rank-one" (askja_flow_define: rotation along the frozen
-        plaquettes' direction) or "stout" (askja_flow_define_stout: smearing
-        step along the staple force, rho_max < 1/(2(D-1)), delta unused)."""
+        plaquettes' direction), "stout" (askja_flow_define_stout: smearing
+        step along the staple force, rho_max < 1/(2(D-1)), delta unused) or
+        "spectral" (askja_flow_define_spectral, D = 2, any N: `parity` holds
+        the row offsets 0..3, rho_max is t_max <= 1/2, delta unused)."""
         n = len(mu)
         arr = lambda v: (C.c_int * n)(*[int(x) for x in v])
-        if kind == "stout":
+        if kind == "spectral":
+            self._ck(lib().askja_flow_define_spectral(self._h, n, arr(mu), arr(parity), arr(block), int(nblocks),
+                                                      rho_max), "flow_define_spectral")
+        elif kind == "stout":
             self._ck(lib().askja_flow_define_stout(self._h, n, arr(mu), arr(parity), arr(block), int(nblocks), rho_max),
                      "flow_define_stout")
         elif kind == "rank-one":

@@ -82,6 +82,8 @@ struct askja_ctx {
     double *flow_st_wr = nullptr;    // stout: W and rho of every sub-layer's updated links
     double *flow_st_aux = nullptr;   // stout: inverse Jacobian and eigenbasis of every sub-layer
     double *flow_st_lg = nullptr;    // stout: log-det gradient (one sub-layer)
+    double *flow_sp_ptr = nullptr;   // spectral: feature plaquette traces (V doubles)
+    double *flow_sp_dF = nullptr;    // spectral: their gradients (V doubles)
 };
 
 namespace {
@@ -416,7 +418,7 @@ int askja_destroy(askja_ctx *c)
     void *bufs[] = {c->snap, c->mom, c->mom_prev, c->part, c->res, c->matwork, c->staging,
                     c->d_nbr, c->d_dev2lex, c->d_lex2dev, c->flow_w, c->flow_gw, c->flow_g, c->flow_lj,
                     c->flow_out, c->flow_work, c->flow_log, c->flow_contrib, c->flow_save,
-                    c->flow_st_wr, c->flow_st_aux, c->flow_st_lg};
+                    c->flow_st_wr, c->flow_st_aux, c->flow_st_lg, c->flow_sp_ptr, c->flow_sp_dF};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     if (c->cst) (void)hipStreamSynchronize(c->cst);
@@ -1102,9 +1104,16 @@ int flow_ready(askja_ctx *c)
 
 size_t link_bytes(const askja_ctx *c) { return c->link_reals * elem_bytes(c); }
 
+// reals of one sub-layer's input log
+size_t flow_log_stride(const askja_ctx *c)
+{
+    if (c->flow_kind == kFlowSpectral) return (size_t)sflow_active(c->geom) * 2 * c->nc * c->nc;
+    return flow_log_reals(c->geom);
+}
+
 void *flow_log_of(askja_ctx *c, int k)
 {
-    return (char *)c->flow_log + (size_t)k * flow_log_reals(c->geom) * elem_bytes(c);
+    return (char *)c->flow_log + (size_t)k * flow_log_stride(c) * elem_bytes(c);
 }
 
 double *flow_wr_of(askja_ctx *c, int k)
@@ -1122,6 +1131,18 @@ double *flow_aux_of(askja_ctx *c, int k)
 int flow_forward(askja_ctx *c, bool store)
 {
     HIPCHK(c, hipMemcpyAsync(c->flow_out, c->links[c->cur], link_bytes(c), hipMemcpyDeviceToDevice, c->st));
+    if (c->flow_kind == kFlowSpectral) {
+        const long na = sflow_active(c->geom);
+        int rc = ensure_part(c, (size_t)na * 2);
+        if (rc) return rc;
+        for (int k = 0; k < c->flow_nsub; k++) {
+            const FlowSub &f = c->flow_subs[k];
+            HIPCHK(c, launch_sflow_forward(c->geom, c->flow_out, f.mu, f.par, f.w, f.rho_max, c->flow_sp_ptr,
+                                           store ? (double *)flow_log_of(c, k) : nullptr, c->part, c->st));
+            HIPCHK(c, launch_finalize(c->part, na, 1, c->flow_lj + 2 * k, c->st));
+        }
+        return ASKJA_OK;
+    }
     const long np = flow_forward_partials(c->geom, 0) > flow_forward_partials(c->geom, 1)
                         ? flow_forward_partials(c->geom, 0)
                         : flow_forward_partials(c->geom, 1);
@@ -1153,6 +1174,37 @@ int flow_logj(askja_ctx *c, double *logj)
     return ASKJA_OK;
 }
 
+// The spectral kind after its forward: the Wilson gradient of U = F(V) by a
+// kick from zero momenta (P_out = -eps TA(W) with eps = -beta / (2N) is
+// beta TA(W) / (2N), the Lie gradient of beta S_W), then the sub-layers'
+// backward in reverse.  flow_g is in the momentum layout.
+int sflow_gradient(askja_ctx *c, double beta, bool wgrad, double *action)
+{
+    const long np = kick_partials(c->geom);
+    int rc = ensure_part(c, (size_t)np * 6);
+    if (rc) return rc;
+    HIPCHK(c, hipMemsetAsync(c->flow_g, 0, c->mom_reals * sizeof(double), c->st));
+    KickArgs a = kick_args(c, c->flow_out, nullptr, nullptr, -beta / (2.0 * c->nc), 0.0, 0);
+    a.P = c->flow_g;
+    HIPCHK(c, launch_kick(a, c->mode, false, c->st));
+    HIPCHK(c, launch_finalize(c->part, np, 3, c->res, c->st));
+    if (wgrad)
+        HIPCHK(c, hipMemsetAsync(c->flow_gw, 0, (size_t)c->flow_nblocks * c->flow_bsize * sizeof(double), c->st));
+    HIPCHK(c, hipMemcpyAsync(c->flow_work, c->flow_out, link_bytes(c), hipMemcpyDeviceToDevice, c->st));
+    for (int k = c->flow_nsub - 1; k >= 0; k--) {
+        const FlowSub &f = c->flow_subs[k];
+        double *gw = wgrad ? c->flow_gw + (size_t)c->flow_block[k] * c->flow_bsize : nullptr;
+        HIPCHK(c, launch_sflow_backward(c->geom, c->flow_work, f.mu, f.par, f.w, f.rho_max, c->flow_sp_ptr,
+                                        (double *)flow_log_of(c, k), c->flow_g, -1.0, gw, c->flow_sp_dF, c->st));
+    }
+    if (action) {
+        double h[6];
+        if ((rc = read_res(c, h, 6))) return rc;
+        *action = (double)n_plaquettes(c) - pair_val(h, 0) / (4.0 * c->nc);
+    }
+    return ASKJA_OK;
+}
+
 // Gradient of Phi = beta S_W(F(V)) - log J(V) with respect to V into
 // flow_g (and the MLP weights into flow_gw when wgrad); returns S_W(F(V)).
 int flow_gradient(askja_ctx *c, double beta, bool wgrad, double *action)
@@ -1160,6 +1212,7 @@ int flow_gradient(askja_ctx *c, double beta, bool wgrad, double *action)
     // action == nullptr: no host synchronisation (inside an MD trajectory)
     int rc = flow_forward(c, true);
     if (rc) return rc;
+    if (c->flow_kind == kFlowSpectral) return sflow_gradient(c, beta, wgrad, action);
     const long nl = c->NL;
     if ((rc = ensure_part(c, (size_t)((nl + 255) / 256) * 2))) return rc;
     long np = 0;
@@ -1187,6 +1240,13 @@ int flow_gradient(askja_ctx *c, double beta, bool wgrad, double *action)
     return ASKJA_OK;
 }
 
+// P <- P - eps_c flow_g
+hipError_t flow_kick(askja_ctx *c, double eps_c)
+{
+    if (c->flow_kind == kFlowSpectral) return launch_axpy((double *)c->mom, c->flow_g, -eps_c, (long)c->mom_reals, c->st);
+    return launch_kick_g(c->geom, c->mom, c->flow_g, eps_c, c->mode, c->st);
+}
+
 int kinetic_now(askja_ctx *c, double *k)
 {
     long np = 0;
@@ -1203,6 +1263,66 @@ int kinetic_now(askja_ctx *c, double *k)
 }  // namespace
 
 namespace {
+
+int flow_free(askja_ctx *c)
+{
+    void *old[] = {c->flow_w,   c->flow_gw,       c->flow_g,     c->flow_lj,     c->flow_out,    c->flow_work,
+                   c->flow_log, c->flow_contrib,  c->flow_save,  c->flow_st_wr,  c->flow_st_aux, c->flow_st_lg,
+                   c->flow_sp_ptr, c->flow_sp_dF};
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    for (void *p : old)
+        if (p) (void)hipFree(p);
+    c->flow_w = nullptr;
+    c->flow_gw = c->flow_g = c->flow_lj = nullptr;
+    c->flow_out = c->flow_work = c->flow_log = c->flow_save = nullptr;
+    c->flow_st_wr = c->flow_st_aux = c->flow_st_lg = nullptr;
+    c->flow_contrib = c->flow_sp_ptr = c->flow_sp_dF = nullptr;
+    c->flow_nsub = 0;
+    return ASKJA_OK;
+}
+
+// D = 2 spectral map, any N: sub-layer k updates U_mu[k](x) for
+// x_nu = offset[k] mod 4 (nu = 1 - mu), t = t_max tanh(MLP(3 features))
+int sflow_define(askja_ctx *c, int nsub, const int *mu, const int *offset, const int *block, int nblocks,
+                 double t_max)
+{
+    if (c->dim != 2) return fail(c, ASKJA_EUNSUP, "flow_define_spectral: D = 2 only");
+    if (c->mode != kModeF64 && c->mode != kModeF64Exact)
+        return fail(c, ASKJA_EUNSUP, "flow_define_spectral: FP64 precision modes only");
+    if (c->ext[0] % 4 || c->ext[1] % 4)
+        return fail(c, ASKJA_EUNSUP, "flow_define_spectral: both extents must be multiples of 4 (row masks mod 4)");
+    if (nsub < 1 || nsub > 4096 || nblocks < 1 || !mu || !offset || !block)
+        return fail(c, ASKJA_EINVAL, "flow_define_spectral: bad sizes or null arrays");
+    if (!(t_max > 0.0) || t_max > 0.5)
+        return fail(c, ASKJA_EINVAL, "flow_define_spectral: t_max must be in (0, 1/2] (the phase branch's series bound)");
+    for (int k = 0; k < nsub; k++)
+        if (mu[k] < 0 || mu[k] > 1 || offset[k] < 0 || offset[k] > 3 || block[k] < 0 || block[k] >= nblocks)
+            return fail(c, ASKJA_EINVAL, "flow_define_spectral: sub-layer %d out of range", k);
+    int rc = flow_free(c);
+    if (rc) return rc;
+    c->flow_kind = kFlowSpectral;
+    c->flow_bsize = 32 * 3 + 2 * 32 + 1;
+    c->flow_nblocks = nblocks;
+    const size_t nd = c->mom_reals;
+    HIPCHK(c, hipMalloc(&c->flow_w, (size_t)nblocks * c->flow_bsize * sizeof(float)));
+    HIPCHK(c, hipMemset(c->flow_w, 0, (size_t)nblocks * c->flow_bsize * sizeof(float)));
+    HIPCHK(c, hipMalloc(&c->flow_gw, (size_t)nblocks * c->flow_bsize * sizeof(double)));
+    HIPCHK(c, hipMalloc(&c->flow_g, nd * sizeof(double)));
+    HIPCHK(c, hipMalloc(&c->flow_lj, (size_t)2 * nsub * sizeof(double)));
+    HIPCHK(c, hipMalloc(&c->flow_out, link_bytes(c)));
+    HIPCHK(c, hipMalloc(&c->flow_work, link_bytes(c)));
+    HIPCHK(c, hipMalloc(&c->flow_log, (size_t)nsub * flow_log_stride(c) * sizeof(double)));
+    HIPCHK(c, hipMalloc(&c->flow_save, link_bytes(c)));
+    HIPCHK(c, hipMalloc((void **)&c->flow_sp_ptr, (size_t)c->V * sizeof(double)));
+    HIPCHK(c, hipMalloc((void **)&c->flow_sp_dF, (size_t)c->V * sizeof(double)));
+    c->flow_subs.resize(nsub);
+    c->flow_block.assign(block, block + nsub);
+    for (int k = 0; k < nsub; k++)
+        c->flow_subs[k] =
+            FlowSub{mu[k], offset[k], c->flow_w + (size_t)block[k] * c->flow_bsize, 3, t_max, 0.0, kFlowSpectral};
+    c->flow_nsub = nsub;
+    return ASKJA_OK;
+}
 
 int flow_define(askja_ctx *c, int kind, int nsub, const int *mu, const int *parity, const int *block, int nblocks,
                 double rho_max, double delta)
@@ -1222,17 +1342,9 @@ int flow_define(askja_ctx *c, int kind, int nsub, const int *mu, const int *pari
         if (mu[k] < 0 || mu[k] >= c->dim || (parity[k] != 0 && parity[k] != 1) || block[k] < 0 || block[k] >= nblocks)
             return fail(c, ASKJA_EINVAL, "flow_define: sub-layer %d out of range", k);
     const int nf = 3 * (c->dim - 1) * (c->dim - 2) / 2;  // plaquette and two rectangles per plane
-    // (re)allocate
-    void *old[] = {c->flow_w,    c->flow_gw,  c->flow_g,       c->flow_lj,    c->flow_out,  c->flow_work,
-                   c->flow_log,  c->flow_contrib, c->flow_save, c->flow_st_wr, c->flow_st_aux, c->flow_st_lg};
-    for (void *p : old)
-        if (p) (void)hipFree(p);
-    c->flow_w = nullptr;
-    c->flow_gw = c->flow_g = c->flow_lj = nullptr;
-    c->flow_out = c->flow_work = c->flow_log = c->flow_save = nullptr;
-    c->flow_st_wr = c->flow_st_aux = c->flow_st_lg = nullptr;
-    c->flow_contrib = nullptr;
-    c->flow_nsub = 0;
+    int rc = flow_free(c);
+    if (rc) return rc;
+    c->flow_kind = kind;
     c->flow_bsize = 32 * nf + 2 * 32 + 1;
     c->flow_nblocks = nblocks;
     HIPCHK(c, hipMalloc(&c->flow_w, (size_t)nblocks * c->flow_bsize * sizeof(float)));
@@ -1255,7 +1367,6 @@ int flow_define(askja_ctx *c, int kind, int nsub, const int *mu, const int *pari
     for (int k = 0; k < nsub; k++)
         c->flow_subs[k] =
             FlowSub{mu[k], parity[k], c->flow_w + (size_t)block[k] * c->flow_bsize, nf, rho_max, delta, kind};
-    c->flow_kind = kind;
     c->flow_nsub = nsub;
     return ASKJA_OK;
 }
@@ -1274,6 +1385,13 @@ int askja_flow_define_stout(askja_ctx *c, int nsub, const int *mu, const int *pa
 {
     ENTER(c);
     return flow_define(c, kFlowStout, nsub, mu, parity, block, nblocks, rho_max, 0.0);
+}
+
+int askja_flow_define_spectral(askja_ctx *c, int nsub, const int *mu, const int *offset, const int *block,
+                               int nblocks, double t_max)
+{
+    ENTER(c);
+    return sflow_define(c, nsub, mu, offset, block, nblocks, t_max);
 }
 
 int askja_flow_block_size(askja_ctx *c, int *bsize)
@@ -1342,6 +1460,7 @@ int askja_flow_gradient_download(askja_ctx *c, double *aos)
     int rc = flow_ready(c);
     if (rc) return rc;
     if (!aos) return fail(c, ASKJA_EINVAL, "flow_gradient_download: null buffer");
+    if (c->flow_kind == kFlowSpectral) return mom_down(c, c->flow_g, aos);
     std::vector<double> g((size_t)c->NL * 9);
     HIPCHK(c, hipMemcpyAsync(g.data(), c->flow_g, g.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
@@ -1391,7 +1510,7 @@ int askja_flow_hmc_trajectory(askja_ctx *c, const askja_hmc_params *p, uint64_t 
     if ((rc = kinetic_now(c, &k0))) return rc;
     if ((rc = flow_gradient(c, p->beta, false, &s0))) return rc;
     if ((rc = flow_logj(c, &lj0))) return rc;
-    HIPCHK(c, launch_kick_g(c->geom, c->mom, c->flow_g, 0.5 * p->dt * cf, c->mode, c->st));
+    HIPCHK(c, flow_kick(c, 0.5 * p->dt * cf));
     double ci[9];
     fill_cinv(ci);
     s1 = s0;
@@ -1399,7 +1518,7 @@ int askja_flow_hmc_trajectory(askja_ctx *c, const askja_hmc_params *p, uint64_t 
     for (int t = 1; t <= p->ndrift; t++) {
         HIPCHK(c, launch_drift(c->geom, c->links[c->cur], c->mom, p->dt, p->level, ci, c->mode, c->st));
         if ((rc = flow_gradient(c, p->beta, false, t == p->ndrift ? &s1 : nullptr))) return rc;
-        HIPCHK(c, launch_kick_g(c->geom, c->mom, c->flow_g, (t < p->ndrift ? 1.0 : 0.5) * p->dt * cf, c->mode, c->st));
+        HIPCHK(c, flow_kick(c, (t < p->ndrift ? 1.0 : 0.5) * p->dt * cf));
     }
     if ((rc = flow_logj(c, &lj1))) return rc;
     if ((rc = kinetic_now(c, &k1))) return rc;

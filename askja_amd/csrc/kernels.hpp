@@ -152,13 +152,14 @@ hipError_t launch_line_phases(const Geom &g, const void *U, int dir, int mode, i
                               double *wbuf, double *out, hipStream_t st);
 
 // ---- learned field transformation (flow.hip, N = 3) -------------------------
-enum { kFlowRankOne = 0, kFlowStout = 1 };
+enum { kFlowRankOne = 0, kFlowStout = 1, kFlowSpectral = 2 };
 struct FlowSub {
     int mu, par;     // updated links: direction mu on sites of parity par
     const float *w;  // device MLP block of the sub-layer
     int nf;          // features = planes not containing mu
     double rho_max, delta;
-    int kind;        // kFlowRankOne or kFlowStout
+    int kind;        // kFlowRankOne, kFlowStout or kFlowSpectral (then par = the row offset
+                     // off: links U_mu(x) with x_nu = off mod 4, rho_max = t_max)
 };
 long flow_forward_partials(const Geom &g, int par);
 // one sub-layer in place; log (may be null) receives its input links; for
@@ -191,5 +192,22 @@ hipError_t launch_wilson_grad(const Geom &g, const void *U, double coef, int mod
                               long *np, hipStream_t st);
 // P <- P - eps_c * gr
 hipError_t launch_kick_g(const Geom &g, void *P, const double *gr, double eps_c, int mode, hipStream_t st);
+
+// ---- spectral field transformation of D = 2 lattices (flow2d.hip, any N) -----
+// updated links per sub-layer: V / 4 (rows x_nu = off mod 4)
+long sflow_active(const Geom &g);
+// one sub-layer (mu, off) in place on U (FP64): feature plaquettes into ptr
+// (V doubles), the updated links' inputs into log (2 N^2 per updated link,
+// may be null), log J per updated link into part ((s, c) pairs)
+hipError_t launch_sflow_forward(const Geom &g, void *U, int mu, int off, const float *w, double tmax, double *ptr,
+                                double *log, double *part, hipStream_t st);
+// backward of one sub-layer: gr (momentum layout) from the gradient at its
+// output to the gradient at its input, c_logj d log J included; gw (may be
+// null) += the MLP weight gradient; restores the input links from log into
+// U.  ptr, dF: V doubles of scratch.
+hipError_t launch_sflow_backward(const Geom &g, void *U, int mu, int off, const float *w, double tmax, double *ptr,
+                                 double *log, double *gr, double c_logj, double *gw, double *dF, hipStream_t st);
+// y += s x (n doubles)
+hipError_t launch_axpy(double *y, const double *x, double s, long n, hipStream_t st);
 
 }  // namespace askja

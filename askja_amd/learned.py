@@ -40,6 +40,19 @@ def sweep_masks(dim, layers):
     return mu, par, blk
 
 
+def spectral_masks(layers):
+    """Sub-layers of `layers` sweeps of the two-dimensional spectral map (both
+    directions, the four row offsets); every sweep shares one weight block."""
+    mu, off, blk = [], [], []
+    for layer in range(layers):
+        for o in range(4):
+            for m in (0, 1):
+                mu.append(m)
+                off.append(o)
+                blk.append(layer)
+    return mu, off, blk
+
+
 def init_weights(nblocks, bsize, nf, seed=0, scale=0.1):
     """Small random W1, b1; w2 and b2 zero, so the untrained map is F = id.
     Block layout W1[32][nf], b1[32], w2[32], b2 (bsize = 32 nf + 65)."""
