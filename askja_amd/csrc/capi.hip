@@ -84,6 +84,7 @@ struct askja_ctx {
     double *flow_st_lg = nullptr;    // stout: log-det gradient (one sub-layer)
     double *flow_sp_ptr = nullptr;   // spectral: feature plaquette traces (V doubles)
     double *flow_sp_dF = nullptr;    // spectral: their gradients (V doubles)
+    double *flow_sp_inv = nullptr;   // spectral: (1 - tW)^{-1} of every sub-layer's updated links
 };
 
 namespace {
@@ -418,7 +419,7 @@ int askja_destroy(askja_ctx *c)
     void *bufs[] = {c->snap, c->mom, c->mom_prev, c->part, c->res, c->matwork, c->staging,
                     c->d_nbr, c->d_dev2lex, c->d_lex2dev, c->flow_w, c->flow_gw, c->flow_g, c->flow_lj,
                     c->flow_out, c->flow_work, c->flow_log, c->flow_contrib, c->flow_save,
-                    c->flow_st_wr, c->flow_st_aux, c->flow_st_lg, c->flow_sp_ptr, c->flow_sp_dF};
+                    c->flow_st_wr, c->flow_st_aux, c->flow_st_lg, c->flow_sp_ptr, c->flow_sp_dF, c->flow_sp_inv};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     if (c->cst) (void)hipStreamSynchronize(c->cst);
@@ -1137,8 +1138,10 @@ int flow_forward(askja_ctx *c, bool store)
         if (rc) return rc;
         for (int k = 0; k < c->flow_nsub; k++) {
             const FlowSub &f = c->flow_subs[k];
+            const size_t ko = (size_t)k * flow_log_stride(c);
             HIPCHK(c, launch_sflow_forward(c->geom, c->flow_out, f.mu, f.par, f.w, f.rho_max, c->flow_sp_ptr,
-                                           store ? (double *)flow_log_of(c, k) : nullptr, c->part, c->st));
+                                           store ? (double *)flow_log_of(c, k) : nullptr,
+                                           store ? c->flow_sp_inv + ko : nullptr, c->part, c->st));
             HIPCHK(c, launch_finalize(c->part, na, 1, c->flow_lj + 2 * k, c->st));
         }
         return ASKJA_OK;
@@ -1195,7 +1198,8 @@ int sflow_gradient(askja_ctx *c, double beta, bool wgrad, double *action)
         const FlowSub &f = c->flow_subs[k];
         double *gw = wgrad ? c->flow_gw + (size_t)c->flow_block[k] * c->flow_bsize : nullptr;
         HIPCHK(c, launch_sflow_backward(c->geom, c->flow_work, f.mu, f.par, f.w, f.rho_max, c->flow_sp_ptr,
-                                        (double *)flow_log_of(c, k), c->flow_g, -1.0, gw, c->flow_sp_dF, c->st));
+                                        (double *)flow_log_of(c, k), c->flow_sp_inv + (size_t)k * flow_log_stride(c),
+                                        c->flow_g, -1.0, gw, c->flow_sp_dF, c->st));
     }
     if (action) {
         double h[6];
@@ -1268,7 +1272,7 @@ int flow_free(askja_ctx *c)
 {
     void *old[] = {c->flow_w,   c->flow_gw,       c->flow_g,     c->flow_lj,     c->flow_out,    c->flow_work,
                    c->flow_log, c->flow_contrib,  c->flow_save,  c->flow_st_wr,  c->flow_st_aux, c->flow_st_lg,
-                   c->flow_sp_ptr, c->flow_sp_dF};
+                   c->flow_sp_ptr, c->flow_sp_dF, c->flow_sp_inv};
     HIPCHK(c, hipStreamSynchronize(c->st));
     for (void *p : old)
         if (p) (void)hipFree(p);
@@ -1276,7 +1280,7 @@ int flow_free(askja_ctx *c)
     c->flow_gw = c->flow_g = c->flow_lj = nullptr;
     c->flow_out = c->flow_work = c->flow_log = c->flow_save = nullptr;
     c->flow_st_wr = c->flow_st_aux = c->flow_st_lg = nullptr;
-    c->flow_contrib = c->flow_sp_ptr = c->flow_sp_dF = nullptr;
+    c->flow_contrib = c->flow_sp_ptr = c->flow_sp_dF = c->flow_sp_inv = nullptr;
     c->flow_nsub = 0;
     return ASKJA_OK;
 }
@@ -1315,6 +1319,7 @@ int sflow_define(askja_ctx *c, int nsub, const int *mu, const int *offset, const
     HIPCHK(c, hipMalloc(&c->flow_save, link_bytes(c)));
     HIPCHK(c, hipMalloc((void **)&c->flow_sp_ptr, (size_t)c->V * sizeof(double)));
     HIPCHK(c, hipMalloc((void **)&c->flow_sp_dF, (size_t)c->V * sizeof(double)));
+    HIPCHK(c, hipMalloc((void **)&c->flow_sp_inv, (size_t)nsub * flow_log_stride(c) * sizeof(double)));
     c->flow_subs.resize(nsub);
     c->flow_block.assign(block, block + nsub);
     for (int k = 0; k < nsub; k++)

@@ -27,7 +27,8 @@
 // Backward (G' = Lie gradient of Phi at the output link; AH(M) =
 // (M - M^+) / (2i), TL = traceless part, c = -1 for the -log J of Phi,
 // m = 1 + t^2 - (2t/N) ReTr W):
-//   own link:    g_U = TL[(1 - t^2) B^{-+} G' B^{-1} + c g_L],
+//   own link:    g_U = TL[(1 - t^2) B^{-+} G' B^{-1} + c g_L]  (B^{-1} kept
+//                from the forward, which forms it by LU and substitution),
 //                g_L = -N t AH(h) / (1 - t^2) + t / (N m) AH(W);
 //   staples, with g_s = g_U - E^+ G' E, E = U' U^+:
 //                U_nu(x+mu) += U^+ g_s U,  U_mu(x+nu) += -A^+ g_s A
@@ -41,6 +42,7 @@
 // buffers are ordinary device memory, where the hardware add is exact).
 #include "kernels.hpp"
 #include "common.hpp"
+#include "mfma_impl.hpp"
 
 namespace askja {
 namespace sflow {
@@ -56,6 +58,7 @@ struct Args {
     double tmax;
     double *ptr;        // ReTr P / N per device site (rows off+1)
     double *log;        // the updated links' inputs, 2 N^2 reals each (nullable in the forward)
+    double *inv;        // B^{-1} of every updated link, 2 N^2 reals each (nullable in the forward)
     double *part;       // forward: log J per updated link as (s, c) pairs
     double *gr;         // backward: Lie gradient, momentum layout
     double c_logj;
@@ -63,9 +66,14 @@ struct Args {
     double *dF;         // backward: dPhi/df per feature plaquette (device site)
 };
 
+// Size classes NMAX >= 16 multiply on the FP64 MFMA units (mfma::cmul, the
+// kick's product: planar slots with row stride NMAX + 2, its workgroup
+// shape NT); the smaller ones on the VALU.
 template <int NMAX, int NT>
 struct Cfg {
-    static constexpr int LD = NMAX + 1;  // odd leading dimension: transposed reads spread over banks
+    static constexpr bool MF = NMAX >= 16;
+    static_assert(!MF || NT == mfma::Cfg<mfma::kF64, NMAX>::NT, "MFMA size class: workgroup shape");
+    static constexpr int LD = NMAX + 2;
     static constexpr int SZ = NMAX * LD;
     static constexpr int EPL = (NMAX * NMAX + NT - 1) / NT;
     static constexpr int NW = NT / 64;
@@ -83,21 +91,31 @@ __device__ inline long act_site(const Geom &g, int mu, int row, long b)
     return g.lex2dev[x0 * g.ext[1] + x1];
 }
 
+// Owned output elements of a product.  MFMA classes: the FP64 MFMA
+// accumulator layout (element q = 4 t + u: row acc_row(u), column
+// acc_col(t)).  VALU classes: column c = tid % NMAX, rows r0 + RS q
+// (r0 = tid / NMAX, RS = NT / NMAX), so that a wavefront covers whole rows.
+// (i, j) of an element outside the matrix is (0, 0), so that reads of it
+// stay in the slot.
 template <int NMAX, int NT>
 struct Own {
+    static constexpr bool MF = Cfg<NMAX, NT>::MF;
+    static constexpr int RS = NT / NMAX;
     static constexpr int EPL = Cfg<NMAX, NT>::EPL;
-    int i[EPL], j[EPL];
-    bool ok[EPL];
-    __device__ inline void init(int n)
+    static_assert(MF ? EPL == 4 * mfma::Cfg<mfma::kF64, NMAX>::NCT : (NT % NMAX == 0 && EPL == (NMAX + RS - 1) / RS),
+                  "size class");
+    int r0, c, n;
+    __device__ inline void init(int n_)
     {
-#pragma unroll
-        for (int q = 0; q < EPL; q++) {
-            const int e = threadIdx.x + q * NT;
-            ok[q] = e < n * n;
-            i[q] = ok[q] ? e / n : 0;
-            j[q] = ok[q] ? e % n : 0;
-        }
+        r0 = threadIdx.x / NMAX;
+        c = threadIdx.x % NMAX;
+        n = n_;
     }
+    __device__ inline int ri(int q) const { return MF ? mfma::acc_row<mfma::kF64, NMAX>(q & 3) : r0 + RS * q; }
+    __device__ inline int rj(int q) const { return MF ? mfma::acc_col<mfma::kF64, NMAX>(q >> 2) : c; }
+    __device__ inline bool ok(int q) const { return ri(q) < n && rj(q) < n; }
+    __device__ inline int i(int q) const { return ok(q) ? ri(q) : 0; }
+    __device__ inline int j(int q) const { return ok(q) ? rj(q) : 0; }
 };
 
 template <int NMAX, int NT>
@@ -105,15 +123,27 @@ struct Reg {
     double r[Cfg<NMAX, NT>::EPL], i[Cfg<NMAX, NT>::EPL];
 };
 
-// global link -> LDS matrix
+// global link -> LDS matrix.  Every load of the thread is issued before its
+// first LDS store (a compile-time count per thread), so that the loads
+// overlap instead of each waiting for the previous store.
 template <int NMAX, int NT>
 __device__ inline void ld(const Geom &g, double *re, double *im, const double *f, long l, int n)
 {
-    constexpr int LD = Cfg<NMAX, NT>::LD;
-    for (int r = threadIdx.x; r < 2 * n * n; r += NT) {
-        const double v = f[loff(g, l, r, n)];
-        const int e = r >> 1;
-        (r & 1 ? im : re)[(e / n) * LD + e % n] = v;
+    constexpr int LD = Cfg<NMAX, NT>::LD, K = (2 * NMAX * NMAX + NT - 1) / NT;
+    const int nr = 2 * n * n;
+    double v[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int r = threadIdx.x + k * NT;
+        v[k] = r < nr ? f[loff(g, l, r, n)] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int r = threadIdx.x + k * NT;
+        if (r < nr) {
+            const int e = r >> 1;
+            (r & 1 ? im : re)[(e / n) * LD + e % n] = v[k];
+        }
     }
 }
 
@@ -121,10 +151,41 @@ __device__ inline void ld(const Geom &g, double *re, double *im, const double *f
 template <int NMAX, int NT>
 __device__ inline void ld_log(double *re, double *im, const double *b, int n)
 {
-    constexpr int LD = Cfg<NMAX, NT>::LD;
-    for (int r = threadIdx.x; r < 2 * n * n; r += NT) {
-        const int e = r >> 1;
-        (r & 1 ? im : re)[(e / n) * LD + e % n] = b[r];
+    constexpr int LD = Cfg<NMAX, NT>::LD, K = (2 * NMAX * NMAX + NT - 1) / NT;
+    const int nr = 2 * n * n;
+    double v[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int r = threadIdx.x + k * NT;
+        v[k] = r < nr ? b[r] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int r = threadIdx.x + k * NT;
+        if (r < nr) {
+            const int e = r >> 1;
+            (r & 1 ? im : re)[(e / n) * LD + e % n] = v[k];
+        }
+    }
+}
+
+// global -> global copy of one link's 2 N^2 reals (src in the field layout
+// when FL, dst contiguous; or the reverse)
+template <int NMAX, int NT, bool FromField>
+__device__ inline void copy_link(const Geom &g, double *dst, const double *src, long l, int n)
+{
+    constexpr int K = (2 * NMAX * NMAX + NT - 1) / NT;
+    const int nr = 2 * n * n;
+    double v[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int r = threadIdx.x + k * NT;
+        v[k] = r < nr ? src[FromField ? loff(g, l, r, n) : (size_t)r] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int r = threadIdx.x + k * NT;
+        if (r < nr) dst[FromField ? (size_t)r : loff(g, l, r, n)] = v[k];
     }
 }
 
@@ -134,32 +195,80 @@ __device__ inline void st(const Own<NMAX, NT> &o, double *re, double *im, const 
     constexpr int LD = Cfg<NMAX, NT>::LD;
 #pragma unroll
     for (int q = 0; q < Own<NMAX, NT>::EPL; q++)
-        if (o.ok[q]) {
-            re[o.i[q] * LD + o.j[q]] = c.r[q];
-            im[o.i[q] * LD + o.j[q]] = c.i[q];
+        if (o.ok(q)) {
+            re[o.i(q) * LD + o.j(q)] = c.r[q];
+            im[o.i(q) * LD + o.j(q)] = c.i[q];
         }
 }
 
-// c = op(A) op(B) on the owned elements (op = ^+ where DA / DB)
+// c = op(A) op(B) on the owned elements (op = ^+ where DA / DB).  MFMA
+// classes need the slots zero outside n x n (zero_slots at kernel start:
+// nothing writes there afterwards).
 template <bool DA, bool DB, int NMAX, int NT>
 __device__ inline void mm(const Own<NMAX, NT> &o, const double *ar_, const double *ai_, const double *br_,
                           const double *bi_, int n, Reg<NMAX, NT> &c)
 {
     constexpr int LD = Cfg<NMAX, NT>::LD, EPL = Cfg<NMAX, NT>::EPL;
+    if constexpr (Cfg<NMAX, NT>::MF) {
+        // the kick's stacked product (mfma::cmul_vec) with a shorter unroll:
+        // these kernels hold more matrices in registers than the kick
+        typedef mfma::f64x4 V4;
+        constexpr int NCT = mfma::Cfg<mfma::kF64, NMAX>::NCT;
+        V4 accr[NCT], acci[NCT];
 #pragma unroll
-    for (int q = 0; q < EPL; q++) c.r[q] = c.i[q] = 0.0;
-    for (int k = 0; k < n; k++) {
+        for (int t = 0; t < NCT; t++) accr[t] = acci[t] = V4{0.0, 0.0, 0.0, 0.0};
+        const int lane = threadIdx.x & 63;
+        const int row = 16 * mfma::wave_rowblock<mfma::kF64, NMAX>() + (lane & 15);
+        const int t0 = mfma::wave_tile0<mfma::kF64, NMAX>();
+#pragma unroll 1
+        for (int ks = 0; ks < NMAX / 4; ks++) {
+            const int k = 4 * ks + (lane >> 4);
+            const double ar = DA ? ar_[k * LD + row] : ar_[row * LD + k];
+            const double ai = DA ? -ai_[k * LD + row] : ai_[row * LD + k];
+            const double nai = -ai;
+#pragma unroll
+            for (int t = 0; t < NCT; t++) {
+                const int col = 16 * (t0 + t) + (lane & 15);
+                const double br = DB ? br_[col * LD + k] : br_[k * LD + col];
+                const double bi = DB ? -bi_[col * LD + k] : bi_[k * LD + col];
+                accr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, accr[t], 0, 0, 0);
+                acci[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, acci[t], 0, 0, 0);
+                accr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai, bi, accr[t], 0, 0, 0);
+                acci[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, acci[t], 0, 0, 0);
+            }
+        }
 #pragma unroll
         for (int q = 0; q < EPL; q++) {
-            const int ia = DA ? k * LD + o.i[q] : o.i[q] * LD + k;
-            const int ib = DB ? o.j[q] * LD + k : k * LD + o.j[q];
-            const double ar = ar_[ia], ai = DA ? -ai_[ia] : ai_[ia];
-            const double br = br_[ib], bi = DB ? -bi_[ib] : bi_[ib];
-            c.r[q] = fma(ar, br, c.r[q]);
-            c.r[q] = fma(-ai, bi, c.r[q]);
-            c.i[q] = fma(ar, bi, c.i[q]);
-            c.i[q] = fma(ai, br, c.i[q]);
+            c.r[q] = accr[q >> 2][q & 3];
+            c.i[q] = acci[q >> 2][q & 3];
         }
+    } else {
+#pragma unroll
+        for (int q = 0; q < EPL; q++) c.r[q] = c.i[q] = 0.0;
+        const int jc = o.c < n ? o.c : 0;
+        for (int k = 0; k < n; k++) {
+            const int ib = DB ? jc * LD + k : k * LD + jc;
+            const double br = br_[ib], bi = DB ? -bi_[ib] : bi_[ib];
+#pragma unroll
+            for (int q = 0; q < EPL; q++) {
+                const int ia = DA ? k * LD + o.i(q) : o.i(q) * LD + k;
+                const double ar = ar_[ia], ai = DA ? -ai_[ia] : ai_[ia];
+                c.r[q] = fma(ar, br, c.r[q]);
+                c.r[q] = fma(-ai, bi, c.r[q]);
+                c.i[q] = fma(ar, bi, c.i[q]);
+                c.i[q] = fma(ai, br, c.i[q]);
+            }
+        }
+    }
+}
+
+// zero both operand slots (MFMA classes: the padding outside n x n must be 0)
+template <int NMAX, int NT>
+__device__ inline void zero_slots(double *Ar, double *Ai, double *Br, double *Bi)
+{
+    if constexpr (Cfg<NMAX, NT>::MF) {
+        for (int e = threadIdx.x; e < Cfg<NMAX, NT>::SZ; e += NT) Ar[e] = Ai[e] = Br[e] = Bi[e] = 0.0;
+        __syncthreads();
     }
 }
 
@@ -199,118 +308,109 @@ __device__ inline void crecip(double pr, double pi, double &ir, double &ii)
     ii = -pi * d;
 }
 
-// In-place LU of the matrix in (Ar, Ai) without pivoting, the multipliers
-// left unscaled: L_ik = A_ik / A_kk (i > k), Ub_kj = A_kj (j >= k).
+// Thread map of the factorisation and the triangular solves: thread tid
+// works on column c = tid % NMAX and rows r0 + RS q (r0 = tid / NMAX,
+// q < RP), so that a wavefront covers whole rows (row-k and column-k reads are
+// broadcasts or contiguous) and no index needs a division by a runtime value.
 template <int NMAX, int NT>
-__device__ inline void lu(double *Ar, double *Ai, int n)
+struct Tri {
+    static constexpr int RS = NT / NMAX;                 // rows per pass (NT >= NMAX in every size class)
+    static constexpr int RP = (NMAX + RS - 1) / RS;      // passes
+    static_assert(NT % NMAX == 0, "workgroup not a multiple of the matrix size class");
+    static __device__ inline int col() { return threadIdx.x % NMAX; }
+    static __device__ inline int row(int q) { return threadIdx.x / NMAX + RS * q; }
+};
+
+// In-place LU of the matrix in (Ar, Ai) without pivoting, the multipliers
+// left unscaled: L_ik = A_ik / A_kk (i > k), Ub_kj = A_kj (j >= k); then the
+// pivots' reciprocals into (ipr, ipi) (LDS, NMAX each).
+template <int NMAX, int NT>
+__device__ inline void lu(double *Ar, double *Ai, double *ipr, double *ipi, int n)
 {
     constexpr int LD = Cfg<NMAX, NT>::LD;
+    typedef Tri<NMAX, NT> M;
+    const int c = M::col();
     for (int k = 0; k < n - 1; k++) {
-        double ipr, ipi;
-        crecip(Ar[k * LD + k], Ai[k * LD + k], ipr, ipi);
-        const int m = n - 1 - k;
-        for (int e = threadIdx.x; e < m * m; e += NT) {
-            const int i = k + 1 + e / m, j = k + 1 + e % m;
-            const double ar = Ar[i * LD + k], ai = Ai[i * LD + k];
-            const double lr = ar * ipr - ai * ipi, li = ar * ipi + ai * ipr;
-            const double ur = Ar[k * LD + j], ui = Ai[k * LD + j];
-            Ar[i * LD + j] -= lr * ur - li * ui;
-            Ai[i * LD + j] -= lr * ui + li * ur;
+        if (c > k && c < n) {
+            double pr, pi;
+            crecip(Ar[k * LD + k], Ai[k * LD + k], pr, pi);
+            const double ur = Ar[k * LD + c], ui = Ai[k * LD + c];
+#pragma unroll
+            for (int q = 0; q < M::RP; q++) {
+                const int r = M::row(q);
+                if (r > k && r < n) {
+                    const double ar = Ar[r * LD + k], ai = Ai[r * LD + k];
+                    const double lr = ar * pr - ai * pi, li = ar * pi + ai * pr;
+                    Ar[r * LD + c] -= lr * ur - li * ui;
+                    Ai[r * LD + c] -= lr * ui + li * ur;
+                }
+            }
         }
         __syncthreads();
     }
+    if ((int)threadIdx.x < n) crecip(Ar[threadIdx.x * LD + threadIdx.x], Ai[threadIdx.x * LD + threadIdx.x],
+                                     ipr[threadIdx.x], ipi[threadIdx.x]);
+    __syncthreads();
 }
 
 // (Br, Bi) <- A^{-1} (Br, Bi) with A = L Ub from lu()
 template <int NMAX, int NT>
-__device__ inline void solve_left(const double *Ar, const double *Ai, double *Br, double *Bi, int n)
+__device__ inline void solve_left(const double *Ar, const double *Ai, const double *ipr, const double *ipi,
+                                  double *Br, double *Bi, int n)
 {
     constexpr int LD = Cfg<NMAX, NT>::LD;
-    for (int k = 0; k < n - 1; k++) {  // unit lower L
-        double ipr, ipi;
-        crecip(Ar[k * LD + k], Ai[k * LD + k], ipr, ipi);
-        const int m = n - 1 - k;
-        for (int e = threadIdx.x; e < m * n; e += NT) {
-            const int i = k + 1 + e / n, j = e % n;
-            const double ar = Ar[i * LD + k], ai = Ai[i * LD + k];
-            const double lr = ar * ipr - ai * ipi, li = ar * ipi + ai * ipr;
-            const double br = Br[k * LD + j], bi = Bi[k * LD + j];
-            Br[i * LD + j] -= lr * br - li * bi;
-            Bi[i * LD + j] -= lr * bi + li * br;
+    typedef Tri<NMAX, NT> M;
+    const int c = M::col();
+    for (int k = 0; k < n - 1; k++) {  // unit lower L: rows r > k -= L_rk B_k
+        if (c < n) {
+            const double br = Br[k * LD + c], bi = Bi[k * LD + c];
+            const double pr = ipr[k], pi = ipi[k];
+#pragma unroll
+            for (int q = 0; q < M::RP; q++) {
+                const int r = M::row(q);
+                if (r > k && r < n) {
+                    const double ar = Ar[r * LD + k], ai = Ai[r * LD + k];
+                    const double lr = ar * pr - ai * pi, li = ar * pi + ai * pr;
+                    Br[r * LD + c] -= lr * br - li * bi;
+                    Bi[r * LD + c] -= lr * bi + li * br;
+                }
+            }
         }
         __syncthreads();
     }
-    for (int k = n - 1; k >= 1; k--) {  // upper Ub, row k divided at the end
-        double ipr, ipi;
-        crecip(Ar[k * LD + k], Ai[k * LD + k], ipr, ipi);
-        for (int e = threadIdx.x; e < k * n; e += NT) {
-            const int i = e / n, j = e % n;
-            const double ur = Ar[i * LD + k], ui = Ai[i * LD + k];
-            const double xr0 = Br[k * LD + j], xi0 = Bi[k * LD + j];
-            const double xr = xr0 * ipr - xi0 * ipi, xi = xr0 * ipi + xi0 * ipr;
-            Br[i * LD + j] -= ur * xr - ui * xi;
-            Bi[i * LD + j] -= ur * xi + ui * xr;
+    for (int k = n - 1; k >= 1; k--) {  // upper Ub: rows r < k -= Ub_rk x_k, x_k = B_k / Ub_kk
+        if (c < n) {
+            const double b0 = Br[k * LD + c], b1 = Bi[k * LD + c];
+            const double xr = b0 * ipr[k] - b1 * ipi[k], xi = b0 * ipi[k] + b1 * ipr[k];
+#pragma unroll
+            for (int q = 0; q < M::RP; q++) {
+                const int r = M::row(q);
+                if (r < k) {
+                    const double ur = Ar[r * LD + k], ui = Ai[r * LD + k];
+                    Br[r * LD + c] -= ur * xr - ui * xi;
+                    Bi[r * LD + c] -= ur * xi + ui * xr;
+                }
+            }
         }
         __syncthreads();
     }
-    for (int e = threadIdx.x; e < n * n; e += NT) {
-        const int i = e / n, j = e % n;
-        double ipr, ipi;
-        crecip(Ar[i * LD + i], Ai[i * LD + i], ipr, ipi);
-        const double xr = Br[i * LD + j], xi = Bi[i * LD + j];
-        Br[i * LD + j] = xr * ipr - xi * ipi;
-        Bi[i * LD + j] = xr * ipi + xi * ipr;
+    if (c < n) {
+#pragma unroll
+        for (int q = 0; q < M::RP; q++) {
+            const int r = M::row(q);
+            if (r < n) {
+                const double b0 = Br[r * LD + c], b1 = Bi[r * LD + c];
+                Br[r * LD + c] = b0 * ipr[r] - b1 * ipi[r];
+                Bi[r * LD + c] = b0 * ipi[r] + b1 * ipr[r];
+            }
+        }
     }
     __syncthreads();
-}
-
-// (Rr, Ri) <- (Rr, Ri) A^{-1} with A = L Ub from lu(): Z Ub = R, then Y L = Z
-template <int NMAX, int NT>
-__device__ inline void solve_right(const double *Ar, const double *Ai, double *Rr, double *Ri, int n)
-{
-    constexpr int LD = Cfg<NMAX, NT>::LD;
-    for (int k = 0; k < n - 1; k++) {  // columns j > k -= Z_:k Ub_kj, Z_:k = R_:k / Ub_kk
-        double ipr, ipi;
-        crecip(Ar[k * LD + k], Ai[k * LD + k], ipr, ipi);
-        const int m = n - 1 - k;
-        for (int e = threadIdx.x; e < n * m; e += NT) {
-            const int i = e / m, j = k + 1 + e % m;
-            const double zr0 = Rr[i * LD + k], zi0 = Ri[i * LD + k];
-            const double zr = zr0 * ipr - zi0 * ipi, zi = zr0 * ipi + zi0 * ipr;
-            const double ur = Ar[k * LD + j], ui = Ai[k * LD + j];
-            Rr[i * LD + j] -= zr * ur - zi * ui;
-            Ri[i * LD + j] -= zr * ui + zi * ur;
-        }
-        __syncthreads();
-    }
-    for (int e = threadIdx.x; e < n * n; e += NT) {
-        const int i = e / n, j = e % n;
-        double ipr, ipi;
-        crecip(Ar[j * LD + j], Ai[j * LD + j], ipr, ipi);
-        const double xr = Rr[i * LD + j], xi = Ri[i * LD + j];
-        Rr[i * LD + j] = xr * ipr - xi * ipi;
-        Ri[i * LD + j] = xr * ipi + xi * ipr;
-    }
-    __syncthreads();
-    for (int k = n - 1; k >= 1; k--) {  // columns j < k -= Y_:k L_kj, L_kj = A_kj / A_jj
-        const int m = k;
-        for (int e = threadIdx.x; e < n * m; e += NT) {
-            const int i = e / m, j = e % m;
-            double ipr, ipi;
-            crecip(Ar[j * LD + j], Ai[j * LD + j], ipr, ipi);
-            const double ar = Ar[k * LD + j], ai = Ai[k * LD + j];
-            const double lr = ar * ipr - ai * ipi, li = ar * ipi + ai * ipr;
-            const double yr = Rr[i * LD + k], yi = Ri[i * LD + k];
-            Rr[i * LD + j] -= yr * lr - yi * li;
-            Ri[i * LD + j] -= yr * li + yi * lr;
-        }
-        __syncthreads();
-    }
 }
 
 // MLP of the sub-layer's block: z = b2 + w2 . tanh(W1 f + b1)
 struct Mlp {
-    double f[kNF], hid[kHid], z;
+    double f[kNF], z;
 };
 
 __device__ inline void mlp_fwd(const float *w, Mlp &m)
@@ -321,8 +421,7 @@ __device__ inline void mlp_fwd(const float *w, Mlp &m)
         double a = (double)w[kHid * kNF + k];
 #pragma unroll
         for (int q = 0; q < kNF; q++) a += (double)w[k * kNF + q] * m.f[q];
-        m.hid[k] = tanh(a);
-        z += (double)w[kHid * kNF + kHid + k] * m.hid[k];
+        z += (double)w[kHid * kNF + kHid + k] * tanh(a);
     }
     m.z = z;
 }
@@ -337,26 +436,36 @@ __device__ inline void features(const Geom &g, const double *ptr, int mu, int nu
     f[2] = ptr[g.nbr[(2 * mu) * V + y]];
 }
 
-// Hermitian matrix of link l from the compact momentum layout -> LDS
+// Hermitian matrix of link l from the compact momentum layout -> LDS (the
+// loads of the thread issued first, as in ld)
 template <int NMAX, int NT>
 __device__ inline void ld_herm(const Geom &g, double *re, double *im, const double *gr, long l, int n)
 {
-    constexpr int LD = Cfg<NMAX, NT>::LD;
-    for (int e = threadIdx.x; e < n * n; e += NT) {
-        const int i = e / n, j = e % n;
-        double vr, vi;
-        if (i == j) {
-            vr = gr[moff(g, l, i, n)];
-            vi = 0.0;
-        } else {
-            const int lo = i < j ? i : j, hi = i < j ? j : i;
-            const int h = hup(n, lo, hi);
-            vr = gr[moff(g, l, h, n)];
-            vi = gr[moff(g, l, h + 1, n)];
-            if (i > j) vi = -vi;
+    constexpr int LD = Cfg<NMAX, NT>::LD, K = (NMAX * NMAX + NT - 1) / NT;
+    double vr[K], vi[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int e = threadIdx.x + k * NT;
+        vr[k] = vi[k] = 0.0;
+        if (e < n * n) {
+            const int i = e / n, j = e % n;
+            if (i == j) {
+                vr[k] = gr[moff(g, l, i, n)];
+            } else {
+                const int lo = i < j ? i : j, hi = i < j ? j : i;
+                const int h = hup(n, lo, hi);
+                vr[k] = gr[moff(g, l, h, n)];
+                vi[k] = i > j ? -gr[moff(g, l, h + 1, n)] : gr[moff(g, l, h + 1, n)];
+            }
         }
-        re[i * LD + j] = vr;
-        im[i * LD + j] = vi;
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int e = threadIdx.x + k * NT;
+        if (e < n * n) {
+            re[(e / n) * LD + e % n] = vr[k];
+            im[(e / n) * LD + e % n] = vi[k];
+        }
     }
 }
 
@@ -368,8 +477,8 @@ __device__ inline void put_herm(const Geom &g, const Own<NMAX, NT> &o, double *g
 {
 #pragma unroll
     for (int q = 0; q < Own<NMAX, NT>::EPL; q++) {
-        if (!o.ok[q] || o.i[q] > o.j[q]) continue;
-        const int i = o.i[q], j = o.j[q];
+        if (!o.ok(q) || o.i(q) > o.j(q)) continue;
+        const int i = o.i(q), j = o.j(q);
         if (i == j) {
             double *p = gr + moff(g, l, i, n);
             if constexpr (Add) unsafeAtomicAdd(p, s * c.r[q]);
@@ -396,7 +505,7 @@ __device__ inline void ah_of(const Own<NMAX, NT> &o, const double *re, const dou
     constexpr int LD = Cfg<NMAX, NT>::LD;
 #pragma unroll
     for (int q = 0; q < Own<NMAX, NT>::EPL; q++) {
-        const int a = o.i[q] * LD + o.j[q], b = o.j[q] * LD + o.i[q];
+        const int a = o.i(q) * LD + o.j(q), b = o.j(q) * LD + o.i(q);
         c.r[q] = 0.5 * (im[a] + im[b]);
         c.i[q] = -0.5 * (re[a] - re[b]);
     }
@@ -415,6 +524,7 @@ __global__ __launch_bounds__(NT) void k_ptr(Args a, int row)
     const long y = act_site(g, a.mu, row, blockIdx.x);
     Own<NMAX, NT> o;
     o.init(n);
+    zero_slots<NMAX, NT>(Ar, Ai, Br, Bi);
     // X = U_0(y) U_1(y+0), Y = U_1(y) U_0(y+1): ReTr P = Re sum X_ij conj(Y_ij)
     ld<NMAX, NT>(g, Ar, Ai, a.U, 0 * V + y, n);
     ld<NMAX, NT>(g, Br, Bi, a.U, 1 * V + g.nbr[0 * V + y], n);
@@ -429,7 +539,7 @@ __global__ __launch_bounds__(NT) void k_ptr(Args a, int row)
     double v[1] = {0.0};
 #pragma unroll
     for (int q = 0; q < Own<NMAX, NT>::EPL; q++)
-        if (o.ok[q]) v[0] += X.r[q] * Y.r[q] + X.i[q] * Y.i[q];
+        if (o.ok(q)) v[0] += X.r[q] * Y.r[q] + X.i[q] * Y.i[q];
     bsum<NT, 1>(v, red);
     if (threadIdx.x == 0) a.ptr[y] = v[0] / n;
 }
@@ -441,6 +551,7 @@ __global__ __launch_bounds__(NT) void k_forward(Args a)
     __shared__ double Ar[Cfg<NMAX, NT>::SZ], Ai[Cfg<NMAX, NT>::SZ], Br[Cfg<NMAX, NT>::SZ],
         Bi[Cfg<NMAX, NT>::SZ];
     __shared__ double red[Cfg<NMAX, NT>::NW * 5];
+    __shared__ double Pr[NMAX], Pi[NMAX];  // pivot reciprocals of the LU
     const Geom &g = a.g;
     const int n = g.N, mu = a.mu, nu = 1 - mu;
     const long V = g.V, b = blockIdx.x;
@@ -453,6 +564,7 @@ __global__ __launch_bounds__(NT) void k_forward(Args a)
     const double t = a.tmax * tanh(m.z);
     Own<NMAX, NT> o;
     o.init(n);
+    zero_slots<NMAX, NT>(Ar, Ai, Br, Bi);
     Reg<NMAX, NT> T, S, W;
     // S = U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+
     ld<NMAX, NT>(g, Ar, Ai, a.U, (long)nu * V + xp, n);
@@ -467,10 +579,7 @@ __global__ __launch_bounds__(NT) void k_forward(Args a)
     __syncthreads();
     st(o, Br, Bi, S);
     ld<NMAX, NT>(g, Ar, Ai, a.U, l, n);
-    if (a.log) {
-        double *lg = a.log + (size_t)b * 2 * n * n;
-        for (int r = threadIdx.x; r < 2 * n * n; r += NT) lg[r] = a.U[loff(g, l, r, n)];
-    }
+    if (a.log) copy_link<NMAX, NT, true>(g, a.log + (size_t)b * 2 * n * n, a.U, l, n);
     __syncthreads();
     mm<false, false>(o, Ar, Ai, Br, Bi, n, W);  // W = U S
     __syncthreads();
@@ -483,9 +592,9 @@ __global__ __launch_bounds__(NT) void k_forward(Args a)
     double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int q = 0; q < EPL; q++) {
-        if (!o.ok[q]) continue;
-        const int tr = o.j[q] * LD + o.i[q];
-        if (o.i[q] == o.j[q]) {
+        if (!o.ok(q)) continue;
+        const int tr = o.j(q) * LD + o.i(q);
+        if (o.i(q) == o.j(q)) {
             v[0] += W.r[q];
             v[1] += W.i[q];
         }
@@ -497,17 +606,24 @@ __global__ __launch_bounds__(NT) void k_forward(Args a)
     // B = 1 - t W into A, W - t into B, then h = B^{-1} (W - t) into B
 #pragma unroll
     for (int q = 0; q < EPL; q++)
-        if (o.ok[q]) {
-            const int e = o.i[q] * LD + o.j[q];
-            const double d = o.i[q] == o.j[q] ? 1.0 : 0.0;
+        if (o.ok(q)) {
+            const int e = o.i(q) * LD + o.j(q);
+            const double d = o.i(q) == o.j(q) ? 1.0 : 0.0;
             Ar[e] = d - t * W.r[q];
             Ai[e] = -t * W.i[q];
-            Br[e] = W.r[q] - t * d;
-            Bi[e] = W.i[q];
+            Br[e] = d;
+            Bi[e] = 0.0;
         }
     __syncthreads();
-    lu<NMAX, NT>(Ar, Ai, n);
-    solve_left<NMAX, NT>(Ar, Ai, Br, Bi, n);
+    lu<NMAX, NT>(Ar, Ai, Pr, Pi, n);
+    solve_left<NMAX, NT>(Ar, Ai, Pr, Pi, Br, Bi, n);  // B^{-1} in slot B
+    if (a.inv) {  // kept for the backward
+        double *iv = a.inv + (size_t)b * 2 * n * n;
+        for (int e = threadIdx.x; e < n * n; e += NT) {
+            iv[2 * e] = Br[(e / n) * LD + e % n];
+            iv[2 * e + 1] = Bi[(e / n) * LD + e % n];
+        }
+    }
     double pv[2] = {0.0, 0.0};
     if ((int)threadIdx.x < n) {
         const double pr = Ar[threadIdx.x * LD + threadIdx.x], pi = Ai[threadIdx.x * LD + threadIdx.x];
@@ -521,14 +637,25 @@ __global__ __launch_bounds__(NT) void k_forward(Args a)
     const double phi = pv[1] + twopi * rint((est - pv[1]) / twopi);
     double cr, ci;
     sincos(2.0 * phi / n, &ci, &cr);
-    // U' = e^{2 i phi / N} h S^+
+    // h = B^{-1} (W - t), then U' = e^{2 i phi / N} h S^+
+#pragma unroll
+    for (int q = 0; q < EPL; q++)
+        if (o.ok(q)) {
+            const int e = o.i(q) * LD + o.j(q);
+            Ar[e] = W.r[q] - (o.i(q) == o.j(q) ? t : 0.0);
+            Ai[e] = W.i[q];
+        }
+    __syncthreads();
+    mm<false, false>(o, Br, Bi, Ar, Ai, n, T);
+    __syncthreads();
+    st(o, Br, Bi, T);
     st(o, Ar, Ai, S);
     __syncthreads();
     mm<false, true>(o, Br, Bi, Ar, Ai, n, T);
 #pragma unroll
     for (int q = 0; q < EPL; q++)
-        if (o.ok[q]) {
-            const int e = o.i[q] * n + o.j[q];
+        if (o.ok(q)) {
+            const int e = o.i(q) * n + o.j(q);
             a.U[loff(g, l, 2 * e, n)] = cr * T.r[q] - ci * T.i[q];
             a.U[loff(g, l, 2 * e + 1, n)] = cr * T.i[q] + ci * T.r[q];
         }
@@ -546,6 +673,7 @@ __global__ __launch_bounds__(NT) void k_backward(Args a)
     __shared__ double Ar[Cfg<NMAX, NT>::SZ], Ai[Cfg<NMAX, NT>::SZ], Br[Cfg<NMAX, NT>::SZ],
         Bi[Cfg<NMAX, NT>::SZ];
     __shared__ double red[Cfg<NMAX, NT>::NW * 5];
+    __shared__ double Pr[NMAX], Pi[NMAX];  // pivot reciprocals of the LU
     const Geom &g = a.g;
     const int n = g.N, mu = a.mu, nu = 1 - mu;
     const long V = g.V, b = blockIdx.x;
@@ -559,32 +687,9 @@ __global__ __launch_bounds__(NT) void k_backward(Args a)
     const double th = tanh(m.z), t = a.tmax * th, t2 = t * t;
     Own<NMAX, NT> o;
     o.init(n);
-    Reg<NMAX, NT> T, S, W, Gp, EGE, GL;
-    // E^+ G' E = U U'^+ G' U' U^+ (U' = the current link, U = its input)
-    ld<NMAX, NT>(g, Ar, Ai, a.U, l, n);
-    ld_herm<NMAX, NT>(g, Br, Bi, a.gr, l, n);
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < EPL; q++) {
-        Gp.r[q] = Br[o.i[q] * LD + o.j[q]];
-        Gp.i[q] = Bi[o.i[q] * LD + o.j[q]];
-    }
-    mm<true, false>(o, Ar, Ai, Br, Bi, n, T);
-    __syncthreads();
-    st(o, Br, Bi, T);
-    __syncthreads();
-    mm<false, false>(o, Br, Bi, Ar, Ai, n, T);
-    __syncthreads();
-    st(o, Br, Bi, T);
-    ld_log<NMAX, NT>(Ar, Ai, uin, n);
-    __syncthreads();
-    mm<false, false>(o, Ar, Ai, Br, Bi, n, T);
-    __syncthreads();
-    st(o, Br, Bi, T);
-    __syncthreads();
-    mm<false, true>(o, Br, Bi, Ar, Ai, n, EGE);
-    __syncthreads();
-    // S, W of the input configuration
+    zero_slots<NMAX, NT>(Ar, Ai, Br, Bi);
+    Reg<NMAX, NT> T, W, X, GL;
+    // S = U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+ and W = U S of the input configuration
     ld<NMAX, NT>(g, Ar, Ai, a.U, (long)nu * V + xp, n);
     ld<NMAX, NT>(g, Br, Bi, a.U, (long)mu * V + xpn, n);
     __syncthreads();
@@ -593,9 +698,9 @@ __global__ __launch_bounds__(NT) void k_backward(Args a)
     st(o, Ar, Ai, T);
     ld<NMAX, NT>(g, Br, Bi, a.U, (long)nu * V + s, n);
     __syncthreads();
-    mm<false, true>(o, Ar, Ai, Br, Bi, n, S);
+    mm<false, true>(o, Ar, Ai, Br, Bi, n, T);
     __syncthreads();
-    st(o, Br, Bi, S);
+    st(o, Br, Bi, T);
     ld_log<NMAX, NT>(Ar, Ai, uin, n);
     __syncthreads();
     mm<false, false>(o, Ar, Ai, Br, Bi, n, W);
@@ -606,37 +711,40 @@ __global__ __launch_bounds__(NT) void k_backward(Args a)
     double v[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int q = 0; q < EPL; q++)
-        if (o.ok[q] && o.i[q] == o.j[q]) {
+        if (o.ok(q) && o.i(q) == o.j(q)) {
             v[0] += W.r[q];
             v[1] += W.i[q];
         }
     __syncthreads();
+    // h = B^{-1} (W - t) with B^{-1} kept by the forward (slot A)
+    ld_log<NMAX, NT>(Ar, Ai, a.inv + (size_t)b * 2 * n * n, n);
 #pragma unroll
     for (int q = 0; q < EPL; q++)
-        if (o.ok[q]) {
-            const int e = o.i[q] * LD + o.j[q];
-            const double d = o.i[q] == o.j[q] ? 1.0 : 0.0;
-            Ar[e] = d - t * W.r[q];
-            Ai[e] = -t * W.i[q];
-            Br[e] = W.r[q] - t * d;
+        if (o.ok(q)) {
+            const int e = o.i(q) * LD + o.j(q);
+            Br[e] = W.r[q] - (o.i(q) == o.j(q) ? t : 0.0);
             Bi[e] = W.i[q];
         }
     __syncthreads();
-    lu<NMAX, NT>(Ar, Ai, n);
-    solve_left<NMAX, NT>(Ar, Ai, Br, Bi, n);  // h in B
-    // Re Tr h, Im Tr h, Tr(G' AH(h)); g_L
-    double tga = 0.0;
-    Reg<NMAX, NT> AHh;
-    ah_of(o, Br, Bi, AHh);
+    mm<false, false>(o, Ar, Ai, Br, Bi, n, X);
+    __syncthreads();
+    st(o, Br, Bi, X);
+    __syncthreads();
+    // AH(h), Re Tr h, Im Tr h; then G' into B and Tr(G' AH(h))
+    ah_of(o, Br, Bi, X);
 #pragma unroll
     for (int q = 0; q < EPL; q++)
-        if (o.ok[q]) {
-            tga += Gp.r[q] * AHh.r[q] + Gp.i[q] * AHh.i[q];
-            if (o.i[q] == o.j[q]) {
-                v[2] += Br[o.i[q] * LD + o.i[q]];
-                v[3] += Bi[o.i[q] * LD + o.i[q]];
-            }
+        if (o.ok(q) && o.i(q) == o.j(q)) {
+            v[2] += Br[o.i(q) * LD + o.i(q)];
+            v[3] += Bi[o.i(q) * LD + o.i(q)];
         }
+    __syncthreads();
+    ld_herm<NMAX, NT>(g, Br, Bi, a.gr, l, n);
+    __syncthreads();
+    double tga = 0.0;
+#pragma unroll
+    for (int q = 0; q < EPL; q++)
+        if (o.ok(q)) tga += Br[o.i(q) * LD + o.j(q)] * X.r[q] + Bi[o.i(q) * LD + o.j(q)] * X.i[q];
     {
         double w5[5] = {v[0], v[1], v[2], v[3], tga};
         bsum<NT, 5>(w5, red);
@@ -646,44 +754,57 @@ __global__ __launch_bounds__(NT) void k_backward(Args a)
     const double ca = -n * t / (1.0 - t2), cb = t / (n * mq);
 #pragma unroll
     for (int q = 0; q < EPL; q++) {
-        GL.r[q] = ca * AHh.r[q] + cb * GL.r[q];
-        GL.i[q] = ca * AHh.i[q] + cb * GL.i[q];
+        GL.r[q] = ca * X.r[q] + cb * GL.r[q];
+        GL.i[q] = ca * X.i[q] + cb * GL.i[q];
     }
     const double trgl = ca * v[3] + cb * v[1];
-    // g1 = B^{-+} G' B^{-1}: R = G' B^{-1}, then (R^+) B^{-1}
-    st(o, Br, Bi, Gp);
-    __syncthreads();
-    solve_right<NMAX, NT>(Ar, Ai, Br, Bi, n);
-#pragma unroll
-    for (int q = 0; q < EPL; q++) {
-        T.r[q] = Br[o.j[q] * LD + o.i[q]];
-        T.i[q] = -Bi[o.j[q] * LD + o.i[q]];
-    }
+    // g1 = B^{-+} (G' B^{-1})
+    mm<false, false>(o, Br, Bi, Ar, Ai, n, T);
     __syncthreads();
     st(o, Br, Bi, T);
     __syncthreads();
-    solve_right<NMAX, NT>(Ar, Ai, Br, Bi, n);
+    mm<true, false>(o, Ar, Ai, Br, Bi, n, T);
     double tr1[1] = {0.0};
 #pragma unroll
-    for (int q = 0; q < EPL; q++) {
-        T.r[q] = Br[o.i[q] * LD + o.j[q]];
-        T.i[q] = Bi[o.i[q] * LD + o.j[q]];
-        if (o.ok[q] && o.i[q] == o.j[q]) tr1[0] += T.r[q];
-    }
+    for (int q = 0; q < EPL; q++)
+        if (o.ok(q) && o.i(q) == o.j(q)) tr1[0] += T.r[q];
     bsum<NT, 1>(tr1, red);
     const double c = a.c_logj, omt = 1.0 - t2;
     const double trU = (omt * tr1[0] + c * trgl) / n;
-    // g_U (T), g_s (EGE <- g_U - E^+ G' E)
+    // g_U in T
 #pragma unroll
     for (int q = 0; q < EPL; q++) {
-        T.r[q] = omt * T.r[q] + c * GL.r[q] - (o.i[q] == o.j[q] ? trU : 0.0);
+        T.r[q] = omt * T.r[q] + c * GL.r[q] - (o.i(q) == o.j(q) ? trU : 0.0);
         T.i[q] = omt * T.i[q] + c * GL.i[q];
-        EGE.r[q] = T.r[q] - EGE.r[q];
-        EGE.i[q] = T.i[q] - EGE.i[q];
+    }
+    // E^+ G' E = U U'^+ G' U' U^+ (U' = the current link, U = its input), then
+    // g_s = g_U - E^+ G' E into X
+    ld<NMAX, NT>(g, Ar, Ai, a.U, l, n);
+    ld_herm<NMAX, NT>(g, Br, Bi, a.gr, l, n);
+    __syncthreads();
+    mm<true, false>(o, Ar, Ai, Br, Bi, n, X);
+    __syncthreads();
+    st(o, Br, Bi, X);
+    __syncthreads();
+    mm<false, false>(o, Br, Bi, Ar, Ai, n, X);
+    __syncthreads();
+    st(o, Br, Bi, X);
+    ld_log<NMAX, NT>(Ar, Ai, uin, n);
+    __syncthreads();
+    mm<false, false>(o, Ar, Ai, Br, Bi, n, X);
+    __syncthreads();
+    st(o, Br, Bi, X);
+    __syncthreads();
+    mm<false, true>(o, Br, Bi, Ar, Ai, n, X);
+#pragma unroll
+    for (int q = 0; q < EPL; q++) {
+        X.r[q] = T.r[q] - X.r[q];
+        X.i[q] = T.i[q] - X.i[q];
     }
     put_herm<false>(g, o, a.gr, l, n, T, 1.0);
+    __syncthreads();
     // U_nu(x+mu) += U^+ g_s U
-    st(o, Br, Bi, EGE);
+    st(o, Br, Bi, X);
     ld_log<NMAX, NT>(Ar, Ai, uin, n);
     __syncthreads();
     mm<true, false>(o, Ar, Ai, Br, Bi, n, T);
@@ -695,7 +816,7 @@ __global__ __launch_bounds__(NT) void k_backward(Args a)
     __syncthreads();
     // U_nu(x) += -W^+ g_s W
     st(o, Ar, Ai, W);
-    st(o, Br, Bi, EGE);
+    st(o, Br, Bi, X);
     __syncthreads();
     mm<true, false>(o, Ar, Ai, Br, Bi, n, T);
     __syncthreads();
@@ -710,7 +831,7 @@ __global__ __launch_bounds__(NT) void k_backward(Args a)
     mm<false, false>(o, Ar, Ai, Br, Bi, n, T);
     __syncthreads();
     st(o, Ar, Ai, T);
-    st(o, Br, Bi, EGE);
+    st(o, Br, Bi, X);
     __syncthreads();
     mm<true, false>(o, Ar, Ai, Br, Bi, n, T);
     __syncthreads();
@@ -719,7 +840,7 @@ __global__ __launch_bounds__(NT) void k_backward(Args a)
     mm<false, false>(o, Br, Bi, Ar, Ai, n, T);
     put_herm<true>(g, o, a.gr, (long)mu * V + xpn, n, T, -1.0);
     // restore the input link
-    for (int r = threadIdx.x; r < 2 * n * n; r += NT) a.U[loff(g, l, r, n)] = uin[r];
+    copy_link<NMAX, NT, false>(g, a.U, uin, l, n);
     // dPhi/dt, MLP backward: weight gradient and dPhi/df of the features
     const double dl = -2.0 * t * n * n / omt + 2.0 * n * (v[2] + n * t) / omt + (2.0 * t - 2.0 / n * v[0]) / mq +
                       2.0 * t / omt;
@@ -727,9 +848,13 @@ __global__ __launch_bounds__(NT) void k_backward(Args a)
     const double dz = dt * a.tmax * (1.0 - th * th);
     if (threadIdx.x == 0) {
         double df[kNF] = {0.0, 0.0, 0.0};
+#pragma unroll 1
+        for (int k = 0; k < kHid; k++) {  // hidden units recomputed (no live array of them)
+            double pre = (double)a.w[kHid * kNF + k];
 #pragma unroll
-        for (int k = 0; k < kHid; k++) {
-            const double dp = dz * (double)a.w[kHid * kNF + kHid + k] * (1.0 - m.hid[k] * m.hid[k]);
+            for (int q = 0; q < kNF; q++) pre += (double)a.w[k * kNF + q] * m.f[q];
+            const double hk = tanh(pre);
+            const double dp = dz * (double)a.w[kHid * kNF + kHid + k] * (1.0 - hk * hk);
 #pragma unroll
             for (int q = 0; q < kNF; q++) df[q] += dp * (double)a.w[k * kNF + q];
         }
@@ -772,11 +897,11 @@ __device__ inline void put_ta(const Geom &g, const Own<NMAX, NT> &o, double *re,
     double tr[1] = {0.0};
 #pragma unroll
     for (int q = 0; q < EPL; q++)
-        if (o.ok[q] && o.i[q] == o.j[q]) tr[0] += P.i[q];
+        if (o.ok(q) && o.i(q) == o.j(q)) tr[0] += P.i[q];
     bsum<NT, 1>(tr, red);  // ends with a barrier: (re, im) free again
 #pragma unroll
     for (int q = 0; q < EPL; q++)
-        if (o.i[q] == o.j[q]) H.r[q] -= tr[0] / n;
+        if (o.i(q) == o.j(q)) H.r[q] -= tr[0] / n;
     put_herm<true>(g, o, gr, l, n, H, co);
 }
 
@@ -794,6 +919,7 @@ __global__ __launch_bounds__(NT) void k_featgrad(Args a, int row)
     const long la = y, lb = V + g.nbr[y], lc = g.nbr[2 * V + y], ld_ = V + y;
     Own<NMAX, NT> o;
     o.init(n);
+    zero_slots<NMAX, NT>(Ar, Ai, Br, Bi);
     Reg<NMAX, NT> AB, DC, P, Q;
     ld<NMAX, NT>(g, Ar, Ai, a.U, la, n);
     ld<NMAX, NT>(g, Br, Bi, a.U, lb, n);
@@ -850,7 +976,7 @@ hipError_t dispatch(int n, F f)
     if (n <= 4) return f.template run<4, 64>();
     if (n <= 8) return f.template run<8, 64>();
     if (n <= 16) return f.template run<16, 64>();
-    if (n <= 32) return f.template run<32, 256>();
+    if (n <= 32) return f.template run<32, 128>();
     return f.template run<64, 512>();
 }
 
@@ -902,7 +1028,7 @@ struct LaunchFeat {
 long sflow_active(const Geom &g) { return g.V / 4; }
 
 hipError_t launch_sflow_forward(const Geom &g, void *U, int mu, int off, const float *w, double tmax, double *ptr,
-                                double *log, double *part, hipStream_t st)
+                                double *log, double *inv, double *part, hipStream_t st)
 {
     sflow::Args a{};
     a.g = g;
@@ -913,6 +1039,7 @@ hipError_t launch_sflow_forward(const Geom &g, void *U, int mu, int off, const f
     a.tmax = tmax;
     a.ptr = ptr;
     a.log = log;
+    a.inv = inv;
     a.part = part;
     hipError_t e = sflow::dispatch(g.N, sflow::LaunchPtr{a, (off + 1) % 4, st});
     if (e != hipSuccess) return e;
@@ -920,7 +1047,8 @@ hipError_t launch_sflow_forward(const Geom &g, void *U, int mu, int off, const f
 }
 
 hipError_t launch_sflow_backward(const Geom &g, void *U, int mu, int off, const float *w, double tmax, double *ptr,
-                                 double *log, double *gr, double c_logj, double *gw, double *dF, hipStream_t st)
+                                 double *log, double *inv, double *gr, double c_logj, double *gw, double *dF,
+                                 hipStream_t st)
 {
     sflow::Args a{};
     a.g = g;
@@ -931,6 +1059,7 @@ hipError_t launch_sflow_backward(const Geom &g, void *U, int mu, int off, const 
     a.tmax = tmax;
     a.ptr = ptr;
     a.log = log;
+    a.inv = inv;
     a.gr = gr;
     a.c_logj = c_logj;
     a.gw = gw;

@@ -197,16 +197,18 @@ hipError_t launch_kick_g(const Geom &g, void *P, const double *gr, double eps_c,
 // updated links per sub-layer: V / 4 (rows x_nu = off mod 4)
 long sflow_active(const Geom &g);
 // one sub-layer (mu, off) in place on U (FP64): feature plaquettes into ptr
-// (V doubles), the updated links' inputs into log (2 N^2 per updated link,
-// may be null), log J per updated link into part ((s, c) pairs)
+// (V doubles), the updated links' inputs into log and (1 - tW)^{-1} into inv
+// (2 N^2 per updated link each, both may be null), log J per updated link
+// into part ((s, c) pairs)
 hipError_t launch_sflow_forward(const Geom &g, void *U, int mu, int off, const float *w, double tmax, double *ptr,
-                                double *log, double *part, hipStream_t st);
+                                double *log, double *inv, double *part, hipStream_t st);
 // backward of one sub-layer: gr (momentum layout) from the gradient at its
 // output to the gradient at its input, c_logj d log J included; gw (may be
 // null) += the MLP weight gradient; restores the input links from log into
-// U.  ptr, dF: V doubles of scratch.
+// U; inv: what the forward kept.  ptr, dF: V doubles of scratch.
 hipError_t launch_sflow_backward(const Geom &g, void *U, int mu, int off, const float *w, double tmax, double *ptr,
-                                 double *log, double *gr, double c_logj, double *gw, double *dF, hipStream_t st);
+                                 double *log, double *inv, double *gr, double c_logj, double *gw, double *dF,
+                                 hipStream_t st);
 // y += s x (n doubles)
 hipError_t launch_axpy(double *y, const double *x, double s, long n, hipStream_t st);
 
