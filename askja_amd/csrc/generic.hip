@@ -38,6 +38,12 @@ ASKJA_TEAM_DECL(64)
 hipError_t mfma_kick_16(const KickArgs &a, int mode, bool drift, hipStream_t st);
 hipError_t mfma_kick_32(const KickArgs &a, int mode, bool drift, hipStream_t st);
 hipError_t mfma_kick_64(const KickArgs &a, int mode, bool drift, hipStream_t st);
+hipError_t mfma_drift_16(const Geom &g, void *U, const void *P, double eps, int level, const double *cinv, int mode,
+                         hipStream_t st);
+hipError_t mfma_drift_32(const Geom &g, void *U, const void *P, double eps, int level, const double *cinv, int mode,
+                         hipStream_t st);
+hipError_t mfma_drift_64(const Geom &g, void *U, const void *P, double eps, int level, const double *cinv, int mode,
+                         hipStream_t st);
 hipError_t rows_kick(const KickArgs &a, int mode, bool drift, hipStream_t st);
 long rows_kick_blocks(const Geom &g);
 hipError_t mfma_action_16(const KickArgs &a, bool f32_links, hipStream_t st);
@@ -430,6 +436,13 @@ hipError_t launch_drift(const Geom &g, void *U, const void *P, double eps, int l
     if (mode == kModeBF16x3) mode = kModeF32;
     if (n <= 3) return small_drift(g, U, P, eps, level, cinv, mode, st);
     if (n <= 8) return team_drift_8(g, U, P, eps, level, cinv, mode, st);
+    // N >= 9: the Horner products on MFMA, except the exact mode (the
+    // reference's operation order on the VALU)
+    if (mode != kModeF64Exact) {
+        if (n <= 16) return mfma_drift_16(g, U, P, eps, level, cinv, mode, st);
+        if (n <= 32) return mfma_drift_32(g, U, P, eps, level, cinv, mode, st);
+        return mfma_drift_64(g, U, P, eps, level, cinv, mode, st);
+    }
     if (n <= 16) return team_drift_16(g, U, P, eps, level, cinv, mode, st);
     if (n <= 32) return team_drift_32(g, U, P, eps, level, cinv, mode, st);
     return team_drift_64(g, U, P, eps, level, cinv, mode, st);

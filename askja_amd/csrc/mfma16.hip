@@ -12,6 +12,16 @@ hipError_t mfma_kick_16(const KickArgs &a, int mode, bool drift, hipStream_t st)
     }
 }
 
+hipError_t mfma_drift_16(const Geom &g, void *U, const void *P, double eps, int level, const double *cinv, int mode,
+                         hipStream_t st)
+{
+    switch (mode) {
+    case kModeF64: return mfma::drift<mfma::kF64, 16>(g, U, P, eps, level, cinv, st);
+    case kModeF32: return mfma::drift<mfma::kF32, 16>(g, U, P, eps, level, cinv, st);
+    default: return mfma::drift<mfma::kBF16X3, 16>(g, U, P, eps, level, cinv, st);
+    }
+}
+
 hipError_t mfma_action_16(const KickArgs &a, bool f32_links, hipStream_t st)
 {
     return mfma::action_f64<16>(a, f32_links, st);

@@ -12,6 +12,16 @@ hipError_t mfma_kick_64(const KickArgs &a, int mode, bool drift, hipStream_t st)
     }
 }
 
+hipError_t mfma_drift_64(const Geom &g, void *U, const void *P, double eps, int level, const double *cinv, int mode,
+                         hipStream_t st)
+{
+    switch (mode) {
+    case kModeF64: return mfma::drift<mfma::kF64, 64>(g, U, P, eps, level, cinv, st);
+    case kModeF32: return mfma::drift<mfma::kF32, 64>(g, U, P, eps, level, cinv, st);
+    default: return mfma::drift<mfma::kBF16X3, 64>(g, U, P, eps, level, cinv, st);
+    }
+}
+
 hipError_t mfma_action_64(const KickArgs &a, bool f32_links, hipStream_t st)
 {
     return mfma::action_f64<64>(a, f32_links, st);

@@ -684,6 +684,101 @@ __global__ __launch_bounds__((Cfg<P, NMAX>::NT), (kMinWaves<P, NMAX>)) void k_ki
     if (a.part) block_reduce_store<3>(sum, cmp, a.part + 6 * (size_t)lb);
 }
 
+// Standalone drift U <- T_L(i eps P) U of every link in place (the Horner
+// loop of the fused kernel on the MFMA units; one workgroup per link, which
+// reads and writes its own link only).
+template <int P, int NMAX>
+__global__ __launch_bounds__((Cfg<P, NMAX>::NT), (kMinWaves<P, NMAX>)) void k_drift(Geom g, void *Uv, const void *Pv,
+                                                                                     double eps, int level,
+                                                                                     Cinv9 cinv)
+{
+    typedef typename Pol<P>::R R;
+    constexpr int NCT = Cfg<P, NMAX>::NCT;
+    __shared__ __align__(16) R Are[Cfg<P, NMAX>::SLOT];
+    __shared__ __align__(16) R Aim[Cfg<P, NMAX>::SLOT];
+    __shared__ __align__(16) R Bre[Cfg<P, NMAX>::SLOT];
+    __shared__ __align__(16) R Bim[Cfg<P, NMAX>::SLOT];
+    const int n = g.N;
+    const long l = blockIdx.x;
+    R *U = (R *)Uv;
+    const R *Pm = (const R *)Pv;
+    const R ed = (R)eps;
+    if (level < 0 || level > 8) level = 8;
+    R pr[NCT][4], pim[NCT][4];
+#pragma unroll
+    for (int t = 0; t < NCT; t++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int i = acc_row<P, NMAX>(q), j = acc_col<P, NMAX>(t);
+            pr[t][q] = pim[t][q] = R(0);
+            if (i < n && j < n) pr[t][q] = mom_el(Pm, l, n, i, j, pim[t][q]);
+        }
+    // rhs = i eps P into slot A (zero outside n x n); X = 1 + c_L rhs;
+    // Horner steps X = 1 + c_k (rhs X) (sunmat.h:200-248)
+#pragma unroll
+    for (int t = 0; t < NCT; t++)
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            put<P, NMAX, false, R>(Are, Aim, acc_row<P, NMAX>(q), acc_col<P, NMAX>(t), -(pim[t][q] * ed),
+                                   pr[t][q] * ed);
+    Acc<P, NMAX> X;
+    const R cl = level <= 1 ? R(1) : (R)cinv.c[level];
+#pragma unroll
+    for (int t = 0; t < NCT; t++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int i = acc_row<P, NMAX>(q), j = acc_col<P, NMAX>(t);
+            const R d = (i == j && i < n) ? R(1) : R(0);
+            X.r[t][q] = level == 0 ? d : d - cl * (pim[t][q] * ed);
+            X.i[t][q] = level == 0 ? R(0) : cl * (pr[t][q] * ed);
+        }
+    for (int k = level - 1; k >= 1; k--) {
+        __syncthreads();
+        acc_store<true>(X, Bre, Bim);
+        __syncthreads();
+        Acc<P, NMAX> Q;
+        acc_zero(Q);
+        cmul<P, NMAX, false, false>(Are, Aim, Bre, Bim, Q);
+        const R ck = (R)cinv.c[k];
+#pragma unroll
+        for (int t = 0; t < NCT; t++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int i = acc_row<P, NMAX>(q), j = acc_col<P, NMAX>(t);
+                X.r[t][q] = ((i == j && i < n) ? R(1) : R(0)) + ck * Q.r[t][q];
+                X.i[t][q] = ck * Q.i[t][q];
+            }
+    }
+    __syncthreads();
+    acc_store<false>(X, Bre, Bim);
+    ld_link<P, NMAX, true>(Are, Aim, U, l, n);
+    __syncthreads();
+    Acc<P, NMAX> Un;
+    acc_zero(Un);
+    cmul<P, NMAX, false, false>(Bre, Bim, Are, Aim, Un);
+    R *ob = U + (size_t)l * 2 * n * n;
+#pragma unroll
+    for (int t = 0; t < NCT; t++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int i = acc_row<P, NMAX>(q), j = acc_col<P, NMAX>(t);
+            if (i < n && j < n) {
+                ob[2 * (i * n + j)] = Un.r[t][q];
+                ob[2 * (i * n + j) + 1] = Un.i[t][q];
+            }
+        }
+}
+
+template <int P, int NMAX>
+hipError_t drift(const Geom &g, void *U, const void *Pm, double eps, int level, const double *cinv, hipStream_t st)
+{
+    Cinv9 c;
+    for (int k = 0; k < 9; k++) c.c[k] = cinv[k];
+    hipLaunchKernelGGL((k_drift<P, NMAX>), dim3((unsigned)(g.V * g.D)), dim3(Cfg<P, NMAX>::NT), 0, st, g, U, Pm, eps,
+                       level, c);
+    return hipGetLastError();
+}
+
 // sum ReTr W in FP64 arithmetic over FP32 (f32_links) or FP64 links
 // (partials as the kick's)
 template <int NMAX>

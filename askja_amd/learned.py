@@ -111,10 +111,13 @@ def plain_chain(lat, beta, dt, ndrift, level, seed, first, n, measure=True, plaq
     the plaquette series is appended to `plaq` when given."""
     series, acc = [], 0
     lat.sync()
-    t0 = time.perf_counter()
+    t0 = tl = time.perf_counter()
     for k in range(n):
         r = lat.hmc_trajectory(beta, dt, ndrift, level, seed, first + k, reunit=True)
         acc += r.accepted
+        if time.perf_counter() - tl > 30:  # progress of slow chains
+            tl = time.perf_counter()
+            print(f"[learned] plain trajectory {k + 1}/{n} ({tl - t0:.0f} s)", file=sys.stderr, flush=True)
         if measure:
             series.append(abs(lat.polyakov()))
             if plaq is not None:
@@ -137,15 +140,17 @@ def flow_chain(lat, beta, dt, ndrift, level, seed, first, n, log=None, plaq=None
     the physical configuration U = F(V)."""
     series, acc = [], 0
     lat.sync()
-    t0 = time.perf_counter()
+    t0 = tl = time.perf_counter()
     for k in range(n):
         r = lat.flow_hmc_trajectory(beta, dt, ndrift, level, seed, first + k, reunit=True)
         acc += r.accepted
         series.append(abs(lat.flow_polyakov()))
         if plaq is not None:
             plaq.append(1.0 - r.action / lat.n_plaquettes())
-        if log and (k + 1) % 50 == 0:
-            log(f"[learned] flow trajectory {k + 1}/{n} ({time.perf_counter() - t0:.1f} s)")
+        now = time.perf_counter()
+        if log and ((k + 1) % 50 == 0 or now - tl > 30):
+            tl = now
+            log(f"[learned] flow trajectory {k + 1}/{n} ({now - t0:.1f} s, acceptance {acc / (k + 1):.2f})")
     lat.sync()
     return np.array(series), acc / max(n, 1), time.perf_counter() - t0
 
@@ -189,6 +194,99 @@ def train(lat, configs, beta, w0, epochs=2, lr=2e-3, log=None):
     return w.detach().cpu().numpy(), losses
 
 
+def c5_study(args, log):
+    """The two-dimensional large-N workload: SU(64), 128^2, beta = N^2 = 4096
+    (the Gross-Witten point), DT 0.005, LEVEL 8.  A Haar start is
+    thermalised by BF16x3 HMC (exact FP64 Metropolis); the spectral map (one
+    sweep: both directions, four row offsets) is trained on configurations of
+    plain HMC at beta_train, one Adam step per configuration; then plain FP64
+    HMC, BF16x3 HMC and FTHMC with the trained map run from the same start.
+    Each chain's plaquette is compared with the exact single-plaquette value
+    of two-dimensional SU(64) (tests/golden/exact_2d_plaquette.json)."""
+    import torch
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    gold = json.load(open(os.path.join(root, "tests", "golden", "exact_2d_plaquette.json")))
+    n, ext, beta = 64, [128, 128], 4096.0
+    want = [p["plaquette"] for p in gold["points"] if p["nc"] == n and p["beta"] == beta][0]
+    dt, nd, level, seed = 0.005, 200, 8, args.seed
+    lb = A.Lattice(n, ext, A.BF16X3)
+    lb.haar(seed, 7)
+    t0 = time.perf_counter()
+    for t in range(args.therm):
+        r = lb.hmc_trajectory(beta, dt, nd, level, seed, t)
+        if (t + 1) % 10 == 0:
+            log(f"[c5] thermalising {t + 1}/{args.therm}: plaquette {1.0 - r.action / lb.n_plaquettes():.7f} "
+                f"({time.perf_counter() - t0:.0f} s)")
+    start = lb.links()
+    lf = A.Lattice(n, ext, A.FP64)
+    mu, off, blk = spectral_masks(args.layers)
+    tmax = args.rho_max if args.rho_max is not None else 0.4
+    bsize = lf.flow_define(mu, off, blk, args.layers, tmax, kind="spectral")
+    w0 = init_weights(args.layers, bsize, 3, seed=seed, scale=0.1)
+    # training, one configuration at a time (2.1 GB each on the host)
+    fa = flow_action_function()
+    w = torch.tensor(w0, device="cuda", dtype=torch.float32, requires_grad=True)
+    opt = torch.optim.Adam([w], lr=args.lr)
+    losses = []
+    t0 = time.perf_counter()
+    for k in range(args.ntrain):
+        for j in range(2):
+            lb.hmc_trajectory(args.beta_train, dt, nd, level, seed, 10**6 + 2 * k + j)
+        lf.set_links(lb.links())
+        opt.zero_grad()
+        loss = fa.apply(w, lf, beta)
+        loss.backward()
+        opt.step()
+        losses.append(float(loss.detach()))
+        log(f"[c5] training step {k + 1}/{args.ntrain}: loss {losses[-1]:.3f}")
+    t_train = time.perf_counter() - t0
+    wt = w.detach().cpu().numpy()
+    lb.close()
+    res = {}
+
+    def summarise(name, plaq, acc, secs, ntraj, series=None):
+        m, e = mean_err(plaq)
+        d = {"ntraj": ntraj, "traj_per_s": ntraj / secs, "acceptance": acc, "plaquette": [m, e],
+             "z_vs_exact": (m - want) / e if e > 0 else None}
+        if series is not None and len(series) >= 8:
+            d["abs_polyakov"] = mean_err(series)
+            d["tau_int_polyakov"] = tau_int(series)
+        res[name] = d
+        log(f"[c5] {name}: plaquette {m:.7f} +- {e:.7f} (exact {want:.7f}), acceptance {acc:.2f}, "
+            f"{ntraj / secs:.3f} traj/s")
+
+    for name, prec in (("plain_fp64", A.FP64), ("plain_bf16x3", A.BF16X3)):
+        if name not in args.chains.split(","):
+            continue
+        lat = lf if prec == A.FP64 else A.Lattice(n, ext, prec)
+        lat.set_links(start)
+        pq = []
+        ser, acc, secs = plain_chain(lat, beta, dt, nd, level, seed, 3 * 10**6, args.ntraj, plaq=pq)
+        summarise(name, pq, acc, secs, args.ntraj, ser)
+        if lat is not lf:
+            lat.close()
+    if "flow_trained" in args.chains.split(","):
+        lf.set_links(start)
+        lf.flow_set_weights(wt)
+        fq = []
+        ser, acc, secs = flow_chain(lf, beta, args.flow_dt, args.flow_ndrift, level, seed, 4 * 10**6,
+                                    args.ntraj_flow, log=log, plaq=fq)
+        summarise("flow_trained", fq, acc, secs, args.ntraj_flow, ser)
+        lf.sync()
+        lf.timer_start()
+        lf.flow_gradient(beta)
+        res["flow_trained"]["ms_per_flow_gradient"] = lf.timer_stop()
+        lf.timer_start()
+        lf.kick_drift(dt, dt, level)
+        res["flow_trained"]["ms_per_plain_kick_drift"] = lf.timer_stop()
+    out = {"workload": f"SU(64) D=2 128^2 beta=4096 DT={dt} {nd} MD steps LEVEL=8; FTHMC: spectral map, "
+                       f"{args.layers} sweep(s) ({len(mu)} sub-layers), t_max {tmax}, trained on {args.ntrain} "
+                       f"plain-HMC configs at beta={args.beta_train}, flow dt {args.flow_dt} x {args.flow_ndrift}",
+           "exact_plaquette": want, "therm": args.therm, "train_seconds": t_train, "loss": losses, "results": res}
+    print(json.dumps(out))
+
+
 def main():
     import argparse
 
@@ -215,7 +313,21 @@ def main():
     ap.add_argument("--chains", default="plain,flow_untrained,flow_trained",
                     help="comma-separated chains to run: plain, flow_untrained, flow_trained")
     ap.add_argument("--series-out", help="write each chain's |P| and plaquette series to this .npz file")
+    ap.add_argument("--workload", choices=["c4", "c5"], default="c4",
+                    help="c4: SU(3) D=4 (stout / rank-one map); c5: SU(64) D=2 128^2 (spectral map)")
+    ap.add_argument("--ntraj-flow", type=int, default=20, help="c5: FTHMC trajectories")
+    ap.add_argument("--flow-dt", type=float, default=0.005, help="c5: FTHMC step")
+    ap.add_argument("--flow-ndrift", type=int, default=200, help="c5: FTHMC MD steps per trajectory")
     args = ap.parse_args()
+    if args.workload == "c5":
+        import torch
+
+        torch.cuda.init()
+        if args.beta_train == 5.7:
+            args.beta_train = 0.9 * 4096.0
+        if args.chains == "plain,flow_untrained,flow_trained":
+            args.chains = "plain_fp64,plain_bf16x3,flow_trained"
+        return c5_study(args, lambda m: print(m, file=sys.stderr, flush=True))
     # torch first: the library then binds to torch's HIP runtime, so both
     # share one runtime in this process
     import torch

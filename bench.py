@@ -511,6 +511,39 @@ def flow_study(reps=5):
     lat.close()
 
 
+def flow2d_study(reps=3):
+    """HIP-event time of the two-dimensional spectral map at the C5 size
+    (SU(64), 128^2, one sweep = 8 sub-layers of 4096 updated links, random
+    weights): one forward, one flow gradient (one FTHMC MD step's force:
+    forward keeping the inputs and inverses, the Wilson force of F(V), the
+    backward), and the plain FP64 kick+drift of the same lattice."""
+    import askja_amd as A
+    from askja_amd import learned as L
+
+    lat = A.Lattice(64, [128, 128], A.FP64)
+    lat.haar(SEED, 3)
+    mu, off, blk = L.spectral_masks(1)
+    lat.flow_define(mu, off, blk, 1, 0.4, kind="spectral")
+    lat.flow_set_weights((np.random.default_rng(1).standard_normal(161) * 0.3).astype(np.float32))
+    lat.flow_gradient(4096.0)  # warm-up
+    lat.sync()
+    fwd, grad, kd = [], [], []
+    for r in range(reps):
+        lat.timer_start()
+        lat.flow_apply()
+        fwd.append(lat.timer_stop())
+        lat.timer_start()
+        lat.flow_gradient(4096.0)
+        grad.append(lat.timer_stop())
+        lat.timer_start()
+        lat.kick_drift(0.005, 0.005, 8)
+        kd.append(lat.timer_stop())
+    print(json.dumps({"study": "flow2d", "nc": 64, "extents": [128, 128], "sublayers": len(mu),
+                      "ms_forward": min(fwd), "ms_gradient": min(grad), "ms_plain_kick_drift": min(kd),
+                      "gradient_over_kick_drift": min(grad) / min(kd)}), flush=True)
+    lat.close()
+
+
 def cpu_baseline_c4(steps_sample):
     """The oracle (FP64 C restatement of the reference, OpenMP over sites in
     the kick and drift like sun.cpp:679,777) on the host cores, plain HMC at
@@ -663,7 +696,7 @@ def main():
     ap.add_argument("--copy-probe", action="store_true", help="only run a 1 GiB device copy (counter calibration)")
     ap.add_argument("--kernel-only", choices=["c2", "c4", "c3", "c5"],
                     help="only time --roofline-launches fused kick+drift launches of that lattice (for rocprofv3)")
-    ap.add_argument("--study", choices=["precision", "c3scan", "c5exact", "kernels", "example", "flow"],
+    ap.add_argument("--study", choices=["precision", "c3scan", "c5exact", "kernels", "example", "flow", "flow2d"],
                     help="print a precision, beta-scan, exact-2D, per-config kernel or example-params study instead")
     ap.add_argument("--study-configs", default="C2,C3,C5")
     ap.add_argument("--study-therm", type=int, default=20, help="thermalising trajectories (c5exact)")
@@ -697,6 +730,9 @@ def main():
         return
     if args.study == "example":
         example_study()
+        return
+    if args.study == "flow2d":
+        flow2d_study()
         return
     if args.study == "flow":
         flow_study()

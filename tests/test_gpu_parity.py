@@ -117,6 +117,11 @@ def test_mfma_kick_drift_padded_sizes(oracle, n, ext):
         scale_p = np.max(np.abs(o.field("pi")))
         assert np.max(np.abs(lat.momenta() - o.field("pi"))) <= tol * scale_p * n, prec
         assert np.max(np.abs(lat.links() - o.field("u"))) <= tol * n, prec
+        # the standalone drift (its own MFMA kernel for N >= 9)
+        lat = _gpu(n, ext, prec, u, p)
+        lat.updateE(0.05)
+        lat.updateUandcopyDown(0.05, 8)
+        assert np.max(np.abs(lat.links() - o.field("u"))) <= tol * n, prec
 
 
 @pytest.mark.parametrize("prec", [A.FP64_EXACT, A.FP64, A.FP32])
