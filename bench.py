@@ -3,17 +3,21 @@
 Default workload (the configuration the headline metric is quoted on):
 learned-proposal HMC against plain FP64 HMC for SU(3) in D = 4 on a 16^4
 lattice, beta = 6.0, DT = 0.02, 50 MD steps per trajectory (length 1),
-LEVEL 8.  The learned proposal is field-transformation HMC with the map
-U = F(V) of four sweep layers (32 masked sub-layers, MLP of hidden width 32
-over plaquette and rectangle traces), weights trained on 500 plain-HMC
-configurations at beta 5.7 (askja_amd/data/flow_c4_weights.json, written by
-``python -m askja_amd.learned --save-weights``), exact Metropolis with
-log J.  Data are synthetic: a Haar-random start from a fixed seed,
+LEVEL 8.  The learned proposal is field-transformation HMC with the stout
+map U = F(V) of one sweep layer (8 masked sub-layers: every direction and
+parity, MLP of hidden width 32 over plaquette and rectangle traces), weights
+trained on 500 plain-HMC configurations at beta 5.7
+(askja_amd/data/flow_c4_weights_1sweep.json, written by ``python -m
+askja_amd.learned --layers 1 --save-weights``), exact Metropolis with log J.
+Plain HMC decorrelates |P| in about 1.5 trajectories here, which caps what
+any proposal can gain per trajectory, so the map's cost decides the value:
+one sweep costs a quarter of the four-sweep map
+(askja_amd/data/flow_c4_weights.json, ``--weights`` selects it).  Data are synthetic: a Haar-random start from a fixed seed,
 thermalised by plain HMC (untimed) before either chain.
 
 One step = one learned-proposal trajectory: momentum refresh, 50 MD steps
-with the flow force (forward and hand-written backward of all 32
-sub-layers per step), H with log J, Metropolis, the Polyakov loop of U.
+with the flow force (forward and hand-written backward of every sub-layer
+per step), H with log J, Metropolis, the Polyakov loop of U.
 --warmup trajectories of each chain run untimed, then exactly --steps
 learned trajectories are timed; the plain chain is timed over its own
 series.
@@ -367,7 +371,7 @@ C4_THERM = 300            # plain trajectories from the Haar start, untimed
 # no trajectory accepted); 20 trajectories of dt = 0.005 x 200 steps (90 %
 # accepted) bring it to where dt = 0.02 is accepted 70 % of the time.
 C4_RELAX = (20, 0.005, 200)
-C4_WEIGHTS = os.path.join(ROOT, "askja_amd", "data", "flow_c4_weights.json")
+C4_WEIGHTS = os.path.join(ROOT, "askja_amd", "data", "flow_c4_weights_1sweep.json")
 TAU_SPAN = 100            # a tau estimate needs a series of >= 100 tau
 MIN_SERIES = {"plain": 3000, "learned": 300}
 MAX_SERIES = {"plain": 20000, "learned": 2000}
