@@ -1285,22 +1285,23 @@ int flow_free(askja_ctx *c)
     return ASKJA_OK;
 }
 
-// D = 2 spectral map, any N: sub-layer k updates U_mu[k](x) for
-// x_nu = offset[k] mod 4 (nu = 1 - mu), t = t_max tanh(MLP(3 features))
+// Spectral map, any N and D: sub-layer k updates U_mu[k](x) for
+// x_nu = offset[k] mod 4 (nu = (mu + 1) mod D), t = t_max tanh(MLP(3 features))
 int sflow_define(askja_ctx *c, int nsub, const int *mu, const int *offset, const int *block, int nblocks,
                  double t_max)
 {
-    if (c->dim != 2) return fail(c, ASKJA_EUNSUP, "flow_define_spectral: D = 2 only");
+    if (c->dim < 2) return fail(c, ASKJA_EUNSUP, "flow_define_spectral: D >= 2 (plaquettes)");
     if (c->mode != kModeF64 && c->mode != kModeF64Exact)
         return fail(c, ASKJA_EUNSUP, "flow_define_spectral: FP64 precision modes only");
-    if (c->ext[0] % 4 || c->ext[1] % 4)
-        return fail(c, ASKJA_EUNSUP, "flow_define_spectral: both extents must be multiples of 4 (row masks mod 4)");
+    for (int d = 0; d < c->dim; d++)
+        if (c->ext[d] % 4)
+            return fail(c, ASKJA_EUNSUP, "flow_define_spectral: every extent must be a multiple of 4 (row masks mod 4)");
     if (nsub < 1 || nsub > 4096 || nblocks < 1 || !mu || !offset || !block)
         return fail(c, ASKJA_EINVAL, "flow_define_spectral: bad sizes or null arrays");
     if (!(t_max > 0.0) || t_max > 0.5)
         return fail(c, ASKJA_EINVAL, "flow_define_spectral: t_max must be in (0, 1/2] (the phase branch's series bound)");
     for (int k = 0; k < nsub; k++)
-        if (mu[k] < 0 || mu[k] > 1 || offset[k] < 0 || offset[k] > 3 || block[k] < 0 || block[k] >= nblocks)
+        if (mu[k] < 0 || mu[k] >= c->dim || offset[k] < 0 || offset[k] > 3 || block[k] < 0 || block[k] >= nblocks)
             return fail(c, ASKJA_EINVAL, "flow_define_spectral: sub-layer %d out of range", k);
     int rc = flow_free(c);
     if (rc) return rc;

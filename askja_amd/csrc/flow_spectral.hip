@@ -1,13 +1,15 @@
-// flow2d.hip -- the spectral field transformation U = F(V) of
-// two-dimensional SU(N) lattices (2 <= N <= 64), used as the FTHMC proposal
-// map: forward with its log-Jacobian, and the hand-written backward of
+// flow_spectral.hip -- the spectral field transformation U = F(V) of SU(N)
+// lattices (2 <= N <= 64, any D >= 2), used as the FTHMC proposal map:
+// forward with its log-Jacobian, and the hand-written backward of
 // Phi = beta S_W(F(V)) - log J(V).
 //
-// Sub-layer (mu, off), nu = 1 - mu, updates U_mu(x) for x_nu = off (mod 4);
-// every other link is frozen.  For an updated link
+// Sub-layer (mu, off), nu = (mu + 1) mod D (nu = 1 - mu in D = 2), updates
+// U_mu(x) for x_nu = off (mod 4); every other link is frozen.  Everything the
+// map reads lies in the (mu, nu) plane.  For an updated link
 //   S  = U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+   (frozen: x+nu lies in row off+1)
 //   W  = U_mu(x) S                            (the plaquette through the link)
-//   f  = ReTr P(x+nu+k mu) / N, k = -1, 0, 1  (frozen plaquettes of row off+1)
+//   f  = ReTr P_mu,nu(x+nu+k mu) / N, k = -1, 0, 1  (frozen plaquettes of row off+1,
+//        P_mu,nu(y) = U_mu(y) U_nu(y+mu) U_mu(y+nu)^+ U_nu(y)^+)
 //   t  = t_max tanh(b2 + w2 . tanh(W1 f + b1))
 //   B  = 1 - t W,   h = B^{-1} (W - t)
 //   U' = e^{2 i phi / N} h S^+,   phi = Im Tr log B (continuous branch),
@@ -53,7 +55,7 @@ struct Args {
     Geom g;
     double *U;          // field (FP64); forward: updated links written in place;
                         // backward: their inputs restored from `log`
-    int mu, off;
+    int mu, nu, off;
     const float *w;     // MLP block
     double tmax;
     double *ptr;        // ReTr P / N per device site (rows off+1)
@@ -82,21 +84,21 @@ struct Cfg {
 __device__ inline size_t loff(const Geom &g, long l, int r, int n) { return elem_off(l, r, 2 * n * n, g.tile); }
 __device__ inline size_t moff(const Geom &g, long l, int e, int n) { return elem_off(l, e, n * n, g.tile); }
 
-// device site of active index b: x_mu = b % L_mu, x_nu = row + 4 (b / L_mu)
-__device__ inline long act_site(const Geom &g, int mu, int row, long b)
+// device site of active index b: the V / L_nu sites of one nu-slice in
+// lexicographic order of the other coordinates (j = b mod (V / L_nu)), the
+// slices x_nu = row + 4 (b / (V / L_nu))
+__device__ inline long act_site(const Geom &g, int nu, int row, long b)
 {
-    const long Lm = g.ext[mu];
-    const long xm = b % Lm, xn = row + 4 * (b / Lm);
-    const long x0 = mu == 0 ? xm : xn, x1 = mu == 0 ? xn : xm;
-    return g.lex2dev[x0 * g.ext[1] + x1];
+    const long m = g.V / g.ext[nu];
+    long j = b % m, lex = (row + 4 * (b / m)) * g.helparr[nu];
+    for (int d = g.D - 1; d >= 0; d--) {
+        if (d == nu) continue;
+        lex += (j % g.ext[d]) * g.helparr[d];
+        j /= g.ext[d];
+    }
+    return g.lex2dev[lex];
 }
 
-// Owned output elements of a product.  MFMA classes: the FP64 MFMA
-// accumulator layout (element q = 4 t + u: row acc_row(u), column
-// acc_col(t)).  VALU classes: column c = tid % NMAX, rows r0 + RS q
-// (r0 = tid / NMAX, RS = NT / NMAX), so that a wavefront covers whole rows.
-// (i, j) of an element outside the matrix is (0, 0), so that reads of it
-// stay in the slot.
 template <int NMAX, int NT>
 struct Own {
     static constexpr bool MF = Cfg<NMAX, NT>::MF;
@@ -521,19 +523,20 @@ __global__ __launch_bounds__(NT) void k_ptr(Args a, int row)
     const Geom &g = a.g;
     const int n = g.N;
     const long V = g.V;
-    const long y = act_site(g, a.mu, row, blockIdx.x);
+    const int mu = a.mu, nu = a.nu;
+    const long y = act_site(g, nu, row, blockIdx.x);
     Own<NMAX, NT> o;
     o.init(n);
     zero_slots<NMAX, NT>(Ar, Ai, Br, Bi);
-    // X = U_0(y) U_1(y+0), Y = U_1(y) U_0(y+1): ReTr P = Re sum X_ij conj(Y_ij)
-    ld<NMAX, NT>(g, Ar, Ai, a.U, 0 * V + y, n);
-    ld<NMAX, NT>(g, Br, Bi, a.U, 1 * V + g.nbr[0 * V + y], n);
+    // X = U_mu(y) U_nu(y+mu), Y = U_nu(y) U_mu(y+nu): ReTr P = Re sum X_ij conj(Y_ij)
+    ld<NMAX, NT>(g, Ar, Ai, a.U, (long)mu * V + y, n);
+    ld<NMAX, NT>(g, Br, Bi, a.U, (long)nu * V + g.nbr[(2 * mu) * V + y], n);
     __syncthreads();
     Reg<NMAX, NT> X, Y;
     mm<false, false>(o, Ar, Ai, Br, Bi, n, X);
     __syncthreads();
-    ld<NMAX, NT>(g, Ar, Ai, a.U, 1 * V + y, n);
-    ld<NMAX, NT>(g, Br, Bi, a.U, 0 * V + g.nbr[2 * V + y], n);
+    ld<NMAX, NT>(g, Ar, Ai, a.U, (long)nu * V + y, n);
+    ld<NMAX, NT>(g, Br, Bi, a.U, (long)mu * V + g.nbr[(2 * nu) * V + y], n);
     __syncthreads();
     mm<false, false>(o, Ar, Ai, Br, Bi, n, Y);
     double v[1] = {0.0};
@@ -553,9 +556,9 @@ __global__ __launch_bounds__(NT) void k_forward(Args a)
     __shared__ double red[Cfg<NMAX, NT>::NW * 5];
     __shared__ double Pr[NMAX], Pi[NMAX];  // pivot reciprocals of the LU
     const Geom &g = a.g;
-    const int n = g.N, mu = a.mu, nu = 1 - mu;
+    const int n = g.N, mu = a.mu, nu = a.nu;
     const long V = g.V, b = blockIdx.x;
-    const long s = act_site(g, mu, a.off, b);
+    const long s = act_site(g, nu, a.off, b);
     const long l = (long)mu * V + s;
     const long xp = g.nbr[(2 * mu) * V + s], xpn = g.nbr[(2 * nu) * V + s];
     Mlp m;
@@ -675,9 +678,9 @@ __global__ __launch_bounds__(NT) void k_backward(Args a)
     __shared__ double red[Cfg<NMAX, NT>::NW * 5];
     __shared__ double Pr[NMAX], Pi[NMAX];  // pivot reciprocals of the LU
     const Geom &g = a.g;
-    const int n = g.N, mu = a.mu, nu = 1 - mu;
+    const int n = g.N, mu = a.mu, nu = a.nu;
     const long V = g.V, b = blockIdx.x;
-    const long s = act_site(g, mu, a.off, b);
+    const long s = act_site(g, nu, a.off, b);
     const long l = (long)mu * V + s;
     const long xp = g.nbr[(2 * mu) * V + s], xpn = g.nbr[(2 * nu) * V + s];
     const double *uin = a.log + (size_t)b * 2 * n * n;
@@ -880,8 +883,8 @@ __global__ __launch_bounds__(NT) void k_backward(Args a)
     }
 }
 
-// Feature plaquettes P(y) = a b c^+ d^+ of row `row` (a = U_0(y),
-// b = U_1(y+0), c = U_0(y+1), d = U_1(y)): each link += co TA(P_l),
+// Feature plaquettes P(y) = a b c^+ d^+ of row `row` (a = U_mu(y),
+// b = U_nu(y+mu), c = U_mu(y+nu), d = U_nu(y)): each link += co TA(P_l),
 // co = -dPhi/df / (2N), P_l the plaquette rotated to start at the link:
 // a b c^+ d^+, b c^+ d^+ a, c b^+ a^+ d, d c b^+ a^+.
 // put_ta stores P in (re, im), which nobody may be reading on entry.
@@ -914,9 +917,11 @@ __global__ __launch_bounds__(NT) void k_featgrad(Args a, int row)
     const Geom &g = a.g;
     const int n = g.N;
     const long V = g.V;
-    const long y = act_site(g, a.mu, row, blockIdx.x);
+    const int mu = a.mu, nu = a.nu;
+    const long y = act_site(g, nu, row, blockIdx.x);
     const double co = -a.dF[y] / (2.0 * n);
-    const long la = y, lb = V + g.nbr[y], lc = g.nbr[2 * V + y], ld_ = V + y;
+    const long la = (long)mu * V + y, lb = (long)nu * V + g.nbr[(2 * mu) * V + y];
+    const long lc = (long)mu * V + g.nbr[(2 * nu) * V + y], ld_ = (long)nu * V + y;
     Own<NMAX, NT> o;
     o.init(n);
     zero_slots<NMAX, NT>(Ar, Ai, Br, Bi);
@@ -1034,6 +1039,7 @@ hipError_t launch_sflow_forward(const Geom &g, void *U, int mu, int off, const f
     a.g = g;
     a.U = (double *)U;
     a.mu = mu;
+    a.nu = (mu + 1) % g.D;
     a.off = off;
     a.w = w;
     a.tmax = tmax;
@@ -1054,6 +1060,7 @@ hipError_t launch_sflow_backward(const Geom &g, void *U, int mu, int off, const 
     a.g = g;
     a.U = (double *)U;
     a.mu = mu;
+    a.nu = (mu + 1) % g.D;
     a.off = off;
     a.w = w;
     a.tmax = tmax;

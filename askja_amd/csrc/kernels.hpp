@@ -193,7 +193,7 @@ hipError_t launch_wilson_grad(const Geom &g, const void *U, double coef, int mod
 // P <- P - eps_c * gr
 hipError_t launch_kick_g(const Geom &g, void *P, const double *gr, double eps_c, int mode, hipStream_t st);
 
-// ---- spectral field transformation of D = 2 lattices (flow2d.hip, any N) -----
+// ---- spectral field transformation (flow_spectral.hip, any N and D) ----------
 // updated links per sub-layer: V / 4 (rows x_nu = off mod 4)
 long sflow_active(const Geom &g);
 // one sub-layer (mu, off) in place on U (FP64): feature plaquettes into ptr

@@ -40,13 +40,14 @@ def sweep_masks(dim, layers):
     return mu, par, blk
 
 
-def spectral_masks(layers):
-    """Sub-layers of `layers` sweeps of the two-dimensional spectral map (both
-    directions, the four row offsets); every sweep shares one weight block."""
+def spectral_masks(layers, dim=2):
+    """Sub-layers of `layers` sweeps of the spectral map (every direction, the
+    four row offsets; in D = 2 both directions alternate); every sweep shares
+    one weight block."""
     mu, off, blk = [], [], []
     for layer in range(layers):
         for o in range(4):
-            for m in (0, 1):
+            for m in range(dim):
                 mu.append(m)
                 off.append(o)
                 blk.append(layer)

@@ -278,17 +278,18 @@ int askja_flow_define(askja_ctx *ctx, int nsub, const int *mu, const int *parity
  * extents and 0 < rho_max < 1/(2(D-1)) (then |g M| < 1: invertible). */
 int askja_flow_define_stout(askja_ctx *ctx, int nsub, const int *mu, const int *parity, const int *block, int nblocks,
                             double rho_max);
-/* The spectral form of the map for D = 2 and any 2 <= N <= 64 (no reference
- * counterpart; the map of the large-N two-dimensional workload): sub-layer k
- * updates U_mu(x), mu = mu[k], for every x with x_nu = offset[k] mod 4
- * (nu = 1 - mu), through the plaquette W = U_mu(x) S (S = the frozen staple
+/* The spectral form of the map for any D >= 2 and any 2 <= N <= 64 (no
+ * reference counterpart; the map of the large-N two-dimensional workload):
+ * sub-layer k updates U_mu(x), mu = mu[k], for every x with
+ * x_nu = offset[k] mod 4 (nu = (mu + 1) mod D, so nu = 1 - mu in D = 2),
+ * through the plaquette W = U_mu(x) S (S = the frozen staple
  * U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+): every eigenvalue of W goes to
  * e^{2i phi/N} (lambda - t) / (1 - t lambda) (phi restores det = 1), i.e.
  *   U'_mu(x) = e^{2i phi/N} (1 - tW)^{-1} (W - t) S^+,
- *   t = t_max tanh(MLP(f)),  f = ReTr P(x+nu+k mu) / N, k = -1, 0, 1
- * (frozen plaquettes), and per updated link
+ *   t = t_max tanh(MLP(f)),  f = ReTr P_mu,nu(x+nu+k mu) / N, k = -1, 0, 1
+ * (frozen plaquettes of the same plane), and per updated link
  *   log J = N^2 log(1-t^2) - 2N log|det(1-tW)| + log((1+t^2-(2t/N) ReTr W)/(1-t^2)).
- * Requires D = 2, both extents multiples of 4, an FP64 precision mode and
+ * Requires every extent a multiple of 4, an FP64 precision mode and
  * 0 < t_max <= 1/2.  Block layout: W1[32][3], b1[32], w2[32], b2 (161).
  * The gradient's contributions to frozen links are FP64 atomic adds (the
  * order of those sums is not fixed). */

@@ -1,18 +1,19 @@
-"""CPU restatement (numpy, FP64) of the two-dimensional spectral field
-transformation U = F(V) for SU(N), any N >= 2.
+"""CPU restatement (numpy, FP64) of the spectral field transformation
+U = F(V) for SU(N), any N >= 2, in any dimension D >= 2.
 
 TEST INFRASTRUCTURE ONLY: the checker for the HIP kernels of
-askja_amd/csrc/flow2d.hip.  The reference has no counterpart (the learned
+askja_amd/csrc/flow_spectral.hip.  The reference has no counterpart (the learned
 proposal is an addition to its method, whose action is EnergyMag,
 sun.cpp:507-537); the definitions are these.
 
-Lattice D = 2, extents (L0, L1), sites lexicographic (x1 fastest), periodic.
-P(y) = U_0(y) U_1(y+0) U_0(y+1)^+ U_1(y)^+ is the plaquette at y.
-Sub-layer (mu, off), nu = 1 - mu: the links U_mu(x) with x_nu = off (mod 4)
-are updated, every other link is frozen.  For an updated link:
+Extents (L0, ..., L_{D-1}), sites lexicographic (x_{D-1} fastest), periodic.
+P_mu,nu(y) = U_mu(y) U_nu(y+mu) U_mu(y+nu)^+ U_nu(y)^+ is the plaquette at y.
+Sub-layer (mu, off), nu = (mu + 1) mod D (= 1 - mu in D = 2): the links
+U_mu(x) with x_nu = off (mod 4) are updated, every other link is frozen.
+For an updated link:
     S  = U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+     (frozen: x+nu is in row off+1)
     W  = U_mu(x) S                               (the plaquette through the link)
-    f  = ReTr P(x+nu+k mu) / N, k = -1, 0, 1     (frozen plaquettes, row off+1)
+    f  = ReTr P_mu,nu(x+nu+k mu) / N, k = -1, 0, 1  (frozen plaquettes, row off+1)
     t  = t_max tanh(b2 + w2 . tanh(W1 f + b1))    (|t| < 1)
     W' = the spectral map of W: each eigenvalue lambda of W goes to
          e^{2 i phi / N} (lambda - t) / (1 - t lambda),
@@ -33,7 +34,7 @@ eigenvalues, log J = N^2 log(1-t^2) - 2N log|det(1 - tW)|
 the tests check.
 
 Weight block layout (float32): W1[32][3], b1[32], w2[32], b2.  Fields are
-complex128 arrays [V, 2, N, N] in the reference's U-*.conf order.
+complex128 arrays [V, D, N, N] in the reference's U-*.conf order.
 """
 import numpy as np
 
@@ -41,34 +42,40 @@ HID = 32
 NF = 3
 
 
-def site(ext, x0, x1):
-    return (x0 % ext[0]) * ext[1] + (x1 % ext[1])
-
-
 def coords(ext, i):
-    return divmod(i, ext[1])
+    return tuple(int(c) for c in np.unravel_index(i, ext))
+
+
+def site(ext, x):
+    return int(np.ravel_multi_index(tuple(c % e for c, e in zip(x, ext)), ext))
 
 
 def shift(ext, i, d, k=1):
     x = list(coords(ext, i))
     x[d] += k
-    return site(ext, x[0], x[1])
+    return site(ext, x)
 
 
-def plaquette(U, ext, y):
-    """P(y) = U_0(y) U_1(y+0) U_0(y+1)^+ U_1(y)^+."""
-    return U[y, 0] @ U[shift(ext, y, 0), 1] @ U[shift(ext, y, 1), 0].conj().T @ U[y, 1].conj().T
+def partner(ext, mu):
+    """The direction nu of sub-layers of direction mu."""
+    return (mu + 1) % len(ext)
+
+
+def plaquette(U, ext, y, mu=0, nu=1):
+    """P_mu,nu(y) = U_mu(y) U_nu(y+mu) U_mu(y+nu)^+ U_nu(y)^+."""
+    return U[y, mu] @ U[shift(ext, y, mu), nu] @ U[shift(ext, y, nu), mu].conj().T @ U[y, nu].conj().T
 
 
 def wilson_action(U, ext):
-    """S_W = sum_y (1 - ReTr P(y) / N)  (EnergyMag, sun.cpp:507-537, D = 2)."""
-    n = U.shape[-1]
-    return float(sum(1.0 - np.trace(plaquette(U, ext, y)).real / n for y in range(U.shape[0])))
+    """S_W = sum_y sum_{mu < nu} (1 - ReTr P_mu,nu(y) / N)  (EnergyMag, sun.cpp:507-537)."""
+    n, D = U.shape[-1], len(ext)
+    return float(sum(1.0 - np.trace(plaquette(U, ext, y, m, v)).real / n
+                     for y in range(U.shape[0]) for m in range(D) for v in range(m + 1, D)))
 
 
 def active_sites(ext, mu, off):
-    nu = 1 - mu
-    return [i for i in range(ext[0] * ext[1]) if coords(ext, i)[nu] % 4 == off]
+    nu = partner(ext, mu)
+    return [i for i in range(int(np.prod(ext))) if coords(ext, i)[nu] % 4 == off]
 
 
 def mlp(w, f):
@@ -82,11 +89,11 @@ def mlp(w, f):
 
 def link_inputs(U, ext, x, mu):
     """(S, features) of the updated link (x, mu)."""
-    nu = 1 - mu
+    nu = partner(ext, mu)
     n = U.shape[-1]
     S = U[shift(ext, x, mu), nu] @ U[shift(ext, x, nu), mu].conj().T @ U[x, nu].conj().T
     y = shift(ext, x, nu)
-    f = [np.trace(plaquette(U, ext, shift(ext, y, mu, k))).real / n for k in (-1, 0, 1)]
+    f = [np.trace(plaquette(U, ext, shift(ext, y, mu, k), mu, nu)).real / n for k in (-1, 0, 1)]
     return S, f
 
 
@@ -215,11 +222,11 @@ def haar(n, rng):
 
 
 def random_field(ext, n, rng, spread=None):
-    """[V, 2, n, n] field: Haar links, or e^{i spread H} near the identity."""
-    V = ext[0] * ext[1]
-    U = np.zeros((V, 2, n, n), complex)
+    """[V, D, n, n] field: Haar links, or e^{i spread H} near the identity."""
+    V, D = int(np.prod(ext)), len(ext)
+    U = np.zeros((V, D, n, n), complex)
     for i in range(V):
-        for d in range(2):
+        for d in range(D):
             if spread is None:
                 U[i, d] = haar(n, rng)
             else:

@@ -1,6 +1,7 @@
-"""The two-dimensional spectral map (askja_flow_define_spectral, any N) against
-its CPU restatement (oracle/spectral_oracle.py, pinned in
-tests/test_spectral_oracle.py): forward links and log J for N = 2 ... 64, the
+"""The spectral map (askja_flow_define_spectral, any N, any D) against its
+CPU restatement (oracle/spectral_oracle.py, pinned in
+tests/test_spectral_oracle.py): forward links and log J for N = 2 ... 64 in
+D = 2 and for N = 2, 4, 9 in D = 3 and 4, the
 identity at zero weights against the plain Wilson force, the hand-written
 backward against finite differences of the oracle's Phi = beta S_W(F(V)) -
 log J(V) (links and MLP weights, eps = 1e-6, FP64), field-transformation HMC
@@ -69,6 +70,80 @@ def test_spectral_forward_matches_oracle(n):
     assert err < (1e-12 if n <= 8 else 1e-10)
     assert abs(lj - ljo) < 1e-10 * max(1.0, abs(ljo)) and abs(ljo) > 1e-2
     assert np.max(np.abs(U - V)) > 1e-3
+
+
+GEOMS_D34 = {"D3": ((4, 8, 4), [(0, 0, 0), (1, 1, 0), (2, 2, 1), (0, 3, 1), (2, 0, 0)]),
+             "D4": ((4, 4, 4, 4), [(0, 0, 0), (1, 1, 0), (2, 2, 1), (3, 3, 1), (0, 1, 0), (3, 2, 1)])}
+
+
+@pytest.mark.parametrize("geom,n", [("D3", 4), ("D4", 2), ("D4", 4), ("D4", 9)])
+def test_spectral_forward_matches_oracle_d34(geom, n):
+    ext, subs = GEOMS_D34[geom]
+    rng, V, lat, w = _setup(n, 40 + n, ext=ext, subs=subs)
+    lj = lat.flow_apply()
+    U = lat.flow_output()
+    Uo, ljo = S.forward(V, ext, subs, w, TMAX)
+    err = np.max(np.abs(U - Uo))
+    print(f"SU({n}) {geom} spectral forward: max |U - U_oracle| = {err:.2e}, log J = {lj:.10f} ({ljo:.10f})")
+    assert err < 1e-12
+    assert abs(lj - ljo) < 1e-10 * max(1.0, abs(ljo)) and abs(ljo) > 1e-2
+
+
+@pytest.mark.parametrize("geom,n", [("D3", 3), ("D4", 2)])
+def test_spectral_link_gradient_vs_finite_differences_d34(geom, n):
+    ext, subs = GEOMS_D34[geom]
+    beta = 1.5 * n
+    rng, V, lat, w = _setup(n, 50 + n, ext=ext, subs=subs)
+    s, lj = lat.flow_gradient(beta)
+    Uo, ljo = S.forward(V, ext, subs, w, TMAX)
+    assert abs(s - S.wilson_action(Uo, ext)) < 1e-10 * max(1.0, abs(s))
+    g = lat.flow_link_gradient()
+    gens = S.su_generators(n)
+    eps = 1e-6
+    for trial in range(8):
+        x, d = int(rng.integers(V.shape[0])), int(rng.integers(len(ext)))
+        X = sum(c * T for c, T in zip(rng.standard_normal(len(gens)), gens))
+        vals = []
+        for sgn in (1, -1):
+            W = V.copy()
+            W[x, d] = scipy.linalg.expm(1j * sgn * eps * X) @ V[x, d]
+            vals.append(S.phi_action(W, ext, subs, w, TMAX, beta))
+        fd = (vals[0] - vals[1]) / (2 * eps)
+        an = 2 * np.trace(g[x, d] @ X).real
+        assert abs(fd - an) < 2e-6 * max(1.0, abs(an)), (x, d, fd, an)
+
+
+def test_spectral_fthmc_reversible_d4():
+    n, ext, beta = 4, [8, 8, 4, 4], 8.0
+    lat = A.Lattice(n, ext, A.FP64)
+    lat.haar(6, 0)
+    for t in range(3):
+        lat.hmc_trajectory(beta, 0.05, 10, 8, 6, t)
+    mu, off, blk = L.spectral_masks(1, 4)
+    lat.flow_define(mu, off, blk, 1, TMAX, kind="spectral")
+    lat.flow_set_weights((np.random.default_rng(6).standard_normal(161) * 0.5).astype(np.float32))
+    V0 = lat.links()
+    lat.mom_refresh(9, 0, beta)
+    cf = 2 * n / beta
+
+    def kick(eps):
+        lat.flow_gradient(beta)
+        lat.set_momenta(lat.momenta() - eps * cf * lat.flow_link_gradient())
+
+    def md(k, dt):
+        kick(dt / 2)
+        for j in range(k):
+            lat.updateUandcopyDown(dt, 8)
+            kick(dt if j < k - 1 else dt / 2)
+
+    p0 = lat.momenta()
+    md(5, 0.05)
+    assert np.max(np.abs(lat.links() - V0)) > 1e-3
+    lat.mom_negate()
+    md(5, 0.05)
+    du, dp = np.max(np.abs(lat.links() - V0)), np.max(np.abs(lat.momenta() + p0))
+    print(f"SU({n}) D=4 FTHMC forward + reversed: max |V - V0| = {du:.2e}, max |P + P0| = {dp:.2e}")
+    assert du < 3e-11 * n and dp < 3e-10 * n
 
 
 def test_spectral_identity_is_plain_force():

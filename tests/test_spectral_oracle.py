@@ -84,21 +84,21 @@ def test_moebius_angle_map_is_monotone():
         assert abs((m[-1] - m[0]) - 2 * np.pi) < 1e-9
 
 
-@pytest.mark.parametrize("mu", [0, 1])
+@pytest.mark.parametrize("ext", [(8, 12), (4, 8, 4), (4, 4, 4, 8)], ids=["D2", "D3", "D4"])
 @pytest.mark.parametrize("off", [0, 1, 2, 3])
-def test_sublayer_inputs_avoid_updated_links(mu, off):
-    ext = (8, 12)
-    nu = 1 - mu
-    upd = {(x, mu) for x in S.active_sites(ext, mu, off)}
-    assert len(upd) == ext[0] * ext[1] // 4
-    for x, _ in upd:
-        staple = {(S.shift(ext, x, mu), nu), (S.shift(ext, x, nu), mu), (x, nu)}
-        feats = set()
-        y0 = S.shift(ext, x, nu)
-        for k in (-1, 0, 1):
-            y = S.shift(ext, y0, mu, k)
-            feats |= {(y, 0), (S.shift(ext, y, 0), 1), (S.shift(ext, y, 1), 0), (y, 1)}
-        assert not (staple | feats) & upd
+def test_sublayer_inputs_avoid_updated_links(ext, off):
+    for mu in range(len(ext)):
+        nu = S.partner(ext, mu)
+        upd = {(x, mu) for x in S.active_sites(ext, mu, off)}
+        assert len(upd) == int(np.prod(ext)) // 4
+        for x, _ in upd:
+            staple = {(S.shift(ext, x, mu), nu), (S.shift(ext, x, nu), mu), (x, nu)}
+            feats = set()
+            y0 = S.shift(ext, x, nu)
+            for k in (-1, 0, 1):
+                y = S.shift(ext, y0, mu, k)
+                feats |= {(y, mu), (S.shift(ext, y, mu), nu), (S.shift(ext, y, nu), mu), (y, nu)}
+            assert not (staple | feats) & upd
 
 
 def test_forward_log_j_sums_sublayers():
