@@ -106,6 +106,24 @@ __device__ inline bf16x8 ld8(const __bf16 *p)
     return *(const bf16x8 *)p;
 }
 
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+// Two elements x = hi + lo at once: hi = bf16(x) (round to nearest even, one
+// packed conversion), lo = bf16(x - hi).  The two hi values are widened back
+// to FP32 from the packed word by a shift and a mask, so a pair costs six
+// VALU instructions.  No contraction: x - hi must not fuse with a product
+// that formed x.
+__device__ inline void split2(float x0, float x1, bf16x2 &hi, bf16x2 &lo)
+{
+#pragma clang fp contract(off)
+    const f32x2 x = {x0, x1};
+    hi = __builtin_convertvector(x, bf16x2);
+    const unsigned hb = __builtin_bit_cast(unsigned, hi);
+    const f32x2 h = {__builtin_bit_cast(float, hb << 16), __builtin_bit_cast(float, hb & 0xffff0000u)};
+    lo = __builtin_convertvector(x - h, bf16x2);
+}
+
 // Operand slots.  FP64 / FP32: planar re / im matrices, row-major, row
 // stride LD.  BF16x3: every element is split once, when it is written, as
 // x = hi + lo (hi = bf16(x), lo = bf16(x - hi)); each of the re and im
@@ -118,15 +136,14 @@ __device__ inline void put(R *Sr, R *Si, int r, int c, R re, R im)
 {
     constexpr int LD = Cfg<P, NMAX>::LD;
     if constexpr (P == kBF16X3) {
-        // no contraction: x - hi must not fuse with a product that formed x
-#pragma clang fp contract(off)
         const int o = TR ? c * LD + r : r * LD + c;
         __bf16 *rh = (__bf16 *)Sr, *ih = (__bf16 *)Si;
-        const __bf16 hr = (__bf16)re, hi = (__bf16)im;
-        rh[o] = hr;
-        rh[NMAX * LD + o] = (__bf16)(re - (float)hr);
-        ih[o] = hi;
-        ih[NMAX * LD + o] = (__bf16)(im - (float)hi);
+        bf16x2 h, l;
+        split2(re, im, h, l);
+        rh[o] = h[0];
+        rh[NMAX * LD + o] = l[0];
+        ih[o] = h[1];
+        ih[NMAX * LD + o] = l[1];
     } else {
         Sr[r * LD + c] = re;
         Si[r * LD + c] = im;
@@ -251,16 +268,14 @@ __device__ inline int acc_col(int t)
 
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
-// four consecutive elements x -> hi / lo bf16 (see put)
+// four consecutive elements x -> hi / lo bf16 (see split2)
 __device__ inline void split4(const f32x4 &x, bf16x4 &hi, bf16x4 &lo)
 {
-#pragma clang fp contract(off)
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const __bf16 h = (__bf16)x[q];
-        hi[q] = h;
-        lo[q] = (__bf16)(x[q] - (float)h);
-    }
+    bf16x2 h0, l0, h1, l1;
+    split2(x[0], x[1], h0, l0);
+    split2(x[2], x[3], h1, l1);
+    hi = __builtin_shufflevector(h0, h1, 0, 1, 2, 3);
+    lo = __builtin_shufflevector(l0, l1, 0, 1, 2, 3);
 }
 
 // accumulators -> operand slot (orientation TR, see put).  A BF16x3 slot
@@ -307,28 +322,26 @@ __device__ inline void acc_store_full(const Acc<P, NMAX> &a, R *Sr, R *Si)
 }
 
 // Operand items: each thread moves E elements of a slot, in items of W
-// consecutive elements along the slot's row (W = 2 for BF16x3, whose split
-// planes then take one 4-byte write per item and plane; W = 1 otherwise).
-// Item j of this thread covers (r, c) .. (r, c + W - 1) of the stored
-// matrix, i.e. rows r .. r + W - 1 of column c of the link when TR.
+// elements.  FP64 / FP32: W = 1, item p is element (p / NMAX, p % NMAX).
+// BF16x3: W = 4, item p is the 2 x 2 block at rows 2 br, 2 br + 1 and columns
+// 2 bc, 2 bc + 1 (br = p / (NMAX / 2), bc = p % (NMAX / 2)), element w at
+// (2 br + w / 2, 2 bc + w % 2): a block is two 16-byte runs of the link in
+// memory, and either orientation of the slot takes it as two 4-byte writes
+// per plane (rows 2 br, 2 br + 1 of the slot, or columns when TR).
 template <int P, int NMAX>
 struct Items {
     static constexpr int NT = Cfg<P, NMAX>::NT, E = NMAX * NMAX / NT;
-    static constexpr int W = P == kBF16X3 ? 2 : 1, EI = E / W;
+    static constexpr int W = P == kBF16X3 ? 4 : 1, EI = E / W;
     static_assert(E * NT == NMAX * NMAX && EI * W == E, "slot not a multiple of the workgroup");
     // link element (r, c) of item p, element w
-    template <bool TR>
     static __device__ inline void rc(int p, int w, int &r, int &c)
     {
         if constexpr (W == 1) {
             r = p / NMAX;
             c = p % NMAX;
-        } else if constexpr (!TR) {
-            r = p / (NMAX / W);
-            c = W * (p % (NMAX / W)) + w;
         } else {
-            c = p % NMAX;
-            r = W * (p / NMAX) + w;
+            r = 2 * (p / (NMAX / 2)) + (w >> 1);
+            c = 2 * (p % (NMAX / 2)) + (w & 1);
         }
     }
     // write the item's W complex values (link elements of rc) to the slot
@@ -337,47 +350,75 @@ struct Items {
     {
         if constexpr (W == 1) {
             int r, c;
-            rc<TR>(p, 0, r, c);
+            rc(p, 0, r, c);
             put<P, NMAX, TR, R>(Sr, Si, r, c, re[0], im[0]);
         } else {
-#pragma clang fp contract(off)
             constexpr int LD = Cfg<P, NMAX>::LD, PL = NMAX * LD;
-            typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
             int r, c;
-            rc<TR>(p, 0, r, c);
-            const int o = TR ? c * LD + r : r * LD + c;  // the W elements sit at o, o + 1
-            bf16x2 rh, rl, ih, il;
+            rc(p, 0, r, c);
+            // pairs along the stored row: (w0, w1), (w2, w3) as rows r, r + 1;
+            // (w0, w2), (w1, w3) as columns c, c + 1 when TR
+            constexpr int a0 = 0, a1 = TR ? 2 : 1, b0 = TR ? 1 : 2, b1 = 3;
+            const int o0 = TR ? c * LD + r : r * LD + c, o1 = o0 + LD;
+            bf16x2 h, l;
+            __bf16 *rp = (__bf16 *)Sr, *ip = (__bf16 *)Si;
+            split2(re[a0], re[a1], h, l);
+            *(bf16x2 *)(rp + o0) = h;
+            *(bf16x2 *)(rp + PL + o0) = l;
+            split2(re[b0], re[b1], h, l);
+            *(bf16x2 *)(rp + o1) = h;
+            *(bf16x2 *)(rp + PL + o1) = l;
+            split2(im[a0], im[a1], h, l);
+            *(bf16x2 *)(ip + o0) = h;
+            *(bf16x2 *)(ip + PL + o0) = l;
+            split2(im[b0], im[b1], h, l);
+            *(bf16x2 *)(ip + o1) = h;
+            *(bf16x2 *)(ip + PL + o1) = l;
+        }
+    }
+    // global link (matrix contiguous, n x n complex) -> the item's values.
+    // EN (n == NMAX): a BF16x3 block is read as two 16-byte loads with no
+    // bounds test; otherwise element by element, zero in the padding.
+    template <bool EN, typename S, typename V2>
+    static __device__ inline void fetch(const S *__restrict__ b, int p, int n, V2 (&v)[W])
+    {
+        if constexpr (W == 4 && EN) {
+            typedef S S4 __attribute__((ext_vector_type(4)));
+            int r, c;
+            rc(p, 0, r, c);
+            const S4 x = *(const S4 *)(b + 2 * (r * NMAX + c));
+            const S4 y = *(const S4 *)(b + 2 * ((r + 1) * NMAX + c));
+            v[0] = V2{x[0], x[1]};
+            v[1] = V2{x[2], x[3]};
+            v[2] = V2{y[0], y[1]};
+            v[3] = V2{y[2], y[3]};
+        } else {
+            const V2 *b2 = (const V2 *)b;
 #pragma unroll
             for (int w = 0; w < W; w++) {
-                rh[w] = (__bf16)re[w];
-                rl[w] = (__bf16)(re[w] - (float)rh[w]);
-                ih[w] = (__bf16)im[w];
-                il[w] = (__bf16)(im[w] - (float)ih[w]);
+                int r, c;
+                rc(p, w, r, c);
+                v[w] = (EN || (r < n && c < n)) ? b2[r * n + c] : V2{S(0), S(0)};
             }
-            __bf16 *rp = (__bf16 *)Sr, *ip = (__bf16 *)Si;
-            *(bf16x2 *)(rp + o) = rh;
-            *(bf16x2 *)(rp + PL + o) = rl;
-            *(bf16x2 *)(ip + o) = ih;
-            *(bf16x2 *)(ip + PL + o) = il;
         }
     }
 };
 
-// global link (matrix contiguous, n x n complex) -> zero-padded slot
-template <int P, int NMAX, bool TR, typename R, typename S>
+// global link -> zero-padded slot
+template <int P, int NMAX, bool TR, bool EN = false, typename R, typename S>
 __device__ inline void ld_link(R *Sr, R *Si, const S *__restrict__ f, long l, int n)
 {
     typedef Items<P, NMAX> I;
+    typedef S S2 __attribute__((ext_vector_type(2)));
     const S *b = f + (size_t)l * 2 * n * n;
     for (int p = threadIdx.x; p < I::EI * I::NT; p += I::NT) {
+        S2 v[I::W];
+        I::template fetch<EN>(b, p, n, v);
         R re[I::W], im[I::W];
 #pragma unroll
         for (int w = 0; w < I::W; w++) {
-            int r, c;
-            I::template rc<TR>(p, w, r, c);
-            const bool in = r < n && c < n;
-            re[w] = in ? (R)b[2 * (r * n + c)] : R(0);
-            im[w] = in ? (R)b[2 * (r * n + c) + 1] : R(0);
+            re[w] = (R)v[w][0];
+            im[w] = (R)v[w][1];
         }
         I::template write<TR, R>(Sr, Si, p, re, im);
     }
@@ -388,24 +429,17 @@ __device__ inline void ld_link(R *Sr, R *Si, const S *__restrict__ f, long l, in
 // is issued before the current product and the stash follows it, so the
 // loads run under the MFMA work; without PF the operand is loaded straight
 // into its slot at the stash point (fewer live registers).  TR is the
-// orientation of the slot the operand goes to (see put).
-template <int P, int NMAX, typename S>
+// orientation of the slot the operand goes to (see put); EN: n == NMAX.
+template <int P, int NMAX, typename S, bool EN>
 struct Stage {
     typedef Items<P, NMAX> I;
     typedef S S2 __attribute__((ext_vector_type(2)));
-    S2 v[I::E];
-    template <bool TR>
+    S2 v[I::EI][I::W];
     __device__ inline void fetch(const S *__restrict__ f, long l, int n)
     {
-        const S2 *b = (const S2 *)(f + (size_t)l * 2 * n * n);
+        const S *b = f + (size_t)l * 2 * n * n;
 #pragma unroll
-        for (int j = 0; j < I::EI; j++)
-#pragma unroll
-            for (int w = 0; w < I::W; w++) {
-                int r, c;
-                I::template rc<TR>(threadIdx.x + j * I::NT, w, r, c);
-                v[j * I::W + w] = (r < n && c < n) ? b[r * n + c] : S2{S(0), S(0)};
-            }
+        for (int j = 0; j < I::EI; j++) I::template fetch<EN>(b, (int)threadIdx.x + j * I::NT, n, v[j]);
     }
     template <bool TR, typename R>
     __device__ inline void stash(R *Sr, R *Si) const
@@ -415,8 +449,8 @@ struct Stage {
             R re[I::W], im[I::W];
 #pragma unroll
             for (int w = 0; w < I::W; w++) {
-                re[w] = (R)v[j * I::W + w][0];
-                im[w] = (R)v[j * I::W + w][1];
+                re[w] = (R)v[j][w][0];
+                im[w] = (R)v[j][w][1];
             }
             I::template write<TR, R>(Sr, Si, threadIdx.x + j * I::NT, re, im);
         }
@@ -424,7 +458,7 @@ struct Stage {
     template <bool PF, bool TR>
     __device__ inline void early(const S *__restrict__ f, long l, int n)
     {
-        if constexpr (PF) fetch<TR>(f, l, n);
+        if constexpr (PF) fetch(f, l, n);
     }
     template <bool PF, bool TR, typename R>
     __device__ inline void late(const S *__restrict__ f, long l, int n, R *Sr, R *Si)
@@ -432,7 +466,7 @@ struct Stage {
         if constexpr (PF)
             stash<TR>(Sr, Si);
         else
-            ld_link<P, NMAX, TR>(Sr, Si, f, l, n);
+            ld_link<P, NMAX, TR, EN>(Sr, Si, f, l, n);
     }
 };
 
@@ -448,6 +482,79 @@ __device__ inline R mom_el(const R *__restrict__ P, long l, int n, int i, int j,
     const int h = hup(n, lo, hi);
     im = i < j ? b[h + 1] : -b[h + 1];
     return b[h];
+}
+
+// Row and column (i < j) of entry v of the upper triangle in row order:
+// row i starts at s_i = i n - i (i + 1) / 2; i from the quadratic, then
+// corrected by one where the single-precision root rounds across a row start.
+__device__ inline void upper_ij(int n, int v, int &i, int &j)
+{
+    const float m = (float)(2 * n - 1);
+    int r = (int)(0.5f * (m - sqrtf(m * m - 8.0f * (float)v)));
+    r = r < 0 ? 0 : (r > n - 2 ? n - 2 : r);
+    if (r * n - r * (r + 1) / 2 > v) r--;
+    if ((r + 1) * n - (r + 1) * (r + 2) / 2 <= v) r++;
+    i = r;
+    j = v - (r * n - r * (r + 1) / 2) + r + 1;
+}
+
+// The kick on the compact momentum (n diagonal reals, then the upper
+// triangle's (re, im) pairs in row order, see hup): thread-strided units, a
+// unit being one diagonal element or one upper pair, so the momentum is read
+// and written in contiguous runs and each unit by one thread.  W = U_mu(x) S
+// at full precision in (Wr, Wi).  Drift: the new momentum also goes to
+// (Fr, Fi) as a full Hermitian matrix.  sum: {ReTr W, Tr Pin^2, Tr Pout^2}.
+// TAH: sunmat.h:355-371; kick: sun.cpp:716.
+template <int P, int NMAX, bool Drift, typename R>
+__device__ inline void kick_compact(const KickArgs &a, long l, int n, const R *Wr, const R *Wi, R *Fr, R *Fi,
+                                    double (&sum)[3])
+{
+    constexpr int LD = Cfg<P, NMAX>::LD, NT = Cfg<P, NMAX>::NT;
+    const R eps = (R)a.eps_kick, inv_n = (R)a.inv_n;
+    R tri = R(0);
+    for (int i = 0; i < n; i++) tri += Wi[i * LD + i] + Wi[i * LD + i];
+    const R corr = tri * inv_n;
+    R *b = (R *)a.P + (size_t)l * n * n;
+    R *bp = (!Drift && a.Pprev) ? (R *)a.Pprev + (size_t)l * n * n : nullptr;
+    const int units = n + n * (n - 1) / 2;
+    for (int u = threadIdx.x; u < units; u += NT) {
+        if (u < n) {
+            const R ir = b[u];
+            const R mi = (Wi[u * LD + u] + Wi[u * LD + u]) - corr;
+            const R pr = ir - (R(0.5) * mi) * eps;
+            b[u] = pr;
+            if (bp) bp[u] = ir;
+            sum[0] += (double)Wr[u * LD + u];
+            sum[1] += (double)ir * ir;
+            sum[2] += (double)pr * pr;
+            if constexpr (Drift) {
+                Fr[u * LD + u] = pr;
+                Fi[u * LD + u] = R(0);
+            }
+        } else {
+            const int v = u - n, h = n + 2 * v;
+            int i, j;
+            upper_ij(n, v, i, j);
+            const R ur = b[h], ui = b[h + 1];
+            const R dr = Wr[i * LD + j] - Wr[j * LD + i];
+            const R di = Wi[i * LD + j] + Wi[j * LD + i];
+            const R vr = ur - (R(0.5) * di) * eps, vi = ui + (R(0.5) * dr) * eps;
+            b[h] = vr;
+            b[h + 1] = vi;
+            if (bp) {
+                bp[h] = ur;
+                bp[h + 1] = ui;
+            }
+            sum[1] += 2.0 * ((double)ur * ur + (double)ui * ui);
+            sum[2] += 2.0 * ((double)vr * vr + (double)vi * vi);
+            if constexpr (Drift) {
+                Fr[i * LD + j] = vr;
+                Fi[i * LD + j] = vi;
+                Fr[j * LD + i] = vr;
+                Fi[j * LD + i] = -vi;
+            }
+        }
+    }
 }
 
 // LS: link storage type (FP32 links with FP64 arithmetic for the exact
@@ -489,7 +596,7 @@ __global__ __launch_bounds__((Cfg<P, NMAX>::NT), (kMinWaves<P, NMAX>)) void k_ki
     // operands in the order the products consume them, per direction nu:
     // U_nu(x+mu), U_mu(x+nu) | U_nu(x) | U_nu(x+mu-nu), U_mu(x-nu) | U_nu(x-nu)
     constexpr bool PF = Cfg<P, NMAX>::PF;
-    Stage<P, NMAX, LS> s0, s1;
+    Stage<P, NMAX, LS, EN> s0, s1;
     int nu = mu == 0 ? 1 : 0;
     s0.template early<PF, false>(U, nu * V + xp, n);
     s1.template early<PF, false>(U, mu * V + g.nbr[(2 * nu) * V + s], n);
@@ -508,9 +615,18 @@ __global__ __launch_bounds__((Cfg<P, NMAX>::NT), (kMinWaves<P, NMAX>)) void k_ki
         __syncthreads();
         s0.template early<PF, false>(U, nu * V + s, n);
         acc_zero(T);
-        cmul<P, NMAX, false, true>(Are, Aim, Bre, Bim, T);
-        __syncthreads();
-        acc_store<false>(T, Are, Aim);
+        if constexpr (P == kBF16X3) {
+            // the transpose T^T = conj(U_mu(x+nu)) U_nu(x+mu)^T from the slots
+            // as they stand: the transposed store then writes T row-major
+            // with one 8-byte write per plane and tile
+            cmul<P, NMAX, true, false>(Bre, Bim, Are, Aim, T);
+            __syncthreads();
+            acc_store<true>(T, Are, Aim);
+        } else {
+            cmul<P, NMAX, false, true>(Are, Aim, Bre, Bim, T);
+            __syncthreads();
+            acc_store<false>(T, Are, Aim);
+        }
         s0.template late<PF, false>(U, nu * V + s, n, Bre, Bim);
         __syncthreads();
         s0.template early<PF, true>(U, nu * V + xpmn, n);
@@ -523,9 +639,16 @@ __global__ __launch_bounds__((Cfg<P, NMAX>::NT), (kMinWaves<P, NMAX>)) void k_ki
         __syncthreads();
         s0.template early<PF, true>(U, nu * V + xmn, n);
         acc_zero(T);
-        cmul<P, NMAX, true, true>(Are, Aim, Bre, Bim, T);
-        __syncthreads();
-        acc_store<false>(T, Are, Aim);
+        if constexpr (P == kBF16X3) {
+            // T^T = conj(U_mu(x-nu)) conj(U_nu(x+mu-nu)), as above
+            cmul<P, NMAX, true, true>(Bre, Bim, Are, Aim, T);
+            __syncthreads();
+            acc_store<true>(T, Are, Aim);
+        } else {
+            cmul<P, NMAX, true, true>(Are, Aim, Bre, Bim, T);
+            __syncthreads();
+            acc_store<false>(T, Are, Aim);
+        }
         s0.template late<PF, true>(U, nu * V + xmn, n, Bre, Bim);
         __syncthreads();
         if (nx < g.D) {
@@ -559,73 +682,28 @@ __global__ __launch_bounds__((Cfg<P, NMAX>::NT), (kMinWaves<P, NMAX>)) void k_ki
         block_reduce_store<3>(sum, cmp, a.part + 6 * (size_t)lb);
         return;
     }
-    // TAH and kick on the lane's entries (sunmat.h:355-371, sun.cpp:716)
-    const R eps = (R)a.eps_kick, inv_n = (R)a.inv_n;
-    R *Pm = (R *)a.P;
-    R tri = R(0);
-    for (int i = 0; i < n; i++) tri += Aim[i * Cfg<P, NMAX>::LD + i] + Aim[i * Cfg<P, NMAX>::LD + i];
-    const R corr = tri * inv_n;
+    // TAH and kick on the compact momentum (sunmat.h:355-371, sun.cpp:716)
     double sum[3] = {0.0, 0.0, 0.0}, cmp[3] = {0.0, 0.0, 0.0};
-    R pr[NCT][4], pim[NCT][4], opr[NCT][4], opi[NCT][4];
-#pragma unroll
-    for (int t = 0; t < NCT; t++)
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int i = acc_row<P, NMAX>(q), j = acc_col<P, NMAX>(t);
-            pr[t][q] = pim[t][q] = opr[t][q] = opi[t][q] = R(0);
-            if (i >= n || j >= n) continue;
-            R ii;
-            const R ir = mom_el(Pm, l, n, i, j, ii);
-            opr[t][q] = ir;
-            opi[t][q] = ii;
-            if (i == j) {
-                const R mi = (Aim[i * Cfg<P, NMAX>::LD + i] + Aim[i * Cfg<P, NMAX>::LD + i]) - corr;
-                pr[t][q] = ir - (R(0.5) * mi) * eps;
-                sum[0] += (double)Are[i * Cfg<P, NMAX>::LD + i];
-            } else {
-                const int lo = i < j ? i : j, hi = i < j ? j : i;
-                const R dr = Are[lo * Cfg<P, NMAX>::LD + hi] - Are[hi * Cfg<P, NMAX>::LD + lo];
-                const R di = Aim[lo * Cfg<P, NMAX>::LD + hi] + Aim[hi * Cfg<P, NMAX>::LD + lo];
-                R ui;
-                const R ur = mom_el(Pm, l, n, lo, hi, ui);
-                const R vr = ur - (R(0.5) * di) * eps, vi = ui + (R(0.5) * dr) * eps;
-                pr[t][q] = vr;
-                pim[t][q] = i < j ? vi : -vi;
-            }
-            sum[1] += (double)ir * ir + (double)ii * ii;
-            sum[2] += (double)pr[t][q] * pr[t][q] + (double)pim[t][q] * pim[t][q];
-        }
-    __syncthreads();  // every lane has read the old momentum
-    {
-        R *b = Pm + (size_t)l * n * n;
-        R *bp = (!Drift && a.Pprev) ? (R *)a.Pprev + (size_t)l * n * n : nullptr;
+    kick_compact<P, NMAX, Drift>(a, l, n, Are, Aim, Bre, Bim, sum);
+    if constexpr (Drift) {
+        // rhs = i eps_d P into slot A; X = 1 + c_L rhs in registers; Horner
+        // steps X = 1 + c_k (rhs X) (sunmat.h:200-248).  The new momentum is
+        // in slot B as a full matrix.
+        constexpr int LD = Cfg<P, NMAX>::LD;
+        const R ed = (R)a.eps_drift;
+        int level = a.level;
+        if (level < 0 || level > 8) level = 8;
+        __syncthreads();
+        R pr[NCT][4], pim[NCT][4];
 #pragma unroll
         for (int t = 0; t < NCT; t++)
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const int i = acc_row<P, NMAX>(q), j = acc_col<P, NMAX>(t);
-                if (i >= n || j >= n || i > j) continue;
-                if (i == j) {
-                    b[i] = pr[t][q];
-                    if (bp) bp[i] = opr[t][q];
-                } else {
-                    const int h = hup(n, i, j);
-                    b[h] = pr[t][q];
-                    b[h + 1] = pim[t][q];
-                    if (bp) {
-                        bp[h] = opr[t][q];
-                        bp[h + 1] = opi[t][q];
-                    }
-                }
+                const bool in = EN || (i < n && j < n);
+                pr[t][q] = in ? Bre[i * LD + j] : R(0);
+                pim[t][q] = in ? Bim[i * LD + j] : R(0);
             }
-    }
-    if constexpr (Drift) {
-        // rhs = i eps_d P into slot A; X = 1 + c_L rhs in registers; Horner
-        // steps X = 1 + c_k (rhs X) (sunmat.h:200-248)
-        const R ed = (R)a.eps_drift;
-        int level = a.level;
-        if (level < 0 || level > 8) level = 8;
-        __syncthreads();
 #pragma unroll
         for (int t = 0; t < NCT; t++)
 #pragma unroll
@@ -668,18 +746,41 @@ __global__ __launch_bounds__((Cfg<P, NMAX>::NT), (kMinWaves<P, NMAX>)) void k_ki
         __syncthreads();
         Acc<P, NMAX> Un;
         acc_zero(Un);
-        cmul<P, NMAX, false, false>(Bre, Bim, Are, Aim, Un);
         R *ob = (R *)a.Uout + (size_t)l * 2 * n * n;
+        if constexpr (P == kBF16X3) {
+            // the transpose Un^T = U_mu(x)^T X^T (slot A holds U_mu(x)^T
+            // row-major, slot B X row-major): lane (t, q) holds Un[col][row +
+            // q], four consecutive elements of one row of the link
+            cmul<P, NMAX, false, false>(Are, Aim, Bre, Bim, Un);
 #pragma unroll
-        for (int t = 0; t < NCT; t++)
+            for (int t = 0; t < NCT; t++) {
+                const int i = acc_col<P, NMAX>(t), j0 = acc_row<P, NMAX>(0);
+                if constexpr (EN) {
+                    f32x4 *o4 = (f32x4 *)(ob + 2 * (i * NMAX + j0));
+                    o4[0] = f32x4{Un.r[t][0], Un.i[t][0], Un.r[t][1], Un.i[t][1]};
+                    o4[1] = f32x4{Un.r[t][2], Un.i[t][2], Un.r[t][3], Un.i[t][3]};
+                } else {
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int i = acc_row<P, NMAX>(q), j = acc_col<P, NMAX>(t);
-                if (i < n && j < n) {
-                    ob[2 * (i * n + j)] = Un.r[t][q];
-                    ob[2 * (i * n + j) + 1] = Un.i[t][q];
+                    for (int q = 0; q < 4; q++)
+                        if (i < n && j0 + q < n) {
+                            ob[2 * (i * n + j0 + q)] = Un.r[t][q];
+                            ob[2 * (i * n + j0 + q) + 1] = Un.i[t][q];
+                        }
                 }
             }
+        } else {
+            cmul<P, NMAX, false, false>(Bre, Bim, Are, Aim, Un);
+#pragma unroll
+            for (int t = 0; t < NCT; t++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int i = acc_row<P, NMAX>(q), j = acc_col<P, NMAX>(t);
+                    if (i < n && j < n) {
+                        ob[2 * (i * n + j)] = Un.r[t][q];
+                        ob[2 * (i * n + j) + 1] = Un.i[t][q];
+                    }
+                }
+        }
     }
     if (a.part) block_reduce_store<3>(sum, cmp, a.part + 6 * (size_t)lb);
 }
