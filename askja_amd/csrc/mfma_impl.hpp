@@ -485,12 +485,13 @@ __device__ inline R mom_el(const R *__restrict__ P, long l, int n, int i, int j,
 }
 
 // Row and column (i < j) of entry v of the upper triangle in row order:
-// row i starts at s_i = i n - i (i + 1) / 2; i from the quadratic, then
-// corrected by one where the single-precision root rounds across a row start.
+// row i starts at s_i = i n - i (i + 1) / 2; i from the quadratic (the
+// hardware square root, within an ulp), then corrected by one where the root
+// rounds across a row start.
 __device__ inline void upper_ij(int n, int v, int &i, int &j)
 {
     const float m = (float)(2 * n - 1);
-    int r = (int)(0.5f * (m - sqrtf(m * m - 8.0f * (float)v)));
+    int r = (int)(0.5f * (m - __builtin_amdgcn_sqrtf(m * m - 8.0f * (float)v)));
     r = r < 0 ? 0 : (r > n - 2 ? n - 2 : r);
     if (r * n - r * (r + 1) / 2 > v) r--;
     if ((r + 1) * n - (r + 1) * (r + 2) / 2 <= v) r++;
@@ -517,6 +518,7 @@ __device__ inline void kick_compact(const KickArgs &a, long l, int n, const R *W
     R *b = (R *)a.P + (size_t)l * n * n;
     R *bp = (!Drift && a.Pprev) ? (R *)a.Pprev + (size_t)l * n * n : nullptr;
     const int units = n + n * (n - 1) / 2;
+    const bool sums = a.part != nullptr;  // the energies are read from the partials
     for (int u = threadIdx.x; u < units; u += NT) {
         if (u < n) {
             const R ir = b[u];
@@ -524,9 +526,11 @@ __device__ inline void kick_compact(const KickArgs &a, long l, int n, const R *W
             const R pr = ir - (R(0.5) * mi) * eps;
             b[u] = pr;
             if (bp) bp[u] = ir;
-            sum[0] += (double)Wr[u * LD + u];
-            sum[1] += (double)ir * ir;
-            sum[2] += (double)pr * pr;
+            if (sums) {
+                sum[0] += (double)Wr[u * LD + u];
+                sum[1] += (double)ir * ir;
+                sum[2] += (double)pr * pr;
+            }
             if constexpr (Drift) {
                 Fr[u * LD + u] = pr;
                 Fi[u * LD + u] = R(0);
@@ -545,8 +549,10 @@ __device__ inline void kick_compact(const KickArgs &a, long l, int n, const R *W
                 bp[h] = ur;
                 bp[h + 1] = ui;
             }
-            sum[1] += 2.0 * ((double)ur * ur + (double)ui * ui);
-            sum[2] += 2.0 * ((double)vr * vr + (double)vi * vi);
+            if (sums) {
+                sum[1] += 2.0 * ((double)ur * ur + (double)ui * ui);
+                sum[2] += 2.0 * ((double)vr * vr + (double)vi * vi);
+            }
             if constexpr (Drift) {
                 Fr[i * LD + j] = vr;
                 Fi[i * LD + j] = vi;
