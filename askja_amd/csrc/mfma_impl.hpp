@@ -572,9 +572,11 @@ __device__ inline void kick_compact(const KickArgs &a, long l, int n, const R *W
 // faster (SU(32) 8^4 2.18 -> 1.67 ms, SU(24) 8^4 2.11 -> 1.64 ms, SU(32)
 // 96^2 1.63 -> 1.22 ms).  FP64 at NMAX = 16: four instead of three
 // (126 VGPRs + 32 AGPRs before, 16-24 bytes spilled now): SU(16) 6x24^3
-// 4.61 -> 4.46 ms, its action pass over FP32 links 3.12 -> 2.92 ms.
+// 4.61 -> 4.46 ms, its action pass over FP32 links 3.12 -> 2.92 ms.  FP32
+// and BF16x3 at NMAX = 16: five (96 VGPRs, at most a few spilled): SU(16)
+// 6x24^3 FP32 2.25 -> 2.19 ms.
 template <int P, int NMAX>
-constexpr int kMinWaves = (P == kF64 && NMAX == 32) ? 2 : (P == kF64 && NMAX == 16) ? 4 : 1;
+constexpr int kMinWaves = (P == kF64 && NMAX == 32) ? 2 : (P == kF64 && NMAX == 16) ? 4 : (NMAX == 16) ? 5 : 1;
 
 // EN: the link size N equals NMAX (no padding), a compile-time constant, so
 // that every bounds test on the padding folds away.
