@@ -81,7 +81,12 @@ struct askja_ctx {
     int flow_kind = kFlowRankOne;
     double *flow_st_wr = nullptr;    // stout: W and rho of every sub-layer's updated links
     double *flow_st_aux = nullptr;   // stout: inverse Jacobian and eigenbasis of every sub-layer
-    double *flow_st_lg = nullptr;    // stout: log-det gradient (one sub-layer)
+    double *flow_st_lg = nullptr;    // stout: log-det gradient of every sub-layer
+    // stout: the log-det gradients run on a second stream beside the rest of
+    // the flow gradient; per sub-layer an event after its forward (main
+    // stream) and one after its log-det gradient (second stream)
+    hipStream_t fst = nullptr;
+    std::vector<hipEvent_t> flow_ev;
     double *flow_sp_ptr = nullptr;   // spectral: feature plaquette traces (V doubles)
     double *flow_sp_dF = nullptr;    // spectral: their gradients (V doubles)
     double *flow_sp_inv = nullptr;   // spectral: (1 - tW)^{-1} of every sub-layer's updated links
@@ -427,6 +432,9 @@ int askja_destroy(askja_ctx *c)
     if (c->dl_ready) (void)hipEventDestroy(c->dl_ready);
     if (c->dl_done) (void)hipEventDestroy(c->dl_done);
     if (c->cst) (void)hipStreamDestroy(c->cst);
+    if (c->fst) (void)hipStreamSynchronize(c->fst);
+    for (hipEvent_t e : c->flow_ev) (void)hipEventDestroy(e);
+    if (c->fst) (void)hipStreamDestroy(c->fst);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->st) (void)hipStreamDestroy(c->st);
@@ -1127,8 +1135,17 @@ double *flow_aux_of(askja_ctx *c, int k)
     return c->flow_st_aux ? c->flow_st_aux + (size_t)k * stout_aux_doubles(c->geom) : nullptr;
 }
 
+double *flow_lg_of(askja_ctx *c, int k)
+{
+    return c->flow_st_lg ? c->flow_st_lg + (size_t)k * stout_lg_doubles(c->geom) : nullptr;
+}
+
 // U = F(V) into flow_out; with store, the input links of every sub-layer
-// go to flow_log.  Per sub-layer log J into flow_lj.
+// go to flow_log.  Per sub-layer log J into flow_lj.  With store, a stout
+// sub-layer's log-det gradient (which needs only that sub-layer's W, rho and
+// inverse Jacobians) is launched on the second stream as soon as its forward
+// is done, so that it runs beside the later sub-layers' forward and the
+// backward; the backward of the sub-layer waits for it.
 int flow_forward(askja_ctx *c, bool store)
 {
     HIPCHK(c, hipMemcpyAsync(c->flow_out, c->links[c->cur], link_bytes(c), hipMemcpyDeviceToDevice, c->st));
@@ -1158,9 +1175,17 @@ int flow_forward(askja_ctx *c, bool store)
         // stout: the per-link 8 x 8 log det from the W and rho just written
         // (with store, the inverse Jacobians and eigenbases are kept for the
         // backward)
-        if (f.kind == kFlowStout)
+        if (f.kind == kFlowStout) {
             HIPCHK(c, launch_stout_logdet(c->geom, f, flow_wr_of(c, k), c->part, store ? flow_aux_of(c, k) : nullptr,
                                           c->st));
+            if (store) {  // c_logj d log det J of this sub-layer's links (c_logj = -1)
+                HIPCHK(c, hipEventRecord(c->flow_ev[2 * k], c->st));
+                HIPCHK(c, hipStreamWaitEvent(c->fst, c->flow_ev[2 * k], 0));
+                HIPCHK(c, launch_stout_logdet_grad(c->geom, f, flow_wr_of(c, k), flow_aux_of(c, k), -1.0,
+                                                   flow_lg_of(c, k), c->fst));
+                HIPCHK(c, hipEventRecord(c->flow_ev[2 * k + 1], c->fst));
+            }
+        }
         HIPCHK(c, launch_finalize(c->part, flow_forward_partials(c->geom, f.par), 1, c->flow_lj + 2 * k, c->st));
     }
     return ASKJA_OK;
@@ -1230,11 +1255,10 @@ int flow_gradient(askja_ctx *c, double beta, bool wgrad, double *action)
     for (int k = c->flow_nsub - 1; k >= 0; k--) {
         const FlowSub &f = c->flow_subs[k];
         double *gw = wgrad ? c->flow_gw + (size_t)c->flow_block[k] * c->flow_bsize : nullptr;
-        if (f.kind == kFlowStout)  // c_logj d log det J of this sub-layer's links (c_logj = -1)
-            HIPCHK(c, launch_stout_logdet_grad(c->geom, f, flow_wr_of(c, k), flow_aux_of(c, k), -1.0, c->flow_st_lg,
-                                               c->st));
+        if (f.kind == kFlowStout)  // its log-det gradient, from the second stream
+            HIPCHK(c, hipStreamWaitEvent(c->st, c->flow_ev[2 * k + 1], 0));
         HIPCHK(c, launch_flow_backward(c->geom, c->flow_work, f, c->mode, c->flow_g, -1.0, gw, flow_log_of(c, k),
-                                       c->flow_contrib, c->flow_st_lg, c->st));
+                                       c->flow_contrib, flow_lg_of(c, k), c->st));
     }
     if (action) {
         double h[2];
@@ -1274,8 +1298,11 @@ int flow_free(askja_ctx *c)
                    c->flow_log, c->flow_contrib,  c->flow_save,  c->flow_st_wr,  c->flow_st_aux, c->flow_st_lg,
                    c->flow_sp_ptr, c->flow_sp_dF, c->flow_sp_inv};
     HIPCHK(c, hipStreamSynchronize(c->st));
+    if (c->fst) HIPCHK(c, hipStreamSynchronize(c->fst));
     for (void *p : old)
         if (p) (void)hipFree(p);
+    for (hipEvent_t e : c->flow_ev) (void)hipEventDestroy(e);
+    c->flow_ev.clear();
     c->flow_w = nullptr;
     c->flow_gw = c->flow_g = c->flow_lj = nullptr;
     c->flow_out = c->flow_work = c->flow_log = c->flow_save = nullptr;
@@ -1366,7 +1393,10 @@ int flow_define(askja_ctx *c, int kind, int nsub, const int *mu, const int *pari
     if (kind == kFlowStout) {
         HIPCHK(c, hipMalloc((void **)&c->flow_st_wr, (size_t)nsub * stout_wr_doubles(c->geom) * sizeof(double)));
         HIPCHK(c, hipMalloc((void **)&c->flow_st_aux, (size_t)nsub * stout_aux_doubles(c->geom) * sizeof(double)));
-        HIPCHK(c, hipMalloc((void **)&c->flow_st_lg, stout_lg_doubles(c->geom) * sizeof(double)));
+        HIPCHK(c, hipMalloc((void **)&c->flow_st_lg, (size_t)nsub * stout_lg_doubles(c->geom) * sizeof(double)));
+        if (!c->fst) HIPCHK(c, hipStreamCreateWithFlags(&c->fst, hipStreamNonBlocking));
+        c->flow_ev.assign((size_t)2 * nsub, nullptr);
+        for (hipEvent_t &e : c->flow_ev) HIPCHK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
     c->flow_subs.resize(nsub);
     c->flow_block.assign(block, block + nsub);
