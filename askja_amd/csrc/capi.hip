@@ -90,6 +90,7 @@ struct askja_ctx {
     double *flow_sp_ptr = nullptr;   // spectral: feature plaquette traces (V doubles)
     double *flow_sp_dF = nullptr;    // spectral: their gradients (V doubles)
     double *flow_sp_inv = nullptr;   // spectral: (1 - tW)^{-1} of every sub-layer's updated links
+    double *flow_gp = nullptr;       // the plain force's gradient at V (nested trajectory), flow_g's layout
 };
 
 namespace {
@@ -424,7 +425,8 @@ int askja_destroy(askja_ctx *c)
     void *bufs[] = {c->snap, c->mom, c->mom_prev, c->part, c->res, c->matwork, c->staging,
                     c->d_nbr, c->d_dev2lex, c->d_lex2dev, c->flow_w, c->flow_gw, c->flow_g, c->flow_lj,
                     c->flow_out, c->flow_work, c->flow_log, c->flow_contrib, c->flow_save,
-                    c->flow_st_wr, c->flow_st_aux, c->flow_st_lg, c->flow_sp_ptr, c->flow_sp_dF, c->flow_sp_inv};
+                    c->flow_st_wr, c->flow_st_aux, c->flow_st_lg, c->flow_sp_ptr, c->flow_sp_dF, c->flow_sp_inv,
+                    c->flow_gp};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     if (c->cst) (void)hipStreamSynchronize(c->cst);
@@ -1268,11 +1270,71 @@ int flow_gradient(askja_ctx *c, double beta, bool wgrad, double *action)
     return ASKJA_OK;
 }
 
-// P <- P - eps_c flow_g
-hipError_t flow_kick(askja_ctx *c, double eps_c)
+// P <- P - eps_c g, g in flow_g's layout
+hipError_t kick_with(askja_ctx *c, const double *g, double eps_c)
 {
-    if (c->flow_kind == kFlowSpectral) return launch_axpy((double *)c->mom, c->flow_g, -eps_c, (long)c->mom_reals, c->st);
-    return launch_kick_g(c->geom, c->mom, c->flow_g, eps_c, c->mode, c->st);
+    if (c->flow_kind == kFlowSpectral) return launch_axpy((double *)c->mom, g, -eps_c, (long)c->mom_reals, c->st);
+    return launch_kick_g(c->geom, c->mom, g, eps_c, c->mode, c->st);
+}
+
+// P <- P - eps_c flow_g
+hipError_t flow_kick(askja_ctx *c, double eps_c) { return kick_with(c, c->flow_g, eps_c); }
+
+// flow_gp <- the Lie gradient of beta S_W at the chain links V (the plain
+// force, in flow_g's layout: 9 doubles per link, or the momentum layout for
+// the spectral kind, whose gradient is formed by a kick from zero momenta)
+int wilson_grad_v(askja_ctx *c, double beta)
+{
+    const bool sp = c->flow_kind == kFlowSpectral;
+    const size_t n = sp ? c->mom_reals : (size_t)c->NL * 9;
+    if (!c->flow_gp) HIPCHK(c, hipMalloc((void **)&c->flow_gp, n * sizeof(double)));
+    if (sp) {
+        HIPCHK(c, hipMemsetAsync(c->flow_gp, 0, n * sizeof(double), c->st));
+        KickArgs a = kick_args(c, c->links[c->cur], nullptr, nullptr, -beta / (2.0 * c->nc), 0.0, 0);
+        a.P = c->flow_gp;
+        a.part = nullptr;
+        HIPCHK(c, launch_kick(a, c->mode, false, c->st));
+    } else {
+        int rc = ensure_part(c, (size_t)((c->NL + 255) / 256) * 2);
+        if (rc) return rc;
+        long np = 0;
+        HIPCHK(c, launch_wilson_grad(c->geom, c->links[c->cur], beta / (2.0 * c->nc), c->mode, c->flow_gp, c->part,
+                                     &np, c->st));
+    }
+    return ASKJA_OK;
+}
+
+// the kick of the flow's correction Phi - beta S_W: P <- P - eps_c (flow_g - flow_gp)
+int correction_kick(askja_ctx *c, double eps_c)
+{
+    HIPCHK(c, kick_with(c, c->flow_g, eps_c));
+    HIPCHK(c, kick_with(c, c->flow_gp, -eps_c));
+    return ASKJA_OK;
+}
+
+int kinetic_now(askja_ctx *c, double *k);
+
+// H, Metropolis and the reject path of a flow trajectory (links at the end
+// in links[cur], the start in flow_save)
+int flow_decide(askja_ctx *c, const askja_hmc_params *p, uint64_t traj_counter, double s0, double lj0, double k0,
+                double s1, askja_hmc_result *r)
+{
+    double lj1, k1;
+    int rc;
+    if ((rc = flow_logj(c, &lj1))) return rc;
+    if ((rc = kinetic_now(c, &k1))) return rc;
+    r->h_old = p->beta * s0 - lj0 + k0 / (2.0 * c->nc) * p->beta;
+    r->h_new = p->beta * s1 - lj1 + k1 / (2.0 * c->nc) * p->beta;
+    r->dh = r->h_new - r->h_old;
+    r->uniform = metropolis_uniform(p->seed, traj_counter);
+    if (r->h_new < r->h_old) r->accept_case = 1;
+    else if (r->uniform < std::exp(r->h_old - r->h_new)) r->accept_case = 2;
+    else r->accept_case = 0;
+    r->accepted = r->accept_case != 0;
+    if (!r->accepted)
+        HIPCHK(c, hipMemcpyAsync(c->links[c->cur], c->flow_save, link_bytes(c), hipMemcpyDeviceToDevice, c->st));
+    r->action = r->accepted ? s1 : s0;
+    return ASKJA_OK;
 }
 
 int kinetic_now(askja_ctx *c, double *k)
@@ -1296,7 +1358,7 @@ int flow_free(askja_ctx *c)
 {
     void *old[] = {c->flow_w,   c->flow_gw,       c->flow_g,     c->flow_lj,     c->flow_out,    c->flow_work,
                    c->flow_log, c->flow_contrib,  c->flow_save,  c->flow_st_wr,  c->flow_st_aux, c->flow_st_lg,
-                   c->flow_sp_ptr, c->flow_sp_dF, c->flow_sp_inv};
+                   c->flow_sp_ptr, c->flow_sp_dF, c->flow_sp_inv, c->flow_gp};
     HIPCHK(c, hipStreamSynchronize(c->st));
     if (c->fst) HIPCHK(c, hipStreamSynchronize(c->fst));
     for (void *p : old)
@@ -1307,7 +1369,7 @@ int flow_free(askja_ctx *c)
     c->flow_gw = c->flow_g = c->flow_lj = nullptr;
     c->flow_out = c->flow_work = c->flow_log = c->flow_save = nullptr;
     c->flow_st_wr = c->flow_st_aux = c->flow_st_lg = nullptr;
-    c->flow_contrib = c->flow_sp_ptr = c->flow_sp_dF = c->flow_sp_inv = nullptr;
+    c->flow_contrib = c->flow_sp_ptr = c->flow_sp_dF = c->flow_sp_inv = c->flow_gp = nullptr;
     c->flow_nsub = 0;
     return ASKJA_OK;
 }
@@ -1542,7 +1604,7 @@ int askja_flow_hmc_trajectory(askja_ctx *c, const askja_hmc_params *p, uint64_t 
     HIPCHK(c, hipMemcpyAsync(c->flow_save, c->links[c->cur], link_bytes(c), hipMemcpyDeviceToDevice, c->st));
     if ((rc = askja_mom_refresh(c, p->seed, traj_counter, p->beta))) return rc;
     const double cf = 2.0 * c->nc / p->beta;
-    double s0, lj0, k0, s1, lj1, k1;
+    double s0, lj0, k0, s1;
     if ((rc = kinetic_now(c, &k0))) return rc;
     if ((rc = flow_gradient(c, p->beta, false, &s0))) return rc;
     if ((rc = flow_logj(c, &lj0))) return rc;
@@ -1550,26 +1612,48 @@ int askja_flow_hmc_trajectory(askja_ctx *c, const askja_hmc_params *p, uint64_t 
     double ci[9];
     fill_cinv(ci);
     s1 = s0;
-    lj1 = lj0;
     for (int t = 1; t <= p->ndrift; t++) {
         HIPCHK(c, launch_drift(c->geom, c->links[c->cur], c->mom, p->dt, p->level, ci, c->mode, c->st));
         if ((rc = flow_gradient(c, p->beta, false, t == p->ndrift ? &s1 : nullptr))) return rc;
         HIPCHK(c, flow_kick(c, (t < p->ndrift ? 1.0 : 0.5) * p->dt * cf));
     }
-    if ((rc = flow_logj(c, &lj1))) return rc;
-    if ((rc = kinetic_now(c, &k1))) return rc;
-    r->h_old = p->beta * s0 - lj0 + k0 / (2.0 * c->nc) * p->beta;
-    r->h_new = p->beta * s1 - lj1 + k1 / (2.0 * c->nc) * p->beta;
-    r->dh = r->h_new - r->h_old;
-    r->uniform = metropolis_uniform(p->seed, traj_counter);
-    if (r->h_new < r->h_old) r->accept_case = 1;
-    else if (r->uniform < std::exp(r->h_old - r->h_new)) r->accept_case = 2;
-    else r->accept_case = 0;
-    r->accepted = r->accept_case != 0;
-    if (!r->accepted)
-        HIPCHK(c, hipMemcpyAsync(c->links[c->cur], c->flow_save, link_bytes(c), hipMemcpyDeviceToDevice, c->st));
-    r->action = r->accepted ? s1 : s0;
-    return ASKJA_OK;
+    return flow_decide(c, p, traj_counter, s0, lj0, k0, s1, r);
+}
+
+int askja_flow_hmc_trajectory_nested(askja_ctx *c, const askja_hmc_params *p, int inner, uint64_t traj_counter,
+                                     askja_hmc_result *r)
+{
+    ENTER(c);
+    int rc = flow_ready(c);
+    if (rc) return rc;
+    if (!p || !r) return fail(c, ASKJA_EINVAL, "flow_hmc_trajectory_nested: null argument");
+    if (!(p->beta > 0) || !(p->dt > 0) || p->ndrift < 1)
+        return fail(c, ASKJA_EINVAL, "flow_hmc_trajectory_nested: beta, dt and ndrift must be positive");
+    if (inner < 1 || p->ndrift % inner != 0)
+        return fail(c, ASKJA_EINVAL, "flow_hmc_trajectory_nested: inner (%d) must be positive and divide ndrift (%d)",
+                    inner, p->ndrift);
+    if (p->reunit && (rc = askja_reunitarize(c))) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->flow_save, c->links[c->cur], link_bytes(c), hipMemcpyDeviceToDevice, c->st));
+    if ((rc = askja_mom_refresh(c, p->seed, traj_counter, p->beta))) return rc;
+    const double cf = 2.0 * c->nc / p->beta, h = inner * p->dt;
+    const int m = p->ndrift / inner;
+    double s0, lj0, k0, s1;
+    if ((rc = kinetic_now(c, &k0))) return rc;
+    if ((rc = flow_gradient(c, p->beta, false, &s0))) return rc;
+    if ((rc = flow_logj(c, &lj0))) return rc;
+    if ((rc = wilson_grad_v(c, p->beta))) return rc;
+    if ((rc = correction_kick(c, 0.5 * h * cf))) return rc;
+    s1 = s0;
+    for (int o = 1; o <= m; o++) {
+        // inner leapfrog of the plain action, the fused kick+drift kernels
+        int fin = c->cur;
+        if ((rc = md_pingpong(c, p->dt, inner, p->level, &fin))) return rc;
+        c->cur = fin;
+        if ((rc = flow_gradient(c, p->beta, false, o == m ? &s1 : nullptr))) return rc;
+        if ((rc = wilson_grad_v(c, p->beta))) return rc;
+        if ((rc = correction_kick(c, (o < m ? 1.0 : 0.5) * h * cf))) return rc;
+    }
+    return flow_decide(c, p, traj_counter, s0, lj0, k0, s1, r);
 }
 
 int askja_flow_wilson_lines(askja_ctx *c, int dir, double *out)

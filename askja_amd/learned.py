@@ -136,14 +136,15 @@ def flow_relax(lat, beta, level, seed, first, n=20, dt=0.005, ndrift=200):
         lat.flow_hmc_trajectory(beta, dt, ndrift, level, seed, first + k, reunit=True)
 
 
-def flow_chain(lat, beta, dt, ndrift, level, seed, first, n, log=None, plaq=None):
+def flow_chain(lat, beta, dt, ndrift, level, seed, first, n, log=None, plaq=None, inner=None):
     """n FTHMC trajectories; |P| (and the plaquette, into `plaq`) measured on
-    the physical configuration U = F(V)."""
+    the physical configuration U = F(V).  inner: the two-time-scale
+    trajectory, the flow's correction kicking every `inner` fine steps."""
     series, acc = [], 0
     lat.sync()
     t0 = tl = time.perf_counter()
     for k in range(n):
-        r = lat.flow_hmc_trajectory(beta, dt, ndrift, level, seed, first + k, reunit=True)
+        r = lat.flow_hmc_trajectory(beta, dt, ndrift, level, seed, first + k, reunit=True, inner=inner)
         acc += r.accepted
         series.append(abs(lat.flow_polyakov()))
         if plaq is not None:
@@ -272,7 +273,7 @@ def c5_study(args, log):
         lf.flow_set_weights(wt)
         fq = []
         ser, acc, secs = flow_chain(lf, beta, args.flow_dt, args.flow_ndrift, level, seed, 4 * 10**6,
-                                    args.ntraj_flow, log=log, plaq=fq)
+                                    args.ntraj_flow, log=log, plaq=fq, inner=args.flow_inner)
         summarise("flow_trained", fq, acc, secs, args.ntraj_flow, ser)
         lf.sync()
         lf.timer_start()
@@ -319,6 +320,8 @@ def main():
     ap.add_argument("--ntraj-flow", type=int, default=20, help="c5: FTHMC trajectories")
     ap.add_argument("--flow-dt", type=float, default=0.005, help="c5: FTHMC step")
     ap.add_argument("--flow-ndrift", type=int, default=200, help="c5: FTHMC MD steps per trajectory")
+    ap.add_argument("--flow-inner", type=int, default=None,
+                    help="two-time-scale FTHMC: the flow's correction kicks every this many plain MD steps")
     args = ap.parse_args()
     if args.workload == "c5":
         import torch
@@ -393,7 +396,7 @@ def main():
         log(f"[learned] {name} chain")
         fq = []
         fl, acc_f, t_f = flow_chain(lat, args.beta, args.dt, args.ndrift, args.level, args.seed, 4 * 10**6, args.ntraj,
-                                    log=log, plaq=fq)
+                                    log=log, plaq=fq, inner=args.flow_inner)
         res[name] = {"traj_per_s": args.ntraj / t_f, "acceptance": acc_f, "tau": tau_int(fl),
                      "abs_polyakov": mean_err(fl), "plaquette": mean_err(fq), "ntraj": args.ntraj}
         series[name + "_abs_polyakov"], series[name + "_plaquette"] = fl, np.asarray(fq)

@@ -16,8 +16,12 @@ one sweep costs a quarter of the four-sweep map
 thermalised by plain HMC (untimed) before either chain.
 
 One step = one learned-proposal trajectory: momentum refresh, 50 MD steps
-with the flow force (forward and hand-written backward of every sub-layer
-per step), H with log J, Metropolis, the Polyakov loop of U.
+of 0.02 in two time scales (Sexton and Weingarten's nested leapfrog): the
+plain Wilson force on every step (the fused kick+drift), the flow's
+correction Phi - beta S_W every 5 steps (the forward and hand-written
+backward of every sub-layer, 11 flow gradients per trajectory instead of
+51; ``--flow-inner 1`` runs the one-scale trajectory), H with log J,
+Metropolis, the Polyakov loop of U.
 --warmup trajectories of each chain run untimed, then exactly --steps
 learned trajectories are timed; the plain chain is timed over its own
 series.
@@ -371,6 +375,12 @@ C4_THERM = 300            # plain trajectories from the Haar start, untimed
 # no trajectory accepted); 20 trajectories of dt = 0.005 x 200 steps (90 %
 # accepted) bring it to where dt = 0.02 is accepted 70 % of the time.
 C4_RELAX = (20, 0.005, 200)
+# the learned chain's integrator: two time scales, the plain force on every
+# MD step of C4_DT, the flow's correction Phi - beta S_W every C4_FLOW_INNER
+# steps (1: the one-scale trajectory).  Scanned at this workload with the
+# trained one-sweep map: 2 and 5 keep the acceptance at 0.83 and 0.75, 10
+# (a coarse step of 0.2) is unstable.
+C4_FLOW_INNER = 5
 C4_WEIGHTS = os.path.join(ROOT, "askja_amd", "data", "flow_c4_weights_1sweep.json")
 TAU_SPAN = 100            # a tau estimate needs a series of >= 100 tau
 MIN_SERIES = {"plain": 3000, "learned": 300}
@@ -433,9 +443,10 @@ def c4_chain(lat, kind, first, args, log):
     """--warmup untimed trajectories, then a series whose first `ntimed`
     trajectories are timed; the series is extended (untimed) until it spans
     TAU_SPAN tau or reaches the cap."""
+    inner = args.flow_inner if args.flow_inner > 1 else None
     step = (lambda c: lat.hmc_trajectory(C4_BETA, C4_DT, C4_NDRIFT, C4_LEVEL, SEED, c, reunit=True)) \
         if kind == "plain" else \
-        (lambda c: lat.flow_hmc_trajectory(C4_BETA, C4_DT, C4_NDRIFT, C4_LEVEL, SEED, c, reunit=True))
+        (lambda c: lat.flow_hmc_trajectory(C4_BETA, C4_DT, C4_NDRIFT, C4_LEVEL, SEED, c, reunit=True, inner=inner))
     measure = lat.polyakov if kind == "plain" else lat.flow_polyakov
     if kind == "learned":
         n, dt, nd = C4_RELAX
@@ -657,7 +668,11 @@ def main_c4(args):
                                f"({recipe['sublayers']} sub-layers), trained on {recipe['ntrain']} plain configs "
                                f"at beta {recipe['beta_train']}",
                    "nc": C4_NC, "extents": C4_EXT, "beta": C4_BETA, "dt": C4_DT, "md_steps_per_traj": C4_NDRIFT,
-                   "level": C4_LEVEL, "precision": "fp64", "weights": os.path.relpath(args.weights, ROOT)},
+                   "level": C4_LEVEL, "precision": "fp64", "weights": os.path.relpath(args.weights, ROOT),
+                   "learned_integrator": (f"two time scales: plain force every {C4_DT} step, the flow's correction "
+                                          f"every {args.flow_inner} steps ({args.flow_inner * C4_DT:g}); "
+                                          f"{C4_NDRIFT // args.flow_inner + 1} flow gradients per trajectory")
+                   if args.flow_inner > 1 else f"one time scale: {C4_NDRIFT + 1} flow gradients per trajectory"},
         "learned": learned,
         "plain": plain,
         "gain": gain,
@@ -694,6 +709,8 @@ def main():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--weights", default=C4_WEIGHTS)
+    ap.add_argument("--flow-inner", type=int, default=C4_FLOW_INNER,
+                    help="learned chain: plain MD steps per kick of the flow's correction (1: one time scale)")
     ap.add_argument("--layers", type=int, default=None,
                     help="c4: use only the first LAYERS sweep layers of the trained map (default: all)")
     ap.add_argument("--learned-min-series", type=int, default=MIN_SERIES["learned"],
@@ -715,6 +732,8 @@ def main():
     ap.add_argument("--study-therm", type=int, default=20, help="thermalising trajectories (c5exact)")
     ap.add_argument("--study-precisions", default="bf16x3,fp64", help="precisions (c5exact)")
     args = ap.parse_args()
+    if args.flow_inner < 1 or C4_NDRIFT % args.flow_inner:
+        raise SystemExit(f"--flow-inner must be a positive divisor of the {C4_NDRIFT} MD steps")
     if args.copy_probe:
         copy_probe()
         return
