@@ -284,7 +284,8 @@ def c5_study(args, log):
         res["flow_trained"]["ms_per_plain_kick_drift"] = lf.timer_stop()
     out = {"workload": f"SU(64) D=2 128^2 beta=4096 DT={dt} {nd} MD steps LEVEL=8; FTHMC: spectral map, "
                        f"{args.layers} sweep(s) ({len(mu)} sub-layers), t_max {tmax}, trained on {args.ntrain} "
-                       f"plain-HMC configs at beta={args.beta_train}, flow dt {args.flow_dt} x {args.flow_ndrift}",
+                       f"plain-HMC configs at beta={args.beta_train}, flow dt {args.flow_dt} x {args.flow_ndrift}"
+                       + (f", the flow's correction every {args.flow_inner} steps" if args.flow_inner else ""),
            "exact_plaquette": want, "therm": args.therm, "train_seconds": t_train, "loss": losses, "results": res}
     print(json.dumps(out))
 
