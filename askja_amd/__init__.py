@@ -104,7 +104,7 @@ SIGNATURES = {
     "askja_flow_gradient_download": (_I, [_P, _DP]),
     "askja_flow_weight_gradient": (_I, [_P, _DP]),
     "askja_flow_hmc_trajectory": (_I, [_P, C.POINTER(HmcParams), _U64, C.POINTER(HmcResult)]),
-    "askja_flow_hmc_trajectory_nested": (_I, [_P, C.POINTER(HmcParams), _I, _U64, C.POINTER(HmcResult)]),
+    "askja_flow_hmc_trajectory_nested": (_I, [_P, C.POINTER(HmcParams), _I, _D, _U64, C.POINTER(HmcResult)]),
     "askja_flow_wilson_lines": (_I, [_P, _I, _DP]),
     "askja_sync": (_I, [_P]),
     "askja_timer_start": (_I, [_P]),
@@ -404,17 +404,17 @@ class Lattice:
         self._ck(lib().askja_flow_weight_gradient(self._h, _dptr(out)), "flow_weight_gradient")
         return out
 
-    def flow_hmc_trajectory(self, beta, dt, ndrift, level, seed, counter, reunit=True, inner=None):
+    def flow_hmc_trajectory(self, beta, dt, ndrift, level, seed, counter, reunit=True, inner=None, alpha=1.0):
         """One FTHMC trajectory of ndrift steps of dt.  inner: the two-time-scale
         form (askja_flow_hmc_trajectory_nested), the flow's correction to the
-        plain force kicking every `inner` fine steps."""
+        force of alpha beta S_W kicking every `inner` fine steps."""
         p = HmcParams(beta, dt, ndrift, level, 1 if reunit else 0, seed)
         r = HmcResult()
         if inner is None:
             self._ck(lib().askja_flow_hmc_trajectory(self._h, C.byref(p), counter, C.byref(r)), "flow_hmc_trajectory")
         else:
-            self._ck(lib().askja_flow_hmc_trajectory_nested(self._h, C.byref(p), int(inner), counter, C.byref(r)),
-                     "flow_hmc_trajectory_nested")
+            self._ck(lib().askja_flow_hmc_trajectory_nested(self._h, C.byref(p), int(inner), float(alpha), counter,
+                                                            C.byref(r)), "flow_hmc_trajectory_nested")
         return r
 
     def flow_polyakov(self):

@@ -72,7 +72,7 @@ struct askja_ctx {
     float *flow_w = nullptr;     // nblocks * bsize MLP weights
     double *flow_gw = nullptr;   // their gradient
     double *flow_g = nullptr;    // 9 doubles per link: Lie gradient
-    double *flow_lj = nullptr;   // per sub-layer log J (s, c)
+    double *flow_lj = nullptr;   // log J (s, c) of the whole map
     void *flow_out = nullptr;    // U = F(V)
     void *flow_work = nullptr;   // backward working field
     void *flow_log = nullptr;    // input links of every sub-layer
@@ -246,26 +246,29 @@ int exact_action(askja_ctx *c, const void *U, double *s)
 // MD of one trajectory with the chain-state buffer `in0` left untouched;
 // returns the buffer index that holds the final links.  Energies of the
 // first and last kick go to res[0..5] and res[6..11].
-int md_pingpong(askja_ctx *c, double dt, int ndrift, int level, int *final_buf)
+// The plain leapfrog of ndrift steps (kicks scaled by kscale: the force of
+// kscale beta S_W)
+int md_pingpong(askja_ctx *c, double dt, int ndrift, int level, int *final_buf, double kscale = 1.0)
 {
     const int a = c->cur, b = (a + 1) % 3, d = (a + 2) % 3;
+    const double kd = kscale * dt;
     int rc;
     if (ndrift <= 0) {
         // two half kicks on the same links
-        if ((rc = do_kick(c, c->links[a], nullptr, nullptr, dt / 2, 0.0, level, false, 0))) return rc;
-        if ((rc = do_kick(c, c->links[a], nullptr, nullptr, dt / 2, 0.0, level, false, 6))) return rc;
+        if ((rc = do_kick(c, c->links[a], nullptr, nullptr, kd / 2, 0.0, level, false, 0))) return rc;
+        if ((rc = do_kick(c, c->links[a], nullptr, nullptr, kd / 2, 0.0, level, false, 6))) return rc;
         *final_buf = a;
         return ASKJA_OK;
     }
-    if ((rc = do_kick(c, c->links[a], c->links[b], nullptr, dt / 2, dt, level, true, 0))) return rc;
+    if ((rc = do_kick(c, c->links[a], c->links[b], nullptr, kd / 2, dt, level, true, 0))) return rc;
     int in = b, out = d;
     for (int t = 2; t <= ndrift; t++) {
-        if ((rc = do_kick(c, c->links[in], c->links[out], nullptr, dt, dt, level, true, -1))) return rc;
+        if ((rc = do_kick(c, c->links[in], c->links[out], nullptr, kd, dt, level, true, -1))) return rc;
         const int tmp = in;
         in = out;
         out = tmp;
     }
-    if ((rc = do_kick(c, c->links[in], nullptr, nullptr, dt / 2, 0.0, level, false, 6))) return rc;
+    if ((rc = do_kick(c, c->links[in], nullptr, nullptr, kd / 2, 0.0, level, false, 6))) return rc;
     *final_buf = in;
     return ASKJA_OK;
 }
@@ -1143,7 +1146,9 @@ double *flow_lg_of(askja_ctx *c, int k)
 }
 
 // U = F(V) into flow_out; with store, the input links of every sub-layer
-// go to flow_log.  Per sub-layer log J into flow_lj.  With store, a stout
+// go to flow_log.  Every sub-layer writes its log J partials to its own rows
+// of the partial buffer, and one finalize sums them all into flow_lj.  With
+// store, a stout
 // sub-layer's log-det gradient (which needs only that sub-layer's W, rho and
 // inverse Jacobians) is launched on the second stream as soon as its forward
 // is done, so that it runs beside the later sub-layers' forward and the
@@ -1153,32 +1158,35 @@ int flow_forward(askja_ctx *c, bool store)
     HIPCHK(c, hipMemcpyAsync(c->flow_out, c->links[c->cur], link_bytes(c), hipMemcpyDeviceToDevice, c->st));
     if (c->flow_kind == kFlowSpectral) {
         const long na = sflow_active(c->geom);
-        int rc = ensure_part(c, (size_t)na * 2);
+        int rc = ensure_part(c, (size_t)na * 2 * c->flow_nsub);
         if (rc) return rc;
         for (int k = 0; k < c->flow_nsub; k++) {
             const FlowSub &f = c->flow_subs[k];
             const size_t ko = (size_t)k * flow_log_stride(c);
             HIPCHK(c, launch_sflow_forward(c->geom, c->flow_out, f.mu, f.par, f.w, f.rho_max, c->flow_sp_ptr,
                                            store ? (double *)flow_log_of(c, k) : nullptr,
-                                           store ? c->flow_sp_inv + ko : nullptr, c->part, c->st));
-            HIPCHK(c, launch_finalize(c->part, na, 1, c->flow_lj + 2 * k, c->st));
+                                           store ? c->flow_sp_inv + ko : nullptr, c->part + (size_t)2 * na * k,
+                                           c->st));
         }
+        HIPCHK(c, launch_finalize(c->part, na * c->flow_nsub, 1, c->flow_lj, c->st));
         return ASKJA_OK;
     }
-    const long np = flow_forward_partials(c->geom, 0) > flow_forward_partials(c->geom, 1)
-                        ? flow_forward_partials(c->geom, 0)
-                        : flow_forward_partials(c->geom, 1);
-    int rc = ensure_part(c, (size_t)np * 2);
+    long np_all = 0;
+    for (int k = 0; k < c->flow_nsub; k++) np_all += flow_forward_partials(c->geom, c->flow_subs[k].par);
+    int rc = ensure_part(c, (size_t)np_all * 2);
     if (rc) return rc;
+    long off = 0;  // the sub-layer's first partial row
     for (int k = 0; k < c->flow_nsub; k++) {
         const FlowSub &f = c->flow_subs[k];
-        HIPCHK(c, launch_flow_forward(c->geom, c->flow_out, c->flow_out, f, c->mode, c->part,
+        double *pk = c->part + 2 * off;
+        off += flow_forward_partials(c->geom, f.par);
+        HIPCHK(c, launch_flow_forward(c->geom, c->flow_out, c->flow_out, f, c->mode, pk,
                                       store ? flow_log_of(c, k) : nullptr, flow_wr_of(c, k), c->st));
         // stout: the per-link 8 x 8 log det from the W and rho just written
         // (with store, the inverse Jacobians and eigenbases are kept for the
         // backward)
         if (f.kind == kFlowStout) {
-            HIPCHK(c, launch_stout_logdet(c->geom, f, flow_wr_of(c, k), c->part, store ? flow_aux_of(c, k) : nullptr,
+            HIPCHK(c, launch_stout_logdet(c->geom, f, flow_wr_of(c, k), pk, store ? flow_aux_of(c, k) : nullptr,
                                           c->st));
             if (store) {  // c_logj d log det J of this sub-layer's links (c_logj = -1)
                 HIPCHK(c, hipEventRecord(c->flow_ev[2 * k], c->st));
@@ -1188,19 +1196,17 @@ int flow_forward(askja_ctx *c, bool store)
                 HIPCHK(c, hipEventRecord(c->flow_ev[2 * k + 1], c->fst));
             }
         }
-        HIPCHK(c, launch_finalize(c->part, flow_forward_partials(c->geom, f.par), 1, c->flow_lj + 2 * k, c->st));
     }
+    HIPCHK(c, launch_finalize(c->part, np_all, 1, c->flow_lj, c->st));
     return ASKJA_OK;
 }
 
 int flow_logj(askja_ctx *c, double *logj)
 {
-    std::vector<double> h((size_t)2 * c->flow_nsub);
-    HIPCHK(c, hipMemcpyAsync(h.data(), c->flow_lj, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    double h[2];
+    HIPCHK(c, hipMemcpyAsync(h, c->flow_lj, sizeof h, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
-    double s = 0.0;
-    for (int k = 0; k < c->flow_nsub; k++) s += h[2 * k] + h[2 * k + 1];
-    *logj = s;
+    *logj = h[0] + h[1];
     return ASKJA_OK;
 }
 
@@ -1304,11 +1310,12 @@ int wilson_grad_v(askja_ctx *c, double beta)
     return ASKJA_OK;
 }
 
-// the kick of the flow's correction Phi - beta S_W: P <- P - eps_c (flow_g - flow_gp)
-int correction_kick(askja_ctx *c, double eps_c)
+// the kick of the flow's correction Phi - alpha beta S_W:
+// P <- P - eps_c (flow_g - alpha flow_gp)
+int correction_kick(askja_ctx *c, double eps_c, double alpha)
 {
     HIPCHK(c, kick_with(c, c->flow_g, eps_c));
-    HIPCHK(c, kick_with(c, c->flow_gp, -eps_c));
+    HIPCHK(c, kick_with(c, c->flow_gp, -alpha * eps_c));
     return ASKJA_OK;
 }
 
@@ -1620,8 +1627,8 @@ int askja_flow_hmc_trajectory(askja_ctx *c, const askja_hmc_params *p, uint64_t 
     return flow_decide(c, p, traj_counter, s0, lj0, k0, s1, r);
 }
 
-int askja_flow_hmc_trajectory_nested(askja_ctx *c, const askja_hmc_params *p, int inner, uint64_t traj_counter,
-                                     askja_hmc_result *r)
+int askja_flow_hmc_trajectory_nested(askja_ctx *c, const askja_hmc_params *p, int inner, double alpha,
+                                     uint64_t traj_counter, askja_hmc_result *r)
 {
     ENTER(c);
     int rc = flow_ready(c);
@@ -1632,6 +1639,8 @@ int askja_flow_hmc_trajectory_nested(askja_ctx *c, const askja_hmc_params *p, in
     if (inner < 1 || p->ndrift % inner != 0)
         return fail(c, ASKJA_EINVAL, "flow_hmc_trajectory_nested: inner (%d) must be positive and divide ndrift (%d)",
                     inner, p->ndrift);
+    if (!(alpha > 0) || !std::isfinite(alpha))
+        return fail(c, ASKJA_EINVAL, "flow_hmc_trajectory_nested: alpha must be positive and finite");
     if (p->reunit && (rc = askja_reunitarize(c))) return rc;
     HIPCHK(c, hipMemcpyAsync(c->flow_save, c->links[c->cur], link_bytes(c), hipMemcpyDeviceToDevice, c->st));
     if ((rc = askja_mom_refresh(c, p->seed, traj_counter, p->beta))) return rc;
@@ -1642,16 +1651,16 @@ int askja_flow_hmc_trajectory_nested(askja_ctx *c, const askja_hmc_params *p, in
     if ((rc = flow_gradient(c, p->beta, false, &s0))) return rc;
     if ((rc = flow_logj(c, &lj0))) return rc;
     if ((rc = wilson_grad_v(c, p->beta))) return rc;
-    if ((rc = correction_kick(c, 0.5 * h * cf))) return rc;
+    if ((rc = correction_kick(c, 0.5 * h * cf, alpha))) return rc;
     s1 = s0;
     for (int o = 1; o <= m; o++) {
-        // inner leapfrog of the plain action, the fused kick+drift kernels
+        // inner leapfrog of alpha beta S_W, the fused kick+drift kernels
         int fin = c->cur;
-        if ((rc = md_pingpong(c, p->dt, inner, p->level, &fin))) return rc;
+        if ((rc = md_pingpong(c, p->dt, inner, p->level, &fin, alpha))) return rc;
         c->cur = fin;
         if ((rc = flow_gradient(c, p->beta, false, o == m ? &s1 : nullptr))) return rc;
         if ((rc = wilson_grad_v(c, p->beta))) return rc;
-        if ((rc = correction_kick(c, (o < m ? 1.0 : 0.5) * h * cf))) return rc;
+        if ((rc = correction_kick(c, (o < m ? 1.0 : 0.5) * h * cf, alpha))) return rc;
     }
     return flow_decide(c, p, traj_counter, s0, lj0, k0, s1, r);
 }

@@ -136,7 +136,23 @@ def flow_relax(lat, beta, level, seed, first, n=20, dt=0.005, ndrift=200):
         lat.flow_hmc_trajectory(beta, dt, ndrift, level, seed, first + k, reunit=True)
 
 
-def flow_chain(lat, beta, dt, ndrift, level, seed, first, n, log=None, plaq=None, inner=None):
+def plain_force_scale(lat, beta):
+    """alpha = <F_Phi, F_W> / <F_W, F_W> at the current V: the multiple of the
+    plain Wilson force closest (least squares over every link) to the flow
+    force, for the two-time-scale trajectory, whose coarse step then carries
+    only Phi - alpha beta S_W.  The momenta are left as they were."""
+    cf = 2 * lat.nc / beta
+    lat.flow_gradient(beta)
+    fphi = lat.flow_link_gradient() * cf
+    p0 = lat.momenta()
+    lat.set_momenta(np.zeros_like(p0))
+    lat.updateE(1.0)  # P = -TA(W): minus the plain force
+    fw = -lat.momenta()
+    lat.set_momenta(p0)
+    return float(np.vdot(fw, fphi).real / np.vdot(fw, fw).real)
+
+
+def flow_chain(lat, beta, dt, ndrift, level, seed, first, n, log=None, plaq=None, inner=None, alpha=1.0):
     """n FTHMC trajectories; |P| (and the plaquette, into `plaq`) measured on
     the physical configuration U = F(V).  inner: the two-time-scale
     trajectory, the flow's correction kicking every `inner` fine steps."""
@@ -144,7 +160,7 @@ def flow_chain(lat, beta, dt, ndrift, level, seed, first, n, log=None, plaq=None
     lat.sync()
     t0 = tl = time.perf_counter()
     for k in range(n):
-        r = lat.flow_hmc_trajectory(beta, dt, ndrift, level, seed, first + k, reunit=True, inner=inner)
+        r = lat.flow_hmc_trajectory(beta, dt, ndrift, level, seed, first + k, reunit=True, inner=inner, alpha=alpha)
         acc += r.accepted
         series.append(abs(lat.flow_polyakov()))
         if plaq is not None:

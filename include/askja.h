@@ -322,17 +322,19 @@ int askja_flow_weight_gradient(askja_ctx *ctx, double *out);
 int askja_flow_hmc_trajectory(askja_ctx *ctx, const askja_hmc_params *p, uint64_t traj_counter,
                               askja_hmc_result *res);
 /* The same trajectory with two time scales (Sexton and Weingarten's nested
- * leapfrog): Phi = beta S_W(V) + [Phi(V) - beta S_W(V)].  The plain action
- * moves on the fine step dt with the fused kick+drift of askja_hmc_trajectory,
- * `inner` steps per coarse step; the flow's correction Phi - beta S_W kicks on
- * the coarse step inner * dt, so a trajectory of ndrift fine steps (inner
- * must divide ndrift) evaluates the flow gradient ndrift / inner + 1 times
- * instead of ndrift + 1.  The splitting is symplectic and reversible and H is
- * the same, so the accept test keeps the chain exact; inner = 1 is
- * askja_flow_hmc_trajectory up to rounding.  No counterpart in the reference
- * (no learned proposal there). */
-int askja_flow_hmc_trajectory_nested(askja_ctx *ctx, const askja_hmc_params *p, int inner, uint64_t traj_counter,
-                                     askja_hmc_result *res);
+ * leapfrog): Phi = alpha beta S_W(V) + [Phi(V) - alpha beta S_W(V)].  The
+ * scaled plain action moves on the fine step dt with the fused kick+drift of
+ * askja_hmc_trajectory, `inner` steps per coarse step; the flow's correction
+ * kicks on the coarse step inner * dt, so a trajectory of ndrift fine steps
+ * (inner must divide ndrift) evaluates the flow gradient ndrift / inner + 1
+ * times instead of ndrift + 1.  alpha > 0 is free (the split is exact for
+ * any): the closer alpha beta S_W's force is to Phi's, the smaller the
+ * correction and the longer the coarse step it allows.  The splitting is
+ * symplectic and reversible and H is the same, so the accept test keeps the
+ * chain exact; inner = 1, alpha = 1 is askja_flow_hmc_trajectory up to
+ * rounding.  No counterpart in the reference (no learned proposal there). */
+int askja_flow_hmc_trajectory_nested(askja_ctx *ctx, const askja_hmc_params *p, int inner, double alpha,
+                                     uint64_t traj_counter, askja_hmc_result *res);
 /* askja_wilson_lines of U = F(V). */
 int askja_flow_wilson_lines(askja_ctx *ctx, int dir, double *out);
 

@@ -3,10 +3,11 @@
 (the fused kick+drift), the flow's correction Phi - beta S_W on the coarse
 step.  With one inner step it is the one-scale trajectory up to rounding
 (same links, dH, accept decision); its energy error scales as dt^2 when both
-time scales are halved; a non-divisor inner count is refused; and its chain
-samples the target: with the spectral map on 32^2 SU(3) at beta 6 the
-plaquette of U = F(V) agrees with the exact two-dimensional value
-(tests/golden/exact_2d_plaquette.json)."""
+time scales are halved, with the plain force unscaled and scaled
+(alpha != 1); a non-divisor inner count is refused; and its chain samples
+the target: with the spectral map on 32^2 SU(3) at beta 6, with the plain
+force scaled by 0.8, the plaquette of U = F(V) agrees with the exact
+two-dimensional value (tests/golden/exact_2d_plaquette.json)."""
 import json
 import math
 import os
@@ -65,7 +66,8 @@ def test_one_inner_step_is_the_one_scale_trajectory(kind):
     assert np.max(np.abs(V1 - V2)) < 1e-11
 
 
-def test_nested_energy_error_scales_as_dt_squared():
+@pytest.mark.parametrize("alpha", [1.0, 0.9])
+def test_nested_energy_error_scales_as_dt_squared(alpha):
     """Fine step and coarse step halved together (4 fine steps per coarse
     step, trajectory length 0.2): |dH| drops about fourfold."""
     lat = _stout(5)
@@ -73,9 +75,9 @@ def test_nested_energy_error_scales_as_dt_squared():
     dh = []
     for dt, nd in ((0.04, 5 * 4), (0.02, 10 * 4)):
         lat.set_links(V0)
-        r = lat.flow_hmc_trajectory(6.0, dt / 4, nd, 8, 7, 11, reunit=False, inner=4)
+        r = lat.flow_hmc_trajectory(6.0, dt / 4, nd, 8, 7, 11, reunit=False, inner=4, alpha=alpha)
         dh.append(abs(r.dh))
-    print(f"nested FTHMC |dH| at coarse step 0.04 / 0.02: {dh}")
+    print(f"nested FTHMC (alpha {alpha}) |dH| at coarse step 0.04 / 0.02: {dh}")
     assert 2.5 < dh[0] / dh[1] < 6.0, dh
 
 
@@ -96,21 +98,25 @@ def _mean_err(x):
 
 def test_nested_spectral_su3_exact_plaquette():
     """32^2 SU(3) at beta 6, one spectral sweep (random weights), trajectory
-    length 1.0 as 40 fine steps of 0.025 with the flow's correction every 4:
-    11 flow gradients per trajectory, the plaquette of U = F(V) against the
-    exact value."""
+    length 1.0 as 40 fine steps of 0.025 with the flow's correction every 4
+    (11 flow gradients per trajectory) against 0.8 times the plain force
+    (any scale keeps the chain exact): the plaquette of U = F(V) against
+    the exact value."""
     n, beta = 3, 6.0
     lat = _spectral(n, (32, 32), beta, 11, 60, 0.1, 10, 0.2)
+    alpha = 0.8
+    print(f"least-squares plain-force scale of this map: {L.plain_force_scale(lat, beta):.4f}")
     for t in range(20):
-        lat.flow_hmc_trajectory(beta, 0.025, 40, 8, 12, t, inner=4)
+        lat.flow_hmc_trajectory(beta, 0.025, 40, 8, 12, t, inner=4, alpha=alpha)
     plaq, acc = [], 0
     for t in range(500):
-        r = lat.flow_hmc_trajectory(beta, 0.025, 40, 8, 12, 100 + t, inner=4)
+        r = lat.flow_hmc_trajectory(beta, 0.025, 40, 8, 12, 100 + t, inner=4, alpha=alpha)
         acc += r.accepted
         plaq.append(1.0 - r.action / lat.n_plaquettes())
     m, e = _mean_err(plaq)
     gold = json.load(open(os.path.join(ROOT, "tests", "golden", "exact_2d_plaquette.json")))
     want = [p["plaquette"] for p in gold["points"] if p["nc"] == n and p["beta"] == beta][0]
-    print(f"SU(3) 32^2 nested FTHMC: plaquette {m:.6f} +- {e:.6f} (exact {want:.6f}), acceptance {acc / 500:.2f}")
+    print(f"SU(3) 32^2 nested FTHMC (alpha {alpha:.4f}): plaquette {m:.6f} +- {e:.6f} (exact {want:.6f}), "
+          f"acceptance {acc / 500:.2f}")
     assert acc / 500 > 0.5
     assert abs(m - want) < 4 * e + 1e-4, (m, e, want)
