@@ -548,6 +548,34 @@ def flow_study(reps=5):
     lat.close()
 
 
+def fthmc_study(args):
+    """The bench's learned chain alone, for profiling: 20 plain trajectories
+    from a Haar start, the trained map of --weights, then --steps
+    learned-proposal trajectories (the correction every --flow-inner MD
+    steps; no relaxation, so the acceptance is the first trajectories')."""
+    import askja_amd as A
+    from askja_amd import learned as L
+
+    w, recipe = L.load_weights(args.weights)
+    lat = A.Lattice(C4_NC, C4_EXT, A.FP64)
+    lat.haar(SEED, 3)
+    for t in range(20):
+        lat.hmc_trajectory(C4_BETA, C4_DT, C4_NDRIFT, C4_LEVEL, SEED, t, reunit=True)
+    mu, par, blk = L.sweep_masks(4, recipe["layers"])
+    lat.flow_define(mu, par, blk, recipe["layers"], recipe["rho_max"], recipe["delta"], kind=recipe.get("kind"))
+    lat.flow_set_weights(w)
+    inner = args.flow_inner if args.flow_inner > 1 else None
+    lat.sync()
+    t0 = time.perf_counter()
+    acc = sum(lat.flow_hmc_trajectory(C4_BETA, C4_DT, C4_NDRIFT, C4_LEVEL, SEED, 100 + k, reunit=True,
+                                      inner=inner).accepted for k in range(args.steps))
+    lat.sync()
+    T = time.perf_counter() - t0
+    print(json.dumps({"study": "fthmc", "flow_inner": args.flow_inner, "trajectories": args.steps,
+                      "traj_per_s": args.steps / T, "acceptance": acc / args.steps}), flush=True)
+    lat.close()
+
+
 def flow2d_study(reps=3):
     """HIP-event time of the two-dimensional spectral map at the C5 size
     (SU(64), 128^2, one sweep = 8 sub-layers of 4096 updated links, random
@@ -745,7 +773,8 @@ def main():
     ap.add_argument("--copy-probe", action="store_true", help="only run a 1 GiB device copy (counter calibration)")
     ap.add_argument("--kernel-only", choices=["c2", "c4", "c3", "c5"],
                     help="only time --roofline-launches fused kick+drift launches of that lattice (for rocprofv3)")
-    ap.add_argument("--study", choices=["precision", "c3scan", "c5exact", "kernels", "example", "flow", "flow2d"],
+    ap.add_argument("--study", choices=["precision", "c3scan", "c5exact", "kernels", "example", "flow", "flow2d",
+                                        "fthmc"],
                     help="print a precision, beta-scan, exact-2D, per-config kernel or example-params study instead")
     ap.add_argument("--study-configs", default="C2,C3,C5")
     ap.add_argument("--study-therm", type=int, default=20, help="thermalising trajectories (c5exact)")
@@ -787,6 +816,9 @@ def main():
         return
     if args.study == "flow":
         flow_study()
+        return
+    if args.study == "fthmc":
+        fthmc_study(args)
         return
     if args.study == "c5exact":
         c5_exact(max(args.steps, 2), args.study_therm, tuple(args.study_precisions.split(",")))
