@@ -566,6 +566,97 @@ __device__ inline void kick_compact(const KickArgs &a, long l, int n, const R *W
 // LS: link storage type (FP32 links with FP64 arithmetic for the exact
 // action of the reduced-precision modes).  ActionOnly: only sum ReTr W
 // (= 4 sum of the plaquettes touching the link) into the partials.
+// The drift's truncated exponential X = sum_{j <= L} R^j / j! of R = i eps P
+// (the Taylor series of sunmat.h:200-248) in accumulator layout.  On entry
+// slot A holds R (stored as a left operand) and (rr, ri) are R's lane
+// entries; slots A and B are overwritten.  The caller has a barrier between
+// its last read of slot B and this call.  L <= 3: Horner, X = 1 + (R / k) X,
+// L - 1 products.  L >= 4: the even/odd split X = sum_i R^{2i} (a_{2i} + a_{2i+1} R),
+// a_j = 1 / j!, by Horner in R^2: 1 + ceil((L + 1) / 2) - 1 products (5
+// instead of 7 at L = 8, 4 instead of 5 at L = 6).  Same polynomial, other
+// rounding: the non-exact modes only.
+template <int P, int NMAX, typename R>
+__device__ inline void taylor_poly(const R (&rr)[Cfg<P, NMAX>::NCT][4], const R (&ri)[Cfg<P, NMAX>::NCT][4], int level,
+                                   const double *cinv, int n, R *Are, R *Aim, R *Bre, R *Bim, Acc<P, NMAX> &X)
+{
+    constexpr int NCT = Cfg<P, NMAX>::NCT;
+    if (level < 4) {
+        const R cl = level <= 1 ? R(1) : (R)cinv[level];
+#pragma unroll
+        for (int t = 0; t < NCT; t++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int i = acc_row<P, NMAX>(q), j = acc_col<P, NMAX>(t);
+                const R d = (i == j && i < n) ? R(1) : R(0);
+                X.r[t][q] = level == 0 ? d : d + cl * rr[t][q];
+                X.i[t][q] = level == 0 ? R(0) : cl * ri[t][q];
+            }
+        for (int k = level - 1; k >= 1; k--) {
+            __syncthreads();
+            acc_store<true>(X, Bre, Bim);
+            __syncthreads();
+            Acc<P, NMAX> Q;
+            acc_zero(Q);
+            cmul<P, NMAX, false, false>(Are, Aim, Bre, Bim, Q);
+            const R ck = (R)cinv[k];
+#pragma unroll
+            for (int t = 0; t < NCT; t++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int i = acc_row<P, NMAX>(q), j = acc_col<P, NMAX>(t);
+                    X.r[t][q] = ((i == j && i < n) ? R(1) : R(0)) + ck * Q.r[t][q];
+                    X.i[t][q] = ck * Q.i[t][q];
+                }
+        }
+        return;
+    }
+    // a_j = 1 / j! = cinv[1] ... cinv[j], zero beyond the level (uniform
+    // scalars, recomputed per block rather than indexed from a local array)
+    auto fact_inv = [&](int j) {
+        double f = 1.0;
+        for (int k = 1; k <= j; k++) f *= cinv[k];
+        return j <= level ? f : 0.0;
+    };
+    // R^2 = R R: R also as the right operand, then R^2 as the left one
+#pragma unroll
+    for (int t = 0; t < NCT; t++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            X.r[t][q] = rr[t][q];
+            X.i[t][q] = ri[t][q];
+        }
+    acc_store<true>(X, Bre, Bim);
+    __syncthreads();
+    Acc<P, NMAX> Q;
+    acc_zero(Q);
+    cmul<P, NMAX, false, false>(Are, Aim, Bre, Bim, Q);
+    __syncthreads();
+    acc_store<false>(Q, Are, Aim);
+    // Horner in R^2 over the blocks a_{2i} + a_{2i+1} R, from the last
+    const int nb = (level + 2) / 2;
+    // X <- a_{2b} + a_{2b+1} R (+ Q)
+    auto block = [&](int b, bool addq) {
+        const R c0 = (R)fact_inv(2 * b), c1 = (R)fact_inv(2 * b + 1);
+#pragma unroll
+        for (int t = 0; t < NCT; t++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int i = acc_row<P, NMAX>(q), j = acc_col<P, NMAX>(t);
+                X.r[t][q] = ((i == j && i < n) ? c0 : R(0)) + c1 * rr[t][q] + (addq ? Q.r[t][q] : R(0));
+                X.i[t][q] = c1 * ri[t][q] + (addq ? Q.i[t][q] : R(0));
+            }
+    };
+    block(nb - 1, false);
+    for (int b = nb - 2; b >= 0; b--) {
+        __syncthreads();
+        acc_store<true>(X, Bre, Bim);
+        __syncthreads();
+        acc_zero(Q);
+        cmul<P, NMAX, false, false>(Are, Aim, Bre, Bim, Q);
+        block(b, true);
+    }
+}
+
 // Register budgets (launch-bounds minimum of wavefronts per SIMD).  FP64 at
 // NMAX = 32 needs 236 VGPRs plus 64 AGPRs, one wavefront per SIMD; compiled
 // for two (a few registers spilled to scratch) the kick+drift runs 22-25 %
@@ -720,34 +811,18 @@ __global__ __launch_bounds__((Cfg<P, NMAX>::NT), (kMinWaves<P, NMAX>)) void k_ki
                 put<P, NMAX, false, R>(Are, Aim, i, j, -(pim[t][q] * ed), pr[t][q] * ed);  // rhs as A
             }
         s0.template early<PF, true>(U, l, n);  // U_mu(x) for the final product
-        Acc<P, NMAX> X;
-        const R cl = level <= 1 ? R(1) : (R)a.cinv[level];
+        // R = i eps_d P in registers; every lane has read slot B (the momentum)
 #pragma unroll
         for (int t = 0; t < NCT; t++)
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const int i = acc_row<P, NMAX>(q), j = acc_col<P, NMAX>(t);
-                const R d = (i == j && i < n) ? R(1) : R(0);
-                X.r[t][q] = level == 0 ? d : d - cl * (pim[t][q] * ed);
-                X.i[t][q] = level == 0 ? R(0) : cl * (pr[t][q] * ed);
+                const R re = -(pim[t][q] * ed), im = pr[t][q] * ed;
+                pr[t][q] = re;
+                pim[t][q] = im;
             }
-        for (int k = level - 1; k >= 1; k--) {
-            __syncthreads();
-            acc_store<true>(X, Bre, Bim);
-            __syncthreads();
-            Acc<P, NMAX> Q;
-            acc_zero(Q);
-            cmul<P, NMAX, false, false>(Are, Aim, Bre, Bim, Q);
-            const R ck = (R)a.cinv[k];
-#pragma unroll
-            for (int t = 0; t < NCT; t++)
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const int i = acc_row<P, NMAX>(q), j = acc_col<P, NMAX>(t);
-                    X.r[t][q] = ((i == j && i < n) ? R(1) : R(0)) + ck * Q.r[t][q];
-                    X.i[t][q] = ck * Q.i[t][q];
-                }
-        }
+        __syncthreads();
+        Acc<P, NMAX> X;
+        taylor_poly<P, NMAX, R>(pr, pim, level, a.cinv, n, Are, Aim, Bre, Bim, X);
         __syncthreads();
         acc_store<false>(X, Bre, Bim);
         s0.template late<PF, true>(U, l, n, Are, Aim);
@@ -822,42 +897,19 @@ __global__ __launch_bounds__((Cfg<P, NMAX>::NT), (kMinWaves<P, NMAX>)) void k_dr
             pr[t][q] = pim[t][q] = R(0);
             if (i < n && j < n) pr[t][q] = mom_el(Pm, l, n, i, j, pim[t][q]);
         }
-    // rhs = i eps P into slot A (zero outside n x n); X = 1 + c_L rhs;
-    // Horner steps X = 1 + c_k (rhs X) (sunmat.h:200-248)
-#pragma unroll
-    for (int t = 0; t < NCT; t++)
-#pragma unroll
-        for (int q = 0; q < 4; q++)
-            put<P, NMAX, false, R>(Are, Aim, acc_row<P, NMAX>(q), acc_col<P, NMAX>(t), -(pim[t][q] * ed),
-                                   pr[t][q] * ed);
-    Acc<P, NMAX> X;
-    const R cl = level <= 1 ? R(1) : (R)cinv.c[level];
+    // R = i eps P into slot A (zero outside n x n) and in registers; the
+    // truncated exponential of R (sunmat.h:200-248)
 #pragma unroll
     for (int t = 0; t < NCT; t++)
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const int i = acc_row<P, NMAX>(q), j = acc_col<P, NMAX>(t);
-            const R d = (i == j && i < n) ? R(1) : R(0);
-            X.r[t][q] = level == 0 ? d : d - cl * (pim[t][q] * ed);
-            X.i[t][q] = level == 0 ? R(0) : cl * (pr[t][q] * ed);
+            const R re = -(pim[t][q] * ed), im = pr[t][q] * ed;
+            put<P, NMAX, false, R>(Are, Aim, acc_row<P, NMAX>(q), acc_col<P, NMAX>(t), re, im);
+            pr[t][q] = re;
+            pim[t][q] = im;
         }
-    for (int k = level - 1; k >= 1; k--) {
-        __syncthreads();
-        acc_store<true>(X, Bre, Bim);
-        __syncthreads();
-        Acc<P, NMAX> Q;
-        acc_zero(Q);
-        cmul<P, NMAX, false, false>(Are, Aim, Bre, Bim, Q);
-        const R ck = (R)cinv.c[k];
-#pragma unroll
-        for (int t = 0; t < NCT; t++)
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int i = acc_row<P, NMAX>(q), j = acc_col<P, NMAX>(t);
-                X.r[t][q] = ((i == j && i < n) ? R(1) : R(0)) + ck * Q.r[t][q];
-                X.i[t][q] = ck * Q.i[t][q];
-            }
-    }
+    Acc<P, NMAX> X;
+    taylor_poly<P, NMAX, R>(pr, pim, level, cinv.c, n, Are, Aim, Bre, Bim, X);
     __syncthreads();
     acc_store<false>(X, Bre, Bim);
     ld_link<P, NMAX, true>(Are, Aim, U, l, n);

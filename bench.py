@@ -387,6 +387,18 @@ MIN_SERIES = {"plain": 3000, "learned": 300}
 MAX_SERIES = {"plain": 20000, "learned": 2000}
 
 
+def drift_products(nc, level, precision):
+    """Complex N x N products of one drift: the Taylor series to LEVEL by
+    Horner (LEVEL - 1) then U (1); the MFMA kernels (N >= 9, not the exact
+    mode) evaluate it for LEVEL >= 4 as even/odd blocks by Horner in R^2:
+    1 + ceil((LEVEL + 1) / 2) - 1 products, then U."""
+    import askja_amd as A
+
+    if nc >= 9 and precision != A.FP64_EXACT and level >= 4:
+        return (level + 2) // 2 + 1
+    return max(level, 1)
+
+
 def kick_drift_roofline(lat, nc, level, dt, launches):
     """The fused kick+drift launch timed with HIP events on the library's
     stream.  Algorithmic bytes of one MD step: six field passes of 2 N^2
@@ -394,14 +406,15 @@ def kick_drift_roofline(lat, nc, level, dt, launches):
     write U).  The fused kernel's own compulsory traffic is less (U read
     once, momenta compact Hermitian: read + write link, read + write N^2
     momentum reals) and is reported beside it.  Flops: (4(D-1) + 1 + LEVEL)
-    8 N^3 per link, over the average launch."""
+    8 N^3 per link (LEVEL: the drift's products, fewer on the MFMA kernels, see
+    drift_products), over the average launch."""
     import askja_amd as A
 
     nl = lat.V * lat.dim
     esz = 8 if lat.precision in (A.FP64, A.FP64_EXACT) else 4
     bytes_per_link = 6 * (2 * nc * nc * esz)
     compulsory_per_link = 2 * (2 * nc * nc * esz) + 2 * (nc * nc * esz)
-    flops_per_link = (4 * (lat.dim - 1) + 1 + level) * 8 * nc ** 3
+    flops_per_link = (4 * (lat.dim - 1) + 1 + drift_products(nc, level, lat.precision)) * 8 * nc ** 3
     lat.mom_refresh(SEED, 10**9, 5.9)
     for _ in range(5):
         lat.kick_drift(dt, dt, level)
