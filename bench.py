@@ -265,7 +265,7 @@ def c5_exact(ntraj, therm=20, precs=("bf16x3", "fp64")):
 def kernel_study(reps=None):
     """Per-config time of one fused kick+drift launch (HIP events on the
     library's stream) and of one reunitarisation, every precision; flops per
-    link (4(D-1) + 1 + LEVEL) 8 N^3."""
+    link (4(D-1) + 1 + the drift's products) 8 N^3 (drift_products)."""
     import askja_amd as A
 
     cfgs = [("C2", 3, [8, 32, 32, 32], 6), ("C4", 3, [16, 16, 16, 16], 8), ("C1", 8, [10, 10, 10, 10], 8),
@@ -288,7 +288,7 @@ def kernel_study(reps=None):
             lat.timer_start()
             lat.reunitarize()
             ms_re = lat.timer_stop()
-            flops = (4 * (lat.dim - 1) + 1 + level) * 8 * n ** 3 * lat.V * lat.dim
+            flops = (4 * (lat.dim - 1) + 1 + drift_products(n, level, prec)) * 8 * n ** 3 * lat.V * lat.dim
             tf = flops / ms / 1e9
             # the peak of the unit the arithmetic runs on: FP64 / FP32 vector or
             # matrix (the same figures on MI355X), or for BF16x3 the dense BF16
