@@ -79,7 +79,7 @@ struct askja_ctx {
     double *flow_contrib = nullptr;  // backward neighbour contributions
     void *flow_save = nullptr;       // chain state at the start of a flow trajectory
     int flow_kind = kFlowRankOne;
-    double *flow_st_wr = nullptr;    // stout: W and rho of every sub-layer's updated links
+    double *flow_st_wr = nullptr;    // stout: W, rho and the features of every sub-layer's updated links
     double *flow_st_aux = nullptr;   // stout: inverse Jacobian and eigenbasis of every sub-layer
     double *flow_st_lg = nullptr;    // stout: log-det gradient of every sub-layer
     // stout: the log-det gradients run on a second stream beside the rest of
@@ -1266,7 +1266,7 @@ int flow_gradient(askja_ctx *c, double beta, bool wgrad, double *action)
         if (f.kind == kFlowStout)  // its log-det gradient, from the second stream
             HIPCHK(c, hipStreamWaitEvent(c->st, c->flow_ev[2 * k + 1], 0));
         HIPCHK(c, launch_flow_backward(c->geom, c->flow_work, f, c->mode, c->flow_g, -1.0, gw, flow_log_of(c, k),
-                                       c->flow_contrib, flow_lg_of(c, k), c->st));
+                                       c->flow_contrib, flow_lg_of(c, k), flow_wr_of(c, k), c->st));
     }
     if (action) {
         double h[2];

@@ -722,8 +722,9 @@ __device__ inline bool site_of(const Geom &g, int par, long t, long &s)
 // One sub-layer in place (Uin may equal Uout: no thread reads another
 // thread's updated link).  With log, the sub-layer's input links are kept,
 // link of parity-site t at index t (64-link tiles), for the backward.
-// ST: the stout update U' = exp(-i rho TAH(W)) U instead, with W and rho of
-// every updated link written to wr (stride T) for k_stout_logdet; no partials.
+// ST: the stout update U' = exp(-i rho TAH(W)) U instead, with W, rho and the
+// features of every updated link written to wr (stride T: rows 0-17 W, 18
+// rho, 19 on the features) for k_stout_logdet and the backward; no partials.
 template <typename R, int D, int MU, bool ST>
 __global__ __launch_bounds__(2 * kLinks) void k_forward(Geom g, const R *Uin, R *Uout, Sub sb, double *part, R *log,
                                                         double *wr, long T)
@@ -751,6 +752,8 @@ __global__ __launch_bounds__(2 * kLinks) void k_forward(Geom g, const R *Uin, R 
                 wr[(size_t)(9 + e) * T + t] = F.W.i[e / 3][e % 3];
             }
             wr[(size_t)18 * T + t] = F.rho;
+#pragma unroll
+            for (int q = 0; q < nfeat(D); q++) wr[(size_t)(19 + q) * T + t] = F.f[q];
         }
     } else {
         if (ok && role == 0) {
@@ -775,7 +778,9 @@ __device__ inline double wave_sum(double v)
 // with respect to its input on exit.  c_logj = dPhi/dlogJ (-1 for
 // Phi = beta S - log J).  gw (may be null) accumulates dPhi/dweights of the
 // sub-layer's MLP block.  Every forward quantity is recomputed from the
-// sub-layer input (layer recomputation; nothing but the input is stored).
+// sub-layer input (layer recomputation; nothing but the input is stored),
+// except that the stout kind reads W, rho and the features from wr, where
+// its forward wrote them (the same values: the same code on the same links).
 // Lie-gradient contributions of c ReTr R (c = dPhi/df / 6 with
 // f = ReTr R / 3) to the six links of rectangle R = A_1 ... A_6 (A_1..3 = X,
 // A_4..6 = X^+): with the cyclic rotations M_k = A_k ... A_6 A_1 ... A_{k-1},
@@ -810,7 +815,8 @@ __device__ inline void rect_grad(const R *U, const Rect &r, double c, double *co
 // comes from k_stout_logdet_grad in lg (stride T), c_logj already applied.
 template <typename R, int D, int MU, bool ST>
 __global__ __launch_bounds__(2 * kLinks) void k_backward(Geom g, R *U, Sub sb, double *gr, double c_logj, double *gw,
-                                                         const R *log, double *contrib, long T, const double *lg)
+                                                         const R *log, double *contrib, long T, const double *lg,
+                                                         const double *wr)
 {
     constexpr int nf = nfeat(D), np = nplanes(D);
     __shared__ Xch<D> x;
@@ -829,7 +835,20 @@ __global__ __launch_bounds__(2 * kLinks) void k_backward(Geom g, R *U, Sub sb, d
         F.U = ld(log, t);
         if (role == 0) st(U, l, F.U);
     }
-    forward_link<R, D, MU, ST>(g, U, sb, s, ok, role, lane, x, F);
+    if constexpr (ST) {
+        if (ok) {
+#pragma unroll
+            for (int e = 0; e < 9; e++) {
+                F.W.r[e / 3][e % 3] = wr[(size_t)e * T + t];
+                F.W.i[e / 3][e % 3] = wr[(size_t)(9 + e) * T + t];
+            }
+            F.rho = wr[(size_t)18 * T + t];
+#pragma unroll
+            for (int q = 0; q < nf; q++) F.f[q] = wr[(size_t)(19 + q) * T + t];
+        }
+    } else {
+        forward_link<R, D, MU, ST>(g, U, sb, s, ok, role, lane, x, F);
+    }
     // Both roles need dPhi/dz; role 0 then needs dPhi/dC (through Lhat, the
     // dexp adjoint series), role 1 the own link's gradient (through E) and
     // dPhi/dS.
@@ -1391,7 +1410,7 @@ struct BwdKS {
 // parity-site stride of the per-link structure-of-arrays buffers
 static long sub_stride(const Geom &g) { return (long)(flow_log_reals(g) / 18); }
 
-size_t stout_wr_doubles(const Geom &g) { return (size_t)19 * sub_stride(g); }
+size_t stout_wr_doubles(const Geom &g) { return (size_t)(19 + flow::nfeat(g.D)) * sub_stride(g); }
 size_t stout_aux_doubles(const Geom &g) { return (size_t)85 * sub_stride(g); }
 size_t stout_lg_doubles(const Geom &g) { return (size_t)19 * sub_stride(g); }
 
@@ -1438,25 +1457,26 @@ hipError_t launch_flow_forward(const Geom &g, const void *Uin, void *Uout, const
 }
 
 hipError_t launch_flow_backward(const Geom &g, void *U, const FlowSub &f, int mode, double *gr, double c_logj,
-                                double *gw, const void *log, double *contrib, const double *lg, hipStream_t st)
+                                double *gw, const void *log, double *contrib, const double *lg, const double *st_wr,
+                                hipStream_t st)
 {
     const unsigned nb = par_blocks(g, f.par);
     const long T = sub_stride(g);
     const unsigned nl = (unsigned)((g.V * g.D + flow::kB - 1) / flow::kB);
     const bool stout = f.kind == kFlowStout;
-    if (stout && !lg) return hipErrorInvalidValue;
+    if (stout && (!lg || !st_wr)) return hipErrorInvalidValue;
     hipError_t e;
     if (f32_mode(mode)) {
         e = stout ? flow_dispatch<BwdKS, float>(g.D, f.mu, nb, st, g, (float *)U, to_sub(f), gr, c_logj, gw,
-                                                (const float *)log, contrib, T, lg)
+                                                (const float *)log, contrib, T, lg, st_wr)
                   : flow_dispatch<BwdK, float>(g.D, f.mu, nb, st, g, (float *)U, to_sub(f), gr, c_logj, gw,
-                                               (const float *)log, contrib, T, lg);
+                                               (const float *)log, contrib, T, lg, st_wr);
         if (e == hipSuccess) e = flow_dispatch<GatK, float>(g.D, f.mu, nl, st, g, gr, (const double *)contrib, f.par, T);
     } else {
         e = stout ? flow_dispatch<BwdKS, double>(g.D, f.mu, nb, st, g, (double *)U, to_sub(f), gr, c_logj, gw,
-                                                 (const double *)log, contrib, T, lg)
+                                                 (const double *)log, contrib, T, lg, st_wr)
                   : flow_dispatch<BwdK, double>(g.D, f.mu, nb, st, g, (double *)U, to_sub(f), gr, c_logj, gw,
-                                                (const double *)log, contrib, T, lg);
+                                                (const double *)log, contrib, T, lg, st_wr);
         if (e == hipSuccess)
             e = flow_dispatch<GatK, double>(g.D, f.mu, nl, st, g, gr, (const double *)contrib, f.par, T);
     }

@@ -163,16 +163,20 @@ struct FlowSub {
 };
 long flow_forward_partials(const Geom &g, int par);
 // one sub-layer in place; log (may be null) receives its input links; for
-// the stout kind, st_wr (stout_wr_doubles) receives W = U S and rho of every
-// updated link, which launch_stout_logdet turns into log det J
+// the stout kind, st_wr (stout_wr_doubles) receives W = U S, rho and the MLP
+// features of every updated link: launch_stout_logdet turns W and rho into
+// log det J, and the backward reads all three instead of recomputing them
 hipError_t launch_flow_forward(const Geom &g, const void *Uin, void *Uout, const FlowSub &f, int mode, double *part,
                                void *log, double *st_wr, hipStream_t st);
 // backward of one sub-layer; restores its input links from log into U
 // (contrib: flow_contrib_doubles scratch for the neighbour contributions;
-// lg: for the stout kind, the log-det gradient of launch_stout_logdet_grad)
+// lg: for the stout kind, the log-det gradient of launch_stout_logdet_grad;
+// st_wr: for the stout kind, what the sub-layer's forward wrote there)
 hipError_t launch_flow_backward(const Geom &g, void *U, const FlowSub &f, int mode, double *gr, double c_logj,
-                                double *gw, const void *log, double *contrib, const double *lg, hipStream_t st);
-// stout kind: per updated link W (18 doubles) and rho, structure of arrays
+                                double *gw, const void *log, double *contrib, const double *lg, const double *st_wr,
+                                hipStream_t st);
+// stout kind: per updated link W (18 doubles), rho and the features,
+// structure of arrays
 size_t stout_wr_doubles(const Geom &g);
 size_t stout_aux_doubles(const Geom &g);  // J^{-1}, eigenbasis (85 doubles per link)
 size_t stout_lg_doubles(const Geom &g);   // the log-det gradient (19 per link)
