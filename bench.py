@@ -514,11 +514,13 @@ def flow_roofline(ms_grad, sublayers):
     counts per updated link of the forward, log det, log-det gradient and
     backward kernels; 32,768 updated links per sub-layer at 16^4)."""
     links = int(np.prod(C4_EXT)) // 2  # one direction, one parity
-    flops = (15.6e3 + 6e3 + 15e3 + 59e3) * links * sublayers
+    # the backward reads W, rho and the features its forward stored, so it no
+    # longer repeats the forward's 15.6 k
+    flops = (15.6e3 + 6e3 + 15e3 + 43.4e3) * links * sublayers
     tf = flops / (ms_grad * 1e-3) / 1e12
     return {"bound": "fp64", "achieved": tf, "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s",
             "frac": tf / PEAK_TFLOPS["fp64"], "flops_per_gradient": flops,
-            "note": "latency-bound: the kernels need 294-512 registers (one wavefront per SIMD) and a sub-layer's "
+            "note": "latency-bound: the kernels need 290-512 registers (one wavefront per SIMD) and a sub-layer's "
                     "32,768 links fill the wavefront slots that allows"}
 
 
@@ -739,7 +741,7 @@ def main_c4(args):
         "flow_ms_per_gradient": ms_grad,
         # FP64 work of one flow gradient from the kernels' operation counts per
         # updated link (forward ~15.6 k, log det ~6 k, its gradient ~15 k,
-        # backward ~59 k flops), 1/8 of the links per sub-layer
+        # backward ~43.4 k flops), 1/8 of the links per sub-layer
         "flow_roofline": flow_roofline(ms_grad, recipe["sublayers"]),
         "flow_ms_per_forward": ms_fwd,
         "plain_ms_per_kick_drift": roof["ms_per_launch"],
