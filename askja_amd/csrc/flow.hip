@@ -731,7 +731,8 @@ __global__ __launch_bounds__(2 * kLinks) void k_forward(Geom g, const R *Uin, R 
 {
     __shared__ Xch<D> x;
     const int lane = (int)(threadIdx.x % kLinks), role = (int)(threadIdx.x / kLinks);
-    const long t = (long)blockIdx.x * kLinks + lane;
+    const unsigned lb = xcd_block(blockIdx.x, gridDim.x);  // XCD-contiguous link blocks
+    const long t = (long)lb * kLinks + lane;
     double sum[1] = {0.0}, cmp[1] = {0.0};
     long s;
     const bool ok = site_of(g, sb.par, t, s);
@@ -761,7 +762,7 @@ __global__ __launch_bounds__(2 * kLinks) void k_forward(Geom g, const R *Uin, R 
             st(Uout, MU * g.V + s, mm<false, false>(E, F.U));
             sum[0] = log1p(F.rho * F.K);
         }
-        block_reduce_store<1>(sum, cmp, part + 2 * (size_t)blockIdx.x);
+        block_reduce_store<1>(sum, cmp, part + 2 * (size_t)lb);
     }
 }
 
@@ -823,7 +824,7 @@ __global__ __launch_bounds__(2 * kLinks) void k_backward(Geom g, R *U, Sub sb, d
     __shared__ double sga[18][kLinks];  // stout: Gamma = R^T(G') from role 0
     __shared__ double sdr[kLinks];      //        and its d / d rho part
     const int lane = (int)(threadIdx.x % kLinks), role = (int)(threadIdx.x / kLinks);
-    const long t = (long)blockIdx.x * kLinks + lane;
+    const long t = (long)xcd_block(blockIdx.x, gridDim.x) * kLinks + lane;
     long s;
     const bool ok = site_of(g, sb.par, t, s);
     const long V = g.V;
