@@ -308,7 +308,8 @@ int askja_flow_download(askja_ctx *ctx, double *aos);
  * with wgrad != 0 the gradient with respect to the weights.  The forward pass
  * keeps only each sub-layer's input links (its masked links, 1/(2D) of a
  * field per sub-layer); the backward recomputes every sub-layer's
- * intermediates from that input.  Deterministic (no atomics on the link
+ * intermediates from that input (the stout kind: from W, rho and the
+ * features its forward kept per updated link).  Deterministic (no atomics on the link
  * gradient).  action = S_W(F(V)) (EnergyMag normalisation), logj = log det J(V). */
 int askja_flow_gradient(askja_ctx *ctx, double beta, int wgrad, double *action, double *logj);
 /* host AoS buffer <- Hermitian traceless g_l = sum_a T^a d/dt Phi(e^{i t T^a} V_l) */
