@@ -110,18 +110,29 @@ typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 
 // Two elements x = hi + lo at once: hi = bf16(x) (round to nearest even, one
-// packed conversion), lo = bf16(x - hi).  The two hi values are widened back
-// to FP32 from the packed word by a shift and a mask, so a pair costs six
-// VALU instructions.  No contraction: x - hi must not fuse with a product
-// that formed x.
+// packed conversion), lo = bf16(x - hi).  Each difference x_j - hi_j is one
+// bf16 dot product straight from the packed hi word, x_j + (-1) hi_j + 0 hi_k
+// (v_dot2c_f32_bf16): the products are exact and x_j - hi_j is representable
+// in FP32 (hi_j is x_j rounded to 8 bits, within a factor 2 of it), so the
+// dot product returns it exactly, as the subtraction from the widened hi_j
+// did.  A pair costs four VALU instructions (two conversions, two dot
+// products) instead of six (a shift and a mask to widen hi, two
+// subtractions).  The (-1, 0) and (0, -1) bf16 pairs are moved into SGPRs
+// by hand: written as constants, the compiler encodes 0x0000bf80 as the
+// inline constant -1.0, which the instruction reads as the FP32 pattern
+// 0xbf800000, i.e. the pair (0, -1) (checked on gfx950: the first element's
+// lo then came out as x0 + hi1).
 __device__ inline void split2(float x0, float x1, bf16x2 &hi, bf16x2 &lo)
 {
-#pragma clang fp contract(off)
     const f32x2 x = {x0, x1};
     hi = __builtin_convertvector(x, bf16x2);
-    const unsigned hb = __builtin_bit_cast(unsigned, hi);
-    const f32x2 h = {__builtin_bit_cast(float, hb << 16), __builtin_bit_cast(float, hb & 0xffff0000u)};
-    lo = __builtin_convertvector(x - h, bf16x2);
+    unsigned b0, b1;
+    asm("s_mov_b32 %0, 0xbf80" : "=s"(b0));
+    asm("s_mov_b32 %0, 0xbf800000" : "=s"(b1));
+    const bf16x2 m0 = __builtin_bit_cast(bf16x2, b0), m1 = __builtin_bit_cast(bf16x2, b1);
+    const f32x2 d = {__builtin_amdgcn_fdot2_f32_bf16(hi, m0, x0, false),
+                     __builtin_amdgcn_fdot2_f32_bf16(hi, m1, x1, false)};
+    lo = __builtin_convertvector(d, bf16x2);
 }
 
 // Operand slots.  FP64 / FP32: planar re / im matrices, row-major, row
