@@ -106,6 +106,17 @@ __device__ inline bf16x8 ld8(const __bf16 *p)
     return *(const bf16x8 *)p;
 }
 
+// Column of a BF16x3 slot element as stored: in rows 8-15 of every 16-row
+// block the two 16-byte chunks of each 16 k are swapped (k ^ 8).  A
+// wavefront writing one slot column over rows c = 0..15 (a transposed
+// store: rows c and c + 8 are 12 c dwords apart at row stride 24, the same
+// bank modulo 32) then hits distinct banks, while a product's read of eight
+// consecutive k stays one aligned 16-byte chunk.
+__device__ inline int bsw(int row, int k)
+{
+    return k ^ (row & 8);
+}
+
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 
@@ -147,7 +158,7 @@ __device__ inline void put(R *Sr, R *Si, int r, int c, R re, R im)
 {
     constexpr int LD = Cfg<P, NMAX>::LD;
     if constexpr (P == kBF16X3) {
-        const int o = TR ? c * LD + r : r * LD + c;
+        const int o = TR ? c * LD + bsw(c, r) : r * LD + bsw(r, c);
         __bf16 *rh = (__bf16 *)Sr, *ih = (__bf16 *)Si;
         bf16x2 h, l;
         split2(re, im, h, l);
@@ -227,13 +238,14 @@ __device__ inline void bf_steps(const __bf16 *arh, const __bf16 *aih, const __bf
         // with Ai_eff = -Ai for A^+ and Bi_eff = -Bi for B^+
         const unsigned mr = imag ? ((DA == DB) ? SG : 0u) : 0u;
         const unsigned mi = imag ? (DA ? SG : 0u) : (DB ? SG : 0u);
-        const __bf16 *ap = (imag ? aih : arh) + row * LD + kb;
+        const __bf16 *ap = (imag ? aih : arh) + row * LD + bsw(row, kb);
         const bf16x8 ah = ld8(ap), al = ld8(ap + PL);
         const bf16x8 rh = xsign(ah, mr), rl = xsign(al, mr), ih = xsign(ah, mi), il = xsign(al, mi);
         const __bf16 *bp = imag ? bih : brh, *cp = imag ? brh : bih;
 #pragma unroll
         for (int t = 0; t < NCT; t++) {
-            const int o = (16 * (t0 + t) + (lane & 15)) * LD + kb;
+            const int br = 16 * (t0 + t) + (lane & 15);
+            const int o = br * LD + bsw(br, kb);
             const bf16x8 bh = ld8(bp + o), bl = ld8(bp + PL + o);
             const bf16x8 ch = ld8(cp + o), cl = ld8(cp + PL + o);
             acc.r[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(rh, bh, acc.r[t], 0, 0, 0);
@@ -300,7 +312,7 @@ __device__ inline void acc_store(const Acc<P, NMAX> &a, R *Sr, R *Si)
         __bf16 *rp = (__bf16 *)Sr, *ip = (__bf16 *)Si;
 #pragma unroll
         for (int t = 0; t < Cfg<P, NMAX>::NCT; t++) {
-            const int o = acc_col<P, NMAX>(t) * LD + acc_row<P, NMAX>(0);
+            const int o = acc_col<P, NMAX>(t) * LD + bsw(acc_col<P, NMAX>(t), acc_row<P, NMAX>(0));
             bf16x4 h, l;
             split4(a.r[t], h, l);
             *(bf16x4 *)(rp + o) = h;
@@ -370,7 +382,8 @@ struct Items {
             // pairs along the stored row: (w0, w1), (w2, w3) as rows r, r + 1;
             // (w0, w2), (w1, w3) as columns c, c + 1 when TR
             constexpr int a0 = 0, a1 = TR ? 2 : 1, b0 = TR ? 1 : 2, b1 = 3;
-            const int o0 = TR ? c * LD + r : r * LD + c, o1 = o0 + LD;
+            // rows r, r + 1 (or c, c + 1) lie in one 8-row half: one swizzle
+            const int o0 = TR ? c * LD + bsw(c, r) : r * LD + bsw(r, c), o1 = o0 + LD;
             bf16x2 h, l;
             __bf16 *rp = (__bf16 *)Sr, *ip = (__bf16 *)Si;
             split2(re[a0], re[a1], h, l);
