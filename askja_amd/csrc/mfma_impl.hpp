@@ -16,7 +16,8 @@
 // checked on gfx950 with one-hot operands.  The A operand of lane l is
 // A[l%16][k0 + l/16] and the B operand B[k0 + l/16][l%16] for both 16x16x4
 // forms.
-// Operands are staged in two LDS slots (planar re/im, row stride NMAX + 4).
+// Operands are staged in two LDS slots (planar re/im, row stride Cfg::LD;
+// BF16x3 slots column-swizzled, see bsw).
 #pragma once
 #include "kernels.hpp"
 #include "common.hpp"
