@@ -31,7 +31,7 @@ def test_json_line_contract():
     plain chains' tau_int come from series of at least 100 tau with a Wolff
     window of at least 1, tau >= 1, so the value is a measurement."""
     # the driver's --steps / --warmup on the first sweep layer of the trained
-    # map and a shorter learned series than the default 300 (still extended
+    # map and a shorter learned series than the default 1500 (still extended
     # to 100 tau), so that the test stays within a minute
     r = _run(["--steps", "20", "--warmup", "5", "--roofline-launches", "5", "--cpu-steps", "2",
               "--layers", "1", "--learned-min-series", "120"], 900)
