@@ -24,6 +24,16 @@ def test_more_than_one_gpu_is_refused():
     assert "one GPU" in r.stderr
 
 
+def test_learned_md_steps_must_hold_whole_correction_intervals():
+    # 53 steps cannot be cut into intervals of --flow-inner 5: refused
+    # before any GPU work
+    r = _run(["--learned-md-steps", "53"], 120)
+    assert r.returncode != 0
+    assert "divisor of the 53 learned MD steps" in r.stderr
+    r = _run(["--learned-md-steps", "0"], 120)
+    assert r.returncode != 0 and "positive" in r.stderr
+
+
 @pytest.mark.gpu
 @pytest.mark.timeout(900)
 def test_json_line_contract():
